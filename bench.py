@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""E-step throughput of the gfx950 diagonal-GMM accumulator.
+
+A step is one E-step pass (posteriors + zeroth/first/second-order
+statistics + total log-likelihood, the reference's accumulateStatsMT) over
+the whole resident batch of synthetic frames.  Default workload: UBM-scale
+plain EM, D=60, M=512, N=10,000,000 frames (seeded true mixture), model
+initialised from 512 random frames with the global variance.
+
+Prints ONE JSON line (rank 0).  Frames are generated directly in HBM, so the
+timed region starts with inputs resident.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK = {"fp32": 157.3e12, "bf16": 2.5e15, "fp16": 2.5e15}
+HBM_PEAK = 8.0e12
+
+
+def cpu_baseline(x_sample, w, mu, var, budget_s=12.0, max_frames=10_000_000):
+    """Restated SSE CPU path (oracle/sse_port.c) on the host cores over a
+    bounded prefix of the same frames."""
+    from oracle import binding as orc
+    threads = max(1, min(16, os.cpu_count() or 1))
+    M = mu.shape[0]
+    n0 = min(4000, x_sample.shape[0])
+    t0 = time.perf_counter()
+    orc.sse_accumulate(x_sample[:n0], w, mu, var, threads)
+    dt = time.perf_counter() - t0
+    n = int(min(max_frames, x_sample.shape[0], max(n0, n0 * budget_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    _, _, _, _, used = orc.sse_accumulate(x_sample[:n], w, mu, var, threads)
+    dt = time.perf_counter() - t0
+    return {"value": n * M / dt, "unit": "frame*comp/s", "cores": int(used),
+            "kind": "port",
+            "sample": "first %d frames of the workload, %.1f s, reference SSE CPU path restated "
+                      "(oracle/sse_port.c), %d threads" % (n, dt, used)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="ubm_plain")
+    ap.add_argument("--frames", type=int, default=0, help="override N")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the E-step of this build runs on a single "
+              "MI355X (frames would shard N/K per GPU with one all-reduce of the statistics "
+              "per E-step, not built here). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import gmm_amd
+    from gmm_amd import synth
+
+    cfg = dict(synth.CONFIGS[args.workload])
+    if args.frames:
+        cfg["N"] = args.frames
+    D, M, N = cfg["D"], cfg["M"], cfg["N"]
+    w0, mu0, var0 = synth.true_mixture(**cfg)
+    x = synth.sample_frames_torch(w0, mu0, var0, N, cfg["seed"])
+    w, mu, var = synth.init_model_from_frames(x, M, cfg["seed"])
+    torch.cuda.synchronize()
+
+    acc = gmm_amd.Accumulator(0, profile=1)
+    acc.set_model(w, mu, var)
+    acc.set_frames_device(x.data_ptr(), N, D)
+    for _ in range(args.warmup):
+        acc.accumulate()
+    acc.reset_kernel_times()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = acc.accumulate()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    el = t1 - t0
+    value = N * M * args.steps / el
+    kt = acc.kernel_times()
+
+    flop_fc = 8 * D + 4
+    st_ms, st_n = kt["stats"]
+    ll_ms, ll_n = kt["loglik"]
+    rd_ms, rd_n = kt["reduce"]
+    st_avg = st_ms / max(st_n, 1) / 1e3
+    achieved = flop_fc * N * M / st_avg
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": None,
+                "kernel": "stats_kernel",
+                "avg_launch_ms": st_avg * 1e3,
+                "pipeline_ms": {"loglik": ll_ms / max(ll_n, 1), "stats": st_ms / max(st_n, 1),
+                                "reduce": rd_ms / max(rd_n, 1)},
+                "pipeline_frac": flop_fc * N * M / ((ll_ms + st_ms + rd_ms) / max(st_n, 1) / 1e3)
+                / PEAK["fp32"]}
+
+    line = {
+        "metric": "E-step frame*comp/s (FP32 plain form)",
+        "value": value,
+        "unit": "frame*comp/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded true mixture, frames generated in HBM)",
+        "config": {"workload": args.workload, "D": D, "M": M, "N": N,
+                   "precision": "fp32", "tot_loglike_per_frame": res[3] / N},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        ns = min(N, 10_000_000)
+        xs = x[:ns].cpu().numpy()
+        line["cpu_baseline"] = cpu_baseline(xs, w, mu, var)
+    acc.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

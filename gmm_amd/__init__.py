@@ -1,0 +1,210 @@
+"""gmm_amd -- MI355X (gfx950) diagonal-GMM EM accumulator.
+
+The product is the native library gmm_amd/_build/libgmmhip.so (HIP kernels,
+C ABI declared in include/gmmhip.h, C++ host layer).  This module is a thin
+ctypes binding of that C ABI used by the tests and bench.py; it mirrors the
+reference estimator's calls (insertModel, accumulateStatsMT,
+compVecsLogLikeMT of trainer/OL_GMMStatsEstimator_CUDA.h:43-56).
+
+There is no CPU fallback: if the library is missing or no GPU is present,
+the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "_build", "libgmmhip.so")
+
+EXPORTS = [
+    "gmmhip_default_opts", "gmmhip_device_count", "gmmhip_create",
+    "gmmhip_destroy", "gmmhip_set_opts", "gmmhip_last_error",
+    "gmmhip_set_model", "gmmhip_set_frames", "gmmhip_set_frames_contig",
+    "gmmhip_set_frames_device", "gmmhip_accumulate", "gmmhip_loglikes",
+    "gmmhip_loglikes_device", "gmmhip_stream", "gmmhip_sync",
+    "gmmhip_kernel_times", "gmmhip_reset_kernel_times",
+]
+KT_LOGLIK, KT_STATS, KT_REDUCE, KT_COUNT = 0, 1, 2, 3
+
+OK, ERR_HIP, ERR_OOM, ERR_NAN, ERR_ARG, ERR_VAR, ERR_STATE, ERR_MLLR, ERR_NODEV = range(9)
+PREC_FP32, PREC_BF16X3, PREC_FP16X3 = 0, 1, 2
+ACC_MEAN, ACC_VAR = 1, 2
+
+
+class GmmHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("gmmhip error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("min_loglike", ctypes.c_float), ("min_gamma", ctypes.c_float),
+                ("min_var", ctypes.c_float), ("precision", ctypes.c_int),
+                ("chunk_frames", ctypes.c_int), ("verbosity", ctypes.c_int),
+                ("profile", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libgmmhip.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libgmmhip.so not built: run `python -m gmm_amd.build` "
+                              "(expected at %s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        c_u64, c_dp, c_fp, c_vp = (ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_float), ctypes.c_void_p)
+        L.gmmhip_last_error.restype = ctypes.c_char_p
+        L.gmmhip_last_error.argtypes = [c_vp]
+        L.gmmhip_create.argtypes = [ctypes.c_int, ctypes.POINTER(Opts), ctypes.POINTER(c_vp)]
+        L.gmmhip_destroy.argtypes = [c_vp]
+        L.gmmhip_set_opts.argtypes = [c_vp, ctypes.POINTER(Opts)]
+        L.gmmhip_set_model.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_fp, c_fp, c_fp]
+        L.gmmhip_set_frames.argtypes = [c_vp, ctypes.POINTER(c_fp), c_u64, ctypes.c_int]
+        L.gmmhip_set_frames_contig.argtypes = [c_vp, c_fp, c_u64, ctypes.c_int, c_u64]
+        L.gmmhip_set_frames_device.argtypes = [c_vp, c_vp, c_u64, ctypes.c_int, c_u64]
+        L.gmmhip_accumulate.argtypes = [c_vp, ctypes.c_int, c_dp, c_dp, c_dp, c_dp,
+                                        ctypes.POINTER(c_u64)]
+        L.gmmhip_loglikes.argtypes = [c_vp, c_fp]
+        L.gmmhip_loglikes_device.argtypes = [c_vp, c_vp]
+        L.gmmhip_stream.restype = c_vp
+        L.gmmhip_stream.argtypes = [c_vp]
+        L.gmmhip_sync.argtypes = [c_vp]
+        L.gmmhip_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.gmmhip_default_opts.argtypes = [ctypes.POINTER(Opts)]
+        L.gmmhip_default_opts.restype = None
+        L.gmmhip_kernel_times.argtypes = [c_vp, c_dp, ctypes.POINTER(c_u64), ctypes.c_int]
+        L.gmmhip_reset_kernel_times.argtypes = [c_vp]
+        _lib = L
+    return _lib
+
+
+def default_opts(**kw):
+    o = Opts()
+    lib().gmmhip_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    lib().gmmhip_device_count(ctypes.byref(n))
+    return n.value
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Accumulator:
+    """One handle of the C ABI (one GPU, one stream)."""
+
+    def __init__(self, device=0, **opts):
+        self._h = ctypes.c_void_p()
+        self._opts = default_opts(**opts)
+        self._check(lib().gmmhip_create(device, ctypes.byref(self._opts), ctypes.byref(self._h)))
+        self.M = self.D = 0
+        self.N = 0
+        self._keep = None
+
+    def _check(self, rc):
+        if rc != OK:
+            msg = lib().gmmhip_last_error(self._h).decode() if self._h else ""
+            raise GmmHipError(rc, msg)
+
+    def close(self):
+        if self._h:
+            lib().gmmhip_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_opts(self, **kw):
+        for k, v in kw.items():
+            setattr(self._opts, k, v)
+        self._check(lib().gmmhip_set_opts(self._h, ctypes.byref(self._opts)))
+
+    def set_model(self, w, mu, var):
+        w = np.ascontiguousarray(w, np.float32)
+        mu = np.ascontiguousarray(mu, np.float32)
+        var = np.ascontiguousarray(var, np.float32)
+        M, D = mu.shape
+        self._check(lib().gmmhip_set_model(self._h, M, D, _fp(w), _fp(mu), _fp(var)))
+        self.M, self.D = M, D
+
+    def set_frames(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        N, D = x.shape
+        self._check(lib().gmmhip_set_frames_contig(self._h, _fp(x), N, D, D))
+        self.N = N
+
+    def set_frames_rows(self, rows):
+        """rows: list of float32 1-D arrays (the reference's float** frames)."""
+        arrs = [np.ascontiguousarray(r, np.float32) for r in rows]
+        ptrs = (ctypes.POINTER(ctypes.c_float) * len(arrs))(*[_fp(a) for a in arrs])
+        self._check(lib().gmmhip_set_frames(self._h, ptrs, len(arrs), arrs[0].shape[0]))
+        self.N = len(arrs)
+
+    def set_frames_device(self, ptr, N, D, ld=None):
+        self._check(lib().gmmhip_set_frames_device(self._h, ctypes.c_void_p(ptr), N, D,
+                                                   ld if ld is not None else D))
+        self.N = N
+
+    def accumulate(self, flags=ACC_MEAN | ACC_VAR, into=None):
+        """Returns (n[M], F[M,D], S[M,D], tot_ll, nframes); adds into `into` if given."""
+        if into is None:
+            n = np.zeros(self.M)
+            F = np.zeros((self.M, self.D))
+            S = np.zeros((self.M, self.D))
+            tot = ctypes.c_double(0.0)
+            nf = ctypes.c_uint64(0)
+        else:
+            n, F, S, tot0, nf0 = into
+            tot = ctypes.c_double(tot0)
+            nf = ctypes.c_uint64(nf0)
+        self._check(lib().gmmhip_accumulate(self._h, flags, _dp(n), _dp(F), _dp(S),
+                                            ctypes.byref(tot), ctypes.byref(nf)))
+        return n, F, S, tot.value, nf.value
+
+    def loglikes(self):
+        out = np.zeros(self.N, np.float32)
+        self._check(lib().gmmhip_loglikes(self._h, _fp(out)))
+        return out
+
+    def loglikes_device(self, ptr):
+        self._check(lib().gmmhip_loglikes_device(self._h, ctypes.c_void_p(ptr)))
+
+    def kernel_times(self):
+        """{class: (total_ms, launches)} of a handle created with profile=1."""
+        ms = np.zeros(KT_COUNT)
+        cnt = np.zeros(KT_COUNT, np.uint64)
+        self._check(lib().gmmhip_kernel_times(
+            self._h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), KT_COUNT))
+        return {"loglik": (ms[0], int(cnt[0])), "stats": (ms[1], int(cnt[1])),
+                "reduce": (ms[2], int(cnt[2]))}
+
+    def reset_kernel_times(self):
+        self._check(lib().gmmhip_reset_kernel_times(self._h))
+
+    def stream(self):
+        return lib().gmmhip_stream(self._h)
+
+    def sync(self):
+        self._check(lib().gmmhip_sync(self._h))

@@ -1,0 +1,63 @@
+// Internal interface between the C ABI (gmmhip_api.hip) and the E-step
+// kernels (estep_fp32.hip).  Not installed; the public surface is
+// include/gmmhip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gmmhip {
+
+// Frames per E-step log-likelihood block (4 waves x 2 sub-tiles x 16).
+constexpr int LL_FRAMES_PER_BLOCK = 128;
+// Frames staged per iteration of the statistics kernel.
+constexpr int ST_FRAMES_PER_ITER = 64;
+// Components owned by one statistics block (4 waves x 16).
+constexpr int ST_COMPS_PER_BLOCK = 64;
+
+// Row stride (floats) of the LDS frame tile [1, x, x^2, 0...] for a
+// contraction depth KP: a multiple of 64 plus 4, which keeps both MFMA
+// operand reads of the tile free of bank conflicts.
+constexpr int lds_stride(int kp) { return (kp + 63) / 64 * 64 + 4; }
+
+// Contraction depth for D dims: [1, x(D), x^2(D)] padded to 16.
+inline int kp_for_dim(int D) { return (2 * D + 1 + 15) / 16 * 16; }
+constexpr int KP_MAX = 256;  // D <= 127
+
+struct EstepArgs {
+    // frames, row-major [N][ldx], resident in HBM
+    const float *X;
+    uint64_t ldx;
+    uint64_t N;
+    int D;
+    int KP;
+    // packed model: [Mt][KP/4][64] floats (log2 units), centring offset [D]
+    const float *P;
+    int M;
+    int Mt;
+    const float *off;
+    float min_ll;
+    float min_gamma;
+    // per-frame outputs of the log-likelihood kernel
+    float *LL2;      // [N] log2-domain LL, -inf marks "no statistics"
+    float *LLnat;    // [N] natural LL (may be null)
+    double *blockLL; // [ceil(N/LL_FRAMES_PER_BLOCK)] per-block sums
+    // statistics
+    uint64_t chunk_frames;
+    int nchunks;
+    int ngroups;
+    float *partials; // [nchunks][Mt*16][KP]
+    double *red;     // [Mt*16][KP]
+    double *tot;     // [1]
+};
+
+inline uint64_t ll_blocks(uint64_t N) {
+    return (N + LL_FRAMES_PER_BLOCK - 1) / LL_FRAMES_PER_BLOCK;
+}
+
+hipError_t launch_loglik_fp32(const EstepArgs &a, hipStream_t s);
+hipError_t launch_stats_fp32(const EstepArgs &a, hipStream_t s);
+hipError_t launch_reduce_stats(const EstepArgs &a, hipStream_t s);
+hipError_t launch_sum_blockll(const EstepArgs &a, hipStream_t s);
+
+}  // namespace gmmhip

@@ -1,0 +1,523 @@
+// C ABI of the gfx950 diagonal-GMM accumulator (include/gmmhip.h).
+//
+// One handle = one device + one HIP stream + device buffers it owns.  The
+// model is preprocessed on the host in FP64 (insertModel,
+// trainer/OL_GMMStatsEstimator.cpp:234-313), mean-centred and packed into
+// the MFMA B-operand layout; frames stay resident in HBM row-major, so the
+// reference's host transpose into 8-frame tiles (alignVectors,
+// trainer/OL_GMMStatsEstimator_CUDA.cpp:292-328) has no counterpart here.
+#include "../../include/gmmhip.h"
+#include "estep.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace gmmhip;
+
+struct gmmhip_ctx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    gmmhip_opts opt{};
+    std::string err;
+
+    // model
+    int M = 0, D = 0, KP = 0, Mt = 0;
+    float *d_P = nullptr;
+    size_t P_cap = 0;
+    float *d_off = nullptr;
+    size_t off_cap = 0;
+    std::vector<float> h_off;
+
+    // frames
+    const float *d_x = nullptr;
+    uint64_t N = 0, ld = 0;
+    int xD = 0;
+    float *d_xown = nullptr;
+    size_t xown_cap = 0;
+
+    // scratch
+    float *d_LL2 = nullptr;
+    size_t LL2_cap = 0;
+    float *d_LLnat = nullptr;
+    size_t LLnat_cap = 0;
+    double *d_blockLL = nullptr;
+    size_t blockLL_cap = 0;
+    float *d_part = nullptr;
+    size_t part_cap = 0;
+    double *d_red = nullptr;
+    size_t red_cap = 0;
+    double *d_tot = nullptr;
+    double *h_res = nullptr;  // pinned: [Mp*KP] + 1
+    size_t h_res_cap = 0;
+
+    // kernel timers (opts.profile)
+    hipEvent_t ev[5] = {};
+    bool ev_ok = false;
+    double kt_ms[GMMHIP_KT_COUNT] = {};
+    uint64_t kt_n[GMMHIP_KT_COUNT] = {};
+};
+
+namespace {
+
+int fail(gmmhip_ctx *h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, expr)                                                              \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail((h), e_ == hipErrorOutOfMemory ? GMMHIP_ERR_OOM : GMMHIP_ERR_HIP, \
+                        "%s failed: %s", #expr, hipGetErrorString(e_));             \
+    } while (0)
+
+// Grow-only device buffer.
+template <typename T>
+hipError_t ensure(T **p, size_t *cap, size_t n) {
+    if (*cap >= n && *p) return hipSuccess;
+    if (*p) {
+        hipError_t e = hipFree(*p);
+        if (e != hipSuccess) return e;
+        *p = nullptr;
+        *cap = 0;
+    }
+    hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) *cap = n;
+    return e;
+}
+
+void apply_defaults(gmmhip_opts *o) {
+    if (o->min_loglike == 0.0f) o->min_loglike = -1e20f;
+    if (o->min_gamma == 0.0f) o->min_gamma = 1e-4f;
+    if (o->min_var == 0.0f) o->min_var = 1e-6f;
+}
+
+struct DevGuard {
+    int prev = -1;
+    explicit DevGuard(int d) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != d) hipSetDevice(d);
+        else prev = -1;
+    }
+    ~DevGuard() {
+        if (prev >= 0) hipSetDevice(prev);
+    }
+};
+
+// Frames per FP32 partial accumulator: enough chunks to give >= ~4096
+// blocks, at most 8192 frames per FP32 accumulator, a multiple of the
+// statistics kernel's frame step.
+uint64_t pick_chunk_frames(const gmmhip_ctx *h, int ngroups) {
+    if (h->opt.chunk_frames > 0) {
+        uint64_t cf = (uint64_t)h->opt.chunk_frames;
+        return (cf + ST_FRAMES_PER_ITER - 1) / ST_FRAMES_PER_ITER * ST_FRAMES_PER_ITER;
+    }
+    const uint64_t target_chunks = std::max<uint64_t>(1, 4096 / std::max(1, ngroups));
+    uint64_t cf = (h->N + target_chunks - 1) / target_chunks;
+    cf = (cf + ST_FRAMES_PER_ITER - 1) / ST_FRAMES_PER_ITER * ST_FRAMES_PER_ITER;
+    cf = std::max<uint64_t>(cf, 128);
+    cf = std::min<uint64_t>(cf, 8192);
+    return cf;
+}
+
+int check_ready(gmmhip_ctx *h) {
+    if (!h) return GMMHIP_ERR_ARG;
+    if (h->M <= 0) return fail(h, GMMHIP_ERR_STATE, "no model inserted");
+    if (!h->d_x || h->N == 0) return fail(h, GMMHIP_ERR_STATE, "no frames inserted");
+    if (h->xD != h->D)
+        return fail(h, GMMHIP_ERR_ARG, "frame dimension %d != model dimension %d", h->xD, h->D);
+    return GMMHIP_OK;
+}
+
+EstepArgs make_args(gmmhip_ctx *h) {
+    EstepArgs a{};
+    a.X = h->d_x;
+    a.ldx = h->ld;
+    a.N = h->N;
+    a.D = h->D;
+    a.KP = h->KP;
+    a.P = h->d_P;
+    a.M = h->M;
+    a.Mt = h->Mt;
+    a.off = h->d_off;
+    a.min_ll = h->opt.min_loglike;
+    a.min_gamma = h->opt.min_gamma;
+    return a;
+}
+
+int ensure_events(gmmhip_ctx *h) {
+    if (h->ev_ok) return GMMHIP_OK;
+    for (auto &e : h->ev) HIPCHK(h, hipEventCreate(&e));
+    h->ev_ok = true;
+    return GMMHIP_OK;
+}
+
+void add_time(gmmhip_ctx *h, int cls, int e0, int e1) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, h->ev[e0], h->ev[e1]) == hipSuccess) {
+        h->kt_ms[cls] += ms;
+        h->kt_n[cls] += 1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void gmmhip_default_opts(gmmhip_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    apply_defaults(o);
+    o->precision = GMMHIP_PREC_FP32;
+}
+
+int gmmhip_device_count(int *count) {
+    if (!count) return GMMHIP_ERR_ARG;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return GMMHIP_OK;
+}
+
+int gmmhip_create(int device, const gmmhip_opts *opts, gmmhip_t *out) {
+    if (!out) return GMMHIP_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return GMMHIP_ERR_NODEV;
+    if (device < 0 || device >= n) return GMMHIP_ERR_ARG;
+    gmmhip_ctx *h = new gmmhip_ctx();
+    h->dev = device;
+    if (opts) h->opt = *opts;
+    apply_defaults(&h->opt);
+    DevGuard g(device);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return GMMHIP_ERR_HIP;
+    }
+    if (hipMalloc((void **)&h->d_tot, sizeof(double)) != hipSuccess) {
+        hipStreamDestroy(h->stream);
+        delete h;
+        return GMMHIP_ERR_OOM;
+    }
+    *out = h;
+    return GMMHIP_OK;
+}
+
+int gmmhip_destroy(gmmhip_t h) {
+    if (!h) return GMMHIP_OK;
+    DevGuard g(h->dev);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    hipFree(h->d_P);
+    hipFree(h->d_off);
+    hipFree(h->d_xown);
+    hipFree(h->d_LL2);
+    hipFree(h->d_LLnat);
+    hipFree(h->d_blockLL);
+    hipFree(h->d_part);
+    hipFree(h->d_red);
+    hipFree(h->d_tot);
+    if (h->h_res) hipHostFree(h->h_res);
+    if (h->ev_ok)
+        for (auto &e : h->ev) hipEventDestroy(e);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return GMMHIP_OK;
+}
+
+int gmmhip_set_opts(gmmhip_t h, const gmmhip_opts *opts) {
+    if (!h || !opts) return GMMHIP_ERR_ARG;
+    h->opt = *opts;
+    apply_defaults(&h->opt);
+    return GMMHIP_OK;
+}
+
+const char *gmmhip_last_error(gmmhip_t h) { return h ? h->err.c_str() : "null handle"; }
+
+int gmmhip_kernel_times(gmmhip_t h, double *ms, uint64_t *count, int n) {
+    if (!h || !ms || !count || n < 0) return GMMHIP_ERR_ARG;
+    for (int i = 0; i < n && i < GMMHIP_KT_COUNT; i++) {
+        ms[i] = h->kt_ms[i];
+        count[i] = h->kt_n[i];
+    }
+    return GMMHIP_OK;
+}
+
+int gmmhip_reset_kernel_times(gmmhip_t h) {
+    if (!h) return GMMHIP_ERR_ARG;
+    for (int i = 0; i < GMMHIP_KT_COUNT; i++) {
+        h->kt_ms[i] = 0.0;
+        h->kt_n[i] = 0;
+    }
+    return GMMHIP_OK;
+}
+
+void *gmmhip_stream(gmmhip_t h) { return h ? (void *)h->stream : nullptr; }
+
+int gmmhip_sync(gmmhip_t h) {
+    if (!h) return GMMHIP_ERR_ARG;
+    DevGuard g(h->dev);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return GMMHIP_OK;
+}
+
+int gmmhip_set_model(gmmhip_t h, int M, int D, const float *w, const float *mu,
+                     const float *var) {
+    if (!h) return GMMHIP_ERR_ARG;
+    if (M <= 0 || D <= 0 || !w || !mu || !var)
+        return fail(h, GMMHIP_ERR_ARG, "set_model: bad arguments (M=%d, D=%d)", M, D);
+    if (kp_for_dim(D) > KP_MAX)
+        return fail(h, GMMHIP_ERR_ARG, "set_model: D=%d exceeds the supported maximum 127", D);
+    for (int m = 0; m < M; m++)
+        if (!(w[m] > 0.0f))
+            return fail(h, GMMHIP_ERR_ARG,
+                        "set_model: component %d has weight %g; insert only the active "
+                        "(weight > 0) components in RearrangeMixtures order", m, (double)w[m]);
+    // Per-component constants in FP64 (insertModel); the reference throws on
+    // var < min_var (OL_GMMStatsEstimator.cpp:279-285).
+    const double minus_log_2pi = -1.837877066409345;
+    std::vector<double> gconst(M);
+    for (int m = 0; m < M; m++) {
+        double slv = 0.0;
+        for (int d = 0; d < D; d++) {
+            const float v = var[(size_t)m * D + d];
+            if (!(v >= h->opt.min_var))
+                return fail(h, GMMHIP_ERR_VAR,
+                            "insertModel(): model contains very low variances [%g] (component %d, dim %d)",
+                            (double)v, m, d);
+            slv += std::log((double)v);
+        }
+        gconst[m] = minus_log_2pi * D / 2.0 - slv / 2.0 + std::log((double)w[m]);
+    }
+    // Centring offset: the weighted mean of the component means.
+    std::vector<double> offd(D, 0.0);
+    double wsum = 0.0;
+    for (int m = 0; m < M; m++) wsum += w[m];
+    for (int m = 0; m < M; m++)
+        for (int d = 0; d < D; d++) offd[d] += (double)w[m] * mu[(size_t)m * D + d] / wsum;
+    std::vector<float> off(D);
+    for (int d = 0; d < D; d++) off[d] = (float)offd[d];
+
+    const int KP = kp_for_dim(D);
+    const int KK = KP / 4;
+    const int Mt = (M + 15) / 16;
+    const double log2e = 1.4426950408889634;
+    std::vector<float> P((size_t)Mt * KK * 64, 0.0f);
+    std::vector<double> coef(KP);
+    for (int m = 0; m < M; m++) {
+        std::fill(coef.begin(), coef.end(), 0.0);
+        double cq = 0.0;
+        for (int d = 0; d < D; d++) {
+            const double iv = 1.0 / (double)var[(size_t)m * D + d];
+            const double mc = (double)mu[(size_t)m * D + d] - (double)off[d];
+            coef[1 + d] = mc * iv;
+            coef[1 + D + d] = -0.5 * iv;
+            cq += mc * mc * iv;
+        }
+        coef[0] = gconst[m] - 0.5 * cq;
+        const int t = m / 16, c = m % 16;
+        for (int k = 0; k < KP; k++) {
+            const int kk = k / 4, qq = k % 4;
+            P[((size_t)t * KK + kk) * 64 + qq * 16 + c] = (float)(coef[k] * log2e);
+        }
+    }
+    DevGuard g(h->dev);
+    HIPCHK(h, ensure(&h->d_P, &h->P_cap, P.size()));
+    HIPCHK(h, ensure(&h->d_off, &h->off_cap, (size_t)D));
+    HIPCHK(h, hipMemcpyAsync(h->d_P, P.data(), P.size() * sizeof(float), hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_off, off.data(), D * sizeof(float), hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->M = M;
+    h->D = D;
+    h->KP = KP;
+    h->Mt = Mt;
+    h->h_off = off;
+    return GMMHIP_OK;
+}
+
+int gmmhip_set_frames_contig(gmmhip_t h, const float *x, uint64_t N, int D, uint64_t ld) {
+    if (!h) return GMMHIP_ERR_ARG;
+    if (!x || N == 0 || D <= 0 || ld < (uint64_t)D)
+        return fail(h, GMMHIP_ERR_ARG, "set_frames: bad arguments");
+    DevGuard g(h->dev);
+    HIPCHK(h, ensure(&h->d_xown, &h->xown_cap, (size_t)N * D));
+    if (ld == (uint64_t)D) {
+        HIPCHK(h, hipMemcpy(h->d_xown, x, (size_t)N * D * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(h, hipMemcpy2D(h->d_xown, D * sizeof(float), x, ld * sizeof(float),
+                              D * sizeof(float), N, hipMemcpyHostToDevice));
+    }
+    h->d_x = h->d_xown;
+    h->N = N;
+    h->ld = D;
+    h->xD = D;
+    return GMMHIP_OK;
+}
+
+int gmmhip_set_frames(gmmhip_t h, const float *const *rows, uint64_t N, int D) {
+    if (!h) return GMMHIP_ERR_ARG;
+    if (!rows || N == 0 || D <= 0) return fail(h, GMMHIP_ERR_ARG, "set_frames: bad arguments");
+    bool contig = true;
+    for (uint64_t i = 1; i < N && contig; i++) contig = (rows[i] == rows[0] + i * (uint64_t)D);
+    if (contig) return gmmhip_set_frames_contig(h, rows[0], N, D, D);
+    DevGuard g(h->dev);
+    HIPCHK(h, ensure(&h->d_xown, &h->xown_cap, (size_t)N * D));
+    // Pack row blocks through a bounded host buffer.
+    const uint64_t blk = std::max<uint64_t>(1, (64ull << 20) / (sizeof(float) * D));
+    std::vector<float> buf((size_t)std::min<uint64_t>(blk, N) * D);
+    for (uint64_t s = 0; s < N; s += blk) {
+        const uint64_t e = std::min<uint64_t>(N, s + blk);
+        for (uint64_t i = s; i < e; i++)
+            std::memcpy(&buf[(size_t)(i - s) * D], rows[i], sizeof(float) * D);
+        HIPCHK(h, hipMemcpy(h->d_xown + (size_t)s * D, buf.data(), (size_t)(e - s) * D * sizeof(float),
+                            hipMemcpyHostToDevice));
+    }
+    h->d_x = h->d_xown;
+    h->N = N;
+    h->ld = D;
+    h->xD = D;
+    return GMMHIP_OK;
+}
+
+int gmmhip_set_frames_device(gmmhip_t h, const float *dx, uint64_t N, int D, uint64_t ld) {
+    if (!h) return GMMHIP_ERR_ARG;
+    if (!dx || N == 0 || D <= 0 || ld < (uint64_t)D)
+        return fail(h, GMMHIP_ERR_ARG, "set_frames_device: bad arguments");
+    h->d_x = dx;
+    h->N = N;
+    h->ld = ld;
+    h->xD = D;
+    return GMMHIP_OK;
+}
+
+int gmmhip_loglikes_device(gmmhip_t h, float *d_out) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!d_out) return fail(h, GMMHIP_ERR_ARG, "loglikes_device: null output");
+    DevGuard g(h->dev);
+    EstepArgs a = make_args(h);
+    HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, h->N));
+    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N)));
+    a.LL2 = h->d_LL2;
+    a.LLnat = d_out;
+    a.blockLL = h->d_blockLL;
+    HIPCHK(h, launch_loglik_fp32(a, h->stream));
+    return GMMHIP_OK;
+}
+
+int gmmhip_loglikes(gmmhip_t h, float *out) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!out) return fail(h, GMMHIP_ERR_ARG, "loglikes: null output");
+    DevGuard g(h->dev);
+    HIPCHK(h, ensure(&h->d_LLnat, &h->LLnat_cap, h->N));
+    const bool prof = h->opt.profile != 0;
+    if (prof) {
+        if ((rc = ensure_events(h))) return rc;
+        HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+    }
+    rc = gmmhip_loglikes_device(h, h->d_LLnat);
+    if (rc) return rc;
+    if (prof) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+    HIPCHK(h, hipMemcpyAsync(out, h->d_LLnat, h->N * sizeof(float), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (prof) add_time(h, GMMHIP_KT_LOGLIK, 0, 1);
+    // NaN check of compVecsLogLikeMT (OL_GMMStatsEstimator_CUDA.cpp:383-387);
+    // NaN frames are already clamped to min_loglike on the device.
+    return GMMHIP_OK;
+}
+
+int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, double *tot_ll,
+                      uint64_t *nframes) {
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!n || !tot_ll || !nframes || ((flags & GMMHIP_ACC_MEAN) && !F) ||
+        ((flags & GMMHIP_ACC_VAR) && !S))
+        return fail(h, GMMHIP_ERR_ARG, "accumulate: null output for a requested statistic");
+    DevGuard g(h->dev);
+    EstepArgs a = make_args(h);
+    HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, h->N));
+    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N)));
+    a.LL2 = h->d_LL2;
+    a.LLnat = nullptr;
+    a.blockLL = h->d_blockLL;
+
+    a.ngroups = (h->Mt + 3) / 4;
+    a.chunk_frames = pick_chunk_frames(h, a.ngroups);
+    a.nchunks = (int)((h->N + a.chunk_frames - 1) / a.chunk_frames);
+    const size_t Mp = (size_t)h->Mt * 16;
+    const size_t E = Mp * h->KP;
+    HIPCHK(h, ensure(&h->d_part, &h->part_cap, (size_t)a.nchunks * E));
+    HIPCHK(h, ensure(&h->d_red, &h->red_cap, E));
+    if (h->h_res_cap < E + 1) {
+        if (h->h_res) HIPCHK(h, hipHostFree(h->h_res));
+        h->h_res = nullptr;
+        h->h_res_cap = 0;
+        HIPCHK(h, hipHostMalloc((void **)&h->h_res, (E + 1) * sizeof(double)));
+        h->h_res_cap = E + 1;
+    }
+    a.partials = h->d_part;
+    a.red = h->d_red;
+    a.tot = h->d_tot;
+
+    const bool prof = h->opt.profile != 0;
+    if (prof) {
+        if ((rc = ensure_events(h))) return rc;
+        HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
+    }
+    HIPCHK(h, launch_loglik_fp32(a, h->stream));
+    if (prof) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+    HIPCHK(h, launch_stats_fp32(a, h->stream));
+    if (prof) HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
+    HIPCHK(h, launch_reduce_stats(a, h->stream));
+    HIPCHK(h, launch_sum_blockll(a, h->stream));
+    if (prof) HIPCHK(h, hipEventRecord(h->ev[3], h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_res, h->d_red, E * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_res + E, h->d_tot, sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+
+    if (prof) {
+        add_time(h, GMMHIP_KT_LOGLIK, 0, 1);
+        add_time(h, GMMHIP_KT_STATS, 1, 2);
+        add_time(h, GMMHIP_KT_REDUCE, 2, 3);
+    }
+    const double tot = h->h_res[E];
+    if (std::isnan(tot))
+        return fail(h, GMMHIP_ERR_NAN, "accumulateStatsMT(): overall log-likelihood = NaN!");
+
+    // Undo the centring: F = F' + n o, S = S' + 2 o F' + n o^2 (FP64).
+    const int D = h->D, KP = h->KP;
+    for (int m = 0; m < h->M; m++) {
+        const double *r = h->h_res + (size_t)m * KP;
+        const double nm = r[0];
+        n[m] += nm;
+        for (int d = 0; d < D; d++) {
+            const double o = h->h_off[d];
+            const double f1 = r[1 + d], s1 = r[1 + D + d];
+            if (flags & GMMHIP_ACC_MEAN) F[(size_t)m * D + d] += f1 + nm * o;
+            if (flags & GMMHIP_ACC_VAR) S[(size_t)m * D + d] += s1 + 2.0 * o * f1 + nm * o * o;
+        }
+    }
+    *tot_ll += tot;
+    *nframes += h->N;
+    return GMMHIP_OK;
+}
+
+}  // extern "C"

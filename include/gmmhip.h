@@ -1,0 +1,169 @@
+/*
+ * gmmhip.h -- C ABI of the MI355X (gfx950) diagonal-GMM statistics
+ * accumulator.  Plain C: opaque handle, plain pointers and sizes, int status
+ * codes.  One handle drives one GPU through one HIP stream and is not
+ * thread-safe.
+ *
+ * Each entry point names the reference interface it replaces.  Reference
+ * paths are relative to GMM/lib_bmssr/ of GMMTeam/GMM.
+ */
+#ifndef GMMHIP_H
+#define GMMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gmmhip_ctx *gmmhip_t;
+
+/* Status codes.  The C++ adapter rethrows every non-zero code as
+ * std::runtime_error (or std::logic_error for GMMHIP_ERR_ARG/STATE), which is
+ * what the reference's checkError / NaN checks / insertModel throw
+ * (trainer/CU_GMMStatsEstimator_CUDA.cu:1036-1053,
+ * trainer/OL_GMMStatsEstimator_CUDA.cpp:383-387, :455-456,
+ * trainer/OL_GMMStatsEstimator.cpp:279-285). */
+enum {
+    GMMHIP_OK = 0,
+    GMMHIP_ERR_HIP = 1,      /* a HIP runtime call failed */
+    GMMHIP_ERR_OOM = 2,      /* device allocation failed / over budget */
+    GMMHIP_ERR_NAN = 3,      /* total log-likelihood is NaN */
+    GMMHIP_ERR_ARG = 4,      /* bad argument (dims, null pointer, sizes) */
+    GMMHIP_ERR_VAR = 5,      /* model variance below min_var */
+    GMMHIP_ERR_STATE = 6,    /* no model / no frames inserted */
+    GMMHIP_ERR_MLLR = 7,     /* MLLR under-determined (CheckInvertibility) */
+    GMMHIP_ERR_NODEV = 8     /* no HIP device */
+};
+
+/* Arithmetic of the E-step contractions. */
+enum {
+    GMMHIP_PREC_FP32 = 0,    /* FP32 MFMA throughout (plain form) */
+    GMMHIP_PREC_BF16X3 = 1,  /* split bf16 hi+lo operands, FP32 accumulate */
+    GMMHIP_PREC_FP16X3 = 2   /* split fp16 hi+lo operands, FP32 accumulate */
+};
+
+/* Accumulation flags, setAccFlags(mixProbsOnly, meanStats, varStats, ...)
+ * (trainer/OL_GMMStatsEstimator.h:118-123).  n is always produced. */
+enum {
+    GMMHIP_ACC_MEAN = 1,     /* export F = sum g*x   */
+    GMMHIP_ACC_VAR = 2       /* export S = sum g*x*x */
+};
+
+/* Options, the reference's GMMStatsEstimator_GPU::OPTIONS
+ * (trainer/CU_GMMStatsEstimator_CUDA.h:56-63) plus the estimator's public
+ * knobs _minLogLike/_minGamma/_min_var (trainer/OL_GMMStatsEstimator.h:61-66).
+ * Zero-initialised fields take the defaults noted. */
+typedef struct gmmhip_opts {
+    float min_loglike;   /* default -1e20f */
+    float min_gamma;     /* default 1e-4f  */
+    float min_var;       /* default 1e-6f  */
+    int precision;       /* GMMHIP_PREC_*, default FP32 */
+    int chunk_frames;    /* frames per FP32 partial accumulator, 0 = auto */
+    int verbosity;       /* 0 silent */
+    int profile;         /* 1: time every kernel with HIP events */
+} gmmhip_opts;
+
+/* Fill *o with the defaults above. */
+void gmmhip_default_opts(gmmhip_opts *o);
+
+/* Number of HIP devices; replaces GMMStatsEstimator_GPU::deviceCount
+ * (trainer/CU_GMMStatsEstimator_CUDA.h:199). */
+int gmmhip_device_count(int *count);
+
+/* Create a handle on `device`; replaces the GMMStatsEstimator_GPU(int GPU_id)
+ * constructor (trainer/CU_GMMStatsEstimator_CUDA.h:65) and
+ * updateOptionsOnGPU (trainer/OL_GMMStatsEstimator_CUDA.cpp:114-123).
+ * opts may be NULL (defaults). */
+int gmmhip_create(int device, const gmmhip_opts *opts, gmmhip_t *out);
+
+/* Destroy a handle and free its device memory; replaces
+ * ~GMMStatsEstimator_GPU (trainer/CU_GMMStatsEstimator_CUDA.h:66). */
+int gmmhip_destroy(gmmhip_t h);
+
+/* Replace the options; replaces updateOptionsOnGPU
+ * (trainer/OL_GMMStatsEstimator_CUDA.cpp:114-123). */
+int gmmhip_set_opts(gmmhip_t h, const gmmhip_opts *opts);
+
+/* Text of the last error on this handle ("" if none). */
+const char *gmmhip_last_error(gmmhip_t h);
+
+/* Insert a diagonal model of M active components (all weights > 0, already
+ * in the order produced by GMModel::RearrangeMixtures), D dims, row-major
+ * mu/var [M*D].  Computes log-weights, inverse variances and Gaussian
+ * constants (insertModel, trainer/OL_GMMStatsEstimator.cpp:234-313), packs
+ * them for the MFMA contraction and uploads them (the CUDA insertModel +
+ * uploadModel, trainer/OL_GMMStatsEstimator_CUDA.cpp:127-276,
+ * trainer/CU_GMMStatsEstimator_CUDA.cu:284-318).  Returns GMMHIP_ERR_VAR if a
+ * variance is below min_var. */
+int gmmhip_set_model(gmmhip_t h, int M, int D, const float *w,
+                     const float *mu, const float *var);
+
+/* Upload N frames of D floats given as row pointers (the reference's
+ * float** vectors; rows may be non-contiguous) into device memory, where they
+ * stay resident until replaced.  Replaces alignVectors + uploadParam
+ * (trainer/OL_GMMStatsEstimator_CUDA.cpp:292-328,
+ * trainer/CU_GMMStatsEstimator_CUDA.cu:431-470). */
+int gmmhip_set_frames(gmmhip_t h, const float *const *rows, uint64_t N, int D);
+
+/* Same, from one contiguous host buffer with row stride ld (floats). */
+int gmmhip_set_frames_contig(gmmhip_t h, const float *x, uint64_t N, int D,
+                             uint64_t ld);
+
+/* Use N frames already resident in device memory at dx (row stride ld
+ * floats).  The memory is borrowed: it must stay valid while the handle uses
+ * it.  No reference counterpart (the reference always uploads from host); it
+ * is the same call with the upload removed. */
+int gmmhip_set_frames_device(gmmhip_t h, const float *dx, uint64_t N, int D,
+                             uint64_t ld);
+
+/* Accumulate statistics of the resident frames under the inserted model:
+ * n[m] += sum_t g_tm, F[m*D+d] += sum_t g_tm x_td, S[m*D+d] += sum_t g_tm x_td^2
+ * over posteriors g > min_gamma, *tot_ll += sum_t LL_t, *nframes += N.
+ * Accumulate-into semantics like GMMStats (caller resets).  F/S may be NULL
+ * when the matching flag is off.  Replaces accumulateStatsMT
+ * (trainer/OL_GMMStatsEstimator_CUDA.cpp:396-511) = compAccStats +
+ * getAccStats + export2GMMStats (trainer/CU_GMMStatsEstimator_CUDA.cu:721-865,
+ * trainer/OL_GMMStatsEstimator_CUDA.cpp:593-648).  Returns GMMHIP_ERR_NAN if
+ * the total log-likelihood is NaN. */
+int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S,
+                      double *tot_ll, uint64_t *nframes);
+
+/* Per-frame log-likelihoods of the resident frames into out[N] (host).
+ * Replaces compVecsLogLikeMT (trainer/OL_GMMStatsEstimator_CUDA.cpp:333-391)
+ * = compLogLikes + getLogLikes (trainer/CU_GMMStatsEstimator_CUDA.cu:606-643),
+ * without the reference's truncation beyond 262,136 frames per upload. */
+int gmmhip_loglikes(gmmhip_t h, float *out);
+
+/* Device-side variant: per-frame log-likelihoods into device memory d_out[N]
+ * on the handle's stream (no host copy). */
+int gmmhip_loglikes_device(gmmhip_t h, float *d_out);
+
+/* Kernel timers of a handle created with opts.profile = 1, like the
+ * reference's GPU_CUDA_TIMER accumulators
+ * (trainer/CU_GMMStatsEstimator_CUDA.cu:50-55, :338-352): ms[i] receives the
+ * summed milliseconds and count[i] the number of launches of kernel class i
+ * (GMMHIP_KT_*) since the handle was created or the timers were reset.
+ * n is the capacity of both arrays. */
+enum {
+    GMMHIP_KT_LOGLIK = 0,    /* per-frame log-sum-exp kernel */
+    GMMHIP_KT_STATS = 1,     /* posteriors + statistics kernel */
+    GMMHIP_KT_REDUCE = 2,    /* FP64 partial reduction kernels */
+    GMMHIP_KT_COUNT = 3
+};
+int gmmhip_kernel_times(gmmhip_t h, double *ms, uint64_t *count, int n);
+int gmmhip_reset_kernel_times(gmmhip_t h);
+
+/* The handle's HIP stream (hipStream_t as void*), for callers that order
+ * their own device work with the handle's. */
+void *gmmhip_stream(gmmhip_t h);
+
+/* Synchronise the handle's stream. */
+int gmmhip_sync(gmmhip_t h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GMMHIP_H */

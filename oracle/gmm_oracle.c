@@ -1,0 +1,465 @@
+/*
+ * gmm_oracle.c -- FP64 CPU restatement of the reference's diagonal-GMM EM
+ * hot path.  TEST INFRASTRUCTURE ONLY: it is the checker that the HIP
+ * library is compared against.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load it; the product never links it.
+ *
+ * Every function names the reference file:line it restates.  Paths are
+ * relative to GMM/lib_bmssr/ of the reference tree.
+ *
+ * Arithmetic: all sums, logs and exps are in double.  Inputs that the
+ * reference stores as float (frames, model parameters) are taken as float;
+ * where the reference rounds an intermediate to float before using it again
+ * (the M-step mean, MAP mean), this restatement rounds at the same place so
+ * that updated models agree bit-for-bit in structure and to FP64 rounding in
+ * value.
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   orc_prepare_model   -- closed form vs numpy/scipy normal log-density;
+ *                          example/data.logL golden vector.
+ *   orc_frame_loglik    -- example/data.logL (N=10000, D=20, M=4) to print
+ *                          precision; scipy logsumexp brute force; M=1
+ *                          Gaussian closed form; equal components -> 1/M;
+ *                          translation invariance.
+ *   orc_accumulate      -- brute force numpy on tiny inputs (posterior cut
+ *                          planted both sides of 1e-4); sum of n == N when
+ *                          no posterior is cut; permutation invariance.
+ *   orc_update_em       -- example/myModel.gmm reproduced by the from-scratch
+ *                          split driver (tests/golden/train_from_scratch.py).
+ *   orc_update_map      -- tau=0 equals ML means; n=0 keeps the UBM; alpha
+ *                          closed form on a one-component case.
+ *   orc_mllr_init/kg    -- G symmetric positive definite; brute-force outer
+ *                          product sums in numpy; identity transform when the
+ *                          statistics equal the UBM's own moments.
+ *   orc_gj_invert       -- A * inv(A) == I against numpy.linalg.inv.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Reference thresholds: OL_GMMStatsEstimator.cpp:82-83 (_minLogLike=-1e20f,
+ * _minGamma=1e-4f), OL_GMMStatsEstimator.h:283 (LPDIF=15),
+ * OL_GMMStatsEstimator.h:274 (MINUS_LOG_2PI). */
+#define ORC_MINUS_LOG_2PI (-1.837877066409345)
+static const double ORC_LPDIF = 15.0;
+
+double orc_min_loglike(void) { return (double)(-1e20f); }
+double orc_min_gamma(void) { return (double)(1e-4f); }
+
+/* RearrangeMixtures, OL_GMModel.cpp:857-885: components with weight > 0 keep
+ * their order at the front; the others fill the tail from the back, so their
+ * relative order is reversed.  perm[new] = old.  Returns the number of
+ * zero-weight components. */
+int orc_rearrange_perm(int M, const float *w, int *perm)
+{
+    int front = 0, back = M - 1;
+    for (int m = 0; m < M; m++) {
+        if (w[m] > 0.0f)
+            perm[front++] = m;
+        else
+            perm[back--] = m;
+    }
+    return M - front;
+}
+
+/* insertModel (diagonal), OL_GMMStatsEstimator.cpp:234-313 together with the
+ * GPU packing constant gconst = logGconst + logw,
+ * OL_GMMStatsEstimator_CUDA.cpp:195-203.  Inputs are the M active
+ * (weight > 0) components.  Outputs: gconst[m] and ivar[m*D+d].
+ * Returns 0, or -(1+m) when component m holds a variance below min_var
+ * (the reference throws there). */
+int orc_prepare_model(int M, int D, const float *w, const float *mu,
+                      const float *var, double min_var, double *gconst,
+                      double *ivar)
+{
+    (void)mu;
+    for (int m = 0; m < M; m++) {
+        double logw = (w[m] > 0.0f) ? log((double)w[m]) : orc_min_loglike();
+        double sum_logvar = 0.0;
+        for (int d = 0; d < D; d++) {
+            double v = var[(size_t)m * D + d];
+            if (v < min_var)
+                return -(1 + m);
+            ivar[(size_t)m * D + d] = 1.0 / v;
+            sum_logvar += log(v);
+        }
+        gconst[m] = ORC_MINUS_LOG_2PI * D / 2.0 - sum_logvar / 2.0 + logw;
+    }
+    return 0;
+}
+
+/* Component log-likelihood incl. log-weight: computeMixLogLike
+ * (OL_GMMStatsEstimator_SSE.cpp:206-253, non-finite -> _minLogLike) plus
+ * _logWeights (OL_GMMStatsEstimator.cpp:488). */
+static double comp_ll(const float *x, int D, const float *mu_m,
+                      const double *ivar_m, double gconst_m)
+{
+    double q = 0.0;
+    for (int d = 0; d < D; d++) {
+        double t = (double)x[d] - (double)mu_m[d];
+        q += t * t * ivar_m[d];
+    }
+    double l = gconst_m - 0.5 * q;
+    if (!isfinite(l))
+        return orc_min_loglike();
+    return l;
+}
+
+/* addLog with the LPDIF early-out, OL_GMMStatsEstimator.cpp:1450-1458. */
+static double add_log_lpdif(double p1, double p2)
+{
+    if (p1 - ORC_LPDIF > p2)
+        return p1;
+    if (p1 > p2)
+        return p1 + log(1.0 + exp(p2 - p1));
+    return p2 + log(1.0 + exp(p1 - p2));
+}
+
+/* Per-frame log-likelihood and posteriors, getLikeAndGammas + compGammas
+ * (OL_GMMStatsEstimator.cpp:444-558) with the GPU's LSE
+ * (CU_GMMStatsEstimator_CUDAkernels.cuh:922-995, exact pairwise log-add, no
+ * early-out) when lpdif == 0, or the CPU's sequential addLog with the
+ * LPDIF=15 early-out when lpdif != 0.
+ * LL[n] <= _minLogLike means every component underflowed: LL is clamped to
+ * _minLogLike and the frame contributes no statistics.
+ * gamma (optional, N*M) receives the posteriors before the 1e-4 cut. */
+void orc_frame_loglik(int N, int D, int M, const float *x, const float *mu,
+                      const double *ivar, const double *gconst, int lpdif,
+                      double *LL, double *gamma)
+{
+    const double minLL = orc_min_loglike();
+    double *l = (double *)malloc(sizeof(double) * (M > 0 ? M : 1));
+    for (int n = 0; n < N; n++) {
+        const float *xn = x + (size_t)n * D;
+        double mx = minLL;
+        for (int m = 0; m < M; m++) {
+            l[m] = comp_ll(xn, D, mu + (size_t)m * D, ivar + (size_t)m * D,
+                           gconst[m]);
+            if (l[m] > mx)
+                mx = l[m];
+        }
+        double ll;
+        if (mx <= minLL) {
+            ll = minLL;
+        } else if (lpdif) {
+            ll = minLL;
+            for (int m = 0; m < M; m++)
+                ll = add_log_lpdif(ll, l[m]);
+        } else {
+            double s = 0.0;
+            for (int m = 0; m < M; m++)
+                s += exp(l[m] - mx);
+            ll = mx + log(s);
+        }
+        if (!isfinite(ll) || ll < minLL)
+            ll = minLL;
+        LL[n] = ll;
+        if (gamma) {
+            for (int m = 0; m < M; m++) {
+                double g;
+                if (ll <= minLL)
+                    g = 0.0;
+                else if (M == 1)
+                    g = 1.0;
+                else
+                    g = (l[m] > minLL) ? exp(l[m] - ll) : 0.0;
+                gamma[(size_t)n * M + m] = g;
+            }
+        }
+    }
+    free(l);
+}
+
+/* Statistics accumulation, accumulateStats over frames and per vector
+ * (OL_GMMStatsEstimator.cpp:1105-1193, :1334-1378): for every component with
+ * posterior > _minGamma, n += g, F += g*x, S += g*x*x.  tot_ll receives the
+ * sum of per-frame log-likelihoods (including _minLogLike for frames whose
+ * components all underflowed, as the reference does).  Outputs are
+ * overwritten, not accumulated into. */
+void orc_accumulate(int N, int D, int M, const float *x, const float *mu,
+                    const double *ivar, const double *gconst, int lpdif,
+                    double min_gamma, double *n_out, double *F, double *S,
+                    double *tot_ll)
+{
+    double *g = (double *)malloc(sizeof(double) * (M > 0 ? M : 1));
+    double LL;
+    memset(n_out, 0, sizeof(double) * M);
+    memset(F, 0, sizeof(double) * (size_t)M * D);
+    memset(S, 0, sizeof(double) * (size_t)M * D);
+    double tot = 0.0;
+    for (int n = 0; n < N; n++) {
+        const float *xn = x + (size_t)n * D;
+        orc_frame_loglik(1, D, M, xn, mu, ivar, gconst, lpdif, &LL, g);
+        tot += LL;
+        for (int m = 0; m < M; m++) {
+            if (!(g[m] > min_gamma))
+                continue;
+            n_out[m] += g[m];
+            double *Fm = F + (size_t)m * D, *Sm = S + (size_t)m * D;
+            for (int d = 0; d < D; d++) {
+                double xv = xn[d];
+                Fm[d] += g[m] * xv;
+                Sm[d] += g[m] * xv * xv;
+            }
+        }
+    }
+    *tot_ll = tot;
+    free(g);
+}
+
+/* GMMMixture::Reset for a diagonal mixture, OL_GMModel.cpp:76-86: mean = 0
+ * and the variance bytes set to 0x01 (memset(Var, 1, ...)), i.e. every
+ * variance becomes the float with bit pattern 0x01010101. */
+static void mixture_reset(float *mu_m, float *var_m, int D)
+{
+    memset(mu_m, 0, sizeof(float) * D);
+    memset(var_m, 1, sizeof(float) * D);
+}
+
+/* UpdateModelEM (diagonal, non-robust), OL_ModelAdapt.cpp:823-949.
+ * w/mu/var are updated in place for the first M components (model order),
+ * n/F/S are the statistics, tot_samples = GMMStats::_totalAccSamples.
+ * Returns the number of components left with non-zero weight. */
+int orc_update_em(int M, int D, float *w, float *mu, float *var,
+                  const double *n, const double *F, const double *S,
+                  double tot_samples, double min_var, int upweight,
+                  int upmean, int upvar)
+{
+    const double MIN_IND_SAMPLES = 2.0; /* OL_ModelAdapt.h:54 */
+    int nzero = 0, wchange = 0;
+    double gcorr = 0.0;
+    float *mean = (float *)malloc(sizeof(float) * (D > 0 ? D : 1));
+    for (int j = 0; j < M; j++) {
+        if (w[j] == 0.0f) {
+            nzero++;
+            continue;
+        }
+        double np = n[j];
+        if (np < MIN_IND_SAMPLES) {
+            mixture_reset(mu + (size_t)j * D, var + (size_t)j * D, D);
+            w[j] = 0.0f;
+            wchange = 1;
+            continue;
+        }
+        if (upweight) {
+            w[j] = (float)(np / tot_samples);
+            wchange = 1;
+        }
+        gcorr += w[j];
+        if (upmean || upvar) {
+            for (int k = 0; k < D; k++)
+                mean[k] = (float)(F[(size_t)j * D + k] / np);
+            if (upvar) {
+                for (int k = 0; k < D; k++) {
+                    float v = (float)(S[(size_t)j * D + k] / np -
+                                      pow((double)mean[k], 2));
+                    if (v < (float)min_var)
+                        v = (float)min_var;
+                    var[(size_t)j * D + k] = v;
+                }
+            }
+            if (upmean)
+                memcpy(mu + (size_t)j * D, mean, sizeof(float) * D);
+        }
+    }
+    if (wchange && gcorr > 0.0)
+        for (int j = 0; j < M; j++)
+            w[j] = (float)(w[j] / gcorr);
+    free(mean);
+    return M - nzero;
+}
+
+/* UpdateModelMAP for tau > 0 (diagonal, non-robust),
+ * OL_ModelAdapt.cpp:993-1106.  The model arrays start as a copy of the UBM
+ * and are updated in place; ubm_* are the prior.  tau is an int in the
+ * reference (OL_ModelAdapt.h:75). */
+void orc_update_map(int M, int D, float *w, float *mu, float *var,
+                    const float *ubm_w, const float *ubm_mu,
+                    const float *ubm_var, const double *n, const double *F,
+                    const double *S, double tot_samples, int tau,
+                    double min_var, int upweight, int upmean, int upvar)
+{
+    double mixcorr = 0.0;
+    float *amean = (float *)malloc(sizeof(float) * (D > 0 ? D : 1));
+    for (int j = 0; j < M; j++) {
+        if (ubm_w[j] == 0.0f)
+            continue;
+        double np = n[j];
+        if (!(np > 0.0))
+            continue;
+        const float *um = ubm_mu + (size_t)j * D, *uv = ubm_var + (size_t)j * D;
+        double alfa = np / (np + tau);
+        if (upweight && tot_samples > 0) {
+            float aw = (float)(alfa * (np / tot_samples) + (1 - alfa) * ubm_w[j]);
+            w[j] = aw;
+            mixcorr += aw;
+        }
+        if (upmean || upvar) {
+            for (int k = 0; k < D; k++)
+                amean[k] = (float)(alfa * F[(size_t)j * D + k] / np +
+                                   (1 - alfa) * um[k]);
+            if (upvar) {
+                for (int k = 0; k < D; k++) {
+                    float av = (float)(alfa * S[(size_t)j * D + k] / np +
+                                       (1 - alfa) * (uv[k] + um[k] * um[k]) -
+                                       amean[k] * amean[k]);
+                    if (av < (float)min_var)
+                        av = (float)min_var;
+                    var[(size_t)j * D + k] = av;
+                }
+            }
+            memcpy(mu + (size_t)j * D, amean, sizeof(float) * D);
+        }
+    }
+    if (upweight && mixcorr != 0.0)
+        for (int j = 0; j < M; j++)
+            w[j] = w[j] / (float)mixcorr;
+    free(amean);
+}
+
+/* prepareKiGiInitMatrices, OL_ModelAdapt.cpp:1484-1523: the UBM prior for the
+ * MLLR row statistics, as if every UBM component contributed one frame at
+ * its own mean with its own variance.  G is D blocks of (D+1)^2 (row-major),
+ * k is D blocks of D+1. */
+void orc_mllr_init(int M, int D, const float *ubm_mu, const float *ubm_var,
+                   double min_var, double *G, double *k)
+{
+    const int E = D + 1;
+    memset(G, 0, sizeof(double) * (size_t)D * E * E);
+    memset(k, 0, sizeof(double) * (size_t)D * E);
+    for (int m = 0; m < M; m++) {
+        const float *var = ubm_var + (size_t)m * D, *mean = ubm_mu + (size_t)m * D;
+        for (int i = 0; i < D; i++) {
+            double cG = (var[i] < min_var) ? 1.0 / min_var : 1.0 / var[i];
+            double ck = mean[i] * cG;
+            double *Gi = G + (size_t)i * E * E, *ki = k + (size_t)i * E;
+            for (int d = 0; d < D; d++) {
+                ki[d] += ck * mean[d];
+                Gi[(size_t)D * E + d] += cG * mean[d];
+                Gi[(size_t)d * E + D] += cG * mean[d];
+                for (int dd = 0; dd < D; dd++)
+                    Gi[(size_t)d * E + dd] += cG * mean[d] * mean[dd];
+                Gi[(size_t)d * E + d] += cG * var[d];
+            }
+            Gi[(size_t)E * E - 1] += cG;
+            ki[D] += ck;
+        }
+    }
+}
+
+/* ComputeKiGi_MLLR, OL_ModelAdapt.cpp:1525-1581, for one row i:
+ * G_i = Ginit_i + sum_{n_m>0} (n_m/var_mi) xi xi^T,
+ * k_i = kinit_i + sum_{n_m>0} (F_mi/var_mi) xi, xi = [mu_m; 1].
+ * Ginit/kinit may be NULL (zero prior).  Returns the total occupancy. */
+double orc_mllr_kg(int M, int D, const float *mu, const float *var,
+                   const double *n, const double *F, const double *Ginit,
+                   const double *kinit, int i, double *G, double *k)
+{
+    const int E = D + 1;
+    if (Ginit && kinit) {
+        memcpy(G, Ginit, sizeof(double) * (size_t)E * E);
+        memcpy(k, kinit, sizeof(double) * E);
+    } else {
+        memset(G, 0, sizeof(double) * (size_t)E * E);
+        memset(k, 0, sizeof(double) * E);
+    }
+    double tot = 0.0;
+    for (int m = 0; m < M; m++) {
+        double np = n[m];
+        if (!(np > 0.0))
+            continue;
+        tot += np;
+        const float *mm = mu + (size_t)m * D;
+        double vi = var[(size_t)m * D + i];
+        double cm = F[(size_t)m * D + i] / vi;
+        double cv = np / vi;
+        for (int a = 0; a < D; a++) {
+            k[a] += cm * mm[a];
+            for (int b = a; b < D; b++) {
+                G[(size_t)a * E + b] += cv * mm[a] * mm[b];
+                G[(size_t)b * E + a] = G[(size_t)a * E + b];
+            }
+            G[(size_t)a * E + D] += cv * mm[a];
+            G[(size_t)D * E + a] = G[(size_t)a * E + D];
+        }
+        k[D] += cm;
+        G[(size_t)D * E + D] += cv;
+    }
+    return tot;
+}
+
+/* MATRIX::_Invert_Gen, Matrix4_LMa.cpp:2232-2290: in-place Gauss-Jordan
+ * elimination without pivoting, FP64.  A is n x n row-major. */
+void orc_gj_invert(int n, double *A)
+{
+    for (int k = 0; k < n; k++) {
+        double p = A[(size_t)k * n + k];
+        for (int i = 0; i < n; i++) {
+            if (i == k)
+                continue;
+            double aik = A[(size_t)i * n + k];
+            for (int j = 0; j < n; j++)
+                if (j != k)
+                    A[(size_t)i * n + j] -= A[(size_t)k * n + j] * aik / p;
+        }
+        for (int j = 0; j < n; j++)
+            if (j != k)
+                A[(size_t)k * n + j] = -A[(size_t)k * n + j] / p;
+        for (int i = 0; i < n; i++)
+            if (i != k)
+                A[(size_t)i * n + k] = A[(size_t)i * n + k] / p;
+        A[(size_t)k * n + k] = 1.0 / p;
+    }
+}
+
+/* ComputeW_MLLR row solve, OL_ModelAdapt.cpp:1448-1461: w_i = inv(G_i) k_i
+ * with the Gauss-Jordan inverse above.  G is overwritten. */
+void orc_mllr_solve_row(int E, double *G, const double *k, double *w)
+{
+    orc_gj_invert(E, G);
+    for (int a = 0; a < E; a++) {
+        double s = 0.0;
+        for (int b = 0; b < E; b++)
+            s += G[(size_t)a * E + b] * k[b];
+        w[a] = s;
+    }
+}
+
+/* CheckInvertibility, OL_ModelAdapt.cpp:1851-1878.  Returns 0 when MLLR is
+ * well-posed, 1 (too few components), 2 (too few active components) or
+ * 3 (too little occupancy). */
+int orc_mllr_check(int M, int D, const double *n)
+{
+    if (M < D + 1)
+        return 1;
+    int active = 0;
+    double occ = 0.0;
+    for (int m = 0; m < M; m++)
+        if (n[m] > 0) {
+            active++;
+            occ += n[m];
+        }
+    if (active < D + 1)
+        return 2;
+    if (occ * D < (double)D * (D + 1))
+        return 3;
+    return 0;
+}
+
+/* AdaptModelMLLR, OL_ModelAdapt.cpp:1583-1612: mu' = A mu + b with
+ * W = [A | b] (D x (D+1), row-major), FP64 accumulate, float store. */
+void orc_mllr_apply(int M, int D, const double *W, const float *mu_in,
+                    float *mu_out)
+{
+    const int E = D + 1;
+    for (int m = 0; m < M; m++) {
+        const float *src = mu_in + (size_t)m * D;
+        for (int a = 0; a < D; a++) {
+            double v = 0.0;
+            for (int b = 0; b < D; b++)
+                v += W[(size_t)a * E + b] * src[b];
+            mu_out[(size_t)m * D + a] = (float)(v + W[(size_t)a * E + D]);
+        }
+    }
+}

@@ -1,0 +1,70 @@
+"""From-scratch split training (trainGMM --type 0), restated on top of the
+oracle to reproduce the reference's committed example/myModel.gmm.  Test
+infrastructure: it exists to pin the oracle's E-step, statistics and EM
+update against the reference's own output.
+
+Follows OL_ModelAdapt.cpp:673-821 (TrainFromScratch), :590-660
+(splitMultipleGaussians / splitGaussian, diagonal branch), :965-991
+(GetGaussStats: highest weight, ties to the higher index; zero slots listed
+from the highest index), OL_GMModel.cpp:857-885 (RearrangeMixtures inside
+every insertModel) and :76-86 (GMMMixture::Reset).
+"""
+import numpy as np
+
+import _oracle as orc
+
+
+def _rearrange(w, mu, var):
+    perm, nz = orc.rearrange_perm(w)
+    return w[perm].copy(), mu[perm].copy(), var[perm].copy(), nz
+
+
+def _reset_var(D):
+    return np.frombuffer(b"\x01" * (4 * D), dtype=np.float32).copy()
+
+
+def train_from_scratch(x, nummix, it_em):
+    x = np.ascontiguousarray(x, np.float32)
+    N, D = x.shape
+    # global statistics with a single all-ones posterior
+    n = np.array([float(N)])
+    F = x.astype(np.float64).sum(0)[None]
+    S = (x.astype(np.float64) * x.astype(np.float64)).sum(0)[None]
+    w0, mu0, var0, left = orc.update_em(np.ones(1), np.zeros((1, D)),
+                                        _reset_var(D)[None], n, F, S, N)
+    w = np.zeros(nummix, np.float32)
+    mu = np.zeros((nummix, D), np.float32)
+    var = np.tile(_reset_var(D), (nummix, 1))
+    w[0], mu[0], var[0] = w0[0], mu0[0], var0[0]
+    nonzero = 1
+    for it in range(nummix):
+        for _ in range(it_em):
+            w, mu, var, _ = _rearrange(w, mu, var)
+            act = w > 0
+            na, Fa, Sa, _tot = orc.accumulate(x, w[act], mu[act], var[act])
+            n = np.zeros(nummix); F = np.zeros((nummix, D)); S = np.zeros((nummix, D))
+            n[act], F[act], S[act] = na, Fa, Sa
+            if nonzero > 1:
+                w, mu, var, nonzero = orc.update_em(w, mu, var, n, F, S, N)
+            if nonzero < 2:
+                break
+        if nonzero == 0 or nonzero == nummix:
+            break
+        if it < nummix - 1:
+            # split the heaviest component (ties -> higher index) into the
+            # highest-index zero-weight slot
+            imax, wm = 0, 0.0
+            for m in range(nummix - 1, -1, -1):
+                if wm < w[m]:
+                    wm, imax = w[m], m
+            znew = max(m for m in range(nummix) if w[m] == 0)
+            var[znew] = var[imax]
+            half = np.float32(w[imax] / np.float32(2.0))
+            w[znew] = half
+            w[imax] = half
+            shift = (np.sqrt(var[imax]).astype(np.float64) * 0.5).astype(np.float32)
+            mu[znew] = mu[imax] + shift
+            mu[imax] = mu[imax] - shift
+            nonzero += 1
+    w, mu, var, _ = _rearrange(w, mu, var)
+    return w, mu, var
