@@ -59,16 +59,26 @@ __device__ __forceinline__ float zval(const float *__restrict__ xr,
 }
 
 // ---------------------------------------------------------------------------
+// Contraction index mapping shared by both kernels: MFMA step kk of lane
+// group q (q = lane/16) covers k = q*KK + kk, so a lane's operands for
+// consecutive steps are consecutive in memory and load as float4.  The model
+// is packed as P[t][lane][kk] = B[q*KK + kk][16t + c] (c = lane%16).
+//
 // loglik_kernel: one block = 4 waves x 2 sub-tiles x 16 frames.  Each wave
 // keeps the MFMA A-fragments of its 32 frames in registers for the whole
-// component loop; component tiles of 16 are staged through LDS and shared by
-// the 4 waves.  Output layout of the 16x16x4 MFMA: lane (q = lane/16,
-// c = lane%16) holds rows 4q..4q+3 (frames) of column c (component).
+// component loop; component tiles of 16 are staged through LDS (register
+// prefetch of the next tile overlaps the current tile's MFMAs) and shared by
+// the 4 waves.  Output layout of the 16x16x4 MFMA: lane (q, c) holds rows
+// 4q..4q+3 (frames) of column c (component).
 template <int KP>
 __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
     constexpr int KK = KP / 4;
+    constexpr int KU = KK / 4;              // float4 per lane per tile
+    constexpr int PSTR = KK | 4;            // padded LDS row stride (floats)
     constexpr int SUB = 2;
-    __shared__ float Pl[KK * 64];
+    constexpr int TILE4 = 16 * KK;          // float4 per component tile
+    constexpr int PF = (TILE4 + 255) / 256; // float4 per thread per tile
+    __shared__ float Pl[64 * PSTR];
     __shared__ double red[16];
 
     const int tid = threadIdx.x;
@@ -85,51 +95,77 @@ __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
         const float *xr = a.X + (ok ? fr : 0) * a.ldx;
 #pragma unroll
         for (int kk = 0; kk < KK; kk++)
-            af[s][kk] = ok ? zval(xr, a.off, D, 4 * kk + q) : 0.0f;
+            af[s][kk] = ok ? zval(xr, a.off, D, q * KK + kk) : 0.0f;
     }
 
+    // Online log-sum-exp per (frame, lane): mx is a per-lane reference and
+    // sm = sum 2^(v - mx).  A value more than 32 above the reference (rare
+    // after the first tiles) moves the reference first, so every exp2
+    // argument stays <= 32; the common path is sub + exp2 + add.  Padded
+    // components carry l = -1e30 and vanish in exp2; a NaN frame makes sm NaN
+    // and is clamped at the end.
     float mx[SUB][4], sm[SUB][4];
-    bool bad[SUB][4];
-#pragma unroll
-    for (int s = 0; s < SUB; s++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            mx[s][r] = -INFINITY;
-            sm[s][r] = 0.0f;
-            bad[s][r] = false;
-        }
 
     const float4 *P4 = reinterpret_cast<const float4 *>(a.P);
-    float4 *Pl4 = reinterpret_cast<float4 *>(Pl);
+    float4 pf[PF];
+    auto prefetch = [&](int t) {
+#pragma unroll
+        for (int j = 0; j < PF; j++) {
+            const int i = tid + 256 * j;
+            if (i < TILE4) pf[j] = P4[(size_t)t * TILE4 + i];
+        }
+    };
+    prefetch(0);
     for (int t = 0; t < a.Mt; t++) {
         __syncthreads();
-        for (int i = tid; i < KK * 16; i += 256)
-            Pl4[i] = P4[(size_t)t * KK * 16 + i];
+#pragma unroll
+        for (int j = 0; j < PF; j++) {
+            const int i = tid + 256 * j;
+            if (i < TILE4) {
+                const int ln = i / KU, u = i - ln * KU;
+                *reinterpret_cast<float4 *>(&Pl[ln * PSTR + 4 * u]) = pf[j];
+            }
+        }
         __syncthreads();
+        if (t + 1 < a.Mt) prefetch(t + 1);
 
         f32x4 acc[SUB];
 #pragma unroll
         for (int s = 0; s < SUB; s++) acc[s] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kk = 0; kk < KK; kk++) {
-            const float b = Pl[kk * 64 + lane];
+        for (int u = 0; u < KU; u++) {
+            const float4 b = *reinterpret_cast<const float4 *>(&Pl[lane * PSTR + 4 * u]);
 #pragma unroll
-            for (int s = 0; s < SUB; s++) acc[s] = mfma4(af[s][kk], b, acc[s]);
+            for (int s = 0; s < SUB; s++) {
+                acc[s] = mfma4(af[s][4 * u + 0], b.x, acc[s]);
+                acc[s] = mfma4(af[s][4 * u + 1], b.y, acc[s]);
+                acc[s] = mfma4(af[s][4 * u + 2], b.z, acc[s]);
+                acc[s] = mfma4(af[s][4 * u + 3], b.w, acc[s]);
+            }
         }
 
-        const bool cvalid = (t * 16 + c) < a.M;
+        if (t == 0) {
 #pragma unroll
-        for (int s = 0; s < SUB; s++)
+            for (int s = 0; s < SUB; s++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const float v = cvalid ? acc[s][r] : -INFINITY;
-                bad[s][r] = bad[s][r] || (v != v);
-                const float hi = fmaxf(mx[s][r], v);
-                const float lo = fminf(mx[s][r], v);
-                const float e = (hi == -INFINITY) ? 0.0f : fast_exp2(lo - hi);
-                sm[s][r] = (v > mx[s][r]) ? fmaf(sm[s][r], e, 1.0f) : sm[s][r] + e;
-                mx[s][r] = hi;
-            }
+                for (int r = 0; r < 4; r++) {
+                    mx[s][r] = acc[s][r];
+                    sm[s][r] = 1.0f;
+                }
+        } else {
+#pragma unroll
+            for (int s = 0; s < SUB; s++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float d = acc[s][r] - mx[s][r];
+                    if (d > 32.0f) {
+                        sm[s][r] = fmaf(sm[s][r], fast_exp2(-d), 1.0f);
+                        mx[s][r] = acc[s][r];
+                    } else {
+                        sm[s][r] += fast_exp2(d);
+                    }
+                }
+        }
     }
 
     // combine the 16 component lanes of each frame row
@@ -141,14 +177,9 @@ __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
             for (int o = 1; o < 16; o <<= 1) {
                 const float m2 = __shfl_xor(mx[s][r], o);
                 const float s2 = __shfl_xor(sm[s][r], o);
-                const int b2 = __shfl_xor((int)bad[s][r], o);
                 const float hi = fmaxf(mx[s][r], m2);
-                float v = 0.0f;
-                if (hi != -INFINITY)
-                    v = sm[s][r] * fast_exp2(mx[s][r] - hi) + s2 * fast_exp2(m2 - hi);
-                sm[s][r] = v;
+                sm[s][r] = sm[s][r] * fast_exp2(mx[s][r] - hi) + s2 * fast_exp2(m2 - hi);
                 mx[s][r] = hi;
-                bad[s][r] = bad[s][r] || (b2 != 0);
             }
         }
 
@@ -161,9 +192,11 @@ __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
                 const uint64_t fr = base + (uint64_t)(wave * SUB + s) * 16 + 4 * q + r;
                 float l2 = mx[s][r] + fast_log2(sm[s][r]);
                 float nat = l2 * kLn2;
-                if (bad[s][r] || !(nat > a.min_ll) || isinf(nat)) {
+                if (!(nat > a.min_ll) || isinf(nat)) {
+                    // every component underflowed, or a NaN/Inf frame: the
+                    // log-likelihood is clamped and +inf marks "no statistics"
                     nat = a.min_ll;
-                    l2 = -INFINITY;
+                    l2 = INFINITY;
                 }
                 if (fr < a.N) {
                     a.LL2[fr] = l2;
@@ -186,13 +219,22 @@ __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
 // component tile 4*group + w and its 16 x KP statistics accumulators.  Blocks
 // are numbered so that the groups of one chunk land on the same XCD (the
 // dispatcher deals blocks round-robin over 8 XCDs), so a chunk's frames are
-// fetched from HBM once per XCD L2.
+// fetched from HBM once per XCD L2.  The next 64 frames are prefetched into
+// registers while the current ones are contracted.
+//
+// Statistics column mapping: MFMA jt of the g^T Z contraction covers columns
+// c*JT + jt (c = lane%16), so a lane's B operands are consecutive in the LDS
+// row.  Columns are [1, x', x'^2, 0...]: n, F', S'.
+// Two waves per SIMD (<= 256 VGPRs) up to KP = 160 (D <= 79); larger
+// depths keep their accumulators in registers at one wave per SIMD.
 template <int KP>
-__global__ __launch_bounds__(256) void stats_kernel(EstepArgs a) {
+__global__ __launch_bounds__(256, (KP <= 160 ? 2 : 1)) void stats_kernel(EstepArgs a) {
     constexpr int KK = KP / 4;
+    constexpr int KU = KK / 4;
     constexpr int JT = KP / 16;
     constexpr int S = lds_stride(KP);
     constexpr int FB = ST_FRAMES_PER_ITER;
+    constexpr int NPF = KP / 8;  // >= ceil(FB*D/256) for 2D+1 <= KP
     __shared__ float Zr[FB * S];
     __shared__ float Ls[FB];
 
@@ -209,11 +251,21 @@ __global__ __launch_bounds__(256) void stats_kernel(EstepArgs a) {
     const int tile = group * 4 + wave;
     const bool active = tile < a.Mt;
     const int D = a.D;
+    const int FD = FB * D;
 
     float bf[KK];
+    {
+        const float4 *P4 = reinterpret_cast<const float4 *>(a.P);
 #pragma unroll
-    for (int kk = 0; kk < KK; kk++)
-        bf[kk] = active ? a.P[((size_t)tile * KK + kk) * 64 + lane] : 0.0f;
+        for (int u = 0; u < KU; u++) {
+            const float4 v = active ? P4[((size_t)tile * 64 + lane) * KU + u]
+                                    : make_float4(0.f, 0.f, 0.f, 0.f);
+            bf[4 * u + 0] = v.x;
+            bf[4 * u + 1] = v.y;
+            bf[4 * u + 2] = v.z;
+            bf[4 * u + 3] = v.w;
+        }
+    }
 
     f32x4 acc2[JT];
 #pragma unroll
@@ -224,48 +276,85 @@ __global__ __launch_bounds__(256) void stats_kernel(EstepArgs a) {
     const uint64_t f0 = (uint64_t)chunk * a.chunk_frames;
     uint64_t f1 = f0 + a.chunk_frames;
     if (f1 > a.N) f1 = a.N;
-    const bool cvalid = (tile * 16 + c) < a.M;
+
+    // Frames are contiguous rows (ldx == D): a 64-frame step is one flat run
+    // of 64*D floats.  Element i of the run belongs to frame f = i / D, dim
+    // d = i - f*D; the float quotient is exact because i < 2^15 and the
+    // fractional part of (i + 0.5)/D is at least 1/(2D).
+    __shared__ float offL[KP];
+    for (int i = tid; i < D; i += 256) offL[i] = a.off[i];
+    const float invD = 1.0f / (float)D;
+
+    float pf[NPF];
+    float pl = INFINITY;
+    auto prefetch = [&](uint64_t base) {
+        const float *src = a.X + base * (uint64_t)D;
+        const uint64_t nf = (f1 - base < (uint64_t)FB) ? f1 - base : (uint64_t)FB;
+        const int nvalid = (int)nf * D;
+#pragma unroll
+        for (int j = 0; j < NPF; j++) {
+            const int i = tid + 256 * j;
+            pf[j] = (i < nvalid) ? src[i] : 0.0f;
+        }
+        if (tid < FB) pl = (base + tid < f1) ? a.LL2[base + tid] : INFINITY;
+    };
+    prefetch(f0);
 
     for (uint64_t base = f0; base < f1; base += FB) {
         __syncthreads();
-        for (int i = tid; i < FB * D; i += 256) {
-            const int f = i / D, d = i - f * D;
-            const uint64_t fr = base + f;
-            const float v = (fr < f1) ? a.X[fr * a.ldx + d] - a.off[d] : 0.0f;
-            Zr[f * S + 1 + d] = v;
-            Zr[f * S + 1 + D + d] = v * v;
+#pragma unroll
+        for (int j = 0; j < NPF; j++) {
+            const int i = tid + 256 * j;
+            if (i < FD) {
+                const int f = (int)(((float)i + 0.5f) * invD);
+                const int d = i - f * D;
+                float v = pf[j] - offL[d];
+                // Non-finite or huge values belong to frames whose
+                // log-likelihood was clamped (posterior 0); zero them so that
+                // 0 * NaN cannot reach the contraction over frames.  Frames
+                // past the chunk end were loaded as 0 and stay finite.
+                v = (fabsf(v) <= 1e18f) ? v : 0.0f;
+                Zr[f * S + 1 + d] = v;
+                Zr[f * S + 1 + D + d] = v * v;
+            }
         }
         if (tid < FB) {
-            const uint64_t fr = base + tid;
-            const bool ok = fr < f1;
-            Zr[tid * S] = ok ? 1.0f : 0.0f;
-            Ls[tid] = ok ? a.LL2[fr] : -INFINITY;
+            Zr[tid * S] = (base + tid < f1) ? 1.0f : 0.0f;
+            Ls[tid] = pl;
         }
         __syncthreads();
+        if (base + FB < f1) prefetch(base + FB);
         if (!active) continue;
 
         f32x4 acc1[4];
 #pragma unroll
         for (int s = 0; s < 4; s++) acc1[s] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kk = 0; kk < KK; kk++) {
+        for (int u = 0; u < KU; u++) {
 #pragma unroll
-            for (int s = 0; s < 4; s++)
-                acc1[s] = mfma4(Zr[(16 * s + c) * S + 4 * kk + q], bf[kk], acc1[s]);
+            for (int s = 0; s < 4; s++) {
+                const float4 z =
+                    *reinterpret_cast<const float4 *>(&Zr[(16 * s + c) * S + q * KK + 4 * u]);
+                acc1[s] = mfma4(z.x, bf[4 * u + 0], acc1[s]);
+                acc1[s] = mfma4(z.y, bf[4 * u + 1], acc1[s]);
+                acc1[s] = mfma4(z.z, bf[4 * u + 2], acc1[s]);
+                acc1[s] = mfma4(z.w, bf[4 * u + 3], acc1[s]);
+            }
         }
 
 #pragma unroll
         for (int s = 0; s < 4; s++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
+                // g = 2^(l2 - LL2); LL2 = +inf (no statistics / padding
+                // frame) and l2 = -1e30 (padding component) give 0, a NaN
+                // frame fails the cut.
                 const int row = 16 * s + 4 * q + r;
-                const float L = Ls[row];
-                float g = 0.0f;
-                if (cvalid && L != -INFINITY) g = fast_exp2(acc1[s][r] - L);
+                float g = fast_exp2(acc1[s][r] - Ls[row]);
                 g = (g > a.min_gamma) ? g : 0.0f;
+                const float *zr = &Zr[row * S + c * JT];
 #pragma unroll
-                for (int j = 0; j < JT; j++)
-                    acc2[j] = mfma4(g, Zr[row * S + 16 * j + c], acc2[j]);
+                for (int j = 0; j < JT; j++) acc2[j] = mfma4(g, zr[j], acc2[j]);
             }
         }
     }
@@ -274,10 +363,10 @@ __global__ __launch_bounds__(256) void stats_kernel(EstepArgs a) {
     const size_t Mp = (size_t)a.Mt * 16;
     float *out = a.partials + ((size_t)chunk * Mp + (size_t)tile * 16) * KP;
 #pragma unroll
-    for (int j = 0; j < JT; j++)
+    for (int r = 0; r < 4; r++)
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-            out[(size_t)(4 * q + r) * KP + 16 * j + c] = acc2[j][r];
+        for (int j = 0; j < JT; j++)
+            out[(size_t)(4 * q + r) * KP + c * JT + j] = acc2[j][r];
 }
 
 // ---------------------------------------------------------------------------

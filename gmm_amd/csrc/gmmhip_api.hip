@@ -314,7 +314,16 @@ int gmmhip_set_model(gmmhip_t h, int M, int D, const float *w, const float *mu,
     const int KK = KP / 4;
     const int Mt = (M + 15) / 16;
     const double log2e = 1.4426950408889634;
-    std::vector<float> P((size_t)Mt * KK * 64, 0.0f);
+    // MFMA B-operand packing P[t][lane][kk] = B[q*KK + kk][16t + c] with
+    // lane = 16q + c (the contraction index mapping of the E-step kernels).
+    std::vector<float> P((size_t)Mt * 64 * KK, 0.0f);
+    auto pidx = [&](int m, int k) {
+        const int t = m / 16, c = m % 16, qq = k / KK, kk = k % KK;
+        return ((size_t)t * 64 + qq * 16 + c) * KK + kk;
+    };
+    // Padding components (M..16*Mt-1): constant -1e30 (log2 units), so their
+    // log-likelihood vanishes in every exp2 without a lane mask.
+    for (int m = M; m < Mt * 16; m++) P[pidx(m, 0)] = -1e30f;
     std::vector<double> coef(KP);
     for (int m = 0; m < M; m++) {
         std::fill(coef.begin(), coef.end(), 0.0);
@@ -327,11 +336,7 @@ int gmmhip_set_model(gmmhip_t h, int M, int D, const float *w, const float *mu,
             cq += mc * mc * iv;
         }
         coef[0] = gconst[m] - 0.5 * cq;
-        const int t = m / 16, c = m % 16;
-        for (int k = 0; k < KP; k++) {
-            const int kk = k / 4, qq = k % 4;
-            P[((size_t)t * KK + kk) * 64 + qq * 16 + c] = (float)(coef[k] * log2e);
-        }
+        for (int k = 0; k < KP; k++) P[pidx(m, k)] = (float)(coef[k] * log2e);
     }
     DevGuard g(h->dev);
     HIPCHK(h, ensure(&h->d_P, &h->P_cap, P.size()));
@@ -397,6 +402,17 @@ int gmmhip_set_frames_device(gmmhip_t h, const float *dx, uint64_t N, int D, uin
     if (!h) return GMMHIP_ERR_ARG;
     if (!dx || N == 0 || D <= 0 || ld < (uint64_t)D)
         return fail(h, GMMHIP_ERR_ARG, "set_frames_device: bad arguments");
+    if (ld != (uint64_t)D) {
+        // The kernels stream frames as contiguous rows: compact a strided
+        // view once into the handle's own buffer.
+        DevGuard g(h->dev);
+        HIPCHK(h, ensure(&h->d_xown, &h->xown_cap, (size_t)N * D));
+        HIPCHK(h, hipMemcpy2DAsync(h->d_xown, D * sizeof(float), dx, ld * sizeof(float),
+                                   D * sizeof(float), N, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        dx = h->d_xown;
+        ld = D;
+    }
     h->d_x = dx;
     h->N = N;
     h->ld = ld;

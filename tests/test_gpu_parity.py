@@ -140,6 +140,26 @@ def test_offset_frames(acc):
     check_stats(st, orc.accumulate(x, w, mu, var), x)
 
 
+def test_lse_large_jumps_between_component_tiles(acc):
+    """The first component tiles sit far from the data (log-likelihoods
+    ~ -1e4) and later tiles close to it: the running log-sum-exp must move
+    its reference by far more than the exp2 range."""
+    rng = np.random.default_rng(21)
+    N, D, M = 3000, 20, 80
+    x, w, mu, var = rand_case(21, N, D, M, mean_sd=1.0)
+    mu[:32] += 60.0
+    w = w.copy()
+    w[:32] = 1e-30
+    w /= w.sum()
+    ll, st = run_gpu(acc, x, w, mu, var)
+    ref_ll = orc.frame_loglik(x, w, mu, var)
+    check_ll(ll, ref_ll)
+    check_stats(st, orc.accumulate(x, w, mu, var), x)
+    # reversed order: near components first
+    ll2, _ = run_gpu(acc, x, w[::-1].copy(), mu[::-1].copy(), var[::-1].copy())
+    check_ll(ll2, ref_ll)
+
+
 def test_single_component_posterior_is_one(acc):
     x, w, mu, var = rand_case(5, 3000, 12, 1)
     ll, st = run_gpu(acc, x, w, mu, var)
