@@ -26,6 +26,13 @@ EXPORTS = [
     "gmmhip_kernel_times", "gmmhip_reset_kernel_times",
 ]
 KT_LOGLIK, KT_STATS, KT_REDUCE, KT_COUNT = 0, 1, 2, 3
+# host-layer entry points (include/gmmhost.h), bound in gmm_amd.host
+EXTRA_EXPORTS = [
+    "gmmhost_model_load", "gmmhost_model_save", "gmmhost_model_create",
+    "gmmhost_model_dims", "gmmhost_model_get", "gmmhost_model_rearrange",
+    "gmmhost_model_free", "gmmhost_update_em", "gmmhost_update_map",
+    "gmmhost_mllr_transform", "gmmhost_mllr_apply", "gmmhost_last_error",
+]
 
 OK, ERR_HIP, ERR_OOM, ERR_NAN, ERR_ARG, ERR_VAR, ERR_STATE, ERR_MLLR, ERR_NODEV = range(9)
 PREC_FP32, PREC_BF16X3, PREC_FP16X3 = 0, 1, 2

@@ -24,6 +24,11 @@ COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
           "-Wno-unused-result", "-Wno-unused-value"]
 
 HIP_SOURCES = ["estep_fp32.hip", "gmmhip_api.hip"]
+HOST_SOURCES = ["host/gmm_model.cpp", "host/model_adapt.cpp", "host/feature_io.cpp",
+                "host/gmmhost_api.cpp"]
+TOOLS = ["trainGMM", "getLogLike"]
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
 
 
 def _run(cmd):
@@ -43,13 +48,34 @@ def _compile(src):
     return obj
 
 
+def _compile_host(src):
+    obj = os.path.join(OUT, os.path.basename(os.path.splitext(src)[0]) + ".o")
+    _run([CXX] + HOST_FLAGS + ["-I", os.path.join(ROOT, "include"), "-c",
+                               os.path.join(CSRC, src), "-o", obj])
+    return obj
+
+
+def _link_tool(name):
+    exe = os.path.join(OUT, name)
+    _run([CXX] + HOST_FLAGS + [os.path.join(CSRC, "tools", name + ".cpp"), "-o", exe + ".tmp",
+                               "-L", OUT, "-lgmmhip", "-Wl,-rpath,$ORIGIN",
+                               "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build_lib(jobs=8):
     os.makedirs(OUT, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, HIP_SOURCES))
+        hip = [ex.submit(_compile, s) for s in HIP_SOURCES]
+        host = [ex.submit(_compile_host, s) for s in HOST_SOURCES]
+        objs = [f.result() for f in hip] + [f.result() for f in host]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
+    for t in TOOLS:
+        _link_tool(t)
     return LIB
 
 

@@ -1,0 +1,196 @@
+// GMMStatsEstimator_HIP<TS>: the drop-in for the reference's CUDA
+// accumulator GMMStatsEstimator_CUDA<TS> (trainer/OL_GMMStatsEstimator_CUDA.h:
+// 39-107 of GMMTeam/GMM).  Same method names, arguments and accumulate-into
+// semantics; every statistic is computed by the gfx950 library through the C
+// ABI (include/gmmhip.h).  There is no CPU path: the reference falls back to
+// its SSE estimator without a device or below 8 frames per GPU
+// (OL_GMMStatsEstimator_CUDA.cpp:55-60, :402-408); here the constructor throws
+// without a device and every frame count runs on the GPU.  Errors of the C
+// ABI are rethrown as std::runtime_error (std::logic_error for misuse), the
+// exception classes the reference throws.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/gmmhip.h"
+#include "gmm_model.h"
+#include "gmm_stats.h"
+
+namespace gmm {
+
+template <typename TS>
+class GMMStatsEstimator_HIP {
+public:
+    // public knobs of GMMStatsEstimator (OL_GMMStatsEstimator.h:61-71)
+    float _minLogLike = -1e20f;
+    float _minGamma = 1e-4f;
+    float _min_var = 1e-6f;
+    unsigned int _numThreads = 1;
+    unsigned int _NAccBlocks_CUDA = 8;  // accepted, unused (chunking is automatic)
+    float _memoryBuffDataGB = 0.0f;
+    float _memoryBuffDataGB_GPU = 0.0f;
+    int _verbosity = 1;
+    bool _fNumStabilityTroubles = false;
+    int _precision = GMMHIP_PREC_FP32;
+
+    explicit GMMStatsEstimator_HIP(int GPU_id = -1) {
+        int n = 0;
+        gmmhip_device_count(&n);
+        if (n == 0) throw std::runtime_error("GMMStatsEstimator_HIP: no HIP device");
+        dev_ = (GPU_id >= 0 && GPU_id < n) ? GPU_id : 0;
+        gmmhip_opts o;
+        gmmhip_default_opts(&o);
+        check(gmmhip_create(dev_, &o, &h_));
+    }
+    ~GMMStatsEstimator_HIP() { gmmhip_destroy(h_); }
+    GMMStatsEstimator_HIP(const GMMStatsEstimator_HIP &) = delete;
+    GMMStatsEstimator_HIP &operator=(const GMMStatsEstimator_HIP &) = delete;
+
+    const std::string &getEstimationType() const { return type_; }
+
+    // Rearranges the model (zero-weight components last), computes its
+    // constants and uploads it.  first=false adds it to the current group.
+    void insertModel(GMModel &model, bool first = true) {
+        if (model.GetFullCovStatus())
+            throw std::runtime_error("insertModel(): full-covariance models are not supported");
+        if (!first && !group_.empty() && model.GetDimension() != group_[0].D)
+            throw std::runtime_error("Models (GMMs) have to have same dimensions");
+        if (first) group_.clear();
+        const unsigned nzero = model.RearrangeMixtures();
+        Packed p;
+        p.D = model.GetDimension();
+        p.Malloc = model.GetNumberOfMixtures();
+        p.M = p.Malloc - (int)nzero;
+        p.w.assign(model.weights().begin(), model.weights().begin() + p.M);
+        p.mu.assign(model.means().begin(), model.means().begin() + (size_t)p.M * p.D);
+        p.var.assign(model.vars().begin(), model.vars().begin() + (size_t)p.M * p.D);
+        for (float v : p.var)
+            if (v < _min_var) throw std::runtime_error("insertModel(): model contains very low variances!");
+        group_.push_back(std::move(p));
+        select(group_.size() - 1);
+    }
+
+    void setModelToBeUsed(unsigned int n) {
+        if (n >= group_.size()) throw std::out_of_range("setModelToBeUsed(): no such model");
+        select(n);
+    }
+
+    void setAccFlags(bool mixProbsOnly, bool meanStats, bool varStats, bool varFull = false,
+                     bool auxStats = false) {
+        if (varFull || auxStats)
+            throw std::runtime_error("setAccFlags(): full-covariance and robust statistics are not supported");
+        flagsSet_ = true;
+        fMixProbOnly_ = mixProbsOnly;
+        fMean_ = meanStats && !mixProbsOnly;
+        fVar_ = varStats && !mixProbsOnly;
+    }
+    void resetAccFlags() { flagsSet_ = false; }
+    void setSameDataFlag() { sameData_ = true; }
+
+    // Adds the statistics of the N frames into gmms (allocating it for the
+    // model's components if it is not allocated yet).
+    void accumulateStatsMT(float **vectors, unsigned int NSamples, unsigned int dim, GMMStats<TS> &gmms) {
+        const Packed &p = current();
+        if (!flagsSet_) {
+            fMixProbOnly_ = false;
+            fMean_ = gmms.isAllocated() ? gmms.hasMeans() : true;
+            fVar_ = gmms.isAllocated() ? gmms.hasVars() : false;
+        }
+        if (!gmms.isAllocated()) {
+            gmms.alloc(p.D, p.Malloc, fMean_, fVar_);
+            gmms.reset();
+        }
+        if ((int)dim != p.D || gmms.getDim() != dim)
+            throw std::logic_error("accumulateStatsMT(): Inconsistent feature & statistic dimensions!");
+        if ((fMean_ && !gmms.hasMeans()) || (fVar_ && !gmms.hasVars()))
+            throw std::logic_error("accumulateStatsMT(): statistics not allocated for the requested flags");
+        if (NSamples == 0) return;
+        upload(vectors, NSamples, dim);
+        syncOpts();
+        const int flags = (fMean_ ? GMMHIP_ACC_MEAN : 0) | (fVar_ ? GMMHIP_ACC_VAR : 0);
+        std::vector<double> n(p.M, 0.0), F(fMean_ ? (size_t)p.M * p.D : 0), S(fVar_ ? (size_t)p.M * p.D : 0);
+        double tot = 0.0;
+        uint64_t nf = 0;
+        check(gmmhip_accumulate(h_, flags, n.data(), fMean_ ? F.data() : nullptr,
+                                fVar_ ? S.data() : nullptr, &tot, &nf));
+        TS *gn = gmms.probData();
+        TS *gF = gmms.meanData();
+        TS *gS = gmms.varData();
+        for (int m = 0; m < p.M; m++) {
+            gn[m] += (TS)n[m];
+            for (int d = 0; d < p.D && fMean_; d++) gF[(size_t)m * p.D + d] += (TS)F[(size_t)m * p.D + d];
+            for (int d = 0; d < p.D && fVar_; d++) gS[(size_t)m * p.D + d] += (TS)S[(size_t)m * p.D + d];
+        }
+        gmms._totalAccSamples += nf;
+        gmms._totLogLike += (TS)tot;
+        sameData_ = false;
+    }
+
+    // Per-frame log-likelihoods of the N frames into outLogLikes[N].
+    void compVecsLogLikeMT(float **vecs, unsigned int NSamples, unsigned int dim, float *outLogLikes) {
+        const Packed &p = current();
+        if ((int)dim != p.D) throw std::logic_error("compVecsLogLikeMT(): Inconsistent dimensions!");
+        if (NSamples == 0) return;
+        upload(vecs, NSamples, dim);
+        syncOpts();
+        check(gmmhip_loglikes(h_, outLogLikes));
+        for (unsigned n = 0; n < NSamples; n++)
+            if (outLogLikes[n] <= _minLogLike) _fNumStabilityTroubles = true;
+        sameData_ = false;
+    }
+
+    GMMStatsEstimator_HIP<TS> *getNewInstance() { return new GMMStatsEstimator_HIP<TS>(dev_); }
+
+    gmmhip_t handle() const { return h_; }
+
+private:
+    struct Packed {
+        int D = 0, M = 0, Malloc = 0;
+        std::vector<float> w, mu, var;
+    };
+    gmmhip_t h_ = nullptr;
+    int dev_ = 0;
+    std::string type_ = "GPU HIP";
+    std::vector<Packed> group_;
+    size_t cur_ = 0;
+    bool flagsSet_ = false, fMixProbOnly_ = false, fMean_ = true, fVar_ = false;
+    bool sameData_ = false;
+    bool haveFrames_ = false;
+
+    const Packed &current() const {
+        if (group_.empty()) throw std::logic_error("accumulateStatsMT(): None model inserted!");
+        return group_[cur_];
+    }
+    void select(size_t n) {
+        const Packed &p = group_[n];
+        syncOpts();
+        check(gmmhip_set_model(h_, p.M, p.D, p.w.data(), p.mu.data(), p.var.data()));
+        cur_ = n;
+    }
+    void syncOpts() {
+        gmmhip_opts o;
+        gmmhip_default_opts(&o);
+        o.min_loglike = _minLogLike;
+        o.min_gamma = _minGamma;
+        o.min_var = _min_var;
+        o.precision = _precision;
+        o.verbosity = _verbosity;
+        check(gmmhip_set_opts(h_, &o));
+    }
+    void upload(float **vecs, unsigned int N, unsigned int dim) {
+        if (sameData_ && haveFrames_) return;
+        check(gmmhip_set_frames(h_, vecs, N, (int)dim));
+        haveFrames_ = true;
+    }
+    void check(int rc) const {
+        if (rc == GMMHIP_OK) return;
+        const std::string msg = std::string(h_ ? gmmhip_last_error(h_) : "") + " [gmmhip status " +
+                                std::to_string(rc) + "]";
+        if (rc == GMMHIP_ERR_ARG || rc == GMMHIP_ERR_STATE) throw std::logic_error(msg);
+        throw std::runtime_error(msg);
+    }
+};
+
+}  // namespace gmm

@@ -1,0 +1,153 @@
+// Sufficient-statistics container, the counterpart of GMMStats<T>
+// (model/OL_GMMStats.h:44-148 of GMMTeam/GMM): per component the occupancy
+// n = sum g, first order F = sum g x and diagonal second order S = sum g x^2,
+// plus the number of accumulated frames and the total log-likelihood.
+// Estimators ADD into it (the caller resets), operator+= is the reduction
+// (OL_GMMStats.cpp:99-142), and save()/load() keep the reference's TXT/BIN
+// layout (OL_GMMStats.cpp:699-776).  Storage is flat ([M], [M][D]) so the
+// device layer can accumulate into it directly.
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace gmm {
+
+template <typename T>
+class GMMStats {
+public:
+    GMMStats() = default;
+    GMMStats(unsigned int dim, unsigned int nummix, bool allocMeans = true, bool allocVars = false) {
+        alloc(dim, nummix, allocMeans, allocVars);
+    }
+
+    // alloc does not initialise values -> reset()
+    void alloc(unsigned int dim, unsigned int nummix, bool allocMeans = true,
+               bool allocVars = false) {
+        dim_ = dim;
+        M_ = nummix;
+        allM_ = allocMeans;
+        allV_ = allocVars;
+        n_.assign(nummix, T(0));
+        F_.assign(allocMeans ? (size_t)nummix * dim : 0, T(0));
+        S_.assign(allocVars ? (size_t)nummix * dim : 0, T(0));
+        allocated_ = true;
+    }
+    void reset() {
+        std::fill(n_.begin(), n_.end(), T(0));
+        std::fill(F_.begin(), F_.end(), T(0));
+        std::fill(S_.begin(), S_.end(), T(0));
+        _totalAccSamples = 0;
+        _totLogLike = T(0);
+    }
+    bool isAllocated() const { return allocated_; }
+    bool hasMeans() const { return allM_; }
+    bool hasVars() const { return allV_; }
+    bool hasFullCov() const { return false; }
+    unsigned int getMixNum() const { return M_; }
+    unsigned int getDim() const { return dim_; }
+    uint64_t getTotAccSamples() const { return _totalAccSamples; }
+    T getTotLogLike() const { return _totLogLike; }
+    T getProb(unsigned int m) const { return n_[m]; }
+    const T *getMean(unsigned int m) const { return allM_ ? &F_[(size_t)m * dim_] : nullptr; }
+    const T *getVar(unsigned int m) const { return allV_ ? &S_[(size_t)m * dim_] : nullptr; }
+
+    T *probData() { return n_.data(); }
+    T *meanData() { return allM_ ? F_.data() : nullptr; }
+    T *varData() { return allV_ ? S_.data() : nullptr; }
+
+    // Only statistics allocated on the left are summed (as in the reference).
+    GMMStats &operator+=(const GMMStats &r) {
+        if (!allocated_) {
+            *this = r;
+            return *this;
+        }
+        if (r.M_ != M_ || r.dim_ != dim_) throw std::logic_error("GMMStats::operator+=: shape mismatch");
+        for (size_t i = 0; i < n_.size(); i++) n_[i] += r.n_[i];
+        if (allM_ && r.allM_)
+            for (size_t i = 0; i < F_.size(); i++) F_[i] += r.F_[i];
+        if (allV_ && r.allV_)
+            for (size_t i = 0; i < S_.size(); i++) S_[i] += r.S_[i];
+        _totalAccSamples += r._totalAccSamples;
+        _totLogLike += r._totLogLike;
+        return *this;
+    }
+
+    // Header "saveMProbs sMeans sVars sFullVars dim varsize nummix", then per
+    // component the occupancy, the first-order and the second-order rows.
+    void save(const char *filename, bool txt, bool saveMProbs = true, bool saveMeans = true,
+              bool saveVars = true) const {
+        const bool sM = saveMeans && allM_, sV = saveVars && allV_;
+        std::ofstream f(filename, txt ? std::ios::out : (std::ios::out | std::ios::binary));
+        if (!f) throw std::runtime_error(std::string("GMMStats::save: cannot open ") + filename);
+        if (txt) {
+            f.precision(17);
+            f << (int)saveMProbs << " " << (int)sM << " " << (int)sV << " 0 " << dim_ << " "
+              << dim_ << " " << M_ << "\n\n";
+            for (unsigned m = 0; m < M_; m++) {
+                if (saveMProbs) f << n_[m] << "\n";
+                if (sM) {
+                    for (unsigned d = 0; d < dim_; d++) f << " " << F_[(size_t)m * dim_ + d];
+                    f << "\n";
+                }
+                if (sV) {
+                    for (unsigned d = 0; d < dim_; d++) f << " " << S_[(size_t)m * dim_ + d];
+                    f << "\n";
+                }
+                f << "\n";
+            }
+        } else {
+            const int32_t hdr[7] = {(int32_t)saveMProbs, (int32_t)sM, (int32_t)sV, 0,
+                                    (int32_t)dim_, (int32_t)dim_, (int32_t)M_};
+            f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+            for (unsigned m = 0; m < M_; m++) {
+                if (saveMProbs) f.write(reinterpret_cast<const char *>(&n_[m]), sizeof(T));
+                if (sM) f.write(reinterpret_cast<const char *>(&F_[(size_t)m * dim_]), sizeof(T) * dim_);
+                if (sV) f.write(reinterpret_cast<const char *>(&S_[(size_t)m * dim_]), sizeof(T) * dim_);
+            }
+        }
+        if (!f) throw std::runtime_error(std::string("GMMStats::save: write failed ") + filename);
+    }
+
+    void load(const char *filename, bool txt) {
+        std::ifstream f(filename, txt ? std::ios::in : (std::ios::in | std::ios::binary));
+        if (!f) throw std::runtime_error(std::string("GMMStats::load: cannot open ") + filename);
+        int32_t hdr[7];
+        if (txt) {
+            for (int i = 0; i < 7; i++) f >> hdr[i];
+        } else {
+            f.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
+        }
+        if (!f || hdr[3] != 0 || hdr[4] != hdr[5]) throw std::runtime_error("GMMStats::load: bad header");
+        alloc(hdr[4], hdr[6], hdr[1] != 0, hdr[2] != 0);
+        reset();
+        for (unsigned m = 0; m < M_; m++) {
+            if (txt) {
+                if (hdr[0]) f >> n_[m];
+                if (allM_)
+                    for (unsigned d = 0; d < dim_; d++) f >> F_[(size_t)m * dim_ + d];
+                if (allV_)
+                    for (unsigned d = 0; d < dim_; d++) f >> S_[(size_t)m * dim_ + d];
+            } else {
+                if (hdr[0]) f.read(reinterpret_cast<char *>(&n_[m]), sizeof(T));
+                if (allM_) f.read(reinterpret_cast<char *>(&F_[(size_t)m * dim_]), sizeof(T) * dim_);
+                if (allV_) f.read(reinterpret_cast<char *>(&S_[(size_t)m * dim_]), sizeof(T) * dim_);
+            }
+        }
+        if (!f) throw std::runtime_error(std::string("GMMStats::load: truncated ") + filename);
+    }
+
+    uint64_t _totalAccSamples = 0;
+    T _totLogLike = T(0);
+
+private:
+    bool allocated_ = false, allM_ = false, allV_ = false;
+    unsigned int dim_ = 0, M_ = 0;
+    std::vector<T> n_, F_, S_;
+};
+
+}  // namespace gmm

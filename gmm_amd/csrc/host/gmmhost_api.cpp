@@ -1,0 +1,175 @@
+// C ABI of the host layer (include/gmmhost.h).
+#include "../../../include/gmmhost.h"
+
+#include <cstring>
+#include <string>
+
+#include "gmm_model.h"
+#include "model_adapt.h"
+
+struct gmmhost_model {
+    gmm::GMModel m;
+};
+
+namespace {
+thread_local std::string g_err;
+int fail(const std::string &msg, int code = 1) {
+    g_err = msg;
+    return code;
+}
+
+// Statistics wrapper for the update rules (GMMStats<double> view).
+gmm::GMMStats<double> make_stats(int M, int D, const double *n, const double *F, const double *S,
+                                 uint64_t tot) {
+    gmm::GMMStats<double> st(D, M, F != nullptr, S != nullptr);
+    st.reset();
+    std::memcpy(st.probData(), n, sizeof(double) * M);
+    if (F) std::memcpy(st.meanData(), F, sizeof(double) * M * D);
+    if (S) std::memcpy(st.varData(), S, sizeof(double) * M * D);
+    st._totalAccSamples = tot;
+    return st;
+}
+
+// CModelAdapt exposes the statistics it works on through Stats().
+struct RuleHost : gmm::CModelAdapt {};
+}  // namespace
+
+extern "C" {
+
+const char *gmmhost_last_error(void) { return g_err.c_str(); }
+
+int gmmhost_model_load(const char *path, int txt, gmmhost_model_t *out) {
+    if (!path || !out) return fail("null argument");
+    gmmhost_model *h = new gmmhost_model();
+    if (h->m.Load(path, txt != 0) != 0) {
+        const std::string e = h->m.LastError();
+        delete h;
+        return fail(e);
+    }
+    *out = h;
+    return 0;
+}
+
+int gmmhost_model_save(gmmhost_model_t m, const char *path, int txt) {
+    if (!m || !path) return fail("null argument");
+    if (m->m.Save(path, txt != 0) != 0) return fail(m->m.LastError());
+    return 0;
+}
+
+int gmmhost_model_create(int M, int D, const float *w, const float *mu, const float *var,
+                         gmmhost_model_t *out) {
+    if (M <= 0 || D <= 0 || !w || !mu || !var || !out) return fail("bad arguments");
+    gmmhost_model *h = new gmmhost_model();
+    h->m = gmm::GMModel(D, M);
+    for (int k = 0; k < M; k++) h->m.SetMixParam(k, mu + (size_t)k * D, var + (size_t)k * D, w[k]);
+    *out = h;
+    return 0;
+}
+
+int gmmhost_model_dims(gmmhost_model_t m, int *M, int *D) {
+    if (!m || !M || !D) return fail("null argument");
+    *M = m->m.GetNumberOfMixtures();
+    *D = m->m.GetDimension();
+    return 0;
+}
+
+int gmmhost_model_get(gmmhost_model_t m, float *w, float *mu, float *var) {
+    if (!m) return fail("null argument");
+    if (w) std::memcpy(w, m->m.weights().data(), sizeof(float) * m->m.weights().size());
+    if (mu) std::memcpy(mu, m->m.means().data(), sizeof(float) * m->m.means().size());
+    if (var) std::memcpy(var, m->m.vars().data(), sizeof(float) * m->m.vars().size());
+    return 0;
+}
+
+int gmmhost_model_rearrange(gmmhost_model_t m, int *nzero) {
+    if (!m) return fail("null argument");
+    const unsigned nz = m->m.RearrangeMixtures();
+    if (nzero) *nzero = (int)nz;
+    return 0;
+}
+
+void gmmhost_model_free(gmmhost_model_t m) { delete m; }
+
+int gmmhost_update_em(gmmhost_model_t m, const double *n, const double *F, const double *S,
+                      uint64_t tot_samples, float min_var, int upweight, int upmean, int upvar,
+                      int *left) {
+    if (!m || !n) return fail("null argument");
+    try {
+        const int M = m->m.GetNumberOfMixtures(), D = m->m.GetDimension();
+        gmm::GMMStats<double> st = make_stats(M, D, n, F, S, tot_samples);
+        RuleHost ad;
+        gmm::CModelAdapt::OPTIONS o;
+        o._min_var = min_var;
+        ad.SetOptions(o);
+        const unsigned l = ad.UpdateModelEM(&m->m, st, upweight != 0, upmean != 0, upvar != 0);
+        if (left) *left = (int)l;
+    } catch (std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int gmmhost_update_map(gmmhost_model_t m, gmmhost_model_t ubm, const double *n, const double *F,
+                       const double *S, uint64_t tot_samples, int tau, float min_var, int upweight,
+                       int upmean, int upvar) {
+    if (!m || !ubm || !n) return fail("null argument");
+    try {
+        const int M = m->m.GetNumberOfMixtures(), D = m->m.GetDimension();
+        RuleHost ad;
+        gmm::CModelAdapt::OPTIONS o;
+        o._min_var = min_var;
+        o._tau = tau;
+        o._AdaptWeightBool = upweight != 0;
+        o._AdaptMeanBool = upmean != 0;
+        o._AdaptVarBool = upvar != 0;
+        ad.SetOptions(o);
+        ad.Stats() = make_stats(M, D, n, F, S, tot_samples);
+        ad.UpdateModelMAP(&m->m, &ubm->m);
+    } catch (std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int gmmhost_mllr_transform(gmmhost_model_t m, gmmhost_model_t ubm, const double *n,
+                           const double *F, float min_var, double *W) {
+    if (!m || !n || !F || !W) return fail("null argument");
+    try {
+        const int M = m->m.GetNumberOfMixtures(), D = m->m.GetDimension(), E = D + 1;
+        // CheckInvertibility (OL_ModelAdapt.cpp:1851-1878)
+        int active = 0;
+        double occ = 0.0;
+        for (int k = 0; k < M; k++)
+            if (n[k] > 0) {
+                active++;
+                occ += n[k];
+            }
+        if (M < D + 1 || active < D + 1 || occ * D < (double)D * (D + 1))
+            return fail("CheckInvertibility(): MLLR is under-determined for these statistics", 7);
+        RuleHost ad;
+        ad.Stats() = make_stats(M, D, n, F, nullptr, 0);
+        std::vector<double> Gi, ki;
+        if (ubm) gmm::CModelAdapt::prepareKiGiInitMatrices(ubm->m, min_var, Gi, ki);
+        std::vector<double> G((size_t)E * E), k(E);
+        ad.SetNumMix(M);
+        for (int i = 0; i < D; i++) {
+            ad.ComputeKiGi(m->m, i, Gi.empty() ? nullptr : &Gi[(size_t)i * E * E],
+                           ki.empty() ? nullptr : &ki[(size_t)i * E], G.data(), k.data());
+            gmm::spd_solve(E, G.data(), k.data(), W + (size_t)i * E);
+        }
+    } catch (std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int gmmhost_mllr_apply(gmmhost_model_t m, const double *W) {
+    if (!m || !W) return fail("null argument");
+    gmm::MLLRTransform T;
+    T.D = m->m.GetDimension();
+    T.W.assign(W, W + (size_t)T.D * (T.D + 1));
+    m->m = *gmm::CModelAdapt::AdaptModelMLLR(m->m, T);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,496 @@
+#include "model_adapt.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+
+namespace gmm {
+
+namespace {
+constexpr double MIN_IND_SAMPLES = 2.0;  // OL_ModelAdapt.h:54
+}
+
+void CModelAdapt::SetOptions(const OPTIONS &opt) {
+    opt_ = opt;
+    if (opt_._min_var < 1.17549435e-38f) opt_._min_var = 1.17549435e-38f;  // CheckOptions :2141-2143
+}
+
+void CModelAdapt::InsertData(float **vectors, unsigned int N, unsigned int D) {
+    data_ = vectors;
+    N_ = N;
+    D_ = D;
+    dataChanged_ = true;
+}
+
+void CModelAdapt::initEstimator() {
+    if (!est_) {
+        est_.reset(new GMMStatsEstimator_HIP<double>(opt_._cudaGPUid));
+        est_->_min_var = opt_._min_var;
+        est_->_precision = opt_._precision;
+    }
+    est_->_verbosity = opt_._verbosity;
+    est_->_numThreads = opt_._numThreads;
+}
+
+// AccumulateStats (OL_ModelAdapt.cpp:411-466): insert the model (which
+// rearranges it), set the flags and add the statistics of the inserted data.
+void CModelAdapt::AccumulateStats(GMModel *model, GMMStats<double> &stats, bool mixProbsOnly,
+                                  bool meanStats, bool varStats) {
+    if (!data_ || N_ == 0) throw std::logic_error("AccumulateStats(): Insert data first!");
+    initEstimator();
+    if (model) est_->insertModel(*model);
+    est_->setAccFlags(mixProbsOnly, meanStats, varStats);
+    if (!dataChanged_) est_->setSameDataFlag();
+    est_->accumulateStatsMT(data_, N_, D_, stats);
+    dataChanged_ = false;
+    if (opt_._verbosity)
+        std::cout << "[#frames accumulated = " << stats.getTotAccSamples() << "]" << std::endl;
+}
+
+// UpdateModelEM (OL_ModelAdapt.cpp:823-949), diagonal, non-robust.
+unsigned int CModelAdapt::UpdateModelEM(GMModel *model, GMMStats<double> &stats, bool upweight,
+                                        bool upmean, bool upvar) {
+    const int D = model->GetDimension();
+    const int nummix = std::min<int>(model->GetNumberOfMixtures(), (int)stats.getMixNum());
+    std::vector<float> mean(D);
+    bool wchange = false;
+    unsigned int nzero = 0;
+    double gcorr = 0.0;
+    for (int j = 0; j < nummix; j++) {
+        if (model->GetMixtureWeight(j) == 0.0f) {
+            ++nzero;
+            continue;
+        }
+        const double np = stats.getProb(j);
+        if (opt_._saveMixOcc) model->SetMixOcc(j, (float)np);
+        if (np < MIN_IND_SAMPLES) {
+            model->ResetMixture(j);
+            model->SetMixWeight(j, 0.0f);
+            wchange = true;
+            continue;
+        }
+        if (upweight) {
+            model->SetMixWeight(j, (float)(np / (double)stats.getTotAccSamples()));
+            wchange = true;
+        }
+        gcorr += model->GetMixtureWeight(j);
+        if (upmean || upvar) {
+            const double *F = stats.getMean(j);
+            const double *S = stats.getVar(j);
+            if (!F || (upvar && !S))
+                throw std::logic_error("UpdateModelEM(): statistics not allocated");
+            for (int k = 0; k < D; k++) mean[k] = (float)(F[k] / np);
+            if (upvar) {
+                float *var = model->GetMixtureVar(j);
+                for (int k = 0; k < D; k++) {
+                    float v = (float)(S[k] / np - std::pow((double)mean[k], 2));
+                    if (v < opt_._min_var) v = opt_._min_var;
+                    var[k] = v;
+                }
+            }
+            if (upmean) std::memcpy(model->GetMixtureMean(j), mean.data(), sizeof(float) * D);
+        }
+    }
+    if (wchange && gcorr > 0.0)
+        for (int j = 0; j < nummix; j++)
+            model->SetMixWeight(j, (float)(model->GetMixtureWeight(j) / gcorr));
+    return (unsigned int)(nummix - (int)nzero);
+}
+
+// UpdateModelMAP (OL_ModelAdapt.cpp:993-1106), diagonal, non-robust.
+void CModelAdapt::UpdateModelMAP(GMModel *model, GMModel *UBM) {
+    if (opt_._tau == 0) {
+        if (UpdateModelEM(model, stats_, opt_._AdaptWeightBool, opt_._AdaptMeanBool,
+                          opt_._AdaptVarBool) == 0) {
+            std::cerr << "\n [WARNING: All Gaussians discarded - keeping the initial/global model]"
+                      << std::endl;
+            *model = *UBM;
+        }
+        return;
+    }
+    const int nummix = model->GetNumberOfMixtures();
+    const int D = model->GetDimension();
+    std::vector<float> am(D), av(D);
+    double mixcorr = 0.0;
+    for (int j = 0; j < nummix; j++) {
+        if (UBM->GetMixtureWeight(j) == 0) continue;
+        const double np = stats_.getProb(j);
+        if (opt_._saveMixOcc) model->SetMixOcc(j, (float)np);
+        if (!(np > 0.0)) continue;
+        const float *um = UBM->GetMixtureMean(j);
+        const float *uv = UBM->GetMixtureVar(j);
+        const double alfa = np / (np + opt_._tau);
+        if (opt_._AdaptWeightBool && stats_.getTotAccSamples() > 0) {
+            const float aw = (float)(alfa * (np / (double)stats_.getTotAccSamples()) +
+                                     (1 - alfa) * UBM->GetMixtureWeight(j));
+            model->SetMixWeight(j, aw);
+            mixcorr += aw;
+        }
+        if (opt_._AdaptMeanBool || opt_._AdaptVarBool) {
+            const double *F = stats_.getMean(j);
+            const double *S = stats_.getVar(j);
+            for (int k = 0; k < D; k++) am[k] = (float)(alfa * F[k] / np + (1 - alfa) * um[k]);
+            if (opt_._AdaptVarBool) {
+                for (int k = 0; k < D; k++) {
+                    av[k] = (float)(alfa * S[k] / np + (1 - alfa) * (uv[k] + um[k] * um[k]) -
+                                    am[k] * am[k]);
+                    if (av[k] < opt_._min_var) av[k] = opt_._min_var;
+                }
+                std::memcpy(model->GetMixtureVar(j), av.data(), sizeof(float) * D);
+            }
+            std::memcpy(model->GetMixtureMean(j), am.data(), sizeof(float) * D);
+        }
+    }
+    if (opt_._AdaptWeightBool && mixcorr != 0)
+        for (int j = 0; j < nummix; j++)
+            model->SetMixWeight(j, model->GetMixtureWeight(j) / (float)mixcorr);
+}
+
+void CModelAdapt::IterEM(GMModel *model, int iters) {
+    const bool weightOnly = opt_._AdaptWeightBool && !opt_._AdaptMeanBool && !opt_._AdaptVarBool;
+    for (int i = 0; i < iters; i++) {
+        stats_.reset();
+        AccumulateStats(model, stats_, weightOnly, opt_._AdaptMeanBool, opt_._AdaptVarBool);
+        GMModel tmp(*model);
+        UpdateModelMAP(model, &tmp);
+    }
+}
+
+void CModelAdapt::IterMAP(GMModel *model, int iters) {
+    const bool weightOnly = opt_._AdaptWeightBool && !opt_._AdaptMeanBool && !opt_._AdaptVarBool;
+    for (int i = 0; i < iters; i++) {
+        stats_.reset();
+        AccumulateStats(model, stats_, weightOnly, opt_._AdaptMeanBool, opt_._AdaptVarBool);
+        UpdateModelMAP(model, UBM_);
+    }
+}
+
+std::unique_ptr<GMModel> CModelAdapt::AdaptModelMAP(GMModel *ubm) {
+    std::unique_ptr<GMModel> model(new GMModel(*ubm));
+    if (opt_._iterCountEM == 0 && opt_._iterCountMAP == 0)
+        std::cerr << "AdaptModel(): WARNING: iterCountEM = iterCountMAP = 0 => nothing to do "
+                     "(speaker model == UBM)!"
+                  << std::endl;
+    if (opt_._iterCountEM) IterEM(model.get(), opt_._iterCountEM);
+    if (opt_._iterCountMAP) IterMAP(model.get(), opt_._iterCountMAP);
+    return model;
+}
+
+// ------------------------------------------------------------------- MLLR
+void CModelAdapt::CheckInvertibility() const {
+    const int D = (int)D_;
+    if (nummix_ < D + 1)
+        throw std::runtime_error(
+            "CheckInvertibility(): In order to use MLLR, number of Gaussians in the model has to be "
+            "greater than the dimension of feature vectors (use MAP adaptation)!");
+    int active = 0;
+    double occ = 0.0;
+    for (int m = 0; m < nummix_; m++)
+        if (stats_.getProb(m) > 0) {
+            ++active;
+            occ += stats_.getProb(m);
+        }
+    if (active < D + 1)
+        throw std::runtime_error(
+            "CheckInvertibility(): To ensure invertibility of matrix Gi, count of active mixtures has "
+            "to be >= dimension of feature vectors (increase amount of data!)!");
+    if (occ * D < (double)D * (D + 1))
+        throw std::runtime_error(
+            "CheckInvertibility(): Number of parameters to be estimated is higher than the number of "
+            "available data (increase amount of data!)!");
+}
+
+// prepareKiGiInitMatrices (OL_ModelAdapt.cpp:1484-1523): every UBM component
+// contributes one virtual frame at its mean with its variance.
+void CModelAdapt::prepareKiGiInitMatrices(const GMModel &UBM, float min_var, std::vector<double> &G,
+                                          std::vector<double> &k) {
+    const int M = UBM.GetNumberOfMixtures(), D = UBM.GetDimension(), E = D + 1;
+    G.assign((size_t)D * E * E, 0.0);
+    k.assign((size_t)D * E, 0.0);
+    for (int m = 0; m < M; m++) {
+        const float *var = UBM.GetMixtureVar(m), *mean = UBM.GetMixtureMean(m);
+        for (int i = 0; i < D; i++) {
+            const double cG = (var[i] < min_var) ? 1.0 / min_var : 1.0 / var[i];
+            const double ck = mean[i] * cG;
+            double *Gi = &G[(size_t)i * E * E], *ki = &k[(size_t)i * E];
+            for (int a = 0; a < D; a++) {
+                ki[a] += ck * mean[a];
+                Gi[(size_t)D * E + a] += cG * mean[a];
+                Gi[(size_t)a * E + D] += cG * mean[a];
+                for (int b = 0; b < D; b++) Gi[(size_t)a * E + b] += cG * mean[a] * mean[b];
+                Gi[(size_t)a * E + a] += cG * var[a];
+            }
+            Gi[(size_t)E * E - 1] += cG;
+            ki[D] += ck;
+        }
+    }
+}
+
+// ComputeKiGi_MLLR (OL_ModelAdapt.cpp:1525-1581) for row i.
+void CModelAdapt::ComputeKiGi(const GMModel &model, int i, const double *Ginit, const double *kinit,
+                              double *G, double *k) const {
+    const int D = model.GetDimension(), E = D + 1;
+    if (Ginit && kinit) {
+        std::memcpy(G, Ginit, sizeof(double) * E * E);
+        std::memcpy(k, kinit, sizeof(double) * E);
+    } else {
+        std::memset(G, 0, sizeof(double) * E * E);
+        std::memset(k, 0, sizeof(double) * E);
+    }
+    for (int m = 0; m < nummix_; m++) {
+        const double np = stats_.getProb(m);
+        if (!(np > 0)) continue;
+        const float *mm = model.GetMixtureMean(m);
+        const double vi = model.GetMixtureVar(m)[i];
+        const double cm = stats_.getMean(m)[i] / vi, cv = np / vi;
+        for (int a = 0; a < D; a++) {
+            k[a] += cm * mm[a];
+            const double cva = cv * mm[a];
+            for (int b = a; b < D; b++) G[(size_t)a * E + b] += cva * mm[b];
+            G[(size_t)a * E + D] += cva;
+        }
+        k[D] += cm;
+        G[(size_t)D * E + D] += cv;
+    }
+    for (int a = 0; a < E; a++)
+        for (int b = a + 1; b < E; b++) G[(size_t)b * E + a] = G[(size_t)a * E + b];
+}
+
+void spd_solve(int n, const double *G, const double *k, double *w) {
+    std::vector<double> L((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j <= i; j++) {
+            double s = G[(size_t)i * n + j];
+            for (int p = 0; p < j; p++) s -= L[(size_t)i * n + p] * L[(size_t)j * n + p];
+            if (i == j) {
+                if (!(s > 0.0)) throw std::runtime_error("MLLR: matrix G_i is not positive definite");
+                L[(size_t)i * n + i] = std::sqrt(s);
+            } else {
+                L[(size_t)i * n + j] = s / L[(size_t)j * n + j];
+            }
+        }
+    }
+    std::vector<double> y(n);
+    for (int i = 0; i < n; i++) {
+        double s = k[i];
+        for (int p = 0; p < i; p++) s -= L[(size_t)i * n + p] * y[p];
+        y[i] = s / L[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double s = y[i];
+        for (int p = i + 1; p < n; p++) s -= L[(size_t)p * n + i] * w[p];
+        w[i] = s / L[(size_t)i * n + i];
+    }
+}
+
+// ComputeW_MLLR (OL_ModelAdapt.cpp:1417-1482): statistics of the model, then
+// w_i = G_i^-1 k_i per row (FP64; Cholesky instead of the reference's
+// unpivoted Gauss-Jordan inverse, same solution for the SPD G_i).
+MLLRTransform CModelAdapt::ComputeW_MLLR(GMModel *model, const std::vector<double> &Ginit,
+                                         const std::vector<double> &kinit, bool wantQ) {
+    stats_.reset();
+    AccumulateStats(model, stats_, false, true, false);
+    CheckInvertibility();
+    const int D = (int)D_, E = D + 1;
+    MLLRTransform W;
+    W.D = D;
+    W.W.assign((size_t)D * E, 0.0);
+    std::vector<double> G((size_t)E * E), k(E), w(E);
+    for (int i = 0; i < D; i++) {
+        ComputeKiGi(*model, i, Ginit.empty() ? nullptr : &Ginit[(size_t)i * E * E],
+                    kinit.empty() ? nullptr : &kinit[(size_t)i * E], G.data(), k.data());
+        spd_solve(E, G.data(), k.data(), w.data());
+        std::memcpy(&W.W[(size_t)i * E], w.data(), sizeof(double) * E);
+        if (wantQ) {
+            double Q = 0.0;
+            for (int a = 0; a < E; a++) {
+                double gw = 0.0;
+                for (int b = 0; b < E; b++) gw += G[(size_t)a * E + b] * w[b];
+                Q += w[a] * (k[a] - 0.5 * gw);
+            }
+            W.Q.push_back(Q);
+        }
+    }
+    return W;
+}
+
+// AdaptModelMLLR (OL_ModelAdapt.cpp:1583-1612): mu' = A mu + b.
+std::unique_ptr<GMModel> CModelAdapt::AdaptModelMLLR(const GMModel &model, const MLLRTransform &W) {
+    std::unique_ptr<GMModel> up(new GMModel(model));
+    const int D = W.D, E = D + 1;
+    std::vector<float> src(D);
+    for (int m = 0; m < up->GetNumberOfMixtures(); m++) {
+        float *mean = up->GetMixtureMean(m);
+        std::memcpy(src.data(), mean, sizeof(float) * D);
+        for (int a = 0; a < D; a++) {
+            double v = 0.0;
+            for (int b = 0; b < D; b++) v += W.W[(size_t)a * E + b] * src[b];
+            mean[a] = (float)(v + W.W[(size_t)a * E + D]);
+        }
+    }
+    return up;
+}
+
+// UpdateMLLRMatrix (OL_ModelAdapt.cpp:1319-1340): Wsum <- Wnew o Wsum.
+void CModelAdapt::ComposeMLLR(MLLRTransform &Wsum, const MLLRTransform &Wnew) {
+    const int D = Wsum.D, E = D + 1;
+    std::vector<double> out((size_t)D * E, 0.0);
+    for (int a = 0; a < D; a++) {
+        for (int b = 0; b < E; b++) {
+            double s = 0.0;
+            for (int c = 0; c < D; c++) s += Wnew.W[(size_t)a * E + c] * Wsum.W[(size_t)c * E + b];
+            out[(size_t)a * E + b] = s;
+        }
+        out[(size_t)a * E + D] += Wnew.W[(size_t)a * E + D];
+    }
+    Wsum.W.swap(out);
+}
+
+MLLRTransform CModelAdapt::IterW_MLLR(GMModel *UBM, int iters, bool wantQ) {
+    std::vector<double> Gi, ki;
+    if (opt_._mllrUBMInit) prepareKiGiInitMatrices(*UBM, opt_._min_var, Gi, ki);
+    MLLRTransform W = ComputeW_MLLR(UBM, Gi, ki, wantQ && iters <= 1);
+    if (iters > 1) {
+        std::unique_ptr<GMModel> m = AdaptModelMLLR(*UBM, W);
+        for (int i = 1; i < iters; i++) {
+            MLLRTransform Wt = ComputeW_MLLR(m.get(), Gi, ki, wantQ && i == iters - 1);
+            m = AdaptModelMLLR(*m, Wt);
+            ComposeMLLR(W, Wt);
+            if (i == iters - 1) W.Q = Wt.Q;
+        }
+    }
+    return W;
+}
+
+void CModelAdapt::AdaptModel(GMModel *UBM) {
+    if (!UBM) throw std::logic_error("AdaptModel(): UBM not given!");
+    if (!data_) throw std::logic_error("AdaptModel(): Insert adaptation data first!");
+    UBM_ = UBM;
+    nummix_ = UBM->GetNumberOfMixtures();
+    const int D = UBM->GetDimension();
+    if ((unsigned)D != D_) throw std::logic_error("AdaptModel(): data and UBM dimensions differ");
+    model_.reset();
+    W_.reset();
+    switch (opt_._adaptType) {
+    case MAP:
+        stats_.alloc(D, nummix_, opt_._AdaptMeanBool || opt_._AdaptVarBool, opt_._AdaptVarBool);
+        model_ = AdaptModelMAP(UBM);
+        break;
+    case MLLR: {
+        stats_.alloc(D, nummix_, true, false);
+        W_.reset(new MLLRTransform(IterW_MLLR(UBM, opt_._iterMLLR, opt_._getQvalMLLR)));
+        if (opt_._applyTransforms) model_ = AdaptModelMLLR(*UBM, *W_);
+        break;
+    }
+    case MLLR_MAP: {
+        stats_.alloc(D, nummix_, true, opt_._AdaptVarBool);
+        MLLRTransform W = IterW_MLLR(UBM, opt_._iterMLLR, false);
+        std::unique_ptr<GMModel> mm = AdaptModelMLLR(*UBM, W);
+        model_ = AdaptModelMAP(mm.get());
+        break;
+    }
+    default:
+        throw std::runtime_error("AdaptModel(): adaptation type not supported by this build");
+    }
+}
+
+// TrainFromScratch (OL_ModelAdapt.cpp:673-821) with splitGaussian
+// (:614-660, diagonal: shift +-0.5*sqrt(var)) and splitMultipleGaussians with
+// split-X = 0 (one split per round: heaviest component, ties to the higher
+// index, into the highest-index zero-weight slot, GetGaussStats :965-991).
+void CModelAdapt::TrainFromScratch(unsigned int nummix) {
+    if (!data_ || N_ <= 3) throw std::runtime_error("TrainFromScratch(): Not enough data to model training.");
+    const int D = (int)D_;
+    // Global statistics: a single component with a broad variance has
+    // posterior 1 for every finite frame.  The first pass finds the mean; the
+    // second is centred on it, so E[x^2] - mu^2 does not cancel.
+    GMModel g(D, 1);
+    std::vector<float> z(D, 0.0f), v(D, 1e4f);
+    g.SetMixParam(0, z.data(), v.data(), 1.0f);
+    GMMStats<double> gs(D, 1, true, true);
+    gs.reset();
+    AccumulateStats(&g, gs, false, true, false);
+    for (int d = 0; d < D; d++) z[d] = (float)(gs.getMean(0)[d] / gs.getProb(0));
+    g.SetMixParam(0, z.data(), v.data(), 1.0f);
+    gs.reset();
+    AccumulateStats(&g, gs, false, true, true);
+    GMModel init(D, 1);
+    init.SetMixWeight(0, 1.0f);
+    if (UpdateModelEM(&init, gs) == 0)
+        throw std::runtime_error("trainFromScratch(): amount of data too low to estimate a model");
+    model_.reset(new GMModel(D, (int)nummix));
+    model_->SetMixParam(0, init.GetMixtureMean(0), init.GetMixtureVar(0), init.GetMixtureWeight(0));
+    nummix_ = (int)nummix;
+    stats_.alloc(D, nummix, true, true);
+    const unsigned int maxIter = nummix < 2 ? 0 : nummix;
+    unsigned int nonzero = 1;
+    GMModel *model = model_.get();
+    for (unsigned int it = 0; it < maxIter; it++) {
+        for (int e = 0; e < opt_._iterCountEM; e++) {
+            stats_.reset();
+            AccumulateStats(model, stats_, false, true, true);
+            if (opt_._verbosity)
+                std::cout << "     [LL = " << stats_.getTotLogLike() / (double)stats_.getTotAccSamples()
+                          << "]" << std::endl;
+            if (nonzero > 1) nonzero = UpdateModelEM(model, stats_);
+            if (nonzero < 2) {
+                if (nonzero < 1) *model = init;
+                break;
+            }
+        }
+        if (nonzero == 0 || nonzero == nummix) break;
+        if (it < maxIter - 1) {
+            int imax = 0;
+            float wm = 0.0f;
+            for (int m = (int)nummix - 1; m >= 0; m--)
+                if (wm < model->GetMixtureWeight(m)) {
+                    wm = model->GetMixtureWeight(m);
+                    imax = m;
+                }
+            int znew = -1;
+            for (int m = (int)nummix - 1; m >= 0 && znew < 0; m--)
+                if (model->GetMixtureWeight(m) == 0) znew = m;
+            if (znew < 0) break;
+            std::memcpy(model->GetMixtureVar(znew), model->GetMixtureVar(imax), sizeof(float) * D);
+            const float half = model->GetMixtureWeight(imax) / 2.0f;
+            model->SetMixWeight(znew, half);
+            model->SetMixWeight(imax, half);
+            float *mi = model->GetMixtureMean(imax), *mn = model->GetMixtureMean(znew);
+            const float *vi = model->GetMixtureVar(imax);
+            for (int d = 0; d < D; d++) {
+                const float sh = (float)(std::sqrt(vi[d]) * 0.5);
+                mn[d] = mi[d] + sh;
+                mi[d] -= sh;
+            }
+            nonzero += 1;
+        }
+    }
+    model->RearrangeMixtures();
+}
+
+int CModelAdapt::Save(const char *filename, bool save_txt) {
+    if (W_ && (!model_ || opt_._saveTransforms)) {
+        std::string out(filename);
+        const size_t dot = out.rfind('.');
+        if (dot != std::string::npos) out.erase(dot);
+        out += opt_._transformFileExt;
+        std::ofstream f(out);
+        if (!f) throw std::runtime_error("Save(): Unable to open output file " + out);
+        f.precision(10);
+        f << " MLLR\n 1\t" << W_->D << " " << W_->D + 1 << "\n";
+        for (int a = 0; a < W_->D; a++) {
+            for (int b = 0; b <= W_->D; b++) f << " " << W_->W[(size_t)a * (W_->D + 1) + b];
+            f << "\n";
+        }
+    }
+    if (model_) {
+        if (model_->Save(filename, save_txt) != 0) throw std::runtime_error(model_->LastError());
+        return 0;
+    }
+    return W_ ? 0 : 1;
+}
+
+}  // namespace gmm

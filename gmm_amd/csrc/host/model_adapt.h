@@ -1,0 +1,114 @@
+// Trainer / adaptation driver over the HIP estimator: the subset of
+// CModelAdapt (trainer/OL_ModelAdapt.h:140-180 of GMMTeam/GMM) on the
+// diagonal-GMM hot path -- EM from a UBM or from scratch (Gaussian
+// splitting), MAP adaptation, global MLLR (+ MLLR then MAP) -- with the
+// reference's update rules.  All statistics come from GMMStatsEstimator_HIP.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gmm_estimator.h"
+#include "gmm_model.h"
+#include "gmm_stats.h"
+
+namespace gmm {
+
+enum AdaptType { FROM_SCRATCH = 0, MLLR = 1, MAP = 2, fMLLR = 3, MLLR_MAP = 4 };
+
+// MLLR transform W = [A | b], D x (D+1), row-major FP64.
+struct MLLRTransform {
+    int D = 0;
+    std::vector<double> W;
+    std::vector<double> Q;  // per-row auxiliary-function values (optional)
+};
+
+class CModelAdapt {
+public:
+    // CModelAdapt::OPTIONS (OL_ModelAdapt.h:66-138), hot-path subset.
+    struct OPTIONS {
+        int _adaptType = MAP;
+        int _iterCountEM = 0;
+        int _iterCountMAP = 0;
+        bool _AdaptMeanBool = false;
+        bool _AdaptVarBool = false;
+        bool _AdaptWeightBool = false;
+        int _tau = 0;
+        float _min_var = 1e-6f;
+        bool _mllrUBMInit = false;
+        int _iterMLLR = 1;
+        bool _applyTransforms = false;
+        bool _saveTransforms = false;
+        bool _getQvalMLLR = false;
+        bool _saveMixOcc = false;
+        int _verbosity = 0;
+        int _cudaGPUid = -1;
+        unsigned int _numThreads = 1;
+        int _precision = GMMHIP_PREC_FP32;
+        std::string _transformFileExt = ".mat";
+    };
+
+    CModelAdapt() = default;
+    void SetOptions(const OPTIONS &opt);
+    const OPTIONS &GetOptions() const { return opt_; }
+
+    // Frames are borrowed (row pointers), as in InsertData.
+    void InsertData(float **vectors, unsigned int N, unsigned int D);
+
+    // MAP / MLLR / MLLR_MAP adaptation of UBM (AdaptModel, :1163-1247).
+    void AdaptModel(GMModel *UBM);
+    // EM from scratch with binary splitting up to nummix components
+    // (TrainFromScratch, :673-821, diagonal branch, no log-lik threshold).
+    void TrainFromScratch(unsigned int nummix);
+
+    GMModel *GetClientModel() { return model_.get(); }
+    const MLLRTransform *GetTransform() const { return W_.get(); }
+    // Saves the adapted model, or the transform when no model was produced
+    // or --save-trn was given (Save/SaveTRNMatrices, :1900-1989).
+    int Save(const char *filename, bool save_txt);
+
+    // Exposed update rules (reference names), operating on _stats.
+    unsigned int UpdateModelEM(GMModel *model, GMMStats<double> &stats, bool upweight = true,
+                               bool upmean = true, bool upvar = true);
+    void UpdateModelMAP(GMModel *model, GMModel *UBM);
+    void AccumulateStats(GMModel *model, GMMStats<double> &stats, bool mixProbsOnly = false,
+                         bool meanStats = true, bool varStats = false);
+    GMMStats<double> &Stats() { return stats_; }
+    void SetNumMix(int M) { nummix_ = M; }
+
+    // MLLR pieces (OL_ModelAdapt.cpp:1417-1612).
+    void CheckInvertibility() const;
+    static void prepareKiGiInitMatrices(const GMModel &UBM, float min_var, std::vector<double> &G,
+                                        std::vector<double> &k);
+    void ComputeKiGi(const GMModel &model, int row, const double *Ginit, const double *kinit,
+                     double *G, double *k) const;
+    MLLRTransform ComputeW_MLLR(GMModel *model, const std::vector<double> &Ginit,
+                                const std::vector<double> &kinit, bool wantQ);
+    static std::unique_ptr<GMModel> AdaptModelMLLR(const GMModel &model, const MLLRTransform &W);
+    static void ComposeMLLR(MLLRTransform &Wsum, const MLLRTransform &Wnew);
+
+private:
+    OPTIONS opt_;
+    float **data_ = nullptr;
+    unsigned int N_ = 0, D_ = 0;
+    bool dataChanged_ = true;
+    std::unique_ptr<GMMStatsEstimator_HIP<double>> est_;
+    GMMStats<double> stats_;
+    std::unique_ptr<GMModel> model_;
+    std::unique_ptr<MLLRTransform> W_;
+    GMModel *UBM_ = nullptr;
+    int nummix_ = 0;
+
+    void initEstimator();
+    std::unique_ptr<GMModel> AdaptModelMAP(GMModel *ubm);
+    void IterEM(GMModel *model, int iters);
+    void IterMAP(GMModel *model, int iters);
+    MLLRTransform IterW_MLLR(GMModel *UBM, int iters, bool wantQ);
+};
+
+// FP64 Cholesky solve of the symmetric positive definite system G w = k
+// (n x n, row-major).  Throws std::runtime_error if G is not SPD.
+void spd_solve(int n, const double *G, const double *k, double *w);
+
+}  // namespace gmm

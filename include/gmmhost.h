@@ -1,0 +1,64 @@
+/*
+ * gmmhost.h -- C ABI of the host layer around the gfx950 accumulator: the
+ * `.gmm` model file, the RearrangeMixtures component order and the EM / MAP /
+ * MLLR update rules that turn statistics into models.  No device work; these
+ * entry points run on any host.  Reference paths are relative to
+ * GMM/lib_bmssr/ of GMMTeam/GMM.
+ */
+#ifndef GMMHOST_H
+#define GMMHOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gmmhost_model *gmmhost_model_t;
+
+/* GMModel::Load (model/OL_GMModel.cpp:1048, ReadTXT :627-753, ReadBIN
+ * :757-853).  Returns 0 or 1 (message: gmmhost_last_error()). */
+int gmmhost_model_load(const char *path, int txt, gmmhost_model_t *out);
+/* GMModel::Save (WriteTXT :475-547, WriteBIN :551-624); zero-weight
+ * components are skipped. */
+int gmmhost_model_save(gmmhost_model_t m, const char *path, int txt);
+/* A model from arrays w[M], mu[M*D], var[M*D] (GMModel(dim, nummix) +
+ * SetMixParam, :120-156). */
+int gmmhost_model_create(int M, int D, const float *w, const float *mu, const float *var,
+                         gmmhost_model_t *out);
+int gmmhost_model_dims(gmmhost_model_t m, int *M, int *D);
+int gmmhost_model_get(gmmhost_model_t m, float *w, float *mu, float *var);
+/* RearrangeMixtures (:857-885): zero-weight components to the end,
+ * reversed; *nzero receives their count. */
+int gmmhost_model_rearrange(gmmhost_model_t m, int *nzero);
+void gmmhost_model_free(gmmhost_model_t m);
+
+/* UpdateModelEM (trainer/OL_ModelAdapt.cpp:823-949) on model m with
+ * statistics n[M], F[M*D], S[M*D] over tot_samples frames; returns the
+ * reference's count of components with non-zero weight on entry through
+ * *left. */
+int gmmhost_update_em(gmmhost_model_t m, const double *n, const double *F, const double *S,
+                      uint64_t tot_samples, float min_var, int upweight, int upmean, int upvar,
+                      int *left);
+/* UpdateModelMAP (OL_ModelAdapt.cpp:993-1106): m starts as (a copy of) the
+ * UBM ubm and is adapted with relevance factor tau. */
+int gmmhost_update_map(gmmhost_model_t m, gmmhost_model_t ubm, const double *n, const double *F,
+                       const double *S, uint64_t tot_samples, int tau, float min_var,
+                       int upweight, int upmean, int upvar);
+/* Global MLLR transform W[D*(D+1)] (row-major [A|b]) of model m from
+ * statistics n[M], F[M*D] (ComputeW_MLLR + ComputeKiGi_MLLR, :1417-1581),
+ * optionally with the UBM prior of prepareKiGiInitMatrices (:1484-1523,
+ * ubm may be NULL).  Returns 7 (GMMHIP_ERR_MLLR) when CheckInvertibility
+ * (:1851-1878) rejects the statistics. */
+int gmmhost_mllr_transform(gmmhost_model_t m, gmmhost_model_t ubm, const double *n,
+                           const double *F, float min_var, double *W);
+/* AdaptModelMLLR (:1583-1612): means of m replaced by A mu + b. */
+int gmmhost_mllr_apply(gmmhost_model_t m, const double *W);
+
+const char *gmmhost_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GMMHOST_H */

@@ -1,0 +1,122 @@
+"""The drop-in tools on the GPU: trainGMM / getLogLike (C++ over the C ABI)
+reproduce the reference's example outputs and the oracle's EM / MAP / MLLR
+pipelines."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import _formats as fm
+import _oracle as orc
+import gmm_amd
+from gmm_amd import host, synth
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+BIN = os.path.join(os.path.dirname(HERE), "gmm_amd", "_build")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if gmm_amd.device_count() < 1:
+        pytest.skip("no GPU")
+
+
+def run(tool, *args):
+    r = subprocess.run([os.path.join(BIN, tool)] + [str(a) for a in args], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_getloglike_for_each_reproduces_example(tmp_path, golden):
+    """getLogLike.bat:9 -- per-frame log-likelihoods of data.bin under
+    myModel.gmm, TXT saveMat output."""
+    run("getLogLike", "-i", os.path.join(GOLD, "data.bin"), "--data-T", 2, "--in-GMM",
+        os.path.join(GOLD, "myModel.gmm"), "-L", "--for-each", "-o", tmp_path, "-T")
+    out = fm.read_savemat_txt(tmp_path / "data.logL")
+    assert out.shape == (1, 10000)
+    ref = golden["ll"]
+    tol = 0.5 * 10.0 ** (np.floor(np.log10(np.abs(ref))) - 5) + 2e-6 * np.abs(ref)
+    assert np.all(np.abs(out[0] - ref) <= tol)
+
+
+def test_getloglike_mean(tmp_path, golden):
+    outp = run("getLogLike", "-i", os.path.join(GOLD, "data.bin"), "--data-T", 2, "--in-GMM",
+               os.path.join(GOLD, "myModel.gmm"), "-L")
+    name, val = outp.strip().split("\t")
+    assert name.endswith("data.bin")
+    assert float(val) == pytest.approx(golden["ll"].mean(), rel=1e-5)
+
+
+def test_train_from_scratch_reproduces_example(tmp_path, golden):
+    """trainModel.bat:4 -- --type 0 --nummix 4 --it-EM 8 on data.bin gives
+    myModel.gmm (reference SSE path, FP32 per-frame arithmetic)."""
+    out = tmp_path / "myModel.gmm"
+    run("trainGMM", "-i", os.path.join(GOLD, "data.bin"), "--data-T", 2, "--type", 0,
+        "--nummix", 4, "--it-EM", 8, "-o", out, "-T")
+    w, mu, var, _ = fm.read_gmm_txt(out)
+    assert np.abs(w - golden["w"]).max() / golden["w"].max() < 5e-5
+    assert np.abs(mu - golden["mu"]).max() < 5e-5
+    assert (np.abs(var - golden["var"]) / golden["var"]).max() < 5e-5
+
+
+def _ubm_case(tmp_path, seed=41, M=64, D=12, N=20000):
+    rng = np.random.default_rng(seed)
+    w = rng.dirichlet(np.ones(M))
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    var = rng.uniform(0.5, 2, (M, D)).astype(np.float32)
+    # adaptation data from a shifted mixture
+    x = synth.sample_frames_np(w, mu + rng.normal(0, 0.3, (M, D)).astype(np.float32), var, N,
+                               seed + 1)
+    ubm_p = tmp_path / "ubm.gmm"
+    host.Model.create(w.astype(np.float32), mu, var).save(ubm_p, True)
+    # the file keeps 7 digits: read it back for the oracle
+    uw, umu, uvar, _ = fm.read_gmm_txt(ubm_p)
+    data_p = tmp_path / "data.raw"
+    fm.write_raw(data_p, x)
+    return x, uw, umu, uvar, ubm_p, data_p
+
+
+def test_train_em_from_ubm(tmp_path):
+    x, w, mu, var, ubm_p, data_p = _ubm_case(tmp_path)
+    out = tmp_path / "em.gmm"
+    run("trainGMM", "-i", data_p, "--data-T", 2, "--in-UBM", ubm_p, "-L", "--type", 2,
+        "--it-EM", 3, "--tau", 0, "--mean", "--var", "--weight", "-o", out, "-T")
+    for _ in range(3):
+        n, F, S, _ = orc.accumulate(x, w, mu, var)
+        w, mu, var, _ = orc.update_em(w, mu, var, n, F, S, x.shape[0])
+    gw, gmu, gvar, _ = fm.read_gmm_txt(out)
+    assert np.allclose(gw, w, rtol=1e-4, atol=1e-7)
+    assert np.allclose(gmu, mu, rtol=1e-4, atol=1e-4)
+    assert np.allclose(gvar, var, rtol=1e-4, atol=1e-6)
+
+
+def test_train_map_from_ubm(tmp_path):
+    x, w, mu, var, ubm_p, data_p = _ubm_case(tmp_path, seed=43)
+    out = tmp_path / "map.gmm"
+    run("trainGMM", "-i", data_p, "--data-T", 2, "--in-UBM", ubm_p, "-L", "--type", 2,
+        "--it-MAP", 1, "--tau", 16, "--mean", "-o", out, "-T")
+    n, F, S, _ = orc.accumulate(x, w, mu, var)
+    rw, rmu, rvar = orc.update_map(w, mu, var, n, F, S, x.shape[0], 16)
+    gw, gmu, gvar, _ = fm.read_gmm_txt(out)
+    assert np.allclose(gmu, rmu, rtol=1e-5, atol=1e-5)
+    assert np.array_equal(gvar, var) and np.array_equal(gw, w)
+
+
+def test_train_mllr_from_ubm(tmp_path):
+    x, w, mu, var, ubm_p, data_p = _ubm_case(tmp_path, seed=45, M=64, D=8, N=30000)
+    out = tmp_path / "mllr.gmm"
+    run("trainGMM", "-i", data_p, "--data-T", 2, "--in-UBM", ubm_p, "-L", "--type", 1,
+        "--apply-trn", "--save-trn", "-o", out, "-T")
+    n, F, S, _ = orc.accumulate(x, w, mu, var)
+    W = orc.mllr_transform(mu, var, n, F)
+    lines = (tmp_path / "mllr.mat").read_text().split("\n")
+    assert lines[0].strip() == "MLLR"
+    Wg = np.array([[float(v) for v in ln.split()] for ln in lines[2:2 + 8]])
+    assert np.allclose(Wg, W, rtol=1e-4, atol=1e-4)
+    gw, gmu, gvar, _ = fm.read_gmm_txt(out)
+    assert np.allclose(gmu, orc.mllr_apply(W, mu), rtol=1e-4, atol=1e-4)

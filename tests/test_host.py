@@ -1,0 +1,144 @@
+"""CPU tests of the host layer (include/gmmhost.h): the `.gmm` reader and
+writers against an independent parser and the reference's own model file,
+RearrangeMixtures, and the EM / MAP / MLLR update rules against the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import _formats as fm
+import _oracle as orc
+from gmm_amd import host
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def test_read_reference_model_txt(golden):
+    m = host.Model.load(os.path.join(GOLD, "myModel.gmm"), True)
+    w, mu, var = m.get()
+    assert np.array_equal(w, golden["w"])
+    assert np.array_equal(mu, golden["mu"])
+    assert np.array_equal(var, golden["var"])
+
+
+@pytest.mark.parametrize("txt", [True, False])
+def test_round_trip(tmp_path, golden, txt):
+    m = host.Model.load(os.path.join(GOLD, "myModel.gmm"), True)
+    p = tmp_path / ("m.gmm" if txt else "m.bin")
+    m.save(p, txt)
+    r = host.Model.load(p, txt)
+    a, b = m.get(), r.get()
+    for u, v in zip(a, b):
+        if txt:  # %15e keeps 7 significant digits
+            assert np.allclose(u, v, rtol=6e-7, atol=0)
+        else:
+            assert np.array_equal(u, v)
+    if txt:
+        w2, mu2, var2, _ = fm.read_gmm_txt(p)
+        assert np.allclose(w2, golden["w"], rtol=6e-7)
+
+
+def test_bin_layout(tmp_path):
+    w = np.array([0.5, 0.0, 0.5], np.float32)
+    mu = np.arange(6, dtype=np.float32).reshape(3, 2)
+    var = np.ones((3, 2), np.float32) * 2
+    p = tmp_path / "m.bin"
+    host.Model.create(w, mu, var).save(p, False)
+    raw = p.read_bytes()
+    assert raw[:9] == b"GMM-DB-VN"
+    assert np.frombuffer(raw[9:11], "<i2")[0] == 2
+    assert np.frombuffer(raw[19:21], "<i2")[0] == 2  # zero-weight component skipped
+    body = np.frombuffer(raw[21:], "<f4")
+    assert np.array_equal(body[:2], [0.5, 0.5])
+    assert np.array_equal(body[2:6], [0, 1, 2, 2])
+    assert np.array_equal(body[6:10], [4, 5, 2, 2])
+
+
+def test_rearrange_matches_oracle():
+    w = np.array([0.2, 0, 0.3, 0, 0, 0.5], np.float32)
+    mu = np.arange(12, dtype=np.float32).reshape(6, 2)
+    m = host.Model.create(w, mu, np.ones((6, 2), np.float32))
+    assert m.rearrange() == 3
+    perm, _ = orc.rearrange_perm(w)
+    w2, mu2, _ = m.get()
+    assert np.array_equal(w2, w[perm])
+    assert np.array_equal(mu2, mu[perm])
+
+
+def _stats(seed, M, D):
+    rng = np.random.default_rng(seed)
+    n = rng.uniform(0, 50, M)
+    n[1] = 1.5
+    F = rng.normal(0, 2, (M, D)) * n[:, None]
+    S = (F / n[:, None]) ** 2 * n[:, None] + rng.uniform(0.5, 2, (M, D)) * n[:, None]
+    return n, F, S
+
+
+def test_update_em_matches_oracle():
+    rng = np.random.default_rng(1)
+    M, D = 9, 5
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    w[4] = 0.0
+    mu = rng.normal(0, 1, (M, D)).astype(np.float32)
+    var = rng.uniform(0.5, 2, (M, D)).astype(np.float32)
+    n, F, S = _stats(2, M, D)
+    n[4] = 0
+    m = host.Model.create(w, mu, var)
+    left = m.update_em(n, F, S, 300)
+    rw, rmu, rvar, rleft = orc.update_em(w, mu, var, n, F, S, 300)
+    hw, hmu, hvar = m.get()
+    assert left == rleft
+    assert np.array_equal(hw, rw) and np.array_equal(hmu, rmu) and np.array_equal(hvar, rvar)
+
+
+@pytest.mark.parametrize("flags", [(0, 1, 0), (1, 1, 1), (1, 0, 0)])
+def test_update_map_matches_oracle(flags):
+    rng = np.random.default_rng(3)
+    M, D = 7, 4
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, 1, (M, D)).astype(np.float32)
+    var = rng.uniform(0.5, 2, (M, D)).astype(np.float32)
+    n, F, S = _stats(4, M, D)
+    n[2] = 0.0
+    ubm = host.Model.create(w, mu, var)
+    m = host.Model.create(w, mu, var)
+    m.update_map(ubm, n, F, S, 250, 16, upweight=flags[0], upmean=flags[1], upvar=flags[2])
+    rw, rmu, rvar = orc.update_map(w, mu, var, n, F, S, 250, 16, upweight=flags[0],
+                                   upmean=flags[1], upvar=flags[2])
+    hw, hmu, hvar = m.get()
+    assert np.array_equal(hw, rw) and np.array_equal(hmu, rmu) and np.array_equal(hvar, rvar)
+
+
+@pytest.mark.parametrize("prior", [False, True])
+def test_mllr_transform_matches_oracle(prior):
+    rng = np.random.default_rng(5)
+    M, D = 40, 6
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    var = rng.uniform(0.5, 2, (M, D)).astype(np.float32)
+    A = np.eye(D) + rng.normal(0, 0.1, (D, D))
+    b = rng.normal(0, 0.5, D)
+    n = rng.uniform(2, 20, M)
+    F = n[:, None] * (mu.astype(np.float64) @ A.T + b) + rng.normal(0, 0.1, (M, D))
+    m = host.Model.create(w, mu, var)
+    ubm = m if prior else None
+    W = m.mllr_transform(n, F, ubm=ubm)
+    R = orc.mllr_transform(mu, var, n, F, ubm_mu=mu if prior else None,
+                           ubm_var=var if prior else None)
+    assert np.allclose(W, R, rtol=1e-9, atol=1e-9)
+    m.mllr_apply(W)
+    assert np.array_equal(m.get()[1], orc.mllr_apply(W, mu))
+
+
+def test_mllr_underdetermined_rejected():
+    M, D = 5, 6
+    m = host.Model.create(np.full(M, 0.2, np.float32), np.zeros((M, D), np.float32),
+                          np.ones((M, D), np.float32))
+    with pytest.raises(host.HostError):
+        m.mllr_transform(np.ones(M), np.zeros((M, D)))
+
+
+def test_missing_file_reports_error(tmp_path):
+    with pytest.raises(host.HostError):
+        host.Model.load(tmp_path / "nope.gmm", True)

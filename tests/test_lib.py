@@ -18,7 +18,7 @@ def declared_symbols():
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         src = open(h).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-        syms.update(re.findall(r"\b(gmmhip_[a-z0-9_]+)\s*\(", src))
+        syms.update(re.findall(r"\b(gmm(?:hip|host)_[a-z0-9_]+)\s*\(", src))
     return sorted(syms)
 
 
