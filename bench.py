@@ -53,6 +53,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ubm_plain")
     ap.add_argument("--frames", type=int, default=0, help="override N")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16x3", "bf16x3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -75,7 +76,9 @@ def main():
     w, mu, var = synth.init_model_from_frames(x, M, cfg["seed"])
     torch.cuda.synchronize()
 
-    acc = gmm_amd.Accumulator(0, profile=1)
+    prec = {"fp32": gmm_amd.PREC_FP32, "fp16x3": gmm_amd.PREC_FP16X3,
+            "bf16x3": gmm_amd.PREC_BF16X3}[args.precision]
+    acc = gmm_amd.Accumulator(0, profile=1, precision=prec)
     acc.set_model(w, mu, var)
     acc.set_frames_device(x.data_ptr(), N, D)
     for _ in range(args.warmup):
@@ -98,17 +101,18 @@ def main():
     rd_ms, rd_n = kt["reduce"]
     st_avg = st_ms / max(st_n, 1) / 1e3
     achieved = flop_fc * N * M / st_avg
-    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": None,
+    peak = PEAK["fp32"] if args.precision == "fp32" else PEAK["fp16"]
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                 "kernel": "stats_kernel",
                 "avg_launch_ms": st_avg * 1e3,
                 "pipeline_ms": {"loglik": ll_ms / max(ll_n, 1), "stats": st_ms / max(st_n, 1),
                                 "reduce": rd_ms / max(rd_n, 1)},
                 "pipeline_frac": flop_fc * N * M / ((ll_ms + st_ms + rd_ms) / max(st_n, 1) / 1e3)
-                / PEAK["fp32"]}
+                / peak}
 
     line = {
-        "metric": "E-step frame*comp/s (FP32 plain form)",
+        "metric": "E-step frame*comp/s (%s)" % args.precision,
         "value": value,
         "unit": "frame*comp/s",
         "n_gpus": 1,
@@ -118,10 +122,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": {"fp32": "fp32", "fp16x3": "fp16x3 (split fp16, fp32 accumulate)",
+                  "bf16x3": "bf16x3 (split bf16, fp32 accumulate)"}[args.precision],
         "data": "synthetic (seeded true mixture, frames generated in HBM)",
         "config": {"workload": args.workload, "D": D, "M": M, "N": N,
-                   "precision": "fp32", "tot_loglike_per_frame": res[3] / N},
+                   "precision": args.precision, "tot_loglike_per_frame": res[3] / N},
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

@@ -59,5 +59,9 @@ hipError_t launch_loglik_fp32(const EstepArgs &a, hipStream_t s);
 hipError_t launch_stats_fp32(const EstepArgs &a, hipStream_t s);
 hipError_t launch_reduce_stats(const EstepArgs &a, hipStream_t s);
 hipError_t launch_sum_blockll(const EstepArgs &a, hipStream_t s);
+// out[e] = sum_c part[c*E + e] in FP64, c = 0..nchunks-1 in order
+hipError_t launch_reduce_partials(const float *part, int nchunks, size_t E, double *out,
+                                  hipStream_t s);
+hipError_t launch_sum_doubles(const double *v, uint64_t n, double *out, hipStream_t s);
 
 }  // namespace gmmhip

@@ -434,6 +434,18 @@ hipError_t launch_reduce_stats(const EstepArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_reduce_partials(const float *part, int nchunks, size_t E, double *out,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, s, part,
+                       nchunks, E, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_doubles(const double *v, uint64_t n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(sum_blockll_kernel, dim3(1), dim3(256), 0, s, v, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_sum_blockll(const EstepArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(sum_blockll_kernel, dim3(1), dim3(256), 0, s, a.blockLL,
                        ll_blocks(a.N), a.tot);

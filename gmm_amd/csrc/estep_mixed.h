@@ -1,0 +1,47 @@
+// Internal interface of the mixed-precision E-step (estep_mixed.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gmmhip {
+
+// Contraction depth of the mixed form: [1, z(D), z^2(D)] padded to 32.
+inline int kp_mixed_for_dim(int D) { return (2 * D + 1 + 31) / 32 * 32; }
+
+struct PrepArgs {
+    const float *X;        // [N][D] contiguous
+    uint64_t N, Npad;      // Npad = ceil32(N)
+    int D, KP;
+    const float *off;      // [D] centring
+    const float *scale;    // [D] 1/std
+    uint16_t *za_hi_raw, *za_lo_raw;  // [Npad][KP]
+    uint16_t *zb_hi_raw, *zb_lo_raw;  // [Npad/32][32*KP]
+    uint8_t *bad;          // [N] 1: non-finite frame, 2: out of the 16-bit range
+    unsigned int *nrange;  // count of (frame, dim) values flagged 2
+    float zlimit;          // |z| above this is out of range (fp16: 240)
+};
+
+struct MixedArgs {
+    uint64_t N, Npad;
+    int D, KP;
+    const void *za_hi, *za_lo, *zb_hi, *zb_lo;
+    const uint8_t *bad;
+    const void *P16;       // split model [Mt32][KP/16][2][64] x 16 B
+    int M, Mt32;
+    float min_ll, min_gamma;
+    float *LL2, *LLnat;
+    double *blockLL;
+    uint64_t chunk_frames;
+    int nchunks, ngroups;
+    float *partials;       // [nchunks][Mt32*32][KP]
+};
+
+hipError_t launch_colstats(const float *X, uint64_t N, int D, int blocks, uint64_t per_block,
+                           double *part, hipStream_t s);
+// prec: GMMHIP_PREC_BF16X3 (1) or GMMHIP_PREC_FP16X3 (2)
+hipError_t launch_prep_frames(const PrepArgs &p, int prec, hipStream_t s);
+hipError_t launch_loglik_mixed(const MixedArgs &a, int prec, hipStream_t s);
+hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s);
+
+}  // namespace gmmhip

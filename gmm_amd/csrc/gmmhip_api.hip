@@ -8,6 +8,7 @@
 // trainer/OL_GMMStatsEstimator_CUDA.cpp:292-328) has no counterpart here.
 #include "../../include/gmmhip.h"
 #include "estep.h"
+#include "estep_mixed.h"
 
 #include <hip/hip_runtime.h>
 
@@ -56,6 +57,32 @@ struct gmmhip_ctx {
     double *d_tot = nullptr;
     double *h_res = nullptr;  // pinned: [Mp*KP] + 1
     size_t h_res_cap = 0;
+
+    // host copy of the active model (for the mixed-precision packing)
+    std::vector<float> m_w, m_mu, m_var;
+    std::vector<double> m_gconst;
+    uint64_t model_version = 0, frames_version = 0;
+
+    // mixed-precision frame features (ZA/ZB hi/lo) and split model
+    int z_prec = 0, KPm = 0;
+    uint64_t z_version = 0, z_N = 0, z_Npad = 0;
+    bool z_range_fallback = false;
+    uint16_t *d_za_hi = nullptr, *d_za_lo = nullptr, *d_zb_hi = nullptr, *d_zb_lo = nullptr;
+    size_t z_cap = 0;
+    uint8_t *d_bad = nullptr;
+    size_t bad_cap = 0;
+    unsigned int *d_nrange = nullptr;
+    double *d_colpart = nullptr;
+    size_t colpart_cap = 0;
+    float *d_zoff = nullptr, *d_zscale = nullptr;
+    size_t zos_cap = 0, zscale_cap = 0;
+    std::vector<float> h_zoff, h_zscale;
+    void *d_P16 = nullptr;
+    size_t P16_cap = 0;
+    uint64_t p16_model = 0, p16_z = 0;
+    int p16_prec = 0;
+    bool p16_overflow = false;
+    bool last_used_mixed = false;
 
     // kernel timers (opts.profile)
     hipEvent_t ev[5] = {};
@@ -172,6 +199,171 @@ void add_time(gmmhip_ctx *h, int cls, int e0, int e1) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision form: per-frame-set features and the split model.
+
+// Frames per mixed-precision feature row block (the log-likelihood kernel
+// works on 128 frames per block).
+inline uint64_t pad128(uint64_t n) { return (n + 127) / 128 * 128; }
+
+template <typename T>
+void split_store(double v, uint16_t *hi, uint16_t *lo, bool *overflow) {
+    const T h = (T)(float)v;
+    const T l = (T)(float)(v - (double)(float)h);
+    if (!std::isfinite((float)h)) *overflow = true;
+    *hi = __builtin_bit_cast(uint16_t, h);
+    *lo = __builtin_bit_cast(uint16_t, l);
+}
+
+// Prepares the features and the split model of the current frames and model
+// for opts.precision.  *use_mixed stays false (the call then runs the FP32
+// form, on the GPU) when the precision is FP32, when a frame value lies
+// outside the FP16 range after centring and scaling, or when a model
+// coefficient overflows FP16.
+int prepare_mixed(gmmhip_ctx *h, bool *use_mixed) {
+    *use_mixed = false;
+    const int prec = h->opt.precision;
+    if (prec != GMMHIP_PREC_BF16X3 && prec != GMMHIP_PREC_FP16X3) return GMMHIP_OK;
+    const int D = h->D, KPm = kp_mixed_for_dim(D);
+    const uint64_t N = h->N;
+    if (h->z_prec != prec || h->z_version != h->frames_version || h->KPm != KPm || h->z_N != N) {
+        const uint64_t Npad = pad128(N);
+        // per-dimension mean and standard deviation of the frames (FP64)
+        const int blocks = (int)std::min<uint64_t>(1024, (N + 255) / 256);
+        const uint64_t per_block = (N + blocks - 1) / blocks;
+        HIPCHK(h, ensure(&h->d_colpart, &h->colpart_cap, (size_t)blocks * 2 * D));
+        HIPCHK(h, launch_colstats(h->d_x, N, D, blocks, per_block, h->d_colpart, h->stream));
+        std::vector<double> part((size_t)blocks * 2 * D);
+        HIPCHK(h, hipMemcpyAsync(part.data(), h->d_colpart, part.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        h->h_zoff.assign(D, 0.0f);
+        h->h_zscale.assign(D, 1.0f);
+        for (int d = 0; d < D; d++) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int b = 0; b < blocks; b++) {
+                s1 += part[(size_t)b * 2 * D + d];
+                s2 += part[(size_t)b * 2 * D + D + d];
+            }
+            const double mean = s1 / (double)N, var = s2 / (double)N - mean * mean;
+            if (std::isfinite(mean)) h->h_zoff[d] = (float)mean;
+            if (std::isfinite(var) && var > 0.0) h->h_zscale[d] = (float)(1.0 / std::sqrt(var));
+        }
+        HIPCHK(h, ensure(&h->d_zoff, &h->zos_cap, (size_t)D));
+        HIPCHK(h, ensure(&h->d_zscale, &h->zscale_cap, (size_t)D));
+        HIPCHK(h, hipMemcpyAsync(h->d_zoff, h->h_zoff.data(), D * sizeof(float),
+                                 hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->d_zscale, h->h_zscale.data(), D * sizeof(float),
+                                 hipMemcpyHostToDevice, h->stream));
+        const size_t zn = (size_t)Npad * KPm;
+        if (h->z_cap < zn) {
+            hipFree(h->d_za_hi);
+            hipFree(h->d_za_lo);
+            hipFree(h->d_zb_hi);
+            hipFree(h->d_zb_lo);
+            h->d_za_hi = h->d_za_lo = h->d_zb_hi = h->d_zb_lo = nullptr;
+            h->z_cap = 0;
+            HIPCHK(h, hipMalloc((void **)&h->d_za_hi, zn * 2));
+            HIPCHK(h, hipMalloc((void **)&h->d_za_lo, zn * 2));
+            HIPCHK(h, hipMalloc((void **)&h->d_zb_hi, zn * 2));
+            HIPCHK(h, hipMalloc((void **)&h->d_zb_lo, zn * 2));
+            h->z_cap = zn;
+        }
+        HIPCHK(h, ensure(&h->d_bad, &h->bad_cap, (size_t)N));
+        if (!h->d_nrange) HIPCHK(h, hipMalloc((void **)&h->d_nrange, sizeof(unsigned int)));
+        HIPCHK(h, hipMemsetAsync(h->d_bad, 0, N, h->stream));
+        HIPCHK(h, hipMemsetAsync(h->d_nrange, 0, sizeof(unsigned int), h->stream));
+        PrepArgs p{};
+        p.X = h->d_x;
+        p.N = N;
+        p.Npad = Npad;
+        p.D = D;
+        p.KP = KPm;
+        p.off = h->d_zoff;
+        p.scale = h->d_zscale;
+        p.za_hi_raw = h->d_za_hi;
+        p.za_lo_raw = h->d_za_lo;
+        p.zb_hi_raw = h->d_zb_hi;
+        p.zb_lo_raw = h->d_zb_lo;
+        p.bad = h->d_bad;
+        p.nrange = h->d_nrange;
+        p.zlimit = (prec == GMMHIP_PREC_FP16X3) ? 240.0f : INFINITY;
+        HIPCHK(h, launch_prep_frames(p, prec, h->stream));
+        unsigned int nrange = 0;
+        HIPCHK(h, hipMemcpyAsync(&nrange, h->d_nrange, sizeof(unsigned int), hipMemcpyDeviceToHost,
+                                 h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        h->z_range_fallback = nrange > 0;
+        h->z_prec = prec;
+        h->z_version = h->frames_version;
+        h->KPm = KPm;
+        h->z_N = N;
+        h->z_Npad = Npad;
+    }
+    if (h->z_range_fallback) return GMMHIP_OK;
+    if (h->p16_model != h->model_version || h->p16_z != h->z_version || h->p16_prec != prec) {
+        const int M = h->M, Mt32 = (M + 31) / 32, KS = KPm / 16;
+        std::vector<uint16_t> P((size_t)Mt32 * KS * 2 * 64 * 8, 0);
+        std::vector<double> coef(KPm);
+        const double log2e = 1.4426950408889634;
+        bool overflow = false;
+        for (int m = 0; m < M; m++) {
+            std::fill(coef.begin(), coef.end(), 0.0);
+            double cq = 0.0;
+            for (int d = 0; d < D; d++) {
+                const double sd = h->h_zscale[d];
+                const double nu = sd * ((double)h->m_mu[(size_t)m * D + d] - (double)h->h_zoff[d]);
+                const double om = 1.0 / ((double)h->m_var[(size_t)m * D + d] * sd * sd);
+                coef[1 + d] = om * nu;
+                coef[1 + D + d] = -0.5 * om;
+                cq += om * nu * nu;
+            }
+            coef[0] = h->m_gconst[m] - 0.5 * cq;
+            const int t = m / 32, c = m % 32;
+            for (int k = 0; k < KPm; k++) {
+                const int ks = k / 16, hh = (k % 16) / 8, j = k % 8;
+                const size_t ih = ((((size_t)t * KS + ks) * 2 + 0) * 64 + 32 * hh + c) * 8 + j;
+                const size_t il = ((((size_t)t * KS + ks) * 2 + 1) * 64 + 32 * hh + c) * 8 + j;
+                if (prec == GMMHIP_PREC_FP16X3)
+                    split_store<_Float16>(coef[k] * log2e, &P[ih], &P[il], &overflow);
+                else
+                    split_store<__bf16>(coef[k] * log2e, &P[ih], &P[il], &overflow);
+            }
+        }
+        uint16_t *p16 = static_cast<uint16_t *>(h->d_P16);
+        HIPCHK(h, ensure(&p16, &h->P16_cap, P.size()));
+        h->d_P16 = p16;
+        HIPCHK(h, hipMemcpyAsync(h->d_P16, P.data(), P.size() * sizeof(uint16_t),
+                                 hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        h->p16_model = h->model_version;
+        h->p16_z = h->z_version;
+        h->p16_prec = prec;
+        h->p16_overflow = overflow;
+    }
+    *use_mixed = !h->p16_overflow;
+    return GMMHIP_OK;
+}
+
+MixedArgs make_mixed_args(gmmhip_ctx *h) {
+    MixedArgs m{};
+    m.N = h->N;
+    m.Npad = h->z_Npad;
+    m.D = h->D;
+    m.KP = h->KPm;
+    m.za_hi = h->d_za_hi;
+    m.za_lo = h->d_za_lo;
+    m.zb_hi = h->d_zb_hi;
+    m.zb_lo = h->d_zb_lo;
+    m.bad = h->d_bad;
+    m.P16 = h->d_P16;
+    m.M = h->M;
+    m.Mt32 = (h->M + 31) / 32;
+    m.min_ll = h->opt.min_loglike;
+    m.min_gamma = h->opt.min_gamma;
+    return m;
+}
+
 }  // namespace
 
 extern "C" {
@@ -229,6 +421,16 @@ int gmmhip_destroy(gmmhip_t h) {
     hipFree(h->d_part);
     hipFree(h->d_red);
     hipFree(h->d_tot);
+    hipFree(h->d_za_hi);
+    hipFree(h->d_za_lo);
+    hipFree(h->d_zb_hi);
+    hipFree(h->d_zb_lo);
+    hipFree(h->d_bad);
+    hipFree(h->d_nrange);
+    hipFree(h->d_colpart);
+    hipFree(h->d_zoff);
+    hipFree(h->d_zscale);
+    hipFree(h->d_P16);
     if (h->h_res) hipHostFree(h->h_res);
     if (h->ev_ok)
         for (auto &e : h->ev) hipEventDestroy(e);
@@ -351,6 +553,11 @@ int gmmhip_set_model(gmmhip_t h, int M, int D, const float *w, const float *mu,
     h->KP = KP;
     h->Mt = Mt;
     h->h_off = off;
+    h->m_w.assign(w, w + M);
+    h->m_mu.assign(mu, mu + (size_t)M * D);
+    h->m_var.assign(var, var + (size_t)M * D);
+    h->m_gconst = gconst;
+    h->model_version++;
     return GMMHIP_OK;
 }
 
@@ -370,6 +577,7 @@ int gmmhip_set_frames_contig(gmmhip_t h, const float *x, uint64_t N, int D, uint
     h->N = N;
     h->ld = D;
     h->xD = D;
+    h->frames_version++;
     return GMMHIP_OK;
 }
 
@@ -395,6 +603,7 @@ int gmmhip_set_frames(gmmhip_t h, const float *const *rows, uint64_t N, int D) {
     h->N = N;
     h->ld = D;
     h->xD = D;
+    h->frames_version++;
     return GMMHIP_OK;
 }
 
@@ -417,6 +626,7 @@ int gmmhip_set_frames_device(gmmhip_t h, const float *dx, uint64_t N, int D, uin
     h->N = N;
     h->ld = ld;
     h->xD = D;
+    h->frames_version++;
     return GMMHIP_OK;
 }
 
@@ -425,9 +635,20 @@ int gmmhip_loglikes_device(gmmhip_t h, float *d_out) {
     if (rc) return rc;
     if (!d_out) return fail(h, GMMHIP_ERR_ARG, "loglikes_device: null output");
     DevGuard g(h->dev);
-    EstepArgs a = make_args(h);
     HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, h->N));
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N)));
+    bool mixed = false;
+    if ((rc = prepare_mixed(h, &mixed))) return rc;
+    h->last_used_mixed = mixed;
+    if (mixed) {
+        MixedArgs m = make_mixed_args(h);
+        m.LL2 = h->d_LL2;
+        m.LLnat = d_out;
+        m.blockLL = h->d_blockLL;
+        HIPCHK(h, launch_loglik_mixed(m, h->opt.precision, h->stream));
+        return GMMHIP_OK;
+    }
+    EstepArgs a = make_args(h);
     a.LL2 = h->d_LL2;
     a.LLnat = d_out;
     a.blockLL = h->d_blockLL;
@@ -466,19 +687,22 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
         ((flags & GMMHIP_ACC_VAR) && !S))
         return fail(h, GMMHIP_ERR_ARG, "accumulate: null output for a requested statistic");
     DevGuard g(h->dev);
-    EstepArgs a = make_args(h);
     HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, h->N));
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N)));
-    a.LL2 = h->d_LL2;
-    a.LLnat = nullptr;
-    a.blockLL = h->d_blockLL;
+    bool mixed = false;
+    if ((rc = prepare_mixed(h, &mixed))) return rc;
+    h->last_used_mixed = mixed;
 
-    a.ngroups = (h->Mt + 3) / 4;
-    a.chunk_frames = pick_chunk_frames(h, a.ngroups);
-    a.nchunks = (int)((h->N + a.chunk_frames - 1) / a.chunk_frames);
-    const size_t Mp = (size_t)h->Mt * 16;
-    const size_t E = Mp * h->KP;
-    HIPCHK(h, ensure(&h->d_part, &h->part_cap, (size_t)a.nchunks * E));
+    const int D = h->D;
+    const int KPx = mixed ? h->KPm : h->KP;
+    const int tile = mixed ? 32 : 16;
+    const int Mtx = (h->M + tile - 1) / tile;
+    const int ngroups = (Mtx + 3) / 4;
+    const uint64_t chunk_frames = pick_chunk_frames(h, ngroups);
+    const int nchunks = (int)((h->N + chunk_frames - 1) / chunk_frames);
+    const size_t Mp = (size_t)Mtx * tile;
+    const size_t E = Mp * KPx;
+    HIPCHK(h, ensure(&h->d_part, &h->part_cap, (size_t)nchunks * E));
     HIPCHK(h, ensure(&h->d_red, &h->red_cap, E));
     if (h->h_res_cap < E + 1) {
         if (h->h_res) HIPCHK(h, hipHostFree(h->h_res));
@@ -487,28 +711,46 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
         HIPCHK(h, hipHostMalloc((void **)&h->h_res, (E + 1) * sizeof(double)));
         h->h_res_cap = E + 1;
     }
-    a.partials = h->d_part;
-    a.red = h->d_red;
-    a.tot = h->d_tot;
 
     const bool prof = h->opt.profile != 0;
     if (prof) {
         if ((rc = ensure_events(h))) return rc;
         HIPCHK(h, hipEventRecord(h->ev[0], h->stream));
     }
-    HIPCHK(h, launch_loglik_fp32(a, h->stream));
-    if (prof) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
-    HIPCHK(h, launch_stats_fp32(a, h->stream));
+    if (mixed) {
+        MixedArgs m = make_mixed_args(h);
+        m.LL2 = h->d_LL2;
+        m.LLnat = nullptr;
+        m.blockLL = h->d_blockLL;
+        m.chunk_frames = chunk_frames;
+        m.nchunks = nchunks;
+        m.ngroups = ngroups;
+        m.partials = h->d_part;
+        HIPCHK(h, launch_loglik_mixed(m, h->opt.precision, h->stream));
+        if (prof) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+        HIPCHK(h, launch_stats_mixed(m, h->opt.precision, h->stream));
+    } else {
+        EstepArgs a = make_args(h);
+        a.LL2 = h->d_LL2;
+        a.LLnat = nullptr;
+        a.blockLL = h->d_blockLL;
+        a.ngroups = ngroups;
+        a.chunk_frames = chunk_frames;
+        a.nchunks = nchunks;
+        a.partials = h->d_part;
+        HIPCHK(h, launch_loglik_fp32(a, h->stream));
+        if (prof) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
+        HIPCHK(h, launch_stats_fp32(a, h->stream));
+    }
     if (prof) HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
-    HIPCHK(h, launch_reduce_stats(a, h->stream));
-    HIPCHK(h, launch_sum_blockll(a, h->stream));
+    HIPCHK(h, launch_reduce_partials(h->d_part, nchunks, E, h->d_red, h->stream));
+    HIPCHK(h, launch_sum_doubles(h->d_blockLL, ll_blocks(h->N), h->d_tot, h->stream));
     if (prof) HIPCHK(h, hipEventRecord(h->ev[3], h->stream));
     HIPCHK(h, hipMemcpyAsync(h->h_res, h->d_red, E * sizeof(double), hipMemcpyDeviceToHost,
                              h->stream));
     HIPCHK(h, hipMemcpyAsync(h->h_res + E, h->d_tot, sizeof(double), hipMemcpyDeviceToHost,
                              h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-
     if (prof) {
         add_time(h, GMMHIP_KT_LOGLIK, 0, 1);
         add_time(h, GMMHIP_KT_STATS, 1, 2);
@@ -518,15 +760,16 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
     if (std::isnan(tot))
         return fail(h, GMMHIP_ERR_NAN, "accumulateStatsMT(): overall log-likelihood = NaN!");
 
-    // Undo the centring: F = F' + n o, S = S' + 2 o F' + n o^2 (FP64).
-    const int D = h->D, KP = h->KP;
+    // Undo the centring (and scaling) of the features, in FP64:
+    // x = z/s + o  ->  F = F'/s + n o,  S = S'/s^2 + 2 o F'/s + n o^2.
     for (int m = 0; m < h->M; m++) {
-        const double *r = h->h_res + (size_t)m * KP;
+        const double *r = h->h_res + (size_t)m * KPx;
         const double nm = r[0];
         n[m] += nm;
         for (int d = 0; d < D; d++) {
-            const double o = h->h_off[d];
-            const double f1 = r[1 + d], s1 = r[1 + D + d];
+            const double o = mixed ? (double)h->h_zoff[d] : (double)h->h_off[d];
+            const double sc = mixed ? 1.0 / (double)h->h_zscale[d] : 1.0;
+            const double f1 = r[1 + d] * sc, s1 = r[1 + D + d] * sc * sc;
             if (flags & GMMHIP_ACC_MEAN) F[(size_t)m * D + d] += f1 + nm * o;
             if (flags & GMMHIP_ACC_VAR) S[(size_t)m * D + d] += s1 + 2.0 * o * f1 + nm * o * o;
         }
