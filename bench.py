@@ -5,7 +5,10 @@ A step is one E-step pass (posteriors + zeroth/first/second-order
 statistics + total log-likelihood, the reference's accumulateStatsMT) over
 the whole resident batch of synthetic frames.  Default workload: UBM-scale
 plain EM, D=60, M=512, N=10,000,000 frames (seeded true mixture), model
-initialised from 512 random frames with the global variance.
+initialised from 512 random frames with the global variance.  The headline
+is the mixed-precision form (split fp16 operands, FP32 accumulate; total
+log-likelihood within 1e-6 relative of FP64), timed beside the plain FP32
+form on the same frames (`fp32_compare`).
 
 Prints ONE JSON line (rank 0).  Frames are generated directly in HBM, so the
 timed region starts with inputs resident.
@@ -53,7 +56,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ubm_plain")
     ap.add_argument("--frames", type=int, default=0, help="override N")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16x3", "bf16x3"])
+    ap.add_argument("--precision", default="fp16x3", choices=["fp32", "fp16x3", "bf16x3"])
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP32 comparison leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -76,24 +80,35 @@ def main():
     w, mu, var = synth.init_model_from_frames(x, M, cfg["seed"])
     torch.cuda.synchronize()
 
-    prec = {"fp32": gmm_amd.PREC_FP32, "fp16x3": gmm_amd.PREC_FP16X3,
-            "bf16x3": gmm_amd.PREC_BF16X3}[args.precision]
-    acc = gmm_amd.Accumulator(0, profile=1, precision=prec)
+    precs = {"fp32": gmm_amd.PREC_FP32, "fp16x3": gmm_amd.PREC_FP16X3,
+             "bf16x3": gmm_amd.PREC_BF16X3}
+    acc = gmm_amd.Accumulator(0, profile=1)
     acc.set_model(w, mu, var)
     acc.set_frames_device(x.data_ptr(), N, D)
-    for _ in range(args.warmup):
-        acc.accumulate()
-    acc.reset_kernel_times()
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        res = acc.accumulate()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    el = t1 - t0
+    def run(precision):
+        """K timed E-steps (after W untimed ones) in one precision."""
+        acc.set_opts(profile=1, precision=precs[precision])
+        for _ in range(args.warmup):
+            acc.accumulate()
+        acc.reset_kernel_times()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = acc.accumulate()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        return res, el, acc.kernel_times()
+
+    compare = None
+    if args.precision != "fp32" and not args.no_compare:
+        r32, el32, kt32 = run("fp32")
+        compare = {"precision": "fp32", "value": N * M * args.steps / el32,
+                   "ms_per_step": el32 * 1e3 / args.steps,
+                   "stats_kernel_ms": kt32["stats"][0] / max(kt32["stats"][1], 1),
+                   "tot_loglike": r32[3]}
+    res, el, kt = run(args.precision)
     value = N * M * args.steps / el
-    kt = acc.kernel_times()
 
     flop_fc = 8 * D + 4
     st_ms, st_n = kt["stats"]
@@ -104,15 +119,18 @@ def main():
     peak = PEAK["fp32"] if args.precision == "fp32" else PEAK["fp16"]
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "kernel": "stats_kernel",
+                "kernel": "stats_kernel" if args.precision == "fp32" else "stats_mixed_kernel",
                 "avg_launch_ms": st_avg * 1e3,
                 "pipeline_ms": {"loglik": ll_ms / max(ll_n, 1), "stats": st_ms / max(st_n, 1),
                                 "reduce": rd_ms / max(rd_n, 1)},
                 "pipeline_frac": flop_fc * N * M / ((ll_ms + st_ms + rd_ms) / max(st_n, 1) / 1e3)
                 / peak}
+    if args.precision != "fp32":
+        roofline["executed_products"] = 3
 
     line = {
-        "metric": "E-step frame*comp/s (%s)" % args.precision,
+        "metric": "E-step frame*comp/s (%s%s)" % (
+            args.precision, "" if compare is None else " vs fp32"),
         "value": value,
         "unit": "frame*comp/s",
         "n_gpus": 1,
@@ -129,6 +147,11 @@ def main():
                    "precision": args.precision, "tot_loglike_per_frame": res[3] / N},
         "roofline": roofline,
     }
+    if compare is not None:
+        compare["speedup"] = value / compare["value"]
+        compare["tot_loglike_rel_diff"] = abs(res[3] - compare["tot_loglike"]) / abs(
+            compare["tot_loglike"])
+        line["fp32_compare"] = compare
     if not args.no_cpu_baseline:
         ns = min(N, 10_000_000)
         xs = x[:ns].cpu().numpy()

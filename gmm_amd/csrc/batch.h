@@ -1,0 +1,40 @@
+// Internal interface of the batched per-utterance kernels (batch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gmmhip {
+
+struct SegReduceArgs {
+    const float *partials;   // [chunks][Mp][KP]
+    const int *chunk_first;  // [U+1] first chunk of each utterance
+    const float *off;        // [D] centring of the partials
+    int U, M, Mp, D, KP;
+    double *n;               // [U][M]
+    double *F;               // [U][M][D]
+};
+
+struct MllrSolveArgs {
+    const double *Gp;     // [(u,i)][ldg] packed upper triangle of G
+    const double *Kp;     // [(u,i)][ldk] k
+    const double *Ginit;  // [D][E][E] UBM prior or null
+    const double *kinit;  // [D][E] or null
+    int U, D;
+    int ldg, ldk;
+    int *status;          // [U] in: 0 / 7 (under-determined); out: |= 8 if G not SPD
+    double *W;            // [(u,i)][E]
+};
+
+hipError_t launch_fill_u32(unsigned int *p, unsigned int v, int n, hipStream_t s);
+hipError_t launch_seg_reduce(const SegReduceArgs &a, hipStream_t s);
+hipError_t launch_map_means(const double *n, const double *F, const float *mu, int U, int M, int D,
+                            int tau, float *out, hipStream_t s);
+hipError_t launch_mllr_operands(const double *n, const double *F, const float *var, int U, int M,
+                                int D, int ldk, double *A1, double *A2, hipStream_t s);
+hipError_t launch_mllr_check(const double *n, int U, int M, int D, int *status, hipStream_t s);
+hipError_t launch_gemm_f64(const double *A, int lda, const double *B, int ldb, double *C, int ldc,
+                           int R, int Cn, int K, hipStream_t s);
+hipError_t launch_mllr_solve(const MllrSolveArgs &a, hipStream_t s);
+
+}  // namespace gmmhip

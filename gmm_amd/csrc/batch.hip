@@ -1,0 +1,317 @@
+// Batched per-utterance statistics, MAP means and MLLR transforms on gfx950.
+//
+// Thousands of utterances of uneven length are processed together: the
+// per-frame log-likelihoods of all frames run as one launch (the E-step
+// kernels), the posteriors + statistics of every (utterance chunk,
+// component group) work item run in one persistent launch that pulls items
+// from a device work queue (stats kernel, queue mode), and the kernels below
+// turn the chunk partials into per-utterance statistics, MAP-adapted means
+// (UpdateModelMAP, trainer/OL_ModelAdapt.cpp:993-1106) and MLLR row
+// statistics and transforms (ComputeKiGi_MLLR / prepareKiGiInitMatrices /
+// ComputeW_MLLR, OL_ModelAdapt.cpp:1417-1581).  MLLR statistics are
+// contractions over components:
+//   G[(u,i)][p(a,b)] = sum_m (n_um / var_mi) xi_ma xi_mb,  xi_m = [mu_m; 1]
+//   k[(u,i)][a]      = sum_m (F_umi / var_mi) xi_ma
+// i.e. two FP64 GEMMs against UBM-only matrices (xi xi^T packed, and xi),
+// followed by one FP64 Cholesky solve per (utterance, row).
+#include "batch.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace gmmhip {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+__global__ void fill_u32_kernel(unsigned int *p, unsigned int v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(unsigned int *p, unsigned int v, int n, hipStream_t s) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Per-utterance reduction of the chunk partials [chunk][Mp][KP] (columns
+// [1, x', x'^2] of centred frames) into n[u][m] and un-centred F[u][m][d],
+// FP64, chunks of an utterance summed in order.  One thread per (u, m, j),
+// j = 0..D.
+__global__ __launch_bounds__(256) void seg_reduce_kernel(SegReduceArgs a) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const int J = a.D + 1;
+    const uint64_t total = (uint64_t)a.U * a.M * J;
+    if (i >= total) return;
+    const int j = (int)(i % J);
+    const uint64_t um = i / J;
+    const int m = (int)(um % a.M);
+    const int u = (int)(um / a.M);
+    const size_t E = (size_t)a.Mp * a.KP;
+    double s = 0.0, n = 0.0;
+    for (int c = a.chunk_first[u]; c < a.chunk_first[u + 1]; c++) {
+        const float *p = a.partials + (size_t)c * E + (size_t)m * a.KP;
+        s += (double)p[j];
+        if (j > 0) n += (double)p[0];
+    }
+    if (j == 0) {
+        a.n[um] = s;
+    } else {
+        // x = x' + o: F = F' + n o
+        a.F[um * a.D + (j - 1)] = s + n * (double)a.off[j - 1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MAP mean adaptation, alpha = n/(n+tau), mu' = alpha F/n + (1-alpha) mu for
+// n > 0, the UBM mean otherwise.  One thread per (u, m, d).
+__global__ __launch_bounds__(256) void map_means_kernel(const double *__restrict__ n,
+                                                        const double *__restrict__ F,
+                                                        const float *__restrict__ mu, int U, int M,
+                                                        int D, int tau, float *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t total = (uint64_t)U * M * D;
+    if (i >= total) return;
+    const uint64_t um = i / D;
+    const int d = (int)(i - um * D);
+    const int m = (int)(um % M);
+    const double nm = n[um];
+    const float ub = mu[(size_t)m * D + d];
+    float v = ub;
+    if (nm > 0.0) {
+        const double alfa = nm / (nm + (double)tau);
+        v = (float)(alfa * F[i] / nm + (1.0 - alfa) * (double)ub);
+    }
+    out[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// GEMM operands of the MLLR statistics: A1[(u,i)][m] = n_um / var_mi (zero
+// for n_um <= 0), A2[(u,i)][m] = F_umi / var_mi.  One thread per (u, i, m).
+__global__ __launch_bounds__(256) void mllr_operands_kernel(const double *__restrict__ n,
+                                                            const double *__restrict__ F,
+                                                            const float *__restrict__ var, int U,
+                                                            int M, int D, int ldk,
+                                                            double *__restrict__ A1,
+                                                            double *__restrict__ A2) {
+    const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t total = (uint64_t)U * D * M;
+    if (idx >= total) return;
+    const int m = (int)(idx % M);
+    const uint64_t ui = idx / M;
+    const int i = (int)(ui % D);
+    const int u = (int)(ui / D);
+    const double nm = n[(size_t)u * M + m];
+    const double iv = 1.0 / (double)var[(size_t)m * D + i];
+    const bool act = nm > 0.0;
+    A1[ui * ldk + m] = act ? nm * iv : 0.0;
+    A2[ui * ldk + m] = act ? F[((size_t)u * M + m) * D + i] * iv : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// CheckInvertibility (OL_ModelAdapt.cpp:1851-1878) per utterance: status 0, or
+// 7 when fewer than D+1 components are active or the occupancy is below D+1.
+__global__ __launch_bounds__(256) void mllr_check_kernel(const double *__restrict__ n, int U, int M,
+                                                         int D, int *__restrict__ status) {
+    __shared__ int cnt[256];
+    __shared__ double occ[256];
+    const int u = blockIdx.x;
+    int c = 0;
+    double o = 0.0;
+    for (int m = threadIdx.x; m < M; m += 256) {
+        const double v = n[(size_t)u * M + m];
+        if (v > 0.0) {
+            c++;
+            o += v;
+        }
+    }
+    cnt[threadIdx.x] = c;
+    occ[threadIdx.x] = o;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            cnt[threadIdx.x] += cnt[threadIdx.x + w];
+            occ[threadIdx.x] += occ[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const bool ok = M >= D + 1 && cnt[0] >= D + 1 && occ[0] * D >= (double)D * (D + 1);
+        status[u] = ok ? 0 : 7;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 GEMM C[R][Cn] = A[R][K] B[K][Cn] (row-major, leading dimensions lda,
+// ldb, ldc) on v_mfma_f64_16x16x4f64.  Block tile 64 x 64, four waves of
+// 32 x 32, K staged through LDS 16 at a time; out-of-range rows, columns and
+// K are zero-filled.
+__global__ __launch_bounds__(256) void gemm_f64_kernel(const double *__restrict__ A, int lda,
+                                                       const double *__restrict__ B, int ldb,
+                                                       double *__restrict__ C, int ldc, int R,
+                                                       int Cn, int K) {
+    __shared__ double As[64][17];
+    __shared__ double Bs[16][65];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int q = lane >> 4, c = lane & 15;
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    f64x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = (f64x4){0.0, 0.0, 0.0, 0.0};
+
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        __syncthreads();
+        for (int i = tid; i < 64 * 16; i += 256) {
+            const int rr = i >> 4, kk = i & 15;
+            const int gr = r0 + rr, gk = k0 + kk;
+            As[rr][kk] = (gr < R && gk < K) ? A[(size_t)gr * lda + gk] : 0.0;
+        }
+        for (int i = tid; i < 16 * 64; i += 256) {
+            const int kk = i >> 6, cc = i & 63;
+            const int gk = k0 + kk, gc = c0 + cc;
+            Bs[kk][cc] = (gk < K && gc < Cn) ? B[(size_t)gk * ldb + gc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; kk += 4) {
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const double av = As[wr + 16 * a + c][kk + q];
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const double bv = Bs[kk + q][wc + 16 * b + c];
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[a][b], 0, 0, 0);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gr = r0 + wr + 16 * a + 4 * q + r, gc = c0 + wc + 16 * b + c;
+                if (gr < R && gc < Cn) C[(size_t)gr * ldc + gc] = acc[a][b][r];
+            }
+}
+
+hipError_t launch_gemm_f64(const double *A, int lda, const double *B, int ldb, double *C, int ldc,
+                           int R, int Cn, int K, hipStream_t s) {
+    const dim3 grid((Cn + 63) / 64, (R + 63) / 64);
+    hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, R, Cn, K);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// One wave per (u, i): G (packed upper triangle + optional prior) and k are
+// assembled in LDS, G = L L^T by a left-looking Cholesky (lane = row), then
+// L y = k and L^T w = y.  A non-positive pivot marks the utterance's status.
+__global__ __launch_bounds__(64) void mllr_solve_kernel(MllrSolveArgs a) {
+    extern __shared__ double sh[];
+    const int E = a.D + 1;
+    const int LDS = E + 1;
+    double *G = sh;               // [E][LDS]
+    double *v = sh + E * LDS;     // [E]
+    const uint64_t ui = blockIdx.x;
+    const int u = (int)(ui / a.D), i = (int)(ui % a.D);
+    const int lane = threadIdx.x;
+    if (a.status[u] != 0) {
+        for (int p = lane; p < E; p += 64) a.W[ui * E + p] = 0.0;
+        return;
+    }
+    const double *gp = a.Gp + ui * a.ldg;
+    for (int p = lane; p < E * E; p += 64) {
+        const int r = p / E, cc = p - r * E;
+        const int lo = r < cc ? r : cc, hi = r < cc ? cc : r;
+        // packed index of (lo, hi), lo <= hi: lo*E - lo(lo-1)/2 + (hi - lo)
+        const int pk = lo * E - (lo * (lo - 1)) / 2 + (hi - lo);
+        double g = gp[pk];
+        if (a.Ginit) g += a.Ginit[(size_t)i * E * E + p];
+        G[r * LDS + cc] = g;
+    }
+    for (int p = lane; p < E; p += 64) {
+        double kv = a.Kp[ui * a.ldk + p];
+        if (a.kinit) kv += a.kinit[(size_t)i * E + p];
+        v[p] = kv;
+    }
+    __syncthreads();
+    bool bad = false;
+    for (int j = 0; j < E; j++) {
+        // diagonal
+        double d = G[j * LDS + j];
+        for (int p = 0; p < j; p++) d -= G[j * LDS + p] * G[j * LDS + p];
+        bad = bad || !(d > 0.0);
+        const double ljj = sqrt(d > 0.0 ? d : 1.0);
+        // column below the diagonal, one row per lane
+        for (int r = j + 1 + lane; r < E; r += 64) {
+            double s = G[r * LDS + j];
+            for (int p = 0; p < j; p++) s -= G[r * LDS + p] * G[j * LDS + p];
+            G[r * LDS + j] = s / ljj;
+        }
+        __syncthreads();
+        if (lane == 0) G[j * LDS + j] = ljj;
+        __syncthreads();
+    }
+    // forward substitution L y = k (serial over rows, lanes share the dot)
+    for (int r = 0; r < E; r++) {
+        double s = 0.0;
+        for (int p = lane; p < r; p += 64) s += G[r * LDS + p] * v[p];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) v[r] = (v[r] - s) / G[r * LDS + r];
+        __syncthreads();
+    }
+    // back substitution L^T w = y
+    for (int r = E - 1; r >= 0; r--) {
+        double s = 0.0;
+        for (int p = r + 1 + lane; p < E; p += 64) s += G[p * LDS + r] * v[p];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) v[r] = (v[r] - s) / G[r * LDS + r];
+        __syncthreads();
+    }
+    for (int p = lane; p < E; p += 64) a.W[ui * E + p] = v[p];
+    if (bad && lane == 0) atomicOr(&a.status[u], 8);
+}
+
+hipError_t launch_seg_reduce(const SegReduceArgs &a, hipStream_t s) {
+    const uint64_t total = (uint64_t)a.U * a.M * (a.D + 1);
+    hipLaunchKernelGGL(seg_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       a);
+    return hipGetLastError();
+}
+
+hipError_t launch_map_means(const double *n, const double *F, const float *mu, int U, int M, int D,
+                            int tau, float *out, hipStream_t s) {
+    const uint64_t total = (uint64_t)U * M * D;
+    hipLaunchKernelGGL(map_means_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, n,
+                       F, mu, U, M, D, tau, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mllr_operands(const double *n, const double *F, const float *var, int U, int M,
+                                int D, int ldk, double *A1, double *A2, hipStream_t s) {
+    const uint64_t total = (uint64_t)U * D * M;
+    hipLaunchKernelGGL(mllr_operands_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       s, n, F, var, U, M, D, ldk, A1, A2);
+    return hipGetLastError();
+}
+
+hipError_t launch_mllr_check(const double *n, int U, int M, int D, int *status, hipStream_t s) {
+    hipLaunchKernelGGL(mllr_check_kernel, dim3(U), dim3(256), 0, s, n, U, M, D, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_mllr_solve(const MllrSolveArgs &a, hipStream_t s) {
+    const int E = a.D + 1;
+    const size_t shm = ((size_t)E * (E + 1) + E) * sizeof(double);
+    hipLaunchKernelGGL(mllr_solve_kernel, dim3((unsigned)((uint64_t)a.U * a.D)), dim3(64), shm, s,
+                       a);
+    return hipGetLastError();
+}
+
+}  // namespace gmmhip

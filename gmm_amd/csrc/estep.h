@@ -49,6 +49,13 @@ struct EstepArgs {
     float *partials; // [nchunks][Mt*16][KP]
     double *red;     // [Mt*16][KP]
     double *tot;     // [1]
+    // batched mode: explicit chunk frame ranges [chunk_lo[c], chunk_hi[c])
+    // (null: chunk c covers chunk_frames frames from c*chunk_frames) and a
+    // device work queue of nchunks*ngroups items (null: one block per item)
+    const uint64_t *chunk_lo;
+    const uint64_t *chunk_hi;
+    unsigned int *queue;
+    int queue_blocks;
 };
 
 inline uint64_t ll_blocks(uint64_t N) {
@@ -57,6 +64,9 @@ inline uint64_t ll_blocks(uint64_t N) {
 
 hipError_t launch_loglik_fp32(const EstepArgs &a, hipStream_t s);
 hipError_t launch_stats_fp32(const EstepArgs &a, hipStream_t s);
+// persistent variant: a.queue_blocks blocks pull (chunk, group) items from
+// *a.queue (which must hold 0 at launch)
+hipError_t launch_stats_fp32_queue(const EstepArgs &a, hipStream_t s);
 hipError_t launch_reduce_stats(const EstepArgs &a, hipStream_t s);
 hipError_t launch_sum_blockll(const EstepArgs &a, hipStream_t s);
 // out[e] = sum_c part[c*E + e] in FP64, c = 0..nchunks-1 in order

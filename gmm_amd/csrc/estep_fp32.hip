@@ -227,26 +227,33 @@ __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
 // row.  Columns are [1, x', x'^2, 0...]: n, F', S'.
 // Two waves per SIMD (<= 256 VGPRs) up to KP = 160 (D <= 79); larger
 // depths keep their accumulators in registers at one wave per SIMD.
+#define GMMHIP_STATS_BOUNDS __launch_bounds__(256, (KP <= 160 ? 2 : 1))
+
 template <int KP>
-__global__ __launch_bounds__(256, (KP <= 160 ? 2 : 1)) void stats_kernel(EstepArgs a) {
+struct StatsShared {
+    float Zr[ST_FRAMES_PER_ITER * lds_stride(KP)];
+    float Ls[ST_FRAMES_PER_ITER];
+    float offL[KP];
+    int item;
+};
+
+// One (chunk, group) work item: all 256 threads of the block take part.
+template <int KP>
+__device__ __forceinline__ void stats_item(const EstepArgs &a, StatsShared<KP> &sh, int chunk,
+                                           int group) {
     constexpr int KK = KP / 4;
     constexpr int KU = KK / 4;
     constexpr int JT = KP / 16;
     constexpr int S = lds_stride(KP);
     constexpr int FB = ST_FRAMES_PER_ITER;
     constexpr int NPF = KP / 8;  // >= ceil(FB*D/256) for 2D+1 <= KP
-    __shared__ float Zr[FB * S];
-    __shared__ float Ls[FB];
+    float *Zr = sh.Zr;
+    float *Ls = sh.Ls;
+    const float *offL = sh.offL;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int q = lane >> 4, c = lane & 15;
-
-    const int b = blockIdx.x;
-    const int xcd = b & 7, idx = b >> 3;
-    const int group = idx % a.ngroups;
-    const int chunk = (idx / a.ngroups) * 8 + xcd;
-    if (chunk >= a.nchunks) return;
 
     const int tile = group * 4 + wave;
     const bool active = tile < a.Mt;
@@ -271,34 +278,36 @@ __global__ __launch_bounds__(256, (KP <= 160 ? 2 : 1)) void stats_kernel(EstepAr
 #pragma unroll
     for (int j = 0; j < JT; j++) acc2[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
-    for (int i = tid; i < FB * S; i += 256) Zr[i] = 0.0f;
-
-    const uint64_t f0 = (uint64_t)chunk * a.chunk_frames;
-    uint64_t f1 = f0 + a.chunk_frames;
-    if (f1 > a.N) f1 = a.N;
+    uint64_t f0, f1;
+    if (a.chunk_lo) {
+        f0 = a.chunk_lo[chunk];
+        f1 = a.chunk_hi[chunk];
+    } else {
+        f0 = (uint64_t)chunk * a.chunk_frames;
+        f1 = f0 + a.chunk_frames;
+        if (f1 > a.N) f1 = a.N;
+    }
 
     // Frames are contiguous rows (ldx == D): a 64-frame step is one flat run
     // of 64*D floats.  Element i of the run belongs to frame f = i / D, dim
     // d = i - f*D; the float quotient is exact because i < 2^15 and the
     // fractional part of (i + 0.5)/D is at least 1/(2D).
-    __shared__ float offL[KP];
-    for (int i = tid; i < D; i += 256) offL[i] = a.off[i];
     const float invD = 1.0f / (float)D;
 
     float pf[NPF];
     float pl = INFINITY;
-    auto prefetch = [&](uint64_t base) {
-        const float *src = a.X + base * (uint64_t)D;
-        const uint64_t nf = (f1 - base < (uint64_t)FB) ? f1 - base : (uint64_t)FB;
-        const int nvalid = (int)nf * D;
-#pragma unroll
-        for (int j = 0; j < NPF; j++) {
-            const int i = tid + 256 * j;
-            pf[j] = (i < nvalid) ? src[i] : 0.0f;
-        }
-        if (tid < FB) pl = (base + tid < f1) ? a.LL2[base + tid] : INFINITY;
-    };
-    prefetch(f0);
+#define GMMHIP_STATS_PREFETCH(B_)                                                \
+    {                                                                            \
+        const float *src = a.X + (B_) * (uint64_t)D;                             \
+        const uint64_t nf = (f1 - (B_) < (uint64_t)FB) ? f1 - (B_) : (uint64_t)FB; \
+        const int nvalid = (int)nf * D;                                          \
+        _Pragma("unroll") for (int j = 0; j < NPF; j++) {                        \
+            const int i = tid + 256 * j;                                         \
+            pf[j] = (i < nvalid) ? src[i] : 0.0f;                                \
+        }                                                                        \
+        if (tid < FB) pl = ((B_) + tid < f1) ? a.LL2[(B_) + tid] : INFINITY;     \
+    }
+    GMMHIP_STATS_PREFETCH(f0)
 
     for (uint64_t base = f0; base < f1; base += FB) {
         __syncthreads();
@@ -323,7 +332,7 @@ __global__ __launch_bounds__(256, (KP <= 160 ? 2 : 1)) void stats_kernel(EstepAr
             Ls[tid] = pl;
         }
         __syncthreads();
-        if (base + FB < f1) prefetch(base + FB);
+        if (base + FB < f1) GMMHIP_STATS_PREFETCH(base + FB)
         if (!active) continue;
 
         f32x4 acc1[4];
@@ -358,15 +367,63 @@ __global__ __launch_bounds__(256, (KP <= 160 ? 2 : 1)) void stats_kernel(EstepAr
             }
         }
     }
+#undef GMMHIP_STATS_PREFETCH
 
-    if (!active) return;
-    const size_t Mp = (size_t)a.Mt * 16;
-    float *out = a.partials + ((size_t)chunk * Mp + (size_t)tile * 16) * KP;
+    if (active) {
+        const size_t Mp = (size_t)a.Mt * 16;
+        float *out = a.partials + ((size_t)chunk * Mp + (size_t)tile * 16) * KP;
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+        for (int r = 0; r < 4; r++)
 #pragma unroll
-        for (int j = 0; j < JT; j++)
-            out[(size_t)(4 * q + r) * KP + c * JT + j] = acc2[j][r];
+            for (int j = 0; j < JT; j++)
+                out[(size_t)(4 * q + r) * KP + c * JT + j] = acc2[j][r];
+    }
+}
+
+template <int KP>
+__device__ __forceinline__ void stats_init_shared(const EstepArgs &a, StatsShared<KP> &sh) {
+    for (int i = threadIdx.x; i < ST_FRAMES_PER_ITER * lds_stride(KP); i += 256) sh.Zr[i] = 0.0f;
+    for (int i = threadIdx.x; i < a.D; i += 256) sh.offL[i] = a.off[i];
+}
+
+// stats_kernel: block = (frame chunk, group of 64 components); wave w owns
+// component tile 4*group + w and its 16 x KP statistics accumulators.  Blocks
+// are numbered so that the groups of one chunk land on the same XCD (the
+// dispatcher deals blocks round-robin over 8 XCDs), so a chunk's frames are
+// fetched from HBM once per XCD L2.  The next 64 frames are prefetched into
+// registers while the current ones are contracted.
+//
+// Statistics column mapping: MFMA jt of the g^T Z contraction covers columns
+// c*JT + jt (c = lane%16), so a lane's B operands are consecutive in the LDS
+// row.  Columns are [1, x', x'^2, 0...]: n, F', S'.
+template <int KP>
+__global__ GMMHIP_STATS_BOUNDS void stats_kernel(EstepArgs a) {
+    __shared__ StatsShared<KP> sh;
+    const int b = blockIdx.x;
+    const int xcd = b & 7, idx = b >> 3;
+    const int group = idx % a.ngroups;
+    const int chunk = (idx / a.ngroups) * 8 + xcd;
+    if (chunk >= a.nchunks) return;
+    stats_init_shared<KP>(a, sh);
+    stats_item<KP>(a, sh, chunk, group);
+}
+
+// Persistent form: a fixed grid pulls (chunk, group) items, chunk-major in
+// the order of the chunk table, from a device counter until they run out;
+// every block reaches the exit test after each item.
+template <int KP>
+__global__ GMMHIP_STATS_BOUNDS void stats_queue_kernel(EstepArgs a) {
+    __shared__ StatsShared<KP> sh;
+    stats_init_shared<KP>(a, sh);
+    const int nitems = a.nchunks * a.ngroups;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) sh.item = (int)atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const int item = sh.item;
+        if (item >= nitems) break;
+        stats_item<KP>(a, sh, item / a.ngroups, item % a.ngroups);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -420,6 +477,18 @@ hipError_t launch_stats_fp32(const EstepArgs &a, hipStream_t s) {
     switch (a.KP) {
 #define X(K) \
     case K: hipLaunchKernelGGL(stats_kernel<K>, grid, dim3(256), 0, s, a); break;
+        GMMHIP_KP_CASES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stats_fp32_queue(const EstepArgs &a, hipStream_t s) {
+    const dim3 grid((unsigned)a.queue_blocks);
+    switch (a.KP) {
+#define X(K) \
+    case K: hipLaunchKernelGGL(stats_queue_kernel<K>, grid, dim3(256), 0, s, a); break;
         GMMHIP_KP_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;
