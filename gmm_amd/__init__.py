@@ -23,8 +23,9 @@ EXPORTS = [
     "gmmhip_set_model", "gmmhip_set_frames", "gmmhip_set_frames_contig",
     "gmmhip_set_frames_device", "gmmhip_accumulate", "gmmhip_loglikes",
     "gmmhip_loglikes_device", "gmmhip_stream", "gmmhip_sync",
-    "gmmhip_kernel_times", "gmmhip_reset_kernel_times",
+    "gmmhip_kernel_times", "gmmhip_reset_kernel_times", "gmmhip_batch_adapt",
 ]
+BATCH_QUEUE, BATCH_PER_UTTERANCE = 0, 1
 KT_LOGLIK, KT_STATS, KT_REDUCE, KT_COUNT = 0, 1, 2, 3
 # host-layer entry points (include/gmmhost.h), bound in gmm_amd.host
 EXTRA_EXPORTS = [
@@ -50,6 +51,17 @@ class Opts(ctypes.Structure):
                 ("min_var", ctypes.c_float), ("precision", ctypes.c_int),
                 ("chunk_frames", ctypes.c_int), ("verbosity", ctypes.c_int),
                 ("profile", ctypes.c_int)]
+
+
+class BatchCfg(ctypes.Structure):
+    _fields_ = [("tau", ctypes.c_int), ("mllr_prior", ctypes.c_int), ("mode", ctypes.c_int),
+                ("chunk_frames", ctypes.c_int), ("graph", ctypes.c_int),
+                ("outputs_on_device", ctypes.c_int)]
+
+
+class BatchOut(ctypes.Structure):
+    _fields_ = [("means", ctypes.c_void_p), ("occ", ctypes.c_void_p), ("first", ctypes.c_void_p),
+                ("mllr_W", ctypes.c_void_p), ("mllr_status", ctypes.c_void_p)]
 
 
 _lib = None
@@ -85,6 +97,9 @@ def lib():
         L.gmmhip_default_opts.argtypes = [ctypes.POINTER(Opts)]
         L.gmmhip_default_opts.restype = None
         L.gmmhip_kernel_times.argtypes = [c_vp, c_dp, ctypes.POINTER(c_u64), ctypes.c_int]
+        L.gmmhip_batch_adapt.argtypes = [c_vp, c_vp, ctypes.c_int, ctypes.POINTER(c_u64),
+                                         ctypes.c_int, ctypes.c_int, ctypes.POINTER(BatchCfg),
+                                         ctypes.POINTER(BatchOut)]
         L.gmmhip_reset_kernel_times.argtypes = [c_vp]
         _lib = L
     return _lib
@@ -209,6 +224,51 @@ class Accumulator:
 
     def reset_kernel_times(self):
         self._check(lib().gmmhip_reset_kernel_times(self._h))
+
+    def batch_adapt(self, x, offsets, tau=16, means=True, stats=True, mllr=True, prior=False,
+                    mode=BATCH_QUEUE, chunk_frames=0, graph=False):
+        """Batched per-utterance statistics / MAP means / MLLR transforms of
+        host frames x[offsets[-1], D] split by offsets[U+1].  Returns a dict
+        of numpy arrays."""
+        x = np.ascontiguousarray(x, np.float32)
+        off = np.ascontiguousarray(offsets, np.uint64)
+        U = len(off) - 1
+        M, D = self.M, self.D
+        res = {}
+        o = BatchOut()
+        if means:
+            res["means"] = np.zeros((U, M, D), np.float32)
+            o.means = res["means"].ctypes.data
+        if stats:
+            res["occ"] = np.zeros((U, M))
+            res["first"] = np.zeros((U, M, D))
+            o.occ = res["occ"].ctypes.data
+            o.first = res["first"].ctypes.data
+        if mllr:
+            res["W"] = np.zeros((U, D, D + 1))
+            res["status"] = np.zeros(U, np.int32)
+            o.mllr_W = res["W"].ctypes.data
+            o.mllr_status = res["status"].ctypes.data
+        cfg = BatchCfg(tau, int(prior), mode, chunk_frames, int(graph), 0)
+        self._check(lib().gmmhip_batch_adapt(
+            self._h, x.ctypes.data, 0, off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), U, D,
+            ctypes.byref(cfg), ctypes.byref(o)))
+        self.N = int(off[-1])
+        return res
+
+    def batch_adapt_device(self, x_ptr, offsets, D, out_ptrs, tau=16, prior=False,
+                           mode=BATCH_QUEUE, chunk_frames=0, graph=False):
+        """Device-resident variant: frames and outputs are device pointers
+        (out_ptrs: dict with optional means/occ/first/W/status)."""
+        off = np.ascontiguousarray(offsets, np.uint64)
+        U = len(off) - 1
+        o = BatchOut(out_ptrs.get("means"), out_ptrs.get("occ"), out_ptrs.get("first"),
+                     out_ptrs.get("W"), out_ptrs.get("status"))
+        cfg = BatchCfg(tau, int(prior), mode, chunk_frames, int(graph), 1)
+        self._check(lib().gmmhip_batch_adapt(
+            self._h, ctypes.c_void_p(x_ptr), 1, off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            U, D, ctypes.byref(cfg), ctypes.byref(o)))
+        self.N = int(off[-1])
 
     def stream(self):
         return lib().gmmhip_stream(self._h)

@@ -23,7 +23,8 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
           "-Wno-unused-result", "-Wno-unused-value"]
 
-HIP_SOURCES = ["estep_fp32.hip", "estep_mixed.hip", "gmmhip_api.hip"]
+HIP_SOURCES = ["estep_fp32.hip", "estep_mixed.hip", "batch.hip", "gmmhip_api.hip",
+               "batch_api.hip"]
 HOST_SOURCES = ["host/gmm_model.cpp", "host/model_adapt.cpp", "host/feature_io.cpp",
                 "host/gmmhost_api.cpp"]
 TOOLS = ["trainGMM", "getLogLike"]

@@ -144,7 +144,9 @@ __global__ __launch_bounds__(256) void mllr_check_kernel(const double *__restric
 
 // ---------------------------------------------------------------------------
 // FP64 GEMM C[R][Cn] = A[R][K] B[K][Cn] (row-major, leading dimensions lda,
-// ldb, ldc) on v_mfma_f64_16x16x4f64.  Block tile 64 x 64, four waves of
+// ldb, ldc) on v_mfma_f64_16x16x4f64 (operands as the f32 16x16x4 form:
+// lane (q, c) gives A[c][q] and B[q][c]; the result of lane (q, c) register r
+// is row q + 4r, column c).  Block tile 64 x 64, four waves of
 // 32 x 32, K staged through LDS 16 at a time; out-of-range rows, columns and
 // K are zero-filled.
 __global__ __launch_bounds__(256) void gemm_f64_kernel(const double *__restrict__ A, int lda,
@@ -196,7 +198,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const double *__restrict_
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int gr = r0 + wr + 16 * a + 4 * q + r, gc = c0 + wc + 16 * b + c;
+                // v_mfma_f64_16x16x4f64 result: lane (q, c), register r holds
+                // row q + 4r of the 16 x 16 tile
+                const int gr = r0 + wr + 16 * a + q + 4 * r, gc = c0 + wc + 16 * b + c;
                 if (gr < R && gc < Cn) C[(size_t)gr * ldc + gc] = acc[a][b][r];
             }
 }

@@ -1,0 +1,373 @@
+// gmmhip_batch_adapt: batched per-utterance statistics, MAP means and MLLR
+// transforms (include/gmmhip.h).
+//
+// Pipeline on the handle's stream:
+//   loglik_kernel over all frames of the batch (per-frame LL is independent
+//     of the utterance),
+//   stats_queue_kernel: a persistent grid pulls (chunk, component group)
+//     items from a device counter; chunks never straddle utterances,
+//   seg_reduce_kernel: per-utterance FP64 sums of the chunk partials,
+//   map_means_kernel, and for MLLR mllr_check / mllr_operands / two FP64
+//     GEMMs against UBM-only matrices / mllr_solve.
+// With cfg->graph the whole pipeline is captured once into a hipGraph and
+// replayed while the batch shape and buffers stay the same.
+#include "batch.h"
+#include "ctx.h"
+
+using namespace gmmhip_detail;
+
+struct gmmhip_batch_state {
+    uint64_t *d_lo = nullptr, *d_hi = nullptr;
+    size_t lo_cap = 0, hi_cap = 0;
+    int *d_first = nullptr;
+    size_t first_cap = 0;
+    unsigned int *d_queue = nullptr;
+    double *d_n = nullptr, *d_F = nullptr;
+    size_t n_cap = 0, F_cap = 0;
+    float *d_means = nullptr;
+    size_t means_cap = 0;
+    double *d_A1 = nullptr, *d_A2 = nullptr, *d_G = nullptr, *d_K = nullptr, *d_W = nullptr;
+    size_t A1_cap = 0, A2_cap = 0, G_cap = 0, K_cap = 0, W_cap = 0;
+    int *d_status = nullptr;
+    size_t status_cap = 0;
+    // UBM-only MLLR operands: P[m][packed (a<=b)] = xi_a xi_b, Xi[m][a], prior
+    double *d_P = nullptr, *d_Xi = nullptr, *d_Ginit = nullptr, *d_kinit = nullptr;
+    size_t P_cap = 0, Xi_cap = 0, Ginit_cap = 0, kinit_cap = 0;
+    uint64_t mllr_model = ~0ull;
+    // graph cache
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> key;
+};
+
+namespace gmmhip_detail {
+void batch_state_free(gmmhip_batch_state *b) {
+    if (!b) return;
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (void *p : {(void *)b->d_lo, (void *)b->d_hi, (void *)b->d_first, (void *)b->d_queue,
+                    (void *)b->d_n, (void *)b->d_F, (void *)b->d_means, (void *)b->d_A1,
+                    (void *)b->d_A2, (void *)b->d_G, (void *)b->d_K, (void *)b->d_W,
+                    (void *)b->d_status, (void *)b->d_P, (void *)b->d_Xi, (void *)b->d_Ginit,
+                    (void *)b->d_kinit})
+        if (p) hipFree(p);
+    delete b;
+}
+}  // namespace gmmhip_detail
+
+namespace {
+
+// UBM-only MLLR operands (FP64, host-built once per model).
+int build_mllr_operands(gmmhip_ctx *h, gmmhip_batch_state *b) {
+    if (b->mllr_model == h->model_version) return GMMHIP_OK;
+    const int M = h->M, D = h->D, E = D + 1, NP = E * (E + 1) / 2;
+    std::vector<double> P((size_t)M * NP), Xi((size_t)M * E);
+    std::vector<double> Gi((size_t)D * E * E, 0.0), ki((size_t)D * E, 0.0);
+    std::vector<double> xi(E);
+    const double minv = h->opt.min_var;
+    for (int m = 0; m < M; m++) {
+        for (int a = 0; a < D; a++) xi[a] = h->m_mu[(size_t)m * D + a];
+        xi[D] = 1.0;
+        for (int a = 0, pk = 0; a < E; a++)
+            for (int c = a; c < E; c++, pk++) P[(size_t)m * NP + pk] = xi[a] * xi[c];
+        for (int a = 0; a < E; a++) Xi[(size_t)m * E + a] = xi[a];
+        // prepareKiGiInitMatrices (OL_ModelAdapt.cpp:1484-1523): one virtual
+        // frame per component at its mean, with its variance
+        const float *var = &h->m_var[(size_t)m * D];
+        const float *mean = &h->m_mu[(size_t)m * D];
+        for (int i = 0; i < D; i++) {
+            const double cG = (var[i] < minv) ? 1.0 / minv : 1.0 / var[i];
+            const double ck = mean[i] * cG;
+            double *G = &Gi[(size_t)i * E * E], *k = &ki[(size_t)i * E];
+            for (int a = 0; a < D; a++) {
+                k[a] += ck * mean[a];
+                G[(size_t)D * E + a] += cG * mean[a];
+                G[(size_t)a * E + D] += cG * mean[a];
+                for (int c = 0; c < D; c++) G[(size_t)a * E + c] += cG * mean[a] * mean[c];
+                G[(size_t)a * E + a] += cG * var[a];
+            }
+            G[(size_t)E * E - 1] += cG;
+            k[D] += ck;
+        }
+    }
+    HIPCHK(h, ensure(&b->d_P, &b->P_cap, P.size()));
+    HIPCHK(h, ensure(&b->d_Xi, &b->Xi_cap, Xi.size()));
+    HIPCHK(h, ensure(&b->d_Ginit, &b->Ginit_cap, Gi.size()));
+    HIPCHK(h, ensure(&b->d_kinit, &b->kinit_cap, ki.size()));
+    HIPCHK(h, hipMemcpyAsync(b->d_P, P.data(), P.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(b->d_Xi, Xi.data(), Xi.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(b->d_Ginit, Gi.data(), Gi.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(b->d_kinit, ki.data(), ki.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    b->mllr_model = h->model_version;
+    return GMMHIP_OK;
+}
+
+struct BatchPlan {
+    int U = 0, M = 0, D = 0, KP = 0, Mt = 0, ngroups = 0, nchunks = 0;
+    uint64_t N = 0, cf = 0;
+    bool want_means = false, want_mllr = false, prior = false, per_utt = false;
+    int tau = 0;
+    std::vector<uint64_t> lo, hi;
+    std::vector<int> first;
+};
+
+// Enqueue the device pipeline (no host synchronisation; graph-capturable:
+// kernels only, the queue counter is reset by a kernel).
+int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
+    hipStream_t s = h->stream;
+    EstepArgs a{};
+    a.X = h->d_x;
+    a.ldx = p.D;
+    a.N = p.N;
+    a.D = p.D;
+    a.KP = p.KP;
+    a.P = h->d_P;
+    a.M = p.M;
+    a.Mt = p.Mt;
+    a.off = h->d_off;
+    a.min_ll = h->opt.min_loglike;
+    a.min_gamma = h->opt.min_gamma;
+    a.LL2 = h->d_LL2;
+    a.LLnat = nullptr;
+    a.blockLL = h->d_blockLL;
+    a.ngroups = p.ngroups;
+    a.partials = h->d_part;
+    a.chunk_frames = p.cf;
+    const int blocks_cap = std::max(1, h->num_cus) * 2;
+    if (!p.per_utt) {
+        HIPCHK(h, launch_loglik_fp32(a, s));
+        a.nchunks = p.nchunks;
+        a.chunk_lo = b->d_lo;
+        a.chunk_hi = b->d_hi;
+        a.queue = b->d_queue;
+        a.queue_blocks = std::min(p.nchunks * p.ngroups, blocks_cap);
+        HIPCHK(h, launch_fill_u32(b->d_queue, 0u, 1, s));
+        HIPCHK(h, launch_stats_fp32_queue(a, s));
+    } else {
+        // baseline: the same kernels, one launch set per utterance
+        for (int u = 0; u < p.U; u++) {
+            const int c0 = p.first[u], c1 = p.first[u + 1];
+            if (c1 == c0) continue;
+            const uint64_t f0 = p.lo[c0], f1 = p.hi[c1 - 1];
+            EstepArgs au = a;
+            au.X = h->d_x + f0 * p.D;
+            au.N = f1 - f0;
+            au.LL2 = h->d_LL2 + f0;
+            HIPCHK(h, launch_loglik_fp32(au, s));
+            au.X = h->d_x;
+            au.N = p.N;
+            au.LL2 = h->d_LL2;
+            au.nchunks = c1 - c0;
+            au.chunk_lo = b->d_lo + c0;
+            au.chunk_hi = b->d_hi + c0;
+            au.partials = h->d_part + (size_t)c0 * p.Mt * 16 * p.KP;
+            au.queue = b->d_queue + 1 + (u % 255);
+            au.queue_blocks = std::min((c1 - c0) * p.ngroups, blocks_cap);
+            HIPCHK(h, launch_fill_u32(au.queue, 0u, 1, s));
+            HIPCHK(h, launch_stats_fp32_queue(au, s));
+        }
+    }
+    SegReduceArgs r{};
+    r.partials = h->d_part;
+    r.chunk_first = b->d_first;
+    r.off = h->d_off;
+    r.U = p.U;
+    r.M = p.M;
+    r.Mp = p.Mt * 16;
+    r.D = p.D;
+    r.KP = p.KP;
+    r.n = b->d_n;
+    r.F = b->d_F;
+    HIPCHK(h, launch_seg_reduce(r, s));
+    if (p.want_means)
+        HIPCHK(h, launch_map_means(b->d_n, b->d_F, h->d_mu_f32, p.U, p.M, p.D, p.tau, b->d_means, s));
+    if (p.want_mllr) {
+        const int E = p.D + 1, NP = E * (E + 1) / 2;
+        const int R = p.U * p.D;
+        HIPCHK(h, launch_mllr_check(b->d_n, p.U, p.M, p.D, b->d_status, s));
+        HIPCHK(h, launch_mllr_operands(b->d_n, b->d_F, h->d_var_f32, p.U, p.M, p.D, p.M, b->d_A1,
+                                       b->d_A2, s));
+        HIPCHK(h, launch_gemm_f64(b->d_A1, p.M, b->d_P, NP, b->d_G, NP, R, NP, p.M, s));
+        HIPCHK(h, launch_gemm_f64(b->d_A2, p.M, b->d_Xi, E, b->d_K, E, R, E, p.M, s));
+        MllrSolveArgs m{};
+        m.Gp = b->d_G;
+        m.Kp = b->d_K;
+        m.Ginit = p.prior ? b->d_Ginit : nullptr;
+        m.kinit = p.prior ? b->d_kinit : nullptr;
+        m.U = p.U;
+        m.D = p.D;
+        m.ldg = NP;
+        m.ldk = E;
+        m.status = b->d_status;
+        m.W = b->d_W;
+        HIPCHK(h, launch_mllr_solve(m, s));
+    }
+    return GMMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on_device,
+                                  const uint64_t *offsets, int U, int D,
+                                  const gmmhip_batch_cfg *cfg, gmmhip_batch_out *out) {
+    if (!h) return GMMHIP_ERR_ARG;
+    if (!frames || !offsets || U <= 0 || !cfg || !out)
+        return fail(h, GMMHIP_ERR_ARG, "batch_adapt: null argument");
+    if (h->M <= 0) return fail(h, GMMHIP_ERR_STATE, "no model inserted");
+    if (D != h->D) return fail(h, GMMHIP_ERR_ARG, "batch_adapt: D=%d but the model has D=%d", D, h->D);
+    if (offsets[0] != 0) return fail(h, GMMHIP_ERR_ARG, "batch_adapt: offsets[0] must be 0");
+    for (int u = 0; u < U; u++)
+        if (offsets[u + 1] < offsets[u])
+            return fail(h, GMMHIP_ERR_ARG, "batch_adapt: offsets must be non-decreasing");
+    if (out->mllr_W && !out->mllr_status)
+        return fail(h, GMMHIP_ERR_ARG, "batch_adapt: mllr_W needs mllr_status");
+    const uint64_t N = offsets[U];
+    if (N == 0) return fail(h, GMMHIP_ERR_ARG, "batch_adapt: no frames");
+    DevGuard g(h->dev);
+    if (!h->batch) h->batch = new gmmhip_batch_state();
+    gmmhip_batch_state *b = h->batch;
+    if (h->num_cus == 0) {
+        int n = 0;
+        HIPCHK(h, hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, h->dev));
+        h->num_cus = n > 0 ? n : 256;
+    }
+    int rc;
+    // frames become the handle's frames (borrowed device memory or a copy)
+    if (frames_on_device)
+        rc = gmmhip_set_frames_device(h, frames, N, D, D);
+    else
+        rc = gmmhip_set_frames_contig(h, frames, N, D, D);
+    if (rc) return rc;
+
+    BatchPlan p;
+    p.U = U;
+    p.M = h->M;
+    p.D = D;
+    p.KP = h->KP;
+    p.Mt = h->Mt;
+    p.ngroups = (h->Mt + 3) / 4;
+    p.N = N;
+    p.cf = cfg->chunk_frames > 0 ? (uint64_t)(cfg->chunk_frames + 63) / 64 * 64 : 1024;
+    p.want_means = out->means != nullptr;
+    p.want_mllr = out->mllr_W != nullptr;
+    p.prior = cfg->mllr_prior != 0;
+    p.per_utt = cfg->mode == GMMHIP_BATCH_PER_UTTERANCE;
+    p.tau = cfg->tau;
+    p.first.assign(U + 1, 0);
+    for (int u = 0; u < U; u++) {
+        p.first[u] = (int)p.lo.size();
+        for (uint64_t s0 = offsets[u]; s0 < offsets[u + 1]; s0 += p.cf) {
+            p.lo.push_back(s0);
+            p.hi.push_back(std::min<uint64_t>(offsets[u + 1], s0 + p.cf));
+        }
+    }
+    p.first[U] = (int)p.lo.size();
+    p.nchunks = (int)p.lo.size();
+    const size_t M = p.M, E = D + 1, NP = E * (E + 1) / 2;
+    const size_t Mp = (size_t)p.Mt * 16;
+
+    // buffers
+    HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, N));
+    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(N) + (size_t)U));
+    HIPCHK(h, ensure(&h->d_part, &h->part_cap, (size_t)p.nchunks * Mp * p.KP));
+    HIPCHK(h, ensure(&b->d_lo, &b->lo_cap, (size_t)p.nchunks));
+    HIPCHK(h, ensure(&b->d_hi, &b->hi_cap, (size_t)p.nchunks));
+    HIPCHK(h, ensure(&b->d_first, &b->first_cap, (size_t)U + 1));
+    if (!b->d_queue) HIPCHK(h, hipMalloc((void **)&b->d_queue, 256 * sizeof(unsigned int)));
+    HIPCHK(h, ensure(&b->d_n, &b->n_cap, (size_t)U * M));
+    HIPCHK(h, ensure(&b->d_F, &b->F_cap, (size_t)U * M * D));
+    if (p.want_means) {
+        HIPCHK(h, ensure(&b->d_means, &b->means_cap, (size_t)U * M * D));
+        if (!h->d_mu_f32 || h->mu_f32_version != h->model_version) {
+            HIPCHK(h, ensure(&h->d_mu_f32, &h->mu_f32_cap, M * D));
+            HIPCHK(h, hipMemcpyAsync(h->d_mu_f32, h->m_mu.data(), M * D * 4, hipMemcpyHostToDevice,
+                                     h->stream));
+        }
+    }
+    if (p.want_mllr) {
+        HIPCHK(h, ensure(&h->d_var_f32, &h->var_f32_cap, M * D));
+        HIPCHK(h, hipMemcpyAsync(h->d_var_f32, h->m_var.data(), M * D * 4, hipMemcpyHostToDevice,
+                                 h->stream));
+        if ((rc = build_mllr_operands(h, b))) return rc;
+        HIPCHK(h, ensure(&b->d_A1, &b->A1_cap, (size_t)U * D * M));
+        HIPCHK(h, ensure(&b->d_A2, &b->A2_cap, (size_t)U * D * M));
+        HIPCHK(h, ensure(&b->d_G, &b->G_cap, (size_t)U * D * NP));
+        HIPCHK(h, ensure(&b->d_K, &b->K_cap, (size_t)U * D * E));
+        HIPCHK(h, ensure(&b->d_W, &b->W_cap, (size_t)U * D * E));
+        HIPCHK(h, ensure(&b->d_status, &b->status_cap, (size_t)U));
+    }
+    if (p.want_means || p.want_mllr) {
+        if (!h->d_mu_f32 || h->mu_f32_version != h->model_version) {
+            HIPCHK(h, ensure(&h->d_mu_f32, &h->mu_f32_cap, M * D));
+            HIPCHK(h, hipMemcpyAsync(h->d_mu_f32, h->m_mu.data(), M * D * 4, hipMemcpyHostToDevice,
+                                     h->stream));
+        }
+        h->mu_f32_version = h->model_version;
+    }
+    HIPCHK(h, hipMemcpyAsync(b->d_lo, p.lo.data(), p.lo.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(b->d_hi, p.hi.data(), p.hi.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(b->d_first, p.first.data(), p.first.size() * 4, hipMemcpyHostToDevice,
+                             h->stream));
+
+    // run, directly or through a cached graph
+    if (cfg->graph) {
+        std::vector<uint64_t> key = {N,
+                                     (uint64_t)U,
+                                     (uint64_t)(uintptr_t)h->d_x,
+                                     h->model_version,
+                                     p.cf,
+                                     (uint64_t)p.want_means,
+                                     (uint64_t)p.want_mllr,
+                                     (uint64_t)p.prior,
+                                     (uint64_t)p.per_utt,
+                                     (uint64_t)p.tau,
+                                     (uint64_t)(uintptr_t)h->d_part,
+                                     (uint64_t)(uintptr_t)b->d_n,
+                                     (uint64_t)(uintptr_t)b->d_G};
+        key.insert(key.end(), offsets, offsets + U + 1);
+        if (!b->exec || key != b->key) {
+            if (b->exec) HIPCHK(h, hipGraphExecDestroy(b->exec));
+            if (b->graph) HIPCHK(h, hipGraphDestroy(b->graph));
+            b->exec = nullptr;
+            b->graph = nullptr;
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_pipeline(h, b, p);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+            if (rc) {
+                if (graph) hipGraphDestroy(graph);
+                return rc;
+            }
+            HIPCHK(h, e);
+            b->graph = graph;
+            HIPCHK(h, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->key = key;
+        }
+        HIPCHK(h, hipGraphLaunch(b->exec, h->stream));
+    } else {
+        if ((rc = enqueue_pipeline(h, b, p))) return rc;
+    }
+
+    // outputs
+    const hipMemcpyKind kind = cfg->outputs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (out->occ) HIPCHK(h, hipMemcpyAsync(out->occ, b->d_n, (size_t)U * M * 8, kind, h->stream));
+    if (out->first)
+        HIPCHK(h, hipMemcpyAsync(out->first, b->d_F, (size_t)U * M * D * 8, kind, h->stream));
+    if (out->means)
+        HIPCHK(h, hipMemcpyAsync(out->means, b->d_means, (size_t)U * M * D * 4, kind, h->stream));
+    if (out->mllr_W) {
+        HIPCHK(h, hipMemcpyAsync(out->mllr_W, b->d_W, (size_t)U * D * E * 8, kind, h->stream));
+        if (cfg->outputs_on_device) {
+            HIPCHK(h, hipMemcpyAsync(out->mllr_status, b->d_status, (size_t)U * 4, kind, h->stream));
+        } else {
+            HIPCHK(h, hipMemcpyAsync(out->mllr_status, b->d_status, (size_t)U * 4, kind, h->stream));
+        }
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (out->mllr_W && !cfg->outputs_on_device)
+        for (int u = 0; u < U; u++)
+            if (out->mllr_status[u]) out->mllr_status[u] = GMMHIP_ERR_MLLR;
+    return GMMHIP_OK;
+}

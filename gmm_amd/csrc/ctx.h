@@ -1,0 +1,154 @@
+// Handle state shared by the C ABI translation units (internal).
+#pragma once
+
+#include "../../include/gmmhip.h"
+#include "estep.h"
+#include "estep_mixed.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace gmmhip;
+
+struct gmmhip_batch_state;  // batched per-utterance buffers (batch_api.hip)
+
+struct gmmhip_ctx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    gmmhip_opts opt{};
+    std::string err;
+
+    // model
+    int M = 0, D = 0, KP = 0, Mt = 0;
+    float *d_P = nullptr;
+    size_t P_cap = 0;
+    float *d_off = nullptr;
+    size_t off_cap = 0;
+    std::vector<float> h_off;
+
+    // frames
+    const float *d_x = nullptr;
+    uint64_t N = 0, ld = 0;
+    int xD = 0;
+    float *d_xown = nullptr;
+    size_t xown_cap = 0;
+
+    // scratch
+    float *d_LL2 = nullptr;
+    size_t LL2_cap = 0;
+    float *d_LLnat = nullptr;
+    size_t LLnat_cap = 0;
+    double *d_blockLL = nullptr;
+    size_t blockLL_cap = 0;
+    float *d_part = nullptr;
+    size_t part_cap = 0;
+    double *d_red = nullptr;
+    size_t red_cap = 0;
+    double *d_tot = nullptr;
+    double *h_res = nullptr;  // pinned: [Mp*KP] + 1
+    size_t h_res_cap = 0;
+
+    // host copy of the active model (for the mixed-precision packing)
+    std::vector<float> m_w, m_mu, m_var;
+    std::vector<double> m_gconst;
+    uint64_t model_version = 0, frames_version = 0;
+
+    // mixed-precision frame features (ZA/ZB hi/lo) and split model
+    int z_prec = 0, KPm = 0;
+    uint64_t z_version = 0, z_N = 0, z_Npad = 0;
+    bool z_range_fallback = false;
+    uint16_t *d_za_hi = nullptr, *d_za_lo = nullptr, *d_zb_hi = nullptr, *d_zb_lo = nullptr;
+    size_t z_cap = 0;
+    uint8_t *d_bad = nullptr;
+    size_t bad_cap = 0;
+    unsigned int *d_nrange = nullptr;
+    double *d_colpart = nullptr;
+    size_t colpart_cap = 0;
+    float *d_zoff = nullptr, *d_zscale = nullptr;
+    size_t zos_cap = 0, zscale_cap = 0;
+    std::vector<float> h_zoff, h_zscale;
+    void *d_P16 = nullptr;
+    size_t P16_cap = 0;
+    uint64_t p16_model = 0, p16_z = 0;
+    int p16_prec = 0;
+    bool p16_overflow = false;
+    bool last_used_mixed = false;
+
+    // batched per-utterance state (batch_api.hip)
+    gmmhip_batch_state *batch = nullptr;
+    float *d_mu_f32 = nullptr, *d_var_f32 = nullptr;  // UBM means / variances
+    size_t mu_f32_cap = 0, var_f32_cap = 0;
+    uint64_t mu_f32_version = ~0ull;
+    int num_cus = 0;
+
+    // kernel timers (opts.profile)
+    hipEvent_t ev[5] = {};
+    bool ev_ok = false;
+    double kt_ms[GMMHIP_KT_COUNT] = {};
+    uint64_t kt_n[GMMHIP_KT_COUNT] = {};
+};
+
+namespace gmmhip_detail {
+
+inline int fail(gmmhip_ctx *h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, expr)                                                              \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail((h), e_ == hipErrorOutOfMemory ? GMMHIP_ERR_OOM : GMMHIP_ERR_HIP, \
+                        "%s failed: %s", #expr, hipGetErrorString(e_));             \
+    } while (0)
+
+// Grow-only device buffer.
+template <typename T>
+inline hipError_t ensure(T **p, size_t *cap, size_t n) {
+    if (*cap >= n && *p) return hipSuccess;
+    if (*p) {
+        hipError_t e = hipFree(*p);
+        if (e != hipSuccess) return e;
+        *p = nullptr;
+        *cap = 0;
+    }
+    hipError_t e = hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) *cap = n;
+    return e;
+}
+
+inline void apply_defaults(gmmhip_opts *o) {
+    if (o->min_loglike == 0.0f) o->min_loglike = -1e20f;
+    if (o->min_gamma == 0.0f) o->min_gamma = 1e-4f;
+    if (o->min_var == 0.0f) o->min_var = 1e-6f;
+}
+
+struct DevGuard {
+    int prev = -1;
+    explicit DevGuard(int d) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != d) hipSetDevice(d);
+        else prev = -1;
+    }
+    ~DevGuard() {
+        if (prev >= 0) hipSetDevice(prev);
+    }
+};
+
+// Frees the batched state of a handle (batch_api.hip).
+void batch_state_free(gmmhip_batch_state *b);
+
+}  // namespace gmmhip_detail

@@ -229,9 +229,10 @@ __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
 // depths keep their accumulators in registers at one wave per SIMD.
 #define GMMHIP_STATS_BOUNDS __launch_bounds__(256, (KP <= 160 ? 2 : 1))
 
+// Zr rows are read as 16-byte vectors: keep the tile 16-byte aligned.
 template <int KP>
-struct StatsShared {
-    float Zr[ST_FRAMES_PER_ITER * lds_stride(KP)];
+struct alignas(16) StatsShared {
+    alignas(16) float Zr[ST_FRAMES_PER_ITER * lds_stride(KP)];
     float Ls[ST_FRAMES_PER_ITER];
     float offL[KP];
     int item;

@@ -20,7 +20,43 @@ CONFIGS = {
                       alpha=0.5, offset=False, seed=20261205),
     "mixed_estep": dict(D=60, M=2048, N=20_000_000, mean_sd=2.0, var_lo=0.3, var_hi=3.0,
                         alpha=0.5, offset=True, seed=20261206),
+    # batched MAP + MLLR: 4,096 utterances, lengths lognormal(ln 300, 0.8)
+    # clipped to [50, 3000], each from the UBM mixture with its means shifted
+    # by N(0, 0.3^2)
+    "batch_map": dict(D=60, M=2048, U=4096, mean_sd=2.0, var_lo=0.3, var_hi=3.0, alpha=0.5,
+                      offset=False, len_median=300, len_sigma=0.8, len_lo=50, len_hi=3000,
+                      shift_sd=0.3, seed=20261207),
 }
+
+
+def utterance_lengths(U, len_median, len_sigma, len_lo, len_hi, seed, **_):
+    rng = np.random.default_rng(seed + 3)
+    n = np.exp(rng.normal(np.log(len_median), len_sigma, U))
+    return np.clip(np.round(n), len_lo, len_hi).astype(np.int64)
+
+
+def sample_utterances_torch(w, mu, var, lens, shift_sd, seed, device="cuda", group=64):
+    """Concatenated frames of len(lens) utterances; utterance u is drawn from
+    the mixture with every mean shifted by its own N(0, shift_sd^2) offset."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    M, D = mu.shape
+    wt = torch.as_tensor(w, dtype=torch.float64, device=device)
+    mut = torch.as_tensor(mu, dtype=torch.float32, device=device)
+    sdt = torch.as_tensor(np.sqrt(var), dtype=torch.float32, device=device)
+    N = int(np.sum(lens))
+    x = torch.empty((N, D), dtype=torch.float32, device=device)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    for u0 in range(0, len(lens), group):
+        u1 = min(len(lens), u0 + group)
+        shift = torch.randn((u1 - u0, M, D), generator=g, device=device) * shift_sd
+        for u in range(u0, u1):
+            n = int(lens[u])
+            k = torch.multinomial(wt, n, replacement=True, generator=g)
+            z = torch.randn((n, D), generator=g, device=device)
+            x[off[u]:off[u + 1]] = mut[k] + shift[u - u0, k] + sdt[k] * z
+    return x, off.astype(np.uint64)
 
 
 def true_mixture(D, M, mean_sd, var_lo, var_hi, alpha, offset, seed, **_):

@@ -155,6 +155,45 @@ enum {
 int gmmhip_kernel_times(gmmhip_t h, double *ms, uint64_t *count, int n);
 int gmmhip_reset_kernel_times(gmmhip_t h);
 
+/* ---- Batched per-utterance adaptation --------------------------------
+ * Many utterances of uneven length against the inserted model (the UBM):
+ * per-utterance occupancies and first-order statistics, MAP-adapted means
+ * (UpdateModelEM/UpdateModelMAP with --mean, trainer/OL_ModelAdapt.cpp:
+ * 993-1106) and global MLLR transforms (ComputeW_MLLR with ComputeKiGi_MLLR,
+ * prepareKiGiInitMatrices and CheckInvertibility, OL_ModelAdapt.cpp:
+ * 1417-1581, :1851-1878).  The reference adapts one model per process
+ * (projects/Model_ADAPT/Model_ADAPT.cpp:173-240); this is its batched form. */
+enum {
+    GMMHIP_BATCH_QUEUE = 0,          /* one persistent work-queue launch */
+    GMMHIP_BATCH_PER_UTTERANCE = 1   /* one launch set per utterance */
+};
+
+typedef struct gmmhip_batch_cfg {
+    int tau;                /* MAP relevance factor (an int, as --tau) */
+    int mllr_prior;         /* 1: UBM prior (--mllr-init) */
+    int mode;               /* GMMHIP_BATCH_* */
+    int chunk_frames;       /* frames per work item, 0 = auto */
+    int graph;              /* 1: capture the device pipeline in a hipGraph and
+                               replay it for batches of the same shape and
+                               buffers */
+    int outputs_on_device;  /* 1: the output pointers are device memory */
+} gmmhip_batch_cfg;
+
+typedef struct gmmhip_batch_out {
+    float *means;           /* [U][M][D] MAP-adapted means, or NULL */
+    double *occ;            /* [U][M] occupancies n, or NULL */
+    double *first;          /* [U][M][D] first-order statistics F, or NULL */
+    double *mllr_W;         /* [U][D][D+1] MLLR transforms [A | b], or NULL */
+    int *mllr_status;       /* [U] 0 or GMMHIP_ERR_MLLR; required with mllr_W */
+} gmmhip_batch_out;
+
+/* frames: the U utterances concatenated, [offsets[U]][D] row-major, in host
+ * memory or (frames_on_device = 1) device memory; offsets[U+1] (host) with
+ * offsets[0] = 0.  The frames become the handle's current frames. */
+int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on_device,
+                       const uint64_t *offsets, int U, int D, const gmmhip_batch_cfg *cfg,
+                       gmmhip_batch_out *out);
+
 /* The handle's HIP stream (hipStream_t as void*), for callers that order
  * their own device work with the handle's. */
 void *gmmhip_stream(gmmhip_t h);

@@ -1,0 +1,137 @@
+"""Batched per-utterance statistics, MAP means and MLLR transforms
+(gmmhip_batch_adapt) against the oracle applied utterance by utterance.
+
+Tolerances: statistics as the E-step (1e-5 relative + cut allowance); MAP
+means 1e-5 relative; MLLR transforms: exact FP64 algebra on the GPU's own
+statistics (1e-8), and 1e-3 against the oracle's FP64 statistics (the
+transform amplifies the FP32 statistics' 1e-5 by the conditioning of G).
+The work-queue launch, the per-utterance launches and hipGraph replays give
+bit-identical results.
+"""
+import numpy as np
+import pytest
+
+import _oracle as orc
+import gmm_amd
+from gmm_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def acc():
+    if gmm_amd.device_count() < 1:
+        pytest.skip("no GPU")
+    a = gmm_amd.Accumulator(0)
+    yield a
+    a.close()
+
+
+def make_batch(seed, U, M, D, lo=30, hi=400, extra=()):
+    rng = np.random.default_rng(seed)
+    w = rng.dirichlet(np.ones(M))
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    var = rng.uniform(0.5, 2.0, (M, D)).astype(np.float32)
+    lens = list(rng.integers(lo, hi, U - len(extra))) + list(extra)
+    xs = [synth.sample_frames_np(w, mu + rng.normal(0, 0.3, (M, D)).astype(np.float32), var,
+                                 int(n), seed + 10 + u) if n > 0 else np.zeros((0, D), np.float32)
+          for u, n in enumerate(lens)]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    x = np.concatenate(xs).astype(np.float32)
+    return x, off, w.astype(np.float32), mu, var
+
+
+def per_utt_oracle(x, off, w, mu, var):
+    out = []
+    for u in range(len(off) - 1):
+        xu = x[int(off[u]):int(off[u + 1])]
+        if len(xu) == 0:
+            M, D = mu.shape
+            out.append((np.zeros(M), np.zeros((M, D))))
+        else:
+            n, F, S, _ = orc.accumulate(xu, w, mu, var)
+            out.append((n, F))
+    return out
+
+
+def test_batch_stats_map_mllr(acc):
+    x, off, w, mu, var = make_batch(1, 24, 48, 6, extra=(0, 4))
+    acc.set_model(w, mu, var)
+    res = acc.batch_adapt(x, off, tau=16, prior=False)
+    ref = per_utt_oracle(x, off, w, mu, var)
+    xm = float(np.abs(x).max())
+    for u, (rn, rF) in enumerate(ref):
+        n, F = res["occ"][u], res["first"][u]
+        assert np.all(np.abs(n - rn) <= 1e-5 * rn + 3e-4), u
+        assert np.all(np.abs(F - rF) <= 1e-5 * np.abs(rF) + 3e-4 * xm + 1e-5 * rn[:, None] * xm), u
+        # MAP means from the GPU's statistics and from the oracle's
+        _, rmu, _ = orc.update_map(w, mu, var, rn, rF, rF ** 2, 1, 16)
+        assert np.allclose(res["means"][u], rmu, rtol=1e-5, atol=1e-5 * xm), u
+    # MLLR: under-determined utterances are flagged
+    lens = np.diff(off.astype(np.int64))
+    for u in range(len(lens)):
+        n = res["occ"][u]
+        expect_ok = orc.mllr_check(len(w), mu.shape[1], n) == 0
+        assert (res["status"][u] == 0) == expect_ok, (u, lens[u])
+        if expect_ok:
+            Wg = orc.mllr_transform(mu, var, n, res["first"][u])
+            assert np.allclose(res["W"][u], Wg, rtol=1e-8, atol=1e-8), u
+            Wo = orc.mllr_transform(mu, var, ref[u][0], ref[u][1])
+            assert np.allclose(res["W"][u], Wo, rtol=1e-3, atol=1e-3), u
+        else:
+            assert np.all(res["W"][u] == 0)
+
+
+def test_batch_mllr_prior(acc):
+    x, off, w, mu, var = make_batch(2, 10, 40, 5, lo=40, hi=200)
+    acc.set_model(w, mu, var)
+    res = acc.batch_adapt(x, off, tau=16, prior=True, means=False)
+    for u in range(10):
+        assert res["status"][u] == 0
+        Wg = orc.mllr_transform(mu, var, res["occ"][u], res["first"][u], ubm_mu=mu, ubm_var=var)
+        assert np.allclose(res["W"][u], Wg, rtol=1e-8, atol=1e-8), u
+
+
+def test_batch_modes_and_graph_bit_identical(acc):
+    x, off, w, mu, var = make_batch(3, 40, 100, 12, lo=20, hi=700)
+    acc.set_model(w, mu, var)
+    a = acc.batch_adapt(x, off, mode=gmm_amd.BATCH_QUEUE)
+    b = acc.batch_adapt(x, off, mode=gmm_amd.BATCH_PER_UTTERANCE)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    # graph: capture once, replay several times with eager work in between
+    for rep in range(3):
+        g = acc.batch_adapt(x, off, graph=True)
+        for k in a:
+            assert np.array_equal(a[k], g[k]), (rep, k)
+        e = acc.batch_adapt(x, off, graph=False)
+        for k in a:
+            assert np.array_equal(a[k], e[k]), (rep, k)
+    # a different batch shape rebuilds the graph
+    x2, off2, _, _, _ = make_batch(4, 17, 100, 12)
+    g2 = acc.batch_adapt(x2, off2, graph=True)
+    e2 = acc.batch_adapt(x2, off2, graph=False)
+    for k in g2:
+        assert np.array_equal(g2[k], e2[k]), k
+
+
+def test_batch_chunk_size_invariance(acc):
+    x, off, w, mu, var = make_batch(5, 16, 64, 8, lo=100, hi=3000)
+    acc.set_model(w, mu, var)
+    a = acc.batch_adapt(x, off, chunk_frames=256, mllr=False)
+    b = acc.batch_adapt(x, off, chunk_frames=4096, mllr=False)
+    assert np.allclose(a["occ"], b["occ"], rtol=2e-5, atol=1e-4)
+    assert np.allclose(a["means"], b["means"], rtol=2e-5, atol=1e-4)
+
+
+def test_batch_matches_single_accumulate(acc):
+    """One utterance through the batch path equals the plain accumulate."""
+    x, off, w, mu, var = make_batch(6, 1, 64, 10, lo=5000, hi=5001)
+    acc.set_model(w, mu, var)
+    res = acc.batch_adapt(x, off, mllr=False, chunk_frames=8192)
+    acc.set_opts(chunk_frames=8192)
+    acc.set_frames(x)
+    n, F, S, tot, _ = acc.accumulate()
+    acc.set_opts(chunk_frames=0)
+    assert np.allclose(res["occ"][0], n, rtol=1e-12, atol=1e-12)
+    assert np.allclose(res["first"][0], F, rtol=1e-12, atol=1e-9)
