@@ -105,6 +105,44 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_serve(args):
+    """Serving loop: UBM D=60, M=2048 resident in HBM; open-loop Poisson
+    arrivals (60% score / 30% MAP / 10% MLLR, lengths lognormal(median 300,
+    sigma 1.0) clipped to [20, 6000] frames) at each rate; latency from the
+    scheduled arrival to completion.  Runs the serveLoad tool (C++, links
+    libgmmhip.so) as a child process, one JSON line per rate, and reports the
+    p99 at 5000 req/s as the value."""
+    import subprocess
+    exe = os.path.join(ROOT, "gmm_amd", "_build", "serveLoad")
+    if not os.path.exists(exe):
+        raise SystemExit("serveLoad not built: run `python -m gmm_amd.build`")
+    rows = []
+    precisions = ["fp32"] if args.precision == "fp32" else ["fp32", args.precision]
+    for prec in precisions:
+        cmd = [exe, "--rates", args.rates, "--seconds", str(args.serve_seconds),
+               "--precision", prec]
+        print("bench: " + " ".join(cmd), file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        sys.stderr.write(r.stderr)
+        if r.returncode != 0:
+            raise SystemExit("serveLoad failed (%d): %s" % (r.returncode, r.stdout[-2000:]))
+        rows += [json.loads(ln) for ln in r.stdout.splitlines() if ln.startswith("{")]
+    head = [r for r in rows if r["precision"] == "fp32" and r["rate"] == 5000] or rows[:1]
+    h = head[0]
+    line = {
+        "metric": "serving p99 latency at %d req/s (score/MAP/MLLR mix, M=2048 UBM)" % h["rate"],
+        "value": h["p99_ms"], "unit": "ms", "n_gpus": 1, "steps": len(rows),
+        "warmup": 1, "ms_per_step": h["p50_ms"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": h["precision"],
+        "data": "synthetic (seeded UBM, per-utterance shifted mixtures, Poisson arrivals)",
+        "config": {"workload": "serve", "D": h["D"], "M": h["M"], "rates": args.rates,
+                   "seconds_per_rate": args.serve_seconds, "mix": "0.6/0.3/0.1",
+                   "lengths": "lognormal(300, 1.0) in [20, 6000]"},
+        "rates": rows,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +153,8 @@ def main():
     ap.add_argument("--precision", default="fp16x3", choices=["fp32", "fp16x3", "bf16x3"])
     ap.add_argument("--no-compare", action="store_true", help="skip the FP32 comparison leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--rates", default="2000,5000,10000", help="serve: arrival rates (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="serve: seconds per rate")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -124,6 +164,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch_map":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import torch
     import gmm_amd

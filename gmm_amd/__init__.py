@@ -24,6 +24,9 @@ EXPORTS = [
     "gmmhip_set_frames_device", "gmmhip_accumulate", "gmmhip_loglikes",
     "gmmhip_loglikes_device", "gmmhip_stream", "gmmhip_sync",
     "gmmhip_kernel_times", "gmmhip_reset_kernel_times", "gmmhip_batch_adapt",
+    "gmmhip_server_create", "gmmhip_server_submit", "gmmhip_server_wait",
+    "gmmhip_server_drain", "gmmhip_server_stats_get", "gmmhip_server_last_error",
+    "gmmhip_server_destroy",
 ]
 BATCH_QUEUE, BATCH_PER_UTTERANCE = 0, 1
 KT_LOGLIK, KT_STATS, KT_REDUCE, KT_COUNT = 0, 1, 2, 3
@@ -35,7 +38,10 @@ EXTRA_EXPORTS = [
     "gmmhost_mllr_transform", "gmmhost_mllr_apply", "gmmhost_last_error",
 ]
 
-OK, ERR_HIP, ERR_OOM, ERR_NAN, ERR_ARG, ERR_VAR, ERR_STATE, ERR_MLLR, ERR_NODEV = range(9)
+(OK, ERR_HIP, ERR_OOM, ERR_NAN, ERR_ARG, ERR_VAR, ERR_STATE, ERR_MLLR, ERR_NODEV, ERR_TIMEOUT,
+ ERR_CLOSED) = range(11)
+REQ_SCORE, REQ_MAP, REQ_MLLR = 0, 1, 2
+REQ_PENDING = -1
 PREC_FP32, PREC_BF16X3, PREC_FP16X3 = 0, 1, 2
 ACC_MEAN, ACC_VAR = 1, 2
 
@@ -62,6 +68,28 @@ class BatchCfg(ctypes.Structure):
 class BatchOut(ctypes.Structure):
     _fields_ = [("means", ctypes.c_void_p), ("occ", ctypes.c_void_p), ("first", ctypes.c_void_p),
                 ("mllr_W", ctypes.c_void_p), ("mllr_status", ctypes.c_void_p)]
+
+
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("opts", Opts), ("slots", ctypes.c_int), ("slot_bytes", ctypes.c_uint64),
+                ("max_batch", ctypes.c_int), ("chunk_frames", ctypes.c_int),
+                ("tau", ctypes.c_int), ("mllr_prior", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("frames", ctypes.c_void_p),
+                ("n_frames", ctypes.c_uint64), ("frame_ll", ctypes.c_void_p),
+                ("means", ctypes.c_void_p), ("occ", ctypes.c_void_p),
+                ("mllr_W", ctypes.c_void_p), ("ll_sum", ctypes.c_double),
+                ("latency_ms", ctypes.c_double), ("status", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("submitted", "completed", "failed", "batches", "frames")] + \
+               [(n, ctypes.c_double) for n in
+                ("p50_ms", "p90_ms", "p99_ms", "p999_ms", "max_ms", "mean_ms",
+                 "mean_batch_requests", "mean_batch_gpu_ms", "wall_s")]
 
 
 _lib = None
@@ -101,6 +129,16 @@ def lib():
                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(BatchCfg),
                                          ctypes.POINTER(BatchOut)]
         L.gmmhip_reset_kernel_times.argtypes = [c_vp]
+        L.gmmhip_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerCfg), ctypes.c_int,
+                                           ctypes.c_int, c_fp, c_fp, c_fp, ctypes.POINTER(c_vp)]
+        L.gmmhip_server_submit.argtypes = [c_vp, ctypes.POINTER(Request)]
+        L.gmmhip_server_wait.argtypes = [c_vp, ctypes.POINTER(Request), ctypes.c_double]
+        L.gmmhip_server_drain.argtypes = [c_vp]
+        L.gmmhip_server_stats_get.argtypes = [c_vp, ctypes.POINTER(ServerStats), ctypes.c_int]
+        L.gmmhip_server_last_error.restype = ctypes.c_char_p
+        L.gmmhip_server_last_error.argtypes = [c_vp]
+        L.gmmhip_server_destroy.argtypes = [c_vp]
+        L.gmmhip_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -275,3 +313,97 @@ class Accumulator:
 
     def sync(self):
         self._check(lib().gmmhip_sync(self._h))
+
+
+class PendingRequest:
+    """A submitted server request: the ctypes struct plus the numpy buffers
+    it points to (kept alive until the server has finished it)."""
+
+    def __init__(self, kind, frames, M, D, frame_ll=False, occ=False):
+        self.frames = np.ascontiguousarray(frames, np.float32)
+        assert self.frames.ndim == 2 and self.frames.shape[1] == D
+        self.kind = kind
+        r = Request()
+        r.kind = kind
+        r.frames = self.frames.ctypes.data
+        r.n_frames = self.frames.shape[0]
+        self.frame_ll = self.means = self.occ = self.W = None
+        if frame_ll:
+            self.frame_ll = np.zeros(self.frames.shape[0], np.float32)
+            r.frame_ll = self.frame_ll.ctypes.data
+        if kind == REQ_MAP:
+            self.means = np.zeros((M, D), np.float32)
+            r.means = self.means.ctypes.data
+        if kind == REQ_MLLR:
+            self.W = np.zeros((D, D + 1))
+            r.mllr_W = self.W.ctypes.data
+        if occ and kind != REQ_SCORE:
+            self.occ = np.zeros(M)
+            r.occ = self.occ.ctypes.data
+        self.req = r
+
+    @property
+    def status(self):
+        return self.req.status
+
+    @property
+    def ll_sum(self):
+        return self.req.ll_sum
+
+    @property
+    def latency_ms(self):
+        return self.req.latency_ms
+
+
+class Server:
+    """Continuous-batching request loop over a resident UBM
+    (gmmhip_server_* of include/gmmhip.h)."""
+
+    def __init__(self, w, mu, var, device=0, slots=2, slot_bytes=0, max_batch=0,
+                 chunk_frames=0, tau=16, mllr_prior=False, **opts):
+        w = np.ascontiguousarray(w, np.float32)
+        mu = np.ascontiguousarray(mu, np.float32)
+        var = np.ascontiguousarray(var, np.float32)
+        self.M, self.D = mu.shape
+        cfg = ServerCfg()
+        cfg.opts = default_opts(**opts)
+        cfg.slots, cfg.slot_bytes, cfg.max_batch = slots, slot_bytes, max_batch
+        cfg.chunk_frames, cfg.tau, cfg.mllr_prior = chunk_frames, tau, int(mllr_prior)
+        h = ctypes.c_void_p()
+        rc = lib().gmmhip_server_create(device, ctypes.byref(cfg), self.M, self.D, _fp(w),
+                                        _fp(mu), _fp(var), ctypes.byref(h))
+        if rc:
+            raise GmmHipError(rc, "gmmhip_server_create failed")
+        self._h = h
+
+    def submit(self, kind, frames, frame_ll=False, occ=False):
+        p = PendingRequest(kind, frames, self.M, self.D, frame_ll, occ)
+        rc = lib().gmmhip_server_submit(self._h, ctypes.byref(p.req))
+        if rc:
+            raise GmmHipError(rc, "gmmhip_server_submit failed")
+        return p
+
+    def wait(self, p, timeout_ms=-1.0):
+        return lib().gmmhip_server_wait(self._h, ctypes.byref(p.req), timeout_ms)
+
+    def drain(self):
+        lib().gmmhip_server_drain(self._h)
+
+    def stats(self, reset=False):
+        st = ServerStats()
+        lib().gmmhip_server_stats_get(self._h, ctypes.byref(st), int(reset))
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}
+
+    def last_error(self):
+        return lib().gmmhip_server_last_error(self._h).decode()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gmmhip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

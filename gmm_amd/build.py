@@ -24,10 +24,10 @@ COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
           "-Wno-unused-result", "-Wno-unused-value"]
 
 HIP_SOURCES = ["estep_fp32.hip", "estep_mixed.hip", "batch.hip", "gmmhip_api.hip",
-               "batch_api.hip"]
+               "batch_api.hip", "server.hip"]
 HOST_SOURCES = ["host/gmm_model.cpp", "host/model_adapt.cpp", "host/feature_io.cpp",
                 "host/gmmhost_api.cpp"]
-TOOLS = ["trainGMM", "getLogLike"]
+TOOLS = ["trainGMM", "getLogLike", "serveLoad"]
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
 
@@ -59,7 +59,7 @@ def _compile_host(src):
 def _link_tool(name):
     exe = os.path.join(OUT, name)
     _run([CXX] + HOST_FLAGS + [os.path.join(CSRC, "tools", name + ".cpp"), "-o", exe + ".tmp",
-                               "-L", OUT, "-lgmmhip", "-Wl,-rpath,$ORIGIN",
+                               "-L", OUT, "-lgmmhip", "-pthread", "-Wl,-rpath,$ORIGIN",
                                "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(exe + ".tmp", exe)
     return exe
@@ -73,7 +73,7 @@ def build_lib(jobs=8):
         objs = [f.result() for f in hip] + [f.result() for f in host]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-Wl,-rpath,/opt/rocm/lib"])
+         ["-pthread", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     for t in TOOLS:
         _link_tool(t)

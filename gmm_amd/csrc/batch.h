@@ -36,5 +36,9 @@ hipError_t launch_mllr_check(const double *n, int U, int M, int D, int *status, 
 hipError_t launch_gemm_f64(const double *A, int lda, const double *B, int ldb, double *C, int ldc,
                            int R, int Cn, int K, hipStream_t s);
 hipError_t launch_mllr_solve(const MllrSolveArgs &a, hipStream_t s);
+// out[u] = sum of v[offsets[u] .. offsets[u+1]) in FP64; the summation order
+// is fixed relative to offsets[u] (one block per segment)
+hipError_t launch_seg_sum(const float *v, const uint64_t *offsets, int U, double *out,
+                          hipStream_t s);
 
 }  // namespace gmmhip

@@ -282,6 +282,33 @@ __global__ __launch_bounds__(64) void mllr_solve_kernel(MllrSolveArgs a) {
     if (bad && lane == 0) atomicOr(&a.status[u], 8);
 }
 
+// ---------------------------------------------------------------------------
+// Per-segment FP64 sums: thread t of block u adds frames t, t+256, ... of
+// segment u in order, then a fixed tree combines the 256 partial sums.
+__global__ __launch_bounds__(256) void seg_sum_kernel(const float *__restrict__ v,
+                                                      const uint64_t *__restrict__ offsets,
+                                                      double *__restrict__ out) {
+    __shared__ double red[256];
+    const int u = blockIdx.x;
+    const uint64_t b = offsets[u], e = offsets[u + 1];
+    double s = 0.0;
+    for (uint64_t i = b + threadIdx.x; i < e; i += 256) s += (double)v[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[u] = red[0];
+}
+
+hipError_t launch_seg_sum(const float *v, const uint64_t *offsets, int U, double *out,
+                          hipStream_t s) {
+    if (U <= 0) return hipSuccess;
+    hipLaunchKernelGGL(seg_sum_kernel, dim3(U), dim3(256), 0, s, v, offsets, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_seg_reduce(const SegReduceArgs &a, hipStream_t s) {
     const uint64_t total = (uint64_t)a.U * a.M * (a.D + 1);
     hipLaunchKernelGGL(seg_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,

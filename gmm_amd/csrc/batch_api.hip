@@ -103,6 +103,43 @@ int build_mllr_operands(gmmhip_ctx *h, gmmhip_batch_state *b) {
     return GMMHIP_OK;
 }
 
+}  // namespace
+
+namespace gmmhip_detail {
+
+int model_operands(gmmhip_ctx *h, bool mllr, ModelOperands *o) {
+    DevGuard g(h->dev);
+    if (!h->batch) h->batch = new gmmhip_batch_state();
+    gmmhip_batch_state *b = h->batch;
+    const size_t MD = (size_t)h->M * h->D;
+    if (!h->d_mu_f32 || h->mu_f32_version != h->model_version) {
+        HIPCHK(h, ensure(&h->d_mu_f32, &h->mu_f32_cap, MD));
+        HIPCHK(h, ensure(&h->d_var_f32, &h->var_f32_cap, MD));
+        HIPCHK(h, hipMemcpyAsync(h->d_mu_f32, h->m_mu.data(), MD * 4, hipMemcpyHostToDevice,
+                                 h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->d_var_f32, h->m_var.data(), MD * 4, hipMemcpyHostToDevice,
+                                 h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        h->mu_f32_version = h->model_version;
+    }
+    o->mu = h->d_mu_f32;
+    o->var = h->d_var_f32;
+    o->P = o->Xi = o->Ginit = o->kinit = nullptr;
+    if (mllr) {
+        const int rc = build_mllr_operands(h, b);
+        if (rc) return rc;
+        o->P = b->d_P;
+        o->Xi = b->d_Xi;
+        o->Ginit = b->d_Ginit;
+        o->kinit = b->d_kinit;
+    }
+    return GMMHIP_OK;
+}
+
+}  // namespace gmmhip_detail
+
+namespace {
+
 struct BatchPlan {
     int U = 0, M = 0, D = 0, KP = 0, Mt = 0, ngroups = 0, nchunks = 0;
     uint64_t N = 0, cf = 0;
@@ -277,33 +314,18 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
     if (!b->d_queue) HIPCHK(h, hipMalloc((void **)&b->d_queue, 256 * sizeof(unsigned int)));
     HIPCHK(h, ensure(&b->d_n, &b->n_cap, (size_t)U * M));
     HIPCHK(h, ensure(&b->d_F, &b->F_cap, (size_t)U * M * D));
-    if (p.want_means) {
-        HIPCHK(h, ensure(&b->d_means, &b->means_cap, (size_t)U * M * D));
-        if (!h->d_mu_f32 || h->mu_f32_version != h->model_version) {
-            HIPCHK(h, ensure(&h->d_mu_f32, &h->mu_f32_cap, M * D));
-            HIPCHK(h, hipMemcpyAsync(h->d_mu_f32, h->m_mu.data(), M * D * 4, hipMemcpyHostToDevice,
-                                     h->stream));
-        }
+    if (p.want_means || p.want_mllr) {
+        ModelOperands mo;
+        if ((rc = model_operands(h, p.want_mllr, &mo))) return rc;
     }
+    if (p.want_means) HIPCHK(h, ensure(&b->d_means, &b->means_cap, (size_t)U * M * D));
     if (p.want_mllr) {
-        HIPCHK(h, ensure(&h->d_var_f32, &h->var_f32_cap, M * D));
-        HIPCHK(h, hipMemcpyAsync(h->d_var_f32, h->m_var.data(), M * D * 4, hipMemcpyHostToDevice,
-                                 h->stream));
-        if ((rc = build_mllr_operands(h, b))) return rc;
         HIPCHK(h, ensure(&b->d_A1, &b->A1_cap, (size_t)U * D * M));
         HIPCHK(h, ensure(&b->d_A2, &b->A2_cap, (size_t)U * D * M));
         HIPCHK(h, ensure(&b->d_G, &b->G_cap, (size_t)U * D * NP));
         HIPCHK(h, ensure(&b->d_K, &b->K_cap, (size_t)U * D * E));
         HIPCHK(h, ensure(&b->d_W, &b->W_cap, (size_t)U * D * E));
         HIPCHK(h, ensure(&b->d_status, &b->status_cap, (size_t)U));
-    }
-    if (p.want_means || p.want_mllr) {
-        if (!h->d_mu_f32 || h->mu_f32_version != h->model_version) {
-            HIPCHK(h, ensure(&h->d_mu_f32, &h->mu_f32_cap, M * D));
-            HIPCHK(h, hipMemcpyAsync(h->d_mu_f32, h->m_mu.data(), M * D * 4, hipMemcpyHostToDevice,
-                                     h->stream));
-        }
-        h->mu_f32_version = h->model_version;
     }
     HIPCHK(h, hipMemcpyAsync(b->d_lo, p.lo.data(), p.lo.size() * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(b->d_hi, p.hi.data(), p.hi.size() * 8, hipMemcpyHostToDevice, h->stream));

@@ -1,0 +1,636 @@
+// gmmhip_server_*: a continuous-batching request loop over one resident UBM
+// (include/gmmhip.h).
+//
+// Threads: callers submit into a FIFO; one dispatcher thread forms a batch
+// whenever a slot is free (admission: FIFO order, while the batch's device
+// layout fits the slot's budget and request cap), stages it, enqueues its
+// device pipeline on the slot's stream and hands the slot to that slot's
+// completer thread, which waits for the slot's event, scatters the results
+// into the callers' buffers and frees the slot.  With several slots, batch
+// k+1 is staged and launched while batch k runs.
+//
+// Device pipeline of one batch (requests ordered MAP, MLLR, SCORE):
+//   one H2D copy of [frame offsets | chunk tables | frames],
+//   loglik_kernel over all frames (per-frame LL, LSE in registers),
+//   stats_queue_kernel over the chunks of the MAP and MLLR requests only,
+//   seg_reduce (n, F per request), seg_sum (LL per request),
+//   map_means (MAP requests), mllr_check / mllr_operands / two FP64 GEMMs /
+//   mllr_solve (MLLR requests),
+//   one D2H copy of [LL | MLLR status | n | W | means | per-frame LL].
+// Chunks start at each request's first frame and every per-request sum runs
+// in a fixed order, so results do not depend on the batch-mates.
+#include "batch.h"
+#include "ctx.h"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+
+using namespace gmmhip_detail;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+inline size_t al(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+
+struct Pending {
+    gmmhip_request *r;
+    Clock::time_point t0;
+};
+
+// Sizes of one batch.
+struct Shape {
+    uint64_t N = 0;
+    int U = 0, Umap = 0, Umllr = 0, nchunks = 0;
+    bool frame_ll = false;
+};
+
+// Byte offsets in the slot arena.  IN [offs, lo, hi, first, X] and OUT
+// [ll, status, n, W, means, LLnat] are contiguous so that each moves in one
+// copy; the pinned host staging uses the same offsets.
+struct Layout {
+    size_t offs, lo, hi, first, X, in_end;
+    size_t ll, status, n, W, means, LLnat, out_end, out_copy;
+    size_t queue, LL2, blockLL, part, F, A1, A2, G, K, total;
+};
+
+struct Dims {
+    int M, D, KP, Mp, E, NP;
+};
+
+Layout layout_of(const Shape &s, const Dims &d) {
+    Layout L{};
+    const int Us = s.Umap + s.Umllr;
+    size_t p = 0;
+    auto take = [&p](size_t bytes) {
+        const size_t at = p;
+        p += al(bytes);
+        return at;
+    };
+    L.offs = take((size_t)(s.U + 1) * 8);
+    L.lo = take((size_t)s.nchunks * 8);
+    L.hi = take((size_t)s.nchunks * 8);
+    L.first = take((size_t)(Us + 1) * 4);
+    L.X = take(s.N * d.D * 4);
+    L.in_end = p;
+    L.ll = take((size_t)s.U * 8);
+    L.status = take((size_t)s.Umllr * 4);
+    L.n = take((size_t)Us * d.M * 8);
+    L.W = take((size_t)s.Umllr * d.D * d.E * 8);
+    L.means = take((size_t)s.Umap * d.M * d.D * 4);
+    L.LLnat = take(s.N * 4);
+    L.out_end = p;
+    L.out_copy = (s.frame_ll ? L.out_end : L.LLnat) - L.ll;
+    L.queue = take(4);
+    L.LL2 = take(s.N * 4);
+    L.blockLL = take(ll_blocks(s.N) * 8);
+    L.part = take((size_t)s.nchunks * d.Mp * d.KP * 4);
+    L.F = take((size_t)Us * d.M * d.D * 8);
+    L.A1 = take((size_t)s.Umllr * d.D * d.M * 8);
+    L.A2 = take((size_t)s.Umllr * d.D * d.M * 8);
+    L.G = take((size_t)s.Umllr * d.D * d.NP * 8);
+    L.K = take((size_t)s.Umllr * d.D * d.E * 8);
+    L.total = p;
+    return L;
+}
+
+int kind_rank(int k) { return k == GMMHIP_REQ_MAP ? 0 : (k == GMMHIP_REQ_MLLR ? 1 : 2); }
+
+// FREE -> STAGING (dispatcher stages and launches) -> RUNNING (completer waits
+// and scatters) -> FREE; STOP ends the completer.
+enum SlotState { SLOT_FREE, SLOT_STAGING, SLOT_RUNNING, SLOT_STOP };
+
+}  // namespace
+
+struct gmmhip_server {
+    gmmhip_t base = nullptr;  // owns the packed model
+    int dev = 0;
+    gmmhip_server_cfg cfg{};
+    Dims d{};
+    uint64_t cf = 512;
+    int num_cus = 256;
+    ModelOperands mo{};
+
+    struct Slot {
+        int id = 0;
+        hipStream_t stream = nullptr;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        char *arena = nullptr;
+        char *h_in = nullptr, *h_out = nullptr;
+        size_t h_in_cap = 0, h_out_cap = 0;
+        SlotState state = SLOT_FREE;
+        std::vector<Pending> batch;
+        std::vector<uint64_t> offs;  // per request frame offset in the batch
+        Shape shape;
+        Layout lay{};
+        int err = 0;
+        std::thread completer;
+    };
+    std::vector<Slot> slots;
+
+    std::mutex mu;
+    std::condition_variable cv_dispatch, cv_slot, cv_done;
+    std::deque<Pending> queue;
+    bool closing = false;
+    std::thread dispatcher;
+    std::string err;
+
+    // statistics
+    uint64_t submitted = 0, completed = 0, failed = 0, batches = 0, frames = 0;
+    uint64_t batch_requests = 0;
+    double batch_gpu_ms = 0.0;
+    std::vector<double> lat;
+    Clock::time_point t_reset;
+};
+
+namespace {
+
+void set_err(gmmhip_server *s, const std::string &e) {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->err = e;
+}
+
+// Finish requests with a status (server mutex held by the caller).
+void finish_locked(gmmhip_server *s, gmmhip_request *r, int status, Clock::time_point t0) {
+    r->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    __atomic_store_n(&r->status, status, __ATOMIC_RELEASE);
+    if (status == GMMHIP_OK || status == GMMHIP_ERR_MLLR) {
+        s->completed++;
+        s->lat.push_back(r->latency_ms);
+    } else {
+        s->failed++;
+    }
+}
+
+#define SRV_HIPCHK(expr)                                                                  \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            set_err(s, std::string(#expr " failed: ") + hipGetErrorString(e_));           \
+            return e_ == hipErrorOutOfMemory ? GMMHIP_ERR_OOM : GMMHIP_ERR_HIP;           \
+        }                                                                                 \
+    } while (0)
+
+int grow_pinned(gmmhip_server *s, char **p, size_t *cap, size_t need) {
+    if (*cap >= need) return GMMHIP_OK;
+    const size_t n = std::max(need, *cap * 3 / 2);
+    if (*p) SRV_HIPCHK(hipHostFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    SRV_HIPCHK(hipHostMalloc((void **)p, n, hipHostMallocDefault));
+    *cap = n;
+    return GMMHIP_OK;
+}
+
+// Stage and enqueue one batch on its slot (dispatcher thread).
+int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
+    const Dims &d = s->d;
+    std::stable_sort(sl.batch.begin(), sl.batch.end(), [](const Pending &a, const Pending &b) {
+        return kind_rank(a.r->kind) < kind_rank(b.r->kind);
+    });
+    Shape sh;
+    sh.U = (int)sl.batch.size();
+    sl.offs.assign(sh.U + 1, 0);
+    std::vector<uint64_t> lo, hi;
+    std::vector<int> first;
+    for (int u = 0; u < sh.U; u++) {
+        const gmmhip_request *r = sl.batch[u].r;
+        sl.offs[u + 1] = sl.offs[u] + r->n_frames;
+        if (r->frame_ll) sh.frame_ll = true;
+        if (r->kind == GMMHIP_REQ_SCORE) continue;
+        if (r->kind == GMMHIP_REQ_MAP) sh.Umap++;
+        else sh.Umllr++;
+        first.push_back((int)lo.size());
+        for (uint64_t f = sl.offs[u]; f < sl.offs[u + 1]; f += s->cf) {
+            lo.push_back(f);
+            hi.push_back(std::min(sl.offs[u + 1], f + s->cf));
+        }
+    }
+    first.push_back((int)lo.size());
+    sh.N = sl.offs[sh.U];
+    sh.nchunks = (int)lo.size();
+    const Layout L = layout_of(sh, d);
+    sl.shape = sh;
+    sl.lay = L;
+    int rc;
+    if ((rc = grow_pinned(s, &sl.h_in, &sl.h_in_cap, L.in_end))) return rc;
+    if ((rc = grow_pinned(s, &sl.h_out, &sl.h_out_cap, L.out_copy))) return rc;
+
+    // IN staging
+    std::memcpy(sl.h_in + L.offs, sl.offs.data(), sl.offs.size() * 8);
+    if (sh.nchunks) {
+        std::memcpy(sl.h_in + L.lo, lo.data(), lo.size() * 8);
+        std::memcpy(sl.h_in + L.hi, hi.data(), hi.size() * 8);
+    }
+    std::memcpy(sl.h_in + L.first, first.data(), first.size() * 4);
+    float *hx = (float *)(sl.h_in + L.X);
+    for (int u = 0; u < sh.U; u++) {
+        const gmmhip_request *r = sl.batch[u].r;
+        std::memcpy(hx + sl.offs[u] * d.D, r->frames, r->n_frames * d.D * 4);
+    }
+
+    char *A = sl.arena;
+    hipStream_t st = sl.stream;
+    SRV_HIPCHK(hipEventRecord(sl.ev0, st));
+    SRV_HIPCHK(hipMemcpyAsync(A, sl.h_in, L.in_end, hipMemcpyHostToDevice, st));
+
+    gmmhip_ctx *b = s->base;
+    const int Us = sh.Umap + sh.Umllr;
+    EstepArgs a{};
+    a.X = (const float *)(A + L.X);
+    a.ldx = d.D;
+    a.N = sh.N;
+    a.D = d.D;
+    a.KP = d.KP;
+    a.P = b->d_P;
+    a.M = d.M;
+    a.Mt = b->Mt;
+    a.off = b->d_off;
+    a.min_ll = b->opt.min_loglike;
+    a.min_gamma = b->opt.min_gamma;
+    a.LL2 = (float *)(A + L.LL2);
+    a.LLnat = (float *)(A + L.LLnat);
+    a.blockLL = (double *)(A + L.blockLL);
+    a.ngroups = (b->Mt + 3) / 4;
+    a.partials = (float *)(A + L.part);
+    a.chunk_frames = s->cf;
+    SRV_HIPCHK(launch_loglik_fp32(a, st));
+    if (Us > 0) {
+        a.nchunks = sh.nchunks;
+        a.chunk_lo = (const uint64_t *)(A + L.lo);
+        a.chunk_hi = (const uint64_t *)(A + L.hi);
+        a.queue = (unsigned int *)(A + L.queue);
+        a.queue_blocks = std::min(sh.nchunks * a.ngroups, 2 * s->num_cus);
+        SRV_HIPCHK(launch_fill_u32(a.queue, 0u, 1, st));
+        SRV_HIPCHK(launch_stats_fp32_queue(a, st));
+        SegReduceArgs r{};
+        r.partials = a.partials;
+        r.chunk_first = (const int *)(A + L.first);
+        r.off = b->d_off;
+        r.U = Us;
+        r.M = d.M;
+        r.Mp = d.Mp;
+        r.D = d.D;
+        r.KP = d.KP;
+        r.n = (double *)(A + L.n);
+        r.F = (double *)(A + L.F);
+        SRV_HIPCHK(launch_seg_reduce(r, st));
+    }
+    SRV_HIPCHK(launch_seg_sum(a.LLnat, (const uint64_t *)(A + L.offs), sh.U, (double *)(A + L.ll),
+                              st));
+    if (sh.Umap > 0)
+        SRV_HIPCHK(launch_map_means((const double *)(A + L.n), (const double *)(A + L.F), s->mo.mu,
+                                    sh.Umap, d.M, d.D, s->cfg.tau, (float *)(A + L.means), st));
+    if (sh.Umllr > 0) {
+        const double *n = (const double *)(A + L.n) + (size_t)sh.Umap * d.M;
+        const double *F = (const double *)(A + L.F) + (size_t)sh.Umap * d.M * d.D;
+        int *status = (int *)(A + L.status);
+        double *A1 = (double *)(A + L.A1), *A2 = (double *)(A + L.A2);
+        double *G = (double *)(A + L.G), *K = (double *)(A + L.K);
+        const int R = sh.Umllr * d.D;
+        SRV_HIPCHK(launch_mllr_check(n, sh.Umllr, d.M, d.D, status, st));
+        SRV_HIPCHK(launch_mllr_operands(n, F, s->mo.var, sh.Umllr, d.M, d.D, d.M, A1, A2, st));
+        SRV_HIPCHK(launch_gemm_f64(A1, d.M, s->mo.P, d.NP, G, d.NP, R, d.NP, d.M, st));
+        SRV_HIPCHK(launch_gemm_f64(A2, d.M, s->mo.Xi, d.E, K, d.E, R, d.E, d.M, st));
+        MllrSolveArgs m{};
+        m.Gp = G;
+        m.Kp = K;
+        m.Ginit = s->cfg.mllr_prior ? s->mo.Ginit : nullptr;
+        m.kinit = s->cfg.mllr_prior ? s->mo.kinit : nullptr;
+        m.U = sh.Umllr;
+        m.D = d.D;
+        m.ldg = d.NP;
+        m.ldk = d.E;
+        m.status = status;
+        m.W = (double *)(A + L.W);
+        SRV_HIPCHK(launch_mllr_solve(m, st));
+    }
+    SRV_HIPCHK(hipMemcpyAsync(sl.h_out, A + L.ll, L.out_copy, hipMemcpyDeviceToHost, st));
+    SRV_HIPCHK(hipEventRecord(sl.ev1, st));
+    return GMMHIP_OK;
+}
+
+// Copy the results of a finished batch to the callers (completer thread).
+void scatter(gmmhip_server *s, gmmhip_server::Slot &sl) {
+    const Dims &d = s->d;
+    const Layout &L = sl.lay;
+    // the staging holds the device OUT region from L.ll on
+    auto at = [&](size_t off) { return (const void *)(sl.h_out + (off - L.ll)); };
+    const double *ll = (const double *)at(L.ll);
+    const double *n = (const double *)at(L.n);
+    const double *W = (const double *)at(L.W);
+    const float *means = (const float *)at(L.means);
+    const float *fll = (const float *)at(L.LLnat);
+    const int Umap = sl.shape.Umap;
+    for (int u = 0; u < sl.shape.U; u++) {
+        gmmhip_request *r = sl.batch[u].r;
+        r->ll_sum = ll[u];
+        if (r->frame_ll) std::memcpy(r->frame_ll, fll + sl.offs[u], r->n_frames * 4);
+        if (r->kind == GMMHIP_REQ_SCORE) continue;
+        if (r->occ) std::memcpy(r->occ, n + (size_t)u * d.M, (size_t)d.M * 8);
+        if (r->kind == GMMHIP_REQ_MAP) {
+            std::memcpy(r->means, means + (size_t)u * d.M * d.D, (size_t)d.M * d.D * 4);
+        } else {
+            const int j = u - Umap;
+            std::memcpy(r->mllr_W, W + (size_t)j * d.D * d.E, (size_t)d.D * d.E * 8);
+        }
+    }
+}
+
+void completer_main(gmmhip_server *s, int id) {
+    hipSetDevice(s->dev);
+    gmmhip_server::Slot &sl = s->slots[id];
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_slot.wait(lk, [&] { return sl.state == SLOT_RUNNING || sl.state == SLOT_STOP; });
+            if (sl.state == SLOT_STOP) return;
+        }
+        int st = sl.err;
+        float ms = 0.0f;
+        if (!st) {
+            const hipError_t e = hipEventSynchronize(sl.ev1);
+            if (e != hipSuccess) {
+                set_err(s, std::string("batch failed: ") + hipGetErrorString(e));
+                st = GMMHIP_ERR_HIP;
+            } else {
+                hipEventElapsedTime(&ms, sl.ev0, sl.ev1);
+                scatter(s, sl);
+            }
+        }
+        const Layout &L = sl.lay;
+        const int *status = (const int *)(sl.h_out + (L.status - L.ll));
+        std::lock_guard<std::mutex> g(s->mu);
+        for (int u = 0; u < (int)sl.batch.size(); u++) {
+            gmmhip_request *r = sl.batch[u].r;
+            int rs = st;
+            if (!rs && r->kind == GMMHIP_REQ_MLLR && status[u - sl.shape.Umap] != 0)
+                rs = GMMHIP_ERR_MLLR;
+            finish_locked(s, r, rs, sl.batch[u].t0);
+        }
+        if (!st) {
+            s->batches++;
+            s->batch_requests += sl.batch.size();
+            s->frames += sl.shape.N;
+            s->batch_gpu_ms += ms;
+        }
+        sl.batch.clear();
+        sl.err = 0;
+        sl.state = SLOT_FREE;
+        s->cv_done.notify_all();
+        s->cv_dispatch.notify_all();
+    }
+}
+
+void dispatcher_main(gmmhip_server *s) {
+    hipSetDevice(s->dev);
+    const uint64_t cap = s->cfg.slot_bytes;
+    for (;;) {
+        int id = -1;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_dispatch.wait(lk, [&] {
+                if (s->queue.empty()) return s->closing;
+                for (auto &sl : s->slots)
+                    if (sl.state == SLOT_FREE) return true;
+                return false;
+            });
+            if (s->queue.empty() && s->closing) {
+                // wait for running batches, then stop the completers
+                s->cv_dispatch.wait(lk, [&] {
+                    for (auto &sl : s->slots)
+                        if (sl.state != SLOT_FREE) return false;
+                    return true;
+                });
+                for (auto &sl : s->slots) sl.state = SLOT_STOP;
+                s->cv_slot.notify_all();
+                return;
+            }
+            for (auto &sl : s->slots)
+                if (sl.state == SLOT_FREE) {
+                    id = sl.id;
+                    break;
+                }
+            // admission: FIFO while the batch layout fits the slot budget
+            gmmhip_server::Slot &sl = s->slots[id];
+            Shape sh;
+            while (!s->queue.empty() && (int)sl.batch.size() < s->cfg.max_batch) {
+                const Pending &p = s->queue.front();
+                Shape t = sh;
+                t.U++;
+                t.N += p.r->n_frames;
+                if (p.r->kind != GMMHIP_REQ_SCORE) {
+                    (p.r->kind == GMMHIP_REQ_MAP ? t.Umap : t.Umllr)++;
+                    t.nchunks += (int)((p.r->n_frames + s->cf - 1) / s->cf);
+                }
+                t.frame_ll = t.frame_ll || p.r->frame_ll;
+                if (layout_of(t, s->d).total > cap) {
+                    if (sl.batch.empty()) {
+                        // can never fit
+                        finish_locked(s, p.r, GMMHIP_ERR_OOM, p.t0);
+                        s->err = "request larger than the slot budget";
+                        s->queue.pop_front();
+                        s->cv_done.notify_all();
+                        continue;
+                    }
+                    break;
+                }
+                sh = t;
+                sl.batch.push_back(p);
+                s->queue.pop_front();
+            }
+            if (sl.batch.empty()) continue;
+            sl.state = SLOT_STAGING;  // owned by this thread until handed over
+        }
+        gmmhip_server::Slot &sl = s->slots[id];
+        sl.err = launch_batch(s, sl);
+        if (sl.err) {
+            // let the stream drain before the completer fails the batch
+            hipStreamSynchronize(sl.stream);
+        }
+        std::lock_guard<std::mutex> g(s->mu);
+        sl.state = SLOT_RUNNING;
+        s->cv_slot.notify_all();
+    }
+}
+
+void free_server(gmmhip_server *s) {
+    for (auto &sl : s->slots) {
+        if (sl.arena) hipFree(sl.arena);
+        if (sl.h_in) hipHostFree(sl.h_in);
+        if (sl.h_out) hipHostFree(sl.h_out);
+        if (sl.ev0) hipEventDestroy(sl.ev0);
+        if (sl.ev1) hipEventDestroy(sl.ev1);
+        if (sl.stream) hipStreamDestroy(sl.stream);
+    }
+    if (s->base) gmmhip_destroy(s->base);
+    delete s;
+}
+
+}  // namespace
+
+extern "C" int gmmhip_server_create(int device, const gmmhip_server_cfg *cfg, int M, int D,
+                                    const float *w, const float *mu, const float *var,
+                                    gmmhip_server_t *out) {
+    if (!out) return GMMHIP_ERR_ARG;
+    *out = nullptr;
+    gmmhip_server_cfg c{};
+    if (cfg) c = *cfg;
+    else gmmhip_default_opts(&c.opts);
+    if (c.opts.precision != GMMHIP_PREC_FP32) return GMMHIP_ERR_ARG;
+    if (c.slots <= 0) c.slots = 2;
+    if (c.slots > 8) return GMMHIP_ERR_ARG;
+    if (c.slot_bytes == 0) c.slot_bytes = 1ull << 30;
+    if (c.max_batch <= 0) c.max_batch = 1024;
+    if (c.chunk_frames < 0 || c.tau < 0) return GMMHIP_ERR_ARG;
+    gmmhip_server *s = new gmmhip_server();
+    s->cfg = c;
+    s->dev = device;
+    s->cf = c.chunk_frames > 0 ? (uint64_t)(c.chunk_frames + 63) / 64 * 64 : 512;
+    int rc = gmmhip_create(device, &c.opts, &s->base);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    if ((rc = gmmhip_set_model(s->base, M, D, w, mu, var))) {
+        free_server(s);
+        return rc;
+    }
+    gmmhip_ctx *b = s->base;
+    DevGuard g(device);
+    if ((rc = model_operands(b, true, &s->mo))) {
+        free_server(s);
+        return rc;
+    }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        ncu > 0)
+        s->num_cus = ncu;
+    s->d.M = M;
+    s->d.D = D;
+    s->d.KP = b->KP;
+    s->d.Mp = b->Mt * 16;
+    s->d.E = D + 1;
+    s->d.NP = (D + 1) * (D + 2) / 2;
+    s->slots.resize(c.slots);
+    for (int i = 0; i < c.slots; i++) {
+        auto &sl = s->slots[i];
+        sl.id = i;
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&sl.ev0) != hipSuccess || hipEventCreate(&sl.ev1) != hipSuccess) {
+            free_server(s);
+            return GMMHIP_ERR_HIP;
+        }
+        if (hipMalloc((void **)&sl.arena, c.slot_bytes) != hipSuccess) {
+            free_server(s);
+            return GMMHIP_ERR_OOM;
+        }
+        if (grow_pinned(s, &sl.h_in, &sl.h_in_cap, 16u << 20) ||
+            grow_pinned(s, &sl.h_out, &sl.h_out_cap, 16u << 20)) {
+            free_server(s);
+            return GMMHIP_ERR_OOM;
+        }
+    }
+    s->t_reset = Clock::now();
+    for (int i = 0; i < c.slots; i++) s->slots[i].completer = std::thread(completer_main, s, i);
+    s->dispatcher = std::thread(dispatcher_main, s);
+    *out = s;
+    return GMMHIP_OK;
+}
+
+extern "C" int gmmhip_server_submit(gmmhip_server_t s, gmmhip_request *r) {
+    if (!s || !r) return GMMHIP_ERR_ARG;
+    if (!r->frames || r->n_frames == 0) return GMMHIP_ERR_ARG;
+    if (r->kind == GMMHIP_REQ_MAP && !r->means) return GMMHIP_ERR_ARG;
+    if (r->kind == GMMHIP_REQ_MLLR && !r->mllr_W) return GMMHIP_ERR_ARG;
+    if (r->kind < GMMHIP_REQ_SCORE || r->kind > GMMHIP_REQ_MLLR) return GMMHIP_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->closing) return GMMHIP_ERR_CLOSED;
+    r->ll_sum = 0.0;
+    r->latency_ms = 0.0;
+    __atomic_store_n(&r->status, GMMHIP_REQ_PENDING, __ATOMIC_RELEASE);
+    s->queue.push_back(Pending{r, Clock::now()});
+    s->submitted++;
+    s->cv_dispatch.notify_one();
+    return GMMHIP_OK;
+}
+
+extern "C" int gmmhip_server_wait(gmmhip_server_t s, gmmhip_request *r, double timeout_ms) {
+    if (!s || !r) return GMMHIP_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto done = [&] { return __atomic_load_n(&r->status, __ATOMIC_ACQUIRE) != GMMHIP_REQ_PENDING; };
+    if (timeout_ms < 0) {
+        s->cv_done.wait(lk, done);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms),
+                                    done)) {
+        return GMMHIP_ERR_TIMEOUT;
+    }
+    return r->status;
+}
+
+extern "C" int gmmhip_server_drain(gmmhip_server_t s) {
+    if (!s) return GMMHIP_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->completed + s->failed == s->submitted; });
+    return GMMHIP_OK;
+}
+
+extern "C" int gmmhip_server_stats_get(gmmhip_server_t s, gmmhip_server_stats *st, int reset) {
+    if (!s || !st) return GMMHIP_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    *st = gmmhip_server_stats{};
+    st->submitted = s->submitted;
+    st->completed = s->completed;
+    st->failed = s->failed;
+    st->batches = s->batches;
+    st->frames = s->frames;
+    std::vector<double> v = s->lat;
+    std::sort(v.begin(), v.end());
+    auto pct = [&v](double q) {
+        if (v.empty()) return 0.0;
+        const size_t i = std::min(v.size() - 1, (size_t)std::ceil(q * v.size()) - (q > 0 ? 1 : 0));
+        return v[i];
+    };
+    st->p50_ms = pct(0.50);
+    st->p90_ms = pct(0.90);
+    st->p99_ms = pct(0.99);
+    st->p999_ms = pct(0.999);
+    st->max_ms = v.empty() ? 0.0 : v.back();
+    double sum = 0.0;
+    for (double x : v) sum += x;
+    st->mean_ms = v.empty() ? 0.0 : sum / v.size();
+    st->mean_batch_requests = s->batches ? (double)s->batch_requests / s->batches : 0.0;
+    st->mean_batch_gpu_ms = s->batches ? s->batch_gpu_ms / s->batches : 0.0;
+    st->wall_s = std::chrono::duration<double>(Clock::now() - s->t_reset).count();
+    if (reset) {
+        s->lat.clear();
+        s->submitted -= s->completed + s->failed;
+        s->completed = s->failed = s->batches = s->frames = s->batch_requests = 0;
+        s->batch_gpu_ms = 0.0;
+        s->t_reset = Clock::now();
+    }
+    return GMMHIP_OK;
+}
+
+extern "C" const char *gmmhip_server_last_error(gmmhip_server_t s) {
+    if (!s) return "null server";
+    std::lock_guard<std::mutex> g(s->mu);
+    return s->err.c_str();
+}
+
+extern "C" void gmmhip_server_destroy(gmmhip_server_t s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->closing = true;
+        s->cv_dispatch.notify_all();
+    }
+    if (s->dispatcher.joinable()) s->dispatcher.join();
+    for (auto &sl : s->slots)
+        if (sl.completer.joinable()) sl.completer.join();
+    free_server(s);
+}

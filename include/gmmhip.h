@@ -34,7 +34,9 @@ enum {
     GMMHIP_ERR_VAR = 5,      /* model variance below min_var */
     GMMHIP_ERR_STATE = 6,    /* no model / no frames inserted */
     GMMHIP_ERR_MLLR = 7,     /* MLLR under-determined (CheckInvertibility) */
-    GMMHIP_ERR_NODEV = 8     /* no HIP device */
+    GMMHIP_ERR_NODEV = 8,    /* no HIP device */
+    GMMHIP_ERR_TIMEOUT = 9,  /* a wait ran out of time */
+    GMMHIP_ERR_CLOSED = 10   /* the server is shutting down */
 };
 
 /* Arithmetic of the E-step contractions. */
@@ -193,6 +195,90 @@ typedef struct gmmhip_batch_out {
 int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on_device,
                        const uint64_t *offsets, int U, int D, const gmmhip_batch_cfg *cfg,
                        gmmhip_batch_out *out);
+
+/* ---- Serving loop -------------------------------------------------------
+ * A server keeps one UBM resident in HBM and answers requests of three kinds
+ * and any length, submitted from any thread:
+ *   SCORE  the utterance's log-likelihood (sum over frames, natural log) and
+ *          optionally its per-frame values -- the reference's getLogLike,
+ *          projects/GetLogLikeGMM/main.cpp:150-188;
+ *   MAP    MAP-adapted means (relevance factor tau) -- Model_ADAPT with
+ *          --type 1 --mean, projects/Model_ADAPT/Model_ADAPT.cpp:173-240 and
+ *          trainer/OL_ModelAdapt.cpp:993-1106;
+ *   MLLR   the global MLLR transform [A | b] -- ComputeW_MLLR,
+ *          trainer/OL_ModelAdapt.cpp:1417-1581.
+ * A dispatcher thread admits waiting requests (first come, first served)
+ * into a batch while the batch's device memory fits the slot budget, and
+ * launches it on the slot's stream; up to `slots` batches run at once, so
+ * requests arriving while others are on the GPU are admitted as soon as a
+ * slot frees (continuous batching).  A batch runs the persistent work-queue
+ * pipeline of gmmhip_batch_adapt.  Work items never straddle requests and
+ * every per-request reduction has a fixed order, so a request's results are
+ * bit-identical whichever requests share its batch (and equal to
+ * gmmhip_batch_adapt with the same chunk_frames).  The reference has no
+ * serving mode. */
+typedef struct gmmhip_server *gmmhip_server_t;
+
+enum {
+    GMMHIP_REQ_SCORE = 0,
+    GMMHIP_REQ_MAP = 1,
+    GMMHIP_REQ_MLLR = 2
+};
+#define GMMHIP_REQ_PENDING (-1)
+
+typedef struct gmmhip_server_cfg {
+    gmmhip_opts opts;        /* thresholds and precision (FP32 or mixed) */
+    int slots;               /* batches in flight, 0 = 2 */
+    uint64_t slot_bytes;     /* device budget of one batch: frames, per-frame
+                                values, partial sums and outputs; 0 = 1 GiB */
+    int max_batch;           /* requests per batch, 0 = 1024 */
+    int chunk_frames;        /* frames per statistics work item, 0 = 512 */
+    int tau;                 /* MAP relevance factor */
+    int mllr_prior;          /* 1: UBM prior on the MLLR statistics */
+} gmmhip_server_cfg;
+
+/* One request.  The caller owns it and the buffers it points to until the
+ * server has finished it (status != GMMHIP_REQ_PENDING). */
+typedef struct gmmhip_request {
+    int kind;                /* GMMHIP_REQ_* */
+    const float *frames;     /* [n_frames][D] host memory */
+    uint64_t n_frames;
+    float *frame_ll;         /* optional [n_frames] per-frame log-likelihood */
+    float *means;            /* MAP: [M][D], required */
+    double *occ;             /* MAP / MLLR: optional [M] occupancies */
+    double *mllr_W;          /* MLLR: [D][D+1], required */
+    /* written by the server */
+    double ll_sum;           /* sum over frames of the frame log-likelihood */
+    double latency_ms;       /* submission to completion */
+    volatile int status;     /* GMMHIP_REQ_PENDING, then 0 or GMMHIP_ERR_*
+                                (GMMHIP_ERR_MLLR: transform under-determined,
+                                mllr_W holds zeros) */
+} gmmhip_request;
+
+typedef struct gmmhip_server_stats {
+    uint64_t submitted, completed, failed, batches, frames;
+    double p50_ms, p90_ms, p99_ms, p999_ms, max_ms, mean_ms;  /* latency */
+    double mean_batch_requests;
+    double mean_batch_gpu_ms;  /* device time of one batch (event pair) */
+    double wall_s;             /* since creation or the last reset */
+} gmmhip_server_stats;
+
+/* M active components (weights w[M], means mu[M][D], variances var[M][D]). */
+int gmmhip_server_create(int device, const gmmhip_server_cfg *cfg, int M, int D, const float *w,
+                         const float *mu, const float *var, gmmhip_server_t *out);
+/* Queue a request (non-blocking).  Sets r->status = GMMHIP_REQ_PENDING. */
+int gmmhip_server_submit(gmmhip_server_t s, gmmhip_request *r);
+/* Block until r is finished or timeout_ms (< 0: no limit) passes; returns
+ * r->status, or GMMHIP_ERR_TIMEOUT. */
+int gmmhip_server_wait(gmmhip_server_t s, gmmhip_request *r, double timeout_ms);
+/* Block until every submitted request is finished. */
+int gmmhip_server_drain(gmmhip_server_t s);
+/* Latency percentiles over the requests completed since creation or the
+ * last reset (reset = 1 clears after reading). */
+int gmmhip_server_stats_get(gmmhip_server_t s, gmmhip_server_stats *st, int reset);
+const char *gmmhip_server_last_error(gmmhip_server_t s);
+/* Finishes queued and running requests, then frees the server. */
+void gmmhip_server_destroy(gmmhip_server_t s);
 
 /* The handle's HIP stream (hipStream_t as void*), for callers that order
  * their own device work with the handle's. */
