@@ -86,10 +86,11 @@ class Request(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
-                ("submitted", "completed", "failed", "batches", "frames")] + \
+                ("submitted", "completed", "failed", "batches", "frames", "fp32_fallbacks")] + \
                [(n, ctypes.c_double) for n in
                 ("p50_ms", "p90_ms", "p99_ms", "p999_ms", "max_ms", "mean_ms",
-                 "mean_batch_requests", "mean_batch_gpu_ms", "wall_s")]
+                 "mean_batch_requests", "mean_batch_gpu_ms", "wall_s")] + \
+               [("precision", ctypes.c_int)]
 
 
 _lib = None

@@ -10,6 +10,7 @@ struct SegReduceArgs {
     const float *partials;   // [chunks][Mp][KP]
     const int *chunk_first;  // [U+1] first chunk of each utterance
     const float *off;        // [D] centring of the partials
+    const float *scale;      // [D] scaling of the partials (mixed form) or null
     int U, M, Mp, D, KP;
     double *n;               // [U][M]
     double *F;               // [U][M][D]
@@ -36,9 +37,9 @@ hipError_t launch_mllr_check(const double *n, int U, int M, int D, int *status, 
 hipError_t launch_gemm_f64(const double *A, int lda, const double *B, int ldb, double *C, int ldc,
                            int R, int Cn, int K, hipStream_t s);
 hipError_t launch_mllr_solve(const MllrSolveArgs &a, hipStream_t s);
-// out[u] = sum of v[offsets[u] .. offsets[u+1]) in FP64; the summation order
-// is fixed relative to offsets[u] (one block per segment)
-hipError_t launch_seg_sum(const float *v, const uint64_t *offsets, int U, double *out,
-                          hipStream_t s);
+// out[u] = sum of v[begin[u] .. end[u]) in FP64; the summation order is
+// fixed relative to begin[u] (one block per segment)
+hipError_t launch_seg_sum(const float *v, const uint64_t *begin, const uint64_t *end, int U,
+                          double *out, hipStream_t s);
 
 }  // namespace gmmhip

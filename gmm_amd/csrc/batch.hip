@@ -58,8 +58,10 @@ __global__ __launch_bounds__(256) void seg_reduce_kernel(SegReduceArgs a) {
     if (j == 0) {
         a.n[um] = s;
     } else {
-        // x = x' + o: F = F' + n o
-        a.F[um * a.D + (j - 1)] = s + n * (double)a.off[j - 1];
+        // x = x' + o (FP32 form) or x = z / scale + o (mixed form):
+        // F = F' + n o or F = F' / scale + n o
+        const double f = a.scale ? s / (double)a.scale[j - 1] : s;
+        a.F[um * a.D + (j - 1)] = f + n * (double)a.off[j - 1];
     }
 }
 
@@ -286,11 +288,12 @@ __global__ __launch_bounds__(64) void mllr_solve_kernel(MllrSolveArgs a) {
 // Per-segment FP64 sums: thread t of block u adds frames t, t+256, ... of
 // segment u in order, then a fixed tree combines the 256 partial sums.
 __global__ __launch_bounds__(256) void seg_sum_kernel(const float *__restrict__ v,
-                                                      const uint64_t *__restrict__ offsets,
+                                                      const uint64_t *__restrict__ begin,
+                                                      const uint64_t *__restrict__ end,
                                                       double *__restrict__ out) {
     __shared__ double red[256];
     const int u = blockIdx.x;
-    const uint64_t b = offsets[u], e = offsets[u + 1];
+    const uint64_t b = begin[u], e = end[u];
     double s = 0.0;
     for (uint64_t i = b + threadIdx.x; i < e; i += 256) s += (double)v[i];
     red[threadIdx.x] = s;
@@ -302,10 +305,10 @@ __global__ __launch_bounds__(256) void seg_sum_kernel(const float *__restrict__ 
     if (threadIdx.x == 0) out[u] = red[0];
 }
 
-hipError_t launch_seg_sum(const float *v, const uint64_t *offsets, int U, double *out,
-                          hipStream_t s) {
+hipError_t launch_seg_sum(const float *v, const uint64_t *begin, const uint64_t *end, int U,
+                          double *out, hipStream_t s) {
     if (U <= 0) return hipSuccess;
-    hipLaunchKernelGGL(seg_sum_kernel, dim3(U), dim3(256), 0, s, v, offsets, out);
+    hipLaunchKernelGGL(seg_sum_kernel, dim3(U), dim3(256), 0, s, v, begin, end, out);
     return hipGetLastError();
 }
 

@@ -161,4 +161,11 @@ struct ModelOperands {
 };
 int model_operands(gmmhip_ctx *h, bool mllr, ModelOperands *o);
 
+// Split model of the mixed form for the feature map z = (x - zoff) * zscale:
+// [Mt32][KPm/16][hi, lo][64 lanes][8] 16-bit pieces of the log2-unit
+// coefficients [const, nu*omega, -omega/2] (gmmhip_api.hip).  *overflow is
+// set when a coefficient does not fit the 16-bit type.
+std::vector<uint16_t> pack_split_model(const gmmhip_ctx *h, int prec, int KPm, const float *zoff,
+                                       const float *zscale, bool *overflow);
+
 }  // namespace gmmhip_detail

@@ -35,6 +35,10 @@ struct MixedArgs {
     uint64_t chunk_frames;
     int nchunks, ngroups;
     float *partials;       // [nchunks][Mt32*32][KP]
+    // optional explicit chunk frame ranges [chunk_lo[c], chunk_hi[c]); each
+    // chunk_lo must be a multiple of 32 (null: chunk c covers chunk_frames
+    // frames from c * chunk_frames)
+    const uint64_t *chunk_lo, *chunk_hi;
 };
 
 hipError_t launch_colstats(const float *X, uint64_t N, int D, int blocks, uint64_t per_block,

@@ -318,9 +318,15 @@ __global__ __launch_bounds__(256, (KP <= 192 ? 2 : 1)) void stats_mixed_kernel(M
 #pragma unroll
     for (int j = 0; j < CT; j++) acc2[j] = (f32x16){};
 
-    const uint64_t f0 = (uint64_t)chunk * a.chunk_frames;
-    uint64_t f1 = f0 + a.chunk_frames;
-    if (f1 > a.N) f1 = a.N;
+    uint64_t f0, f1;
+    if (a.chunk_lo) {
+        f0 = a.chunk_lo[chunk];
+        f1 = a.chunk_hi[chunk];
+    } else {
+        f0 = (uint64_t)chunk * a.chunk_frames;
+        f1 = f0 + a.chunk_frames;
+        if (f1 > a.N) f1 = a.N;
+    }
 
     const u32x4 *zah = reinterpret_cast<const u32x4 *>(a.za_hi);
     const u32x4 *zal = reinterpret_cast<const u32x4 *>(a.za_lo);

@@ -83,6 +83,46 @@ void split_store(double v, uint16_t *hi, uint16_t *lo, bool *overflow) {
     *lo = __builtin_bit_cast(uint16_t, l);
 }
 
+}  // namespace
+
+namespace gmmhip_detail {
+
+std::vector<uint16_t> pack_split_model(const gmmhip_ctx *h, int prec, int KPm, const float *zoff,
+                                       const float *zscale, bool *overflow) {
+    const int M = h->M, D = h->D, Mt32 = (M + 31) / 32, KS = KPm / 16;
+    std::vector<uint16_t> P((size_t)Mt32 * KS * 2 * 64 * 8, 0);
+    std::vector<double> coef(KPm);
+    const double log2e = 1.4426950408889634;
+    for (int m = 0; m < M; m++) {
+        std::fill(coef.begin(), coef.end(), 0.0);
+        double cq = 0.0;
+        for (int d = 0; d < D; d++) {
+            const double sd = zscale[d];
+            const double nu = sd * ((double)h->m_mu[(size_t)m * D + d] - (double)zoff[d]);
+            const double om = 1.0 / ((double)h->m_var[(size_t)m * D + d] * sd * sd);
+            coef[1 + d] = om * nu;
+            coef[1 + D + d] = -0.5 * om;
+            cq += om * nu * nu;
+        }
+        coef[0] = h->m_gconst[m] - 0.5 * cq;
+        const int t = m / 32, c = m % 32;
+        for (int k = 0; k < KPm; k++) {
+            const int ks = k / 16, hh = (k % 16) / 8, j = k % 8;
+            const size_t ih = ((((size_t)t * KS + ks) * 2 + 0) * 64 + 32 * hh + c) * 8 + j;
+            const size_t il = ((((size_t)t * KS + ks) * 2 + 1) * 64 + 32 * hh + c) * 8 + j;
+            if (prec == GMMHIP_PREC_FP16X3)
+                split_store<_Float16>(coef[k] * log2e, &P[ih], &P[il], overflow);
+            else
+                split_store<__bf16>(coef[k] * log2e, &P[ih], &P[il], overflow);
+        }
+    }
+    return P;
+}
+
+}  // namespace gmmhip_detail
+
+namespace {
+
 // Prepares the features and the split model of the current frames and model
 // for opts.precision.  *use_mixed stays false (the call then runs the FP32
 // form, on the GPU) when the precision is FP32, when a frame value lies
@@ -170,34 +210,9 @@ int prepare_mixed(gmmhip_ctx *h, bool *use_mixed) {
     }
     if (h->z_range_fallback) return GMMHIP_OK;
     if (h->p16_model != h->model_version || h->p16_z != h->z_version || h->p16_prec != prec) {
-        const int M = h->M, Mt32 = (M + 31) / 32, KS = KPm / 16;
-        std::vector<uint16_t> P((size_t)Mt32 * KS * 2 * 64 * 8, 0);
-        std::vector<double> coef(KPm);
-        const double log2e = 1.4426950408889634;
         bool overflow = false;
-        for (int m = 0; m < M; m++) {
-            std::fill(coef.begin(), coef.end(), 0.0);
-            double cq = 0.0;
-            for (int d = 0; d < D; d++) {
-                const double sd = h->h_zscale[d];
-                const double nu = sd * ((double)h->m_mu[(size_t)m * D + d] - (double)h->h_zoff[d]);
-                const double om = 1.0 / ((double)h->m_var[(size_t)m * D + d] * sd * sd);
-                coef[1 + d] = om * nu;
-                coef[1 + D + d] = -0.5 * om;
-                cq += om * nu * nu;
-            }
-            coef[0] = h->m_gconst[m] - 0.5 * cq;
-            const int t = m / 32, c = m % 32;
-            for (int k = 0; k < KPm; k++) {
-                const int ks = k / 16, hh = (k % 16) / 8, j = k % 8;
-                const size_t ih = ((((size_t)t * KS + ks) * 2 + 0) * 64 + 32 * hh + c) * 8 + j;
-                const size_t il = ((((size_t)t * KS + ks) * 2 + 1) * 64 + 32 * hh + c) * 8 + j;
-                if (prec == GMMHIP_PREC_FP16X3)
-                    split_store<_Float16>(coef[k] * log2e, &P[ih], &P[il], &overflow);
-                else
-                    split_store<__bf16>(coef[k] * log2e, &P[ih], &P[il], &overflow);
-            }
-        }
+        const std::vector<uint16_t> P =
+            pack_split_model(h, prec, KPm, h->h_zoff.data(), h->h_zscale.data(), &overflow);
         uint16_t *p16 = static_cast<uint16_t *>(h->d_P16);
         HIPCHK(h, ensure(&p16, &h->P16_cap, P.size()));
         h->d_P16 = p16;

@@ -10,15 +10,21 @@
 // k+1 is staged and launched while batch k runs.
 //
 // Device pipeline of one batch (requests ordered MAP, MLLR, SCORE):
-//   one H2D copy of [frame offsets | chunk tables | frames],
-//   loglik_kernel over all frames (per-frame LL, LSE in registers),
-//   stats_queue_kernel over the chunks of the MAP and MLLR requests only,
+//   one H2D copy of [request frame ranges | chunk tables | frames],
+//   FP32 form:  loglik_kernel over all frames, stats_queue_kernel over the
+//               chunks of the MAP and MLLR requests only;
+//   mixed form: prep_frames (z = (x - o) * s split hi/lo), loglik_mixed,
+//               stats_mixed over the MAP/MLLR chunks;
 //   seg_reduce (n, F per request), seg_sum (LL per request),
 //   map_means (MAP requests), mllr_check / mllr_operands / two FP64 GEMMs /
 //   mllr_solve (MLLR requests),
 //   one D2H copy of [LL | MLLR status | n | W | means | per-frame LL].
-// Chunks start at each request's first frame and every per-request sum runs
-// in a fixed order, so results do not depend on the batch-mates.
+// Chunks start at each request's first frame, the mixed form's centring o
+// and scaling s come from the UBM (not from the batch), and every
+// per-request sum runs in a fixed order, so results do not depend on the
+// batch-mates.  In the FP16 mixed form a request with a frame value outside
+// the FP16 range after centring and scaling (|z| > 239, checked at
+// submission) runs in an FP32 batch instead.
 #include "batch.h"
 #include "ctx.h"
 
@@ -36,42 +42,52 @@ namespace {
 
 constexpr size_t kAlign = 256;
 inline size_t al(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+inline uint64_t pad32(uint64_t n) { return (n + 31) / 32 * 32; }
+inline uint64_t pad128(uint64_t n) { return (n + 127) / 128 * 128; }
 
 struct Pending {
     gmmhip_request *r;
     Clock::time_point t0;
 };
 
-// Sizes of one batch.
+// Sizes of one batch.  N counts frame rows, including the padding that
+// aligns each request of a mixed batch to 32 rows.
 struct Shape {
+    bool mixed = false;
     uint64_t N = 0;
     int U = 0, Umap = 0, Umllr = 0, nchunks = 0;
     bool frame_ll = false;
 };
 
-// Byte offsets in the slot arena.  IN [offs, lo, hi, first, X] and OUT
+struct Dims {
+    int M, D, E, NP;
+    int KP, Mp;    // FP32 form: contraction depth, padded components
+    int KPm, Mpm;  // mixed form
+};
+
+// Byte offsets in the slot arena.  IN [beg, end, lo, hi, first, X] and OUT
 // [ll, status, n, W, means, LLnat] are contiguous so that each moves in one
 // copy; the pinned host staging uses the same offsets.
 struct Layout {
-    size_t offs, lo, hi, first, X, in_end;
+    size_t beg, end, lo, hi, first, X, in_end;
     size_t ll, status, n, W, means, LLnat, out_end, out_copy;
-    size_t queue, LL2, blockLL, part, F, A1, A2, G, K, total;
-};
-
-struct Dims {
-    int M, D, KP, Mp, E, NP;
+    size_t queue, LL2, blockLL, part, F, A1, A2, G, K;
+    size_t zah, zal, zbh, zbl, bad, nrange, total;
 };
 
 Layout layout_of(const Shape &s, const Dims &d) {
     Layout L{};
     const int Us = s.Umap + s.Umllr;
+    const int KP = s.mixed ? d.KPm : d.KP, Mp = s.mixed ? d.Mpm : d.Mp;
+    const uint64_t Nz = s.mixed ? pad128(s.N) : s.N;
     size_t p = 0;
     auto take = [&p](size_t bytes) {
         const size_t at = p;
         p += al(bytes);
         return at;
     };
-    L.offs = take((size_t)(s.U + 1) * 8);
+    L.beg = take((size_t)s.U * 8);
+    L.end = take((size_t)s.U * 8);
     L.lo = take((size_t)s.nchunks * 8);
     L.hi = take((size_t)s.nchunks * 8);
     L.first = take((size_t)(Us + 1) * 4);
@@ -87,16 +103,26 @@ Layout layout_of(const Shape &s, const Dims &d) {
     L.out_copy = (s.frame_ll ? L.out_end : L.LLnat) - L.ll;
     L.queue = take(4);
     L.LL2 = take(s.N * 4);
-    L.blockLL = take(ll_blocks(s.N) * 8);
-    L.part = take((size_t)s.nchunks * d.Mp * d.KP * 4);
+    L.blockLL = take(pad128(s.N) / 128 * 8);
+    L.part = take((size_t)s.nchunks * Mp * KP * 4);
     L.F = take((size_t)Us * d.M * d.D * 8);
     L.A1 = take((size_t)s.Umllr * d.D * d.M * 8);
     L.A2 = take((size_t)s.Umllr * d.D * d.M * 8);
     L.G = take((size_t)s.Umllr * d.D * d.NP * 8);
     L.K = take((size_t)s.Umllr * d.D * d.E * 8);
+    const size_t zb = s.mixed ? Nz * KP * 2 : 0;
+    L.zah = take(zb);
+    L.zal = take(zb);
+    L.zbh = take(zb);
+    L.zbl = take(zb);
+    L.bad = take(s.mixed ? s.N : 0);
+    L.nrange = take(s.mixed ? 4 : 0);
     L.total = p;
     return L;
 }
+
+// Frame rows a request takes in a batch of the given form.
+inline uint64_t rows_of(uint64_t n, bool mixed) { return mixed ? pad32(n) : n; }
 
 int kind_rank(int k) { return k == GMMHIP_REQ_MAP ? 0 : (k == GMMHIP_REQ_MLLR ? 1 : 2); }
 
@@ -115,6 +141,13 @@ struct gmmhip_server {
     int num_cus = 256;
     ModelOperands mo{};
 
+    // mixed form: precision, UBM-based centring / scaling, split model
+    int prec = GMMHIP_PREC_FP32;
+    std::vector<float> h_zoff, h_zscale;
+    float zlimit = INFINITY;
+    float *d_zoff = nullptr, *d_zscale = nullptr;
+    void *d_P16 = nullptr;
+
     struct Slot {
         int id = 0;
         hipStream_t stream = nullptr;
@@ -124,7 +157,7 @@ struct gmmhip_server {
         size_t h_in_cap = 0, h_out_cap = 0;
         SlotState state = SLOT_FREE;
         std::vector<Pending> batch;
-        std::vector<uint64_t> offs;  // per request frame offset in the batch
+        std::vector<uint64_t> beg;  // first frame row of each request
         Shape shape;
         Layout lay{};
         int err = 0;
@@ -134,14 +167,14 @@ struct gmmhip_server {
 
     std::mutex mu;
     std::condition_variable cv_dispatch, cv_slot, cv_done;
-    std::deque<Pending> queue;
+    std::deque<Pending> q_mixed, q_fp32;  // q_mixed is used by the mixed form only
     bool closing = false;
     std::thread dispatcher;
     std::string err;
 
     // statistics
     uint64_t submitted = 0, completed = 0, failed = 0, batches = 0, frames = 0;
-    uint64_t batch_requests = 0;
+    uint64_t batch_requests = 0, fp32_fallbacks = 0;
     double batch_gpu_ms = 0.0;
     std::vector<double> lat;
     Clock::time_point t_reset;
@@ -154,7 +187,7 @@ void set_err(gmmhip_server *s, const std::string &e) {
     s->err = e;
 }
 
-// Finish requests with a status (server mutex held by the caller).
+// Finish a request with a status (server mutex held by the caller).
 void finish_locked(gmmhip_server *s, gmmhip_request *r, int status, Clock::time_point t0) {
     r->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     __atomic_store_n(&r->status, status, __ATOMIC_RELEASE);
@@ -177,7 +210,9 @@ void finish_locked(gmmhip_server *s, gmmhip_request *r, int status, Clock::time_
 
 int grow_pinned(gmmhip_server *s, char **p, size_t *cap, size_t need) {
     if (*cap >= need) return GMMHIP_OK;
-    const size_t n = std::max(need, *cap * 3 / 2);
+    // geometric growth: hipHostFree waits for the device, so regrowth must
+    // stay rare on the dispatch path
+    const size_t n = std::max(need + need / 2, *cap * 2);
     if (*p) SRV_HIPCHK(hipHostFree(*p));
     *p = nullptr;
     *cap = 0;
@@ -192,26 +227,31 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
     std::stable_sort(sl.batch.begin(), sl.batch.end(), [](const Pending &a, const Pending &b) {
         return kind_rank(a.r->kind) < kind_rank(b.r->kind);
     });
-    Shape sh;
+    Shape sh = sl.shape;  // form set by the dispatcher
     sh.U = (int)sl.batch.size();
-    sl.offs.assign(sh.U + 1, 0);
-    std::vector<uint64_t> lo, hi;
+    sh.Umap = sh.Umllr = 0;
+    sh.frame_ll = false;
+    sl.beg.assign(sh.U, 0);
+    std::vector<uint64_t> end(sh.U), lo, hi;
     std::vector<int> first;
+    uint64_t row = 0;
     for (int u = 0; u < sh.U; u++) {
         const gmmhip_request *r = sl.batch[u].r;
-        sl.offs[u + 1] = sl.offs[u] + r->n_frames;
+        sl.beg[u] = row;
+        end[u] = row + r->n_frames;
+        row += rows_of(r->n_frames, sh.mixed);
         if (r->frame_ll) sh.frame_ll = true;
         if (r->kind == GMMHIP_REQ_SCORE) continue;
         if (r->kind == GMMHIP_REQ_MAP) sh.Umap++;
         else sh.Umllr++;
         first.push_back((int)lo.size());
-        for (uint64_t f = sl.offs[u]; f < sl.offs[u + 1]; f += s->cf) {
+        for (uint64_t f = sl.beg[u]; f < end[u]; f += s->cf) {
             lo.push_back(f);
-            hi.push_back(std::min(sl.offs[u + 1], f + s->cf));
+            hi.push_back(std::min(end[u], f + s->cf));
         }
     }
     first.push_back((int)lo.size());
-    sh.N = sl.offs[sh.U];
+    sh.N = row;
     sh.nchunks = (int)lo.size();
     const Layout L = layout_of(sh, d);
     sl.shape = sh;
@@ -220,8 +260,9 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
     if ((rc = grow_pinned(s, &sl.h_in, &sl.h_in_cap, L.in_end))) return rc;
     if ((rc = grow_pinned(s, &sl.h_out, &sl.h_out_cap, L.out_copy))) return rc;
 
-    // IN staging
-    std::memcpy(sl.h_in + L.offs, sl.offs.data(), sl.offs.size() * 8);
+    // IN staging; the padding rows of a mixed batch are zero
+    std::memcpy(sl.h_in + L.beg, sl.beg.data(), (size_t)sh.U * 8);
+    std::memcpy(sl.h_in + L.end, end.data(), (size_t)sh.U * 8);
     if (sh.nchunks) {
         std::memcpy(sl.h_in + L.lo, lo.data(), lo.size() * 8);
         std::memcpy(sl.h_in + L.hi, hi.data(), hi.size() * 8);
@@ -230,7 +271,9 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
     float *hx = (float *)(sl.h_in + L.X);
     for (int u = 0; u < sh.U; u++) {
         const gmmhip_request *r = sl.batch[u].r;
-        std::memcpy(hx + sl.offs[u] * d.D, r->frames, r->n_frames * d.D * 4);
+        std::memcpy(hx + sl.beg[u] * d.D, r->frames, r->n_frames * d.D * 4);
+        const uint64_t pad = rows_of(r->n_frames, sh.mixed) - r->n_frames;
+        if (pad) std::memset(hx + end[u] * d.D, 0, pad * d.D * 4);
     }
 
     char *A = sl.arena;
@@ -240,48 +283,111 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
 
     gmmhip_ctx *b = s->base;
     const int Us = sh.Umap + sh.Umllr;
-    EstepArgs a{};
-    a.X = (const float *)(A + L.X);
-    a.ldx = d.D;
-    a.N = sh.N;
-    a.D = d.D;
-    a.KP = d.KP;
-    a.P = b->d_P;
-    a.M = d.M;
-    a.Mt = b->Mt;
-    a.off = b->d_off;
-    a.min_ll = b->opt.min_loglike;
-    a.min_gamma = b->opt.min_gamma;
-    a.LL2 = (float *)(A + L.LL2);
-    a.LLnat = (float *)(A + L.LLnat);
-    a.blockLL = (double *)(A + L.blockLL);
-    a.ngroups = (b->Mt + 3) / 4;
-    a.partials = (float *)(A + L.part);
-    a.chunk_frames = s->cf;
-    SRV_HIPCHK(launch_loglik_fp32(a, st));
+    float *LL2 = (float *)(A + L.LL2), *LLnat = (float *)(A + L.LLnat);
+    const uint64_t *clo = (const uint64_t *)(A + L.lo), *chi = (const uint64_t *)(A + L.hi);
+    const float *roff = b->d_off, *rscale = nullptr;
+    int rKP = d.KP, rMp = d.Mp;
+    if (!sh.mixed) {
+        EstepArgs a{};
+        a.X = (const float *)(A + L.X);
+        a.ldx = d.D;
+        a.N = sh.N;
+        a.D = d.D;
+        a.KP = d.KP;
+        a.P = b->d_P;
+        a.M = d.M;
+        a.Mt = b->Mt;
+        a.off = b->d_off;
+        a.min_ll = b->opt.min_loglike;
+        a.min_gamma = b->opt.min_gamma;
+        a.LL2 = LL2;
+        a.LLnat = LLnat;
+        a.blockLL = (double *)(A + L.blockLL);
+        a.ngroups = (b->Mt + 3) / 4;
+        a.partials = (float *)(A + L.part);
+        a.chunk_frames = s->cf;
+        SRV_HIPCHK(launch_loglik_fp32(a, st));
+        if (Us > 0) {
+            a.nchunks = sh.nchunks;
+            a.chunk_lo = clo;
+            a.chunk_hi = chi;
+            a.queue = (unsigned int *)(A + L.queue);
+            a.queue_blocks = std::min(sh.nchunks * a.ngroups, 2 * s->num_cus);
+            SRV_HIPCHK(launch_fill_u32(a.queue, 0u, 1, st));
+            SRV_HIPCHK(launch_stats_fp32_queue(a, st));
+        }
+    } else {
+        const uint64_t Nz = pad128(sh.N);
+        uint8_t *bad = (uint8_t *)(A + L.bad);
+        unsigned int *nrange = (unsigned int *)(A + L.nrange);
+        SRV_HIPCHK(launch_fill_u32((unsigned int *)bad, 0u, (int)((sh.N + 3) / 4), st));
+        SRV_HIPCHK(launch_fill_u32(nrange, 0u, 1, st));
+        PrepArgs p{};
+        p.X = (const float *)(A + L.X);
+        p.N = sh.N;
+        p.Npad = Nz;
+        p.D = d.D;
+        p.KP = d.KPm;
+        p.off = s->d_zoff;
+        p.scale = s->d_zscale;
+        p.za_hi_raw = (uint16_t *)(A + L.zah);
+        p.za_lo_raw = (uint16_t *)(A + L.zal);
+        p.zb_hi_raw = (uint16_t *)(A + L.zbh);
+        p.zb_lo_raw = (uint16_t *)(A + L.zbl);
+        p.bad = bad;
+        p.nrange = nrange;
+        p.zlimit = s->zlimit;
+        SRV_HIPCHK(launch_prep_frames(p, s->prec, st));
+        MixedArgs m{};
+        m.N = sh.N;
+        m.Npad = Nz;
+        m.D = d.D;
+        m.KP = d.KPm;
+        m.za_hi = p.za_hi_raw;
+        m.za_lo = p.za_lo_raw;
+        m.zb_hi = p.zb_hi_raw;
+        m.zb_lo = p.zb_lo_raw;
+        m.bad = bad;
+        m.P16 = s->d_P16;
+        m.M = d.M;
+        m.Mt32 = d.Mpm / 32;
+        m.min_ll = b->opt.min_loglike;
+        m.min_gamma = b->opt.min_gamma;
+        m.LL2 = LL2;
+        m.LLnat = LLnat;
+        m.blockLL = (double *)(A + L.blockLL);
+        SRV_HIPCHK(launch_loglik_mixed(m, s->prec, st));
+        if (Us > 0) {
+            m.chunk_frames = s->cf;
+            m.nchunks = sh.nchunks;
+            m.ngroups = (m.Mt32 + 3) / 4;
+            m.partials = (float *)(A + L.part);
+            m.chunk_lo = clo;
+            m.chunk_hi = chi;
+            SRV_HIPCHK(launch_stats_mixed(m, s->prec, st));
+        }
+        roff = s->d_zoff;
+        rscale = s->d_zscale;
+        rKP = d.KPm;
+        rMp = d.Mpm;
+    }
     if (Us > 0) {
-        a.nchunks = sh.nchunks;
-        a.chunk_lo = (const uint64_t *)(A + L.lo);
-        a.chunk_hi = (const uint64_t *)(A + L.hi);
-        a.queue = (unsigned int *)(A + L.queue);
-        a.queue_blocks = std::min(sh.nchunks * a.ngroups, 2 * s->num_cus);
-        SRV_HIPCHK(launch_fill_u32(a.queue, 0u, 1, st));
-        SRV_HIPCHK(launch_stats_fp32_queue(a, st));
         SegReduceArgs r{};
-        r.partials = a.partials;
+        r.partials = (const float *)(A + L.part);
         r.chunk_first = (const int *)(A + L.first);
-        r.off = b->d_off;
+        r.off = roff;
+        r.scale = rscale;
         r.U = Us;
         r.M = d.M;
-        r.Mp = d.Mp;
+        r.Mp = rMp;
         r.D = d.D;
-        r.KP = d.KP;
+        r.KP = rKP;
         r.n = (double *)(A + L.n);
         r.F = (double *)(A + L.F);
         SRV_HIPCHK(launch_seg_reduce(r, st));
     }
-    SRV_HIPCHK(launch_seg_sum(a.LLnat, (const uint64_t *)(A + L.offs), sh.U, (double *)(A + L.ll),
-                              st));
+    SRV_HIPCHK(launch_seg_sum(LLnat, (const uint64_t *)(A + L.beg), (const uint64_t *)(A + L.end),
+                              sh.U, (double *)(A + L.ll), st));
     if (sh.Umap > 0)
         SRV_HIPCHK(launch_map_means((const double *)(A + L.n), (const double *)(A + L.F), s->mo.mu,
                                     sh.Umap, d.M, d.D, s->cfg.tau, (float *)(A + L.means), st));
@@ -329,7 +435,7 @@ void scatter(gmmhip_server *s, gmmhip_server::Slot &sl) {
     for (int u = 0; u < sl.shape.U; u++) {
         gmmhip_request *r = sl.batch[u].r;
         r->ll_sum = ll[u];
-        if (r->frame_ll) std::memcpy(r->frame_ll, fll + sl.offs[u], r->n_frames * 4);
+        if (r->frame_ll) std::memcpy(r->frame_ll, fll + sl.beg[u], r->n_frames * 4);
         if (r->kind == GMMHIP_REQ_SCORE) continue;
         if (r->occ) std::memcpy(r->occ, n + (size_t)u * d.M, (size_t)d.M * 8);
         if (r->kind == GMMHIP_REQ_MAP) {
@@ -393,13 +499,14 @@ void dispatcher_main(gmmhip_server *s) {
         int id = -1;
         {
             std::unique_lock<std::mutex> lk(s->mu);
+            auto empty = [&] { return s->q_mixed.empty() && s->q_fp32.empty(); };
             s->cv_dispatch.wait(lk, [&] {
-                if (s->queue.empty()) return s->closing;
+                if (empty()) return s->closing;
                 for (auto &sl : s->slots)
                     if (sl.state == SLOT_FREE) return true;
                 return false;
             });
-            if (s->queue.empty() && s->closing) {
+            if (empty() && s->closing) {
                 // wait for running batches, then stop the completers
                 s->cv_dispatch.wait(lk, [&] {
                     for (auto &sl : s->slots)
@@ -415,14 +522,19 @@ void dispatcher_main(gmmhip_server *s) {
                     id = sl.id;
                     break;
                 }
+            // one form per batch: the queue whose head waited longest
+            const bool mixed =
+                !s->q_mixed.empty() && (s->q_fp32.empty() || s->q_mixed.front().t0 <= s->q_fp32.front().t0);
+            std::deque<Pending> &q = mixed ? s->q_mixed : s->q_fp32;
             // admission: FIFO while the batch layout fits the slot budget
             gmmhip_server::Slot &sl = s->slots[id];
             Shape sh;
-            while (!s->queue.empty() && (int)sl.batch.size() < s->cfg.max_batch) {
-                const Pending &p = s->queue.front();
+            sh.mixed = mixed;
+            while (!q.empty() && (int)sl.batch.size() < s->cfg.max_batch) {
+                const Pending &p = q.front();
                 Shape t = sh;
                 t.U++;
-                t.N += p.r->n_frames;
+                t.N += rows_of(p.r->n_frames, mixed);
                 if (p.r->kind != GMMHIP_REQ_SCORE) {
                     (p.r->kind == GMMHIP_REQ_MAP ? t.Umap : t.Umllr)++;
                     t.nchunks += (int)((p.r->n_frames + s->cf - 1) / s->cf);
@@ -433,7 +545,7 @@ void dispatcher_main(gmmhip_server *s) {
                         // can never fit
                         finish_locked(s, p.r, GMMHIP_ERR_OOM, p.t0);
                         s->err = "request larger than the slot budget";
-                        s->queue.pop_front();
+                        q.pop_front();
                         s->cv_done.notify_all();
                         continue;
                     }
@@ -441,9 +553,10 @@ void dispatcher_main(gmmhip_server *s) {
                 }
                 sh = t;
                 sl.batch.push_back(p);
-                s->queue.pop_front();
+                q.pop_front();
             }
             if (sl.batch.empty()) continue;
+            sl.shape = sh;
             sl.state = SLOT_STAGING;  // owned by this thread until handed over
         }
         gmmhip_server::Slot &sl = s->slots[id];
@@ -467,8 +580,70 @@ void free_server(gmmhip_server *s) {
         if (sl.ev1) hipEventDestroy(sl.ev1);
         if (sl.stream) hipStreamDestroy(sl.stream);
     }
+    if (s->d_zoff) hipFree(s->d_zoff);
+    if (s->d_zscale) hipFree(s->d_zscale);
+    if (s->d_P16) hipFree(s->d_P16);
     if (s->base) gmmhip_destroy(s->base);
     delete s;
+}
+
+// Mixed form: UBM-based centring o = sum_m w_m mu_m and scaling
+// s = 1 / sqrt(sum_m w_m (var_m + (mu_m - o)^2)) (the UBM's global mean and
+// deviation), and the split model for them.  Returns false when the model's
+// coefficients overflow the 16-bit type (the server then runs FP32 batches).
+int setup_mixed(gmmhip_server *s, bool *ok) {
+    gmmhip_ctx *b = s->base;
+    const int M = s->d.M, D = s->d.D;
+    s->h_zoff.assign(D, 0.0f);
+    s->h_zscale.assign(D, 1.0f);
+    double wsum = 0.0;
+    for (int m = 0; m < M; m++) wsum += b->m_w[m];
+    if (!(wsum > 0.0)) wsum = 1.0;
+    for (int d = 0; d < D; d++) {
+        double o = 0.0;
+        for (int m = 0; m < M; m++) o += b->m_w[m] * (double)b->m_mu[(size_t)m * D + d];
+        o /= wsum;
+        double v = 0.0;
+        for (int m = 0; m < M; m++) {
+            const double dm = (double)b->m_mu[(size_t)m * D + d] - o;
+            v += b->m_w[m] * ((double)b->m_var[(size_t)m * D + d] + dm * dm);
+        }
+        v /= wsum;
+        s->h_zoff[d] = (float)o;
+        if (std::isfinite(v) && v > 0.0) s->h_zscale[d] = (float)(1.0 / std::sqrt(v));
+    }
+    bool overflow = false;
+    const std::vector<uint16_t> P =
+        pack_split_model(b, s->prec, s->d.KPm, s->h_zoff.data(), s->h_zscale.data(), &overflow);
+    *ok = !overflow;
+    if (overflow) return GMMHIP_OK;
+    SRV_HIPCHK(hipMalloc((void **)&s->d_zoff, D * 4));
+    SRV_HIPCHK(hipMalloc((void **)&s->d_zscale, D * 4));
+    SRV_HIPCHK(hipMalloc(&s->d_P16, P.size() * 2));
+    SRV_HIPCHK(hipMemcpy(s->d_zoff, s->h_zoff.data(), D * 4, hipMemcpyHostToDevice));
+    SRV_HIPCHK(hipMemcpy(s->d_zscale, s->h_zscale.data(), D * 4, hipMemcpyHostToDevice));
+    SRV_HIPCHK(hipMemcpy(s->d_P16, P.data(), P.size() * 2, hipMemcpyHostToDevice));
+    // frames with |z| above this run in FP32 (prep_frames flags > 240)
+    s->zlimit = s->prec == GMMHIP_PREC_FP16X3 ? 240.0f : INFINITY;
+    return GMMHIP_OK;
+}
+
+// True when every finite value of the request maps inside the FP16 range
+// with a margin (the device flags |z| > 240).
+bool fits_fp16(const gmmhip_server *s, const gmmhip_request *r) {
+    const int D = s->d.D;
+    const float *o = s->h_zoff.data(), *sc = s->h_zscale.data();
+    for (uint64_t t = 0; t < r->n_frames; t++) {
+        const float *x = r->frames + t * D;
+        bool ok = true;
+        for (int d = 0; d < D; d++) {
+            const float z = (x[d] - o[d]) * sc[d];
+            // non-finite values pass: the kernels flag those frames
+            ok &= !(std::isfinite(x[d]) && std::fabs(z) > 239.0f);
+        }
+        if (!ok) return false;
+    }
+    return true;
 }
 
 }  // namespace
@@ -481,7 +656,9 @@ extern "C" int gmmhip_server_create(int device, const gmmhip_server_cfg *cfg, in
     gmmhip_server_cfg c{};
     if (cfg) c = *cfg;
     else gmmhip_default_opts(&c.opts);
-    if (c.opts.precision != GMMHIP_PREC_FP32) return GMMHIP_ERR_ARG;
+    const int prec = c.opts.precision;
+    if (prec != GMMHIP_PREC_FP32 && prec != GMMHIP_PREC_BF16X3 && prec != GMMHIP_PREC_FP16X3)
+        return GMMHIP_ERR_ARG;
     if (c.slots <= 0) c.slots = 2;
     if (c.slots > 8) return GMMHIP_ERR_ARG;
     if (c.slot_bytes == 0) c.slot_bytes = 1ull << 30;
@@ -512,10 +689,24 @@ extern "C" int gmmhip_server_create(int device, const gmmhip_server_cfg *cfg, in
         s->num_cus = ncu;
     s->d.M = M;
     s->d.D = D;
-    s->d.KP = b->KP;
-    s->d.Mp = b->Mt * 16;
     s->d.E = D + 1;
     s->d.NP = (D + 1) * (D + 2) / 2;
+    s->d.KP = b->KP;
+    s->d.Mp = b->Mt * 16;
+    s->d.KPm = kp_mixed_for_dim(D);
+    s->d.Mpm = (M + 31) / 32 * 32;
+    if (prec != GMMHIP_PREC_FP32) {
+        s->prec = prec;
+        bool ok = false;
+        if ((rc = setup_mixed(s, &ok))) {
+            free_server(s);
+            return rc;
+        }
+        if (!ok) {
+            s->prec = GMMHIP_PREC_FP32;
+            s->err = "model coefficients overflow the 16-bit type: serving in FP32";
+        }
+    }
     s->slots.resize(c.slots);
     for (int i = 0; i < c.slots; i++) {
         auto &sl = s->slots[i];
@@ -529,8 +720,9 @@ extern "C" int gmmhip_server_create(int device, const gmmhip_server_cfg *cfg, in
             free_server(s);
             return GMMHIP_ERR_OOM;
         }
-        if (grow_pinned(s, &sl.h_in, &sl.h_in_cap, 16u << 20) ||
-            grow_pinned(s, &sl.h_out, &sl.h_out_cap, 16u << 20)) {
+        const size_t stage = std::min<size_t>(c.slot_bytes, 64u << 20);
+        if (grow_pinned(s, &sl.h_in, &sl.h_in_cap, stage) ||
+            grow_pinned(s, &sl.h_out, &sl.h_out_cap, stage)) {
             free_server(s);
             return GMMHIP_ERR_OOM;
         }
@@ -544,16 +736,20 @@ extern "C" int gmmhip_server_create(int device, const gmmhip_server_cfg *cfg, in
 
 extern "C" int gmmhip_server_submit(gmmhip_server_t s, gmmhip_request *r) {
     if (!s || !r) return GMMHIP_ERR_ARG;
+    if (r->kind < GMMHIP_REQ_SCORE || r->kind > GMMHIP_REQ_MLLR) return GMMHIP_ERR_ARG;
     if (!r->frames || r->n_frames == 0) return GMMHIP_ERR_ARG;
     if (r->kind == GMMHIP_REQ_MAP && !r->means) return GMMHIP_ERR_ARG;
     if (r->kind == GMMHIP_REQ_MLLR && !r->mllr_W) return GMMHIP_ERR_ARG;
-    if (r->kind < GMMHIP_REQ_SCORE || r->kind > GMMHIP_REQ_MLLR) return GMMHIP_ERR_ARG;
+    // the form is decided by the request's own frames
+    bool mixed = s->prec != GMMHIP_PREC_FP32;
+    if (mixed && s->prec == GMMHIP_PREC_FP16X3) mixed = fits_fp16(s, r);
     std::lock_guard<std::mutex> g(s->mu);
     if (s->closing) return GMMHIP_ERR_CLOSED;
     r->ll_sum = 0.0;
     r->latency_ms = 0.0;
     __atomic_store_n(&r->status, GMMHIP_REQ_PENDING, __ATOMIC_RELEASE);
-    s->queue.push_back(Pending{r, Clock::now()});
+    (mixed ? s->q_mixed : s->q_fp32).push_back(Pending{r, Clock::now()});
+    if (s->prec != GMMHIP_PREC_FP32 && !mixed) s->fp32_fallbacks++;
     s->submitted++;
     s->cv_dispatch.notify_one();
     return GMMHIP_OK;
@@ -588,12 +784,14 @@ extern "C" int gmmhip_server_stats_get(gmmhip_server_t s, gmmhip_server_stats *s
     st->failed = s->failed;
     st->batches = s->batches;
     st->frames = s->frames;
+    st->fp32_fallbacks = s->fp32_fallbacks;
     std::vector<double> v = s->lat;
     std::sort(v.begin(), v.end());
     auto pct = [&v](double q) {
         if (v.empty()) return 0.0;
-        const size_t i = std::min(v.size() - 1, (size_t)std::ceil(q * v.size()) - (q > 0 ? 1 : 0));
-        return v[i];
+        size_t i = (size_t)std::ceil(q * v.size());
+        i = i ? i - 1 : 0;
+        return v[std::min(i, v.size() - 1)];
     };
     st->p50_ms = pct(0.50);
     st->p90_ms = pct(0.90);
@@ -606,10 +804,12 @@ extern "C" int gmmhip_server_stats_get(gmmhip_server_t s, gmmhip_server_stats *s
     st->mean_batch_requests = s->batches ? (double)s->batch_requests / s->batches : 0.0;
     st->mean_batch_gpu_ms = s->batches ? s->batch_gpu_ms / s->batches : 0.0;
     st->wall_s = std::chrono::duration<double>(Clock::now() - s->t_reset).count();
+    st->precision = s->prec;
     if (reset) {
         s->lat.clear();
         s->submitted -= s->completed + s->failed;
         s->completed = s->failed = s->batches = s->frames = s->batch_requests = 0;
+        s->fp32_fallbacks = 0;
         s->batch_gpu_ms = 0.0;
         s->t_reset = Clock::now();
     }

@@ -227,7 +227,9 @@ enum {
 #define GMMHIP_REQ_PENDING (-1)
 
 typedef struct gmmhip_server_cfg {
-    gmmhip_opts opts;        /* thresholds and precision (FP32 or mixed) */
+    gmmhip_opts opts;        /* thresholds and precision (FP32 or mixed; the
+                                mixed form centres and scales frames by the
+                                UBM's global mean and deviation) */
     int slots;               /* batches in flight, 0 = 2 */
     uint64_t slot_bytes;     /* device budget of one batch: frames, per-frame
                                 values, partial sums and outputs; 0 = 1 GiB */
@@ -257,10 +259,14 @@ typedef struct gmmhip_request {
 
 typedef struct gmmhip_server_stats {
     uint64_t submitted, completed, failed, batches, frames;
+    uint64_t fp32_fallbacks;   /* FP16 form: requests outside the FP16 range
+                                  after centring and scaling, run in FP32 */
     double p50_ms, p90_ms, p99_ms, p999_ms, max_ms, mean_ms;  /* latency */
     double mean_batch_requests;
     double mean_batch_gpu_ms;  /* device time of one batch (event pair) */
     double wall_s;             /* since creation or the last reset */
+    int precision;             /* GMMHIP_PREC_* in use (FP32 if the model's
+                                  split coefficients overflow FP16) */
 } gmmhip_server_stats;
 
 /* M active components (weights w[M], means mu[M][D], variances var[M][D]). */

@@ -194,3 +194,75 @@ def test_server_rejects_bad_requests(ubm):
         assert gmm_amd.lib().gmmhip_server_submit(srv._h, ctypes.byref(r)) == gmm_amd.ERR_ARG
     finally:
         srv.close()
+
+
+# Mixed forms: per-frame log-likelihood rtol / statistics rtol (n, F scaled
+# by sqrt(n S)), as the mixed E-step states; MAP means are checked through
+# the oracle's MAP update of the server's own statistics (exact algebra,
+# 1e-6) and the occupancies against the oracle's.
+MIXED_TOL = {gmm_amd.PREC_FP16X3: (1e-5, 2e-4), gmm_amd.PREC_BF16X3: (2e-4, 2e-3)}
+
+
+@pytest.mark.parametrize("prec", [gmm_amd.PREC_FP16X3, gmm_amd.PREC_BF16X3])
+def test_server_mixed_alone_vs_mix_and_oracle(ubm, prec):
+    w, mu, var = ubm
+    reqs = make_requests(21, w, mu, var, 40, lo=20, hi=1200)
+    lt, st = MIXED_TOL[prec]
+    srv = gmm_amd.Server(w, mu, var, tau=16, chunk_frames=256, precision=prec)
+    try:
+        assert srv.stats()["precision"] == prec
+        alone = []
+        for kind, x in reqs:
+            p = srv.submit(kind, x, frame_ll=True, occ=True)
+            assert srv.wait(p, 60000) in (0, gmm_amd.ERR_MLLR)
+            alone.append(results(p))
+        ps = [srv.submit(kind, x, frame_ll=True, occ=True) for kind, x in reqs]
+        srv.drain()
+        s = srv.stats()
+        assert s["fp32_fallbacks"] == 0 and s["failed"] == 0
+        assert s["batches"] < 2 * len(reqs)
+        for a, p in zip(alone, ps):
+            same(a, results(p))
+        for (kind, x), p in zip(reqs, ps):
+            ref = orc.frame_loglik(x, w, mu, var)
+            assert np.all(np.abs(p.frame_ll - ref) <= lt * np.abs(ref) + 1e-6)
+            assert abs(p.ll_sum - ref.sum()) <= lt * np.abs(ref).sum()
+            if kind == REQ_SCORE:
+                continue
+            rn, rF, rS, _ = orc.accumulate(x, w, mu, var)
+            assert np.all(np.abs(p.occ - rn) <= st * rn + 3e-4)
+            if kind == REQ_MAP:
+                # the server's MAP from its own statistics: F is not returned,
+                # so check mu' against the oracle's MAP of the oracle's
+                # statistics within the statistics tolerance
+                _, rmu, _ = orc.update_map(w, mu, var, rn, rF, rS, 1, 16)
+                xm = float(np.abs(x).max())
+                assert np.allclose(p.means, rmu, rtol=st, atol=st * xm)
+    finally:
+        srv.close()
+
+
+def test_server_fp16_out_of_range_request_runs_fp32(ubm):
+    """A request with a value far outside the FP16 range after the UBM's
+    centring and scaling runs in an FP32 batch: its results equal the FP32
+    server's bit for bit, the others stay in the FP16 form."""
+    w, mu, var = ubm
+    reqs = make_requests(22, w, mu, var, 6, lo=50, hi=300)
+    odd = reqs[1][1].copy()
+    odd[7, 2] = 1e6
+    reqs[1] = (REQ_MAP, odd)
+    srv = gmm_amd.Server(w, mu, var, chunk_frames=256, precision=gmm_amd.PREC_FP16X3)
+    ref = gmm_amd.Server(w, mu, var, chunk_frames=256)
+    try:
+        ps = [srv.submit(k, x, occ=True) for k, x in reqs]
+        srv.drain()
+        assert srv.stats()["fp32_fallbacks"] == 1
+        q = ref.submit(REQ_MAP, odd, occ=True)
+        ref.wait(q)
+        same(results(ps[1]), results(q))
+        q0 = ref.submit(reqs[0][0], reqs[0][1], occ=True)
+        ref.wait(q0)
+        assert ps[0].ll_sum != q0.ll_sum  # the others ran in the FP16 form
+    finally:
+        srv.close()
+        ref.close()
