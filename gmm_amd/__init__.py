@@ -26,8 +26,9 @@ EXPORTS = [
     "gmmhip_kernel_times", "gmmhip_reset_kernel_times", "gmmhip_batch_adapt",
     "gmmhip_server_create", "gmmhip_server_submit", "gmmhip_server_wait",
     "gmmhip_server_drain", "gmmhip_server_stats_get", "gmmhip_server_last_error",
-    "gmmhip_server_destroy",
+    "gmmhip_server_destroy", "gmmhip_tune_load", "gmmhip_tune_lookup", "gmmhip_tiles_used",
 ]
+TUNE_ESTEP, TUNE_BATCH = 0, 1
 BATCH_QUEUE, BATCH_PER_UTTERANCE = 0, 1
 KT_LOGLIK, KT_STATS, KT_REDUCE, KT_COUNT = 0, 1, 2, 3
 # host-layer entry points (include/gmmhost.h), bound in gmm_amd.host
@@ -56,7 +57,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("min_loglike", ctypes.c_float), ("min_gamma", ctypes.c_float),
                 ("min_var", ctypes.c_float), ("precision", ctypes.c_int),
                 ("chunk_frames", ctypes.c_int), ("verbosity", ctypes.c_int),
-                ("profile", ctypes.c_int)]
+                ("profile", ctypes.c_int), ("ll_frames", ctypes.c_int)]
 
 
 class BatchCfg(ctypes.Structure):
@@ -140,6 +141,10 @@ def lib():
         L.gmmhip_server_last_error.argtypes = [c_vp]
         L.gmmhip_server_destroy.argtypes = [c_vp]
         L.gmmhip_server_destroy.restype = None
+        L.gmmhip_tune_load.argtypes = [ctypes.c_char_p]
+        L.gmmhip_tune_lookup.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 2
+        L.gmmhip_tiles_used.argtypes = [c_vp, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
 
@@ -150,6 +155,18 @@ def default_opts(**kw):
     for k, v in kw.items():
         setattr(o, k, v)
     return o
+
+
+def tune_load(path=None):
+    """(Re)load the tuning table; returns the number of entries or -1."""
+    return lib().gmmhip_tune_load(path.encode() if path else None)
+
+
+def tune_lookup(path, precision, D, M):
+    """(ll_frames, chunk_frames) the tuning table picks, or None."""
+    a, b = ctypes.c_int(0), ctypes.c_int(0)
+    rc = lib().gmmhip_tune_lookup(path, precision, D, M, ctypes.byref(a), ctypes.byref(b))
+    return (a.value, b.value) if rc == 0 else None
 
 
 def device_count():
@@ -308,6 +325,12 @@ class Accumulator:
             self._h, ctypes.c_void_p(x_ptr), 1, off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
             U, D, ctypes.byref(cfg), ctypes.byref(o)))
         self.N = int(off[-1])
+
+    def tiles_used(self):
+        """(ll_frames, chunk_frames) of the last accumulate / loglikes / batch call."""
+        a, b = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(lib().gmmhip_tiles_used(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def stream(self):
         return lib().gmmhip_stream(self._h)

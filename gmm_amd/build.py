@@ -26,7 +26,7 @@ COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
 HIP_SOURCES = ["estep_fp32.hip", "estep_mixed.hip", "batch.hip", "gmmhip_api.hip",
                "batch_api.hip", "server.hip"]
 HOST_SOURCES = ["host/gmm_model.cpp", "host/model_adapt.cpp", "host/feature_io.cpp",
-                "host/gmmhost_api.cpp"]
+                "host/gmmhost_api.cpp", "tune.cpp"]
 TOOLS = ["trainGMM", "getLogLike", "serveLoad"]
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
@@ -73,7 +73,7 @@ def build_lib(jobs=8):
         objs = [f.result() for f in hip] + [f.result() for f in host]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-pthread", "-Wl,-rpath,/opt/rocm/lib"])
+         ["-pthread", "-ldl", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     for t in TOOLS:
         _link_tool(t)

@@ -145,6 +145,7 @@ struct BatchPlan {
     uint64_t N = 0, cf = 0;
     bool want_means = false, want_mllr = false, prior = false, per_utt = false;
     int tau = 0;
+    int ll_sub = 2;
     std::vector<uint64_t> lo, hi;
     std::vector<int> first;
 };
@@ -171,6 +172,7 @@ int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
     a.ngroups = p.ngroups;
     a.partials = h->d_part;
     a.chunk_frames = p.cf;
+    a.ll_sub = p.ll_sub;
     const int blocks_cap = std::max(1, h->num_cus) * 2;
     if (!p.per_utt) {
         HIPCHK(h, launch_loglik_fp32(a, s));
@@ -285,7 +287,12 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
     p.Mt = h->Mt;
     p.ngroups = (h->Mt + 3) / 4;
     p.N = N;
-    p.cf = cfg->chunk_frames > 0 ? (uint64_t)(cfg->chunk_frames + 63) / 64 * 64 : 1024;
+    p.cf = cfg->chunk_frames > 0 ? (uint64_t)(cfg->chunk_frames + 63) / 64 * 64
+                                 : tuned_chunk(h, GMMHIP_TUNE_BATCH, GMMHIP_PREC_FP32);
+    if (p.cf == 0) p.cf = 1024;
+    p.ll_sub = choose_ll_sub(h, GMMHIP_TUNE_BATCH);
+    h->used_ll_frames = 64 * p.ll_sub;
+    h->used_chunk = (int)p.cf;
     p.want_means = out->means != nullptr;
     p.want_mllr = out->mllr_W != nullptr;
     p.prior = cfg->mllr_prior != 0;
@@ -306,7 +313,7 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
 
     // buffers
     HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, N));
-    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(N) + (size_t)U));
+    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(N, 1) + (size_t)U));
     HIPCHK(h, ensure(&h->d_part, &h->part_cap, (size_t)p.nchunks * Mp * p.KP));
     HIPCHK(h, ensure(&b->d_lo, &b->lo_cap, (size_t)p.nchunks));
     HIPCHK(h, ensure(&b->d_hi, &b->hi_cap, (size_t)p.nchunks));
@@ -339,6 +346,7 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
                                      (uint64_t)(uintptr_t)h->d_x,
                                      h->model_version,
                                      p.cf,
+                                     (uint64_t)p.ll_sub,
                                      (uint64_t)p.want_means,
                                      (uint64_t)p.want_mllr,
                                      (uint64_t)p.prior,

@@ -89,6 +89,9 @@ struct gmmhip_ctx {
     uint64_t mu_f32_version = ~0ull;
     int num_cus = 0;
 
+    // tiles of the last call (gmmhip_tiles_used)
+    int used_ll_frames = 0, used_chunk = 0;
+
     // kernel timers (opts.profile)
     hipEvent_t ev[5] = {};
     bool ev_ok = false;
@@ -167,5 +170,15 @@ int model_operands(gmmhip_ctx *h, bool mllr, ModelOperands *o);
 // set when a coefficient does not fit the 16-bit type.
 std::vector<uint16_t> pack_split_model(const gmmhip_ctx *h, int prec, int KPm, const float *zoff,
                                        const float *zscale, bool *overflow);
+
+// Log-likelihood frame tile of the FP32 form (in units of 64 frames):
+// opts.ll_frames, else the tuning table's entry for the path (then the
+// E-step entry), else 2; 4 only for contraction depths up to 128
+// (gmmhip_api.hip).
+int choose_ll_sub(const gmmhip_ctx *h, int path);
+// Most frames per statistics work item from the tuning table for (path,
+// precision) and the handle's model, rounded to the 64-frame step; 0 when
+// the table has no entry.
+uint64_t tuned_chunk(const gmmhip_ctx *h, int path, int prec);
 
 }  // namespace gmmhip_detail

@@ -8,7 +8,8 @@
 
 namespace gmmhip {
 
-// Frames per E-step log-likelihood block (4 waves x 2 sub-tiles x 16).
+// Frames per E-step log-likelihood block: 4 waves x SUB sub-tiles x 16,
+// SUB = 1, 2 (default) or 4.
 constexpr int LL_FRAMES_PER_BLOCK = 128;
 // Frames staged per iteration of the statistics kernel.
 constexpr int ST_FRAMES_PER_ITER = 64;
@@ -41,7 +42,8 @@ struct EstepArgs {
     // per-frame outputs of the log-likelihood kernel
     float *LL2;      // [N] log2-domain LL, -inf marks "no statistics"
     float *LLnat;    // [N] natural LL (may be null)
-    double *blockLL; // [ceil(N/LL_FRAMES_PER_BLOCK)] per-block sums
+    double *blockLL; // [ll_blocks(N, ll_sub)] per-block sums
+    int ll_sub;      // log-likelihood frame tile / 64: 1, 2, 4 (0 = 2)
     // statistics
     uint64_t chunk_frames;
     int nchunks;
@@ -58,9 +60,11 @@ struct EstepArgs {
     int queue_blocks;
 };
 
-inline uint64_t ll_blocks(uint64_t N) {
-    return (N + LL_FRAMES_PER_BLOCK - 1) / LL_FRAMES_PER_BLOCK;
+inline uint64_t ll_blocks(uint64_t N, int sub = LL_FRAMES_PER_BLOCK / 64) {
+    const uint64_t f = 64u * (uint64_t)sub;
+    return (N + f - 1) / f;
 }
+inline int ll_sub_of(const EstepArgs &a) { return a.ll_sub > 0 ? a.ll_sub : 2; }
 
 hipError_t launch_loglik_fp32(const EstepArgs &a, hipStream_t s);
 hipError_t launch_stats_fp32(const EstepArgs &a, hipStream_t s);

@@ -64,18 +64,18 @@ __device__ __forceinline__ float zval(const float *__restrict__ xr,
 // consecutive steps are consecutive in memory and load as float4.  The model
 // is packed as P[t][lane][kk] = B[q*KK + kk][16t + c] (c = lane%16).
 //
-// loglik_kernel: one block = 4 waves x 2 sub-tiles x 16 frames.  Each wave
-// keeps the MFMA A-fragments of its 32 frames in registers for the whole
+// loglik_kernel: one block = 4 waves x SUB sub-tiles x 16 frames (SUB = 1,
+// 2 or 4: the frame tile is a tuning choice).  Each wave keeps the MFMA
+// A-fragments of its 16 SUB frames in registers for the whole
 // component loop; component tiles of 16 are staged through LDS (register
 // prefetch of the next tile overlaps the current tile's MFMAs) and shared by
 // the 4 waves.  Output layout of the 16x16x4 MFMA: lane (q, c) holds rows
 // 4q..4q+3 (frames) of column c (component).
-template <int KP>
+template <int KP, int SUB>
 __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
     constexpr int KK = KP / 4;
     constexpr int KU = KK / 4;              // float4 per lane per tile
     constexpr int PSTR = KK | 4;            // padded LDS row stride (floats)
-    constexpr int SUB = 2;
     constexpr int TILE4 = 16 * KK;          // float4 per component tile
     constexpr int PF = (TILE4 + 255) / 256; // float4 per thread per tile
     __shared__ float Pl[64 * PSTR];
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int q = lane >> 4, c = lane & 15;
-    const uint64_t base = (uint64_t)blockIdx.x * LL_FRAMES_PER_BLOCK;
+    const uint64_t base = (uint64_t)blockIdx.x * (64 * SUB);
     const int D = a.D;
 
     float af[SUB][KK];
@@ -460,11 +460,22 @@ __global__ __launch_bounds__(256) void sum_blockll_kernel(const double *__restri
     X(16) X(32) X(48) X(64) X(80) X(96) X(112) X(128) X(144) X(160) X(176) \
     X(192) X(208) X(224) X(240) X(256)
 
+// Frame tiles of 256 (SUB = 4) keep 4 KP/4 frame values per lane in
+// registers; they are built for KP <= 128 only.
 hipError_t launch_loglik_fp32(const EstepArgs &a, hipStream_t s) {
-    const dim3 grid((unsigned)ll_blocks(a.N));
+    const int sub = ll_sub_of(a);
+    if (sub == 4 && a.KP > 128) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)ll_blocks(a.N, sub));
     switch (a.KP) {
-#define X(K) \
-    case K: hipLaunchKernelGGL(loglik_kernel<K>, grid, dim3(256), 0, s, a); break;
+#define X(K)                                                                           \
+    case K:                                                                            \
+        if (sub == 1)                                                                  \
+            hipLaunchKernelGGL((loglik_kernel<K, 1>), grid, dim3(256), 0, s, a);       \
+        else if (sub == 2)                                                             \
+            hipLaunchKernelGGL((loglik_kernel<K, 2>), grid, dim3(256), 0, s, a);       \
+        else                                                                           \
+            hipLaunchKernelGGL((loglik_kernel<(K <= 128 ? K : 128), 4>), grid, dim3(256), 0, s, a); \
+        break;
         GMMHIP_KP_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -518,7 +529,7 @@ hipError_t launch_sum_doubles(const double *v, uint64_t n, double *out, hipStrea
 
 hipError_t launch_sum_blockll(const EstepArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(sum_blockll_kernel, dim3(1), dim3(256), 0, s, a.blockLL,
-                       ll_blocks(a.N), a.tot);
+                       ll_blocks(a.N, ll_sub_of(a)), a.tot);
     return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@
 // reference's host transpose into 8-frame tiles (alignVectors,
 // trainer/OL_GMMStatsEstimator_CUDA.cpp:292-328) has no counterpart here.
 #include "ctx.h"
+#include "tune.h"
 
 namespace {
 using namespace gmmhip_detail;
@@ -14,16 +15,20 @@ using namespace gmmhip_detail;
 // Frames per FP32 partial accumulator: enough chunks to give >= ~4096
 // blocks, at most 8192 frames per FP32 accumulator, a multiple of the
 // statistics kernel's frame step.
-uint64_t pick_chunk_frames(const gmmhip_ctx *h, int ngroups) {
+// The upper bound comes from the tuning table when it has an entry for the
+// precision in use.
+uint64_t pick_chunk_frames(const gmmhip_ctx *h, int ngroups, bool mixed) {
     if (h->opt.chunk_frames > 0) {
         uint64_t cf = (uint64_t)h->opt.chunk_frames;
         return (cf + ST_FRAMES_PER_ITER - 1) / ST_FRAMES_PER_ITER * ST_FRAMES_PER_ITER;
     }
+    uint64_t cap = tuned_chunk(h, GMMHIP_TUNE_ESTEP, mixed ? h->opt.precision : GMMHIP_PREC_FP32);
+    if (cap == 0) cap = 8192;
     const uint64_t target_chunks = std::max<uint64_t>(1, 4096 / std::max(1, ngroups));
     uint64_t cf = (h->N + target_chunks - 1) / target_chunks;
     cf = (cf + ST_FRAMES_PER_ITER - 1) / ST_FRAMES_PER_ITER * ST_FRAMES_PER_ITER;
     cf = std::max<uint64_t>(cf, 128);
-    cf = std::min<uint64_t>(cf, 8192);
+    cf = std::min<uint64_t>(cf, cap);
     return cf;
 }
 
@@ -86,6 +91,28 @@ void split_store(double v, uint16_t *hi, uint16_t *lo, bool *overflow) {
 }  // namespace
 
 namespace gmmhip_detail {
+
+int choose_ll_sub(const gmmhip_ctx *h, int path) {
+    int f = h->opt.ll_frames;
+    if (f <= 0) {
+        gmmhip_tune::Entry e;
+        if (gmmhip_tune::lookup(path, GMMHIP_PREC_FP32, h->D, h->M, &e) ||
+            gmmhip_tune::lookup(GMMHIP_TUNE_ESTEP, GMMHIP_PREC_FP32, h->D, h->M, &e))
+            f = e.ll_frames;
+        else
+            f = 128;
+    }
+    int sub = f >= 256 ? 4 : f >= 128 ? 2 : 1;
+    if (sub == 4 && h->KP > 128) sub = 2;
+    return sub;
+}
+
+uint64_t tuned_chunk(const gmmhip_ctx *h, int path, int prec) {
+    gmmhip_tune::Entry e;
+    if (!gmmhip_tune::lookup(path, prec, h->D, h->M, &e)) return 0;
+    const uint64_t cf = (uint64_t)e.chunk_frames;
+    return (cf + ST_FRAMES_PER_ITER - 1) / ST_FRAMES_PER_ITER * ST_FRAMES_PER_ITER;
+}
 
 std::vector<uint16_t> pack_split_model(const gmmhip_ctx *h, int prec, int KPm, const float *zoff,
                                        const float *zscale, bool *overflow) {
@@ -522,7 +549,7 @@ int gmmhip_loglikes_device(gmmhip_t h, float *d_out) {
     if (!d_out) return fail(h, GMMHIP_ERR_ARG, "loglikes_device: null output");
     DevGuard g(h->dev);
     HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, h->N));
-    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N)));
+    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N, 1)));
     bool mixed = false;
     if ((rc = prepare_mixed(h, &mixed))) return rc;
     h->last_used_mixed = mixed;
@@ -531,6 +558,7 @@ int gmmhip_loglikes_device(gmmhip_t h, float *d_out) {
         m.LL2 = h->d_LL2;
         m.LLnat = d_out;
         m.blockLL = h->d_blockLL;
+        h->used_ll_frames = 128;
         HIPCHK(h, launch_loglik_mixed(m, h->opt.precision, h->stream));
         return GMMHIP_OK;
     }
@@ -538,6 +566,8 @@ int gmmhip_loglikes_device(gmmhip_t h, float *d_out) {
     a.LL2 = h->d_LL2;
     a.LLnat = d_out;
     a.blockLL = h->d_blockLL;
+    a.ll_sub = choose_ll_sub(h, GMMHIP_TUNE_ESTEP);
+    h->used_ll_frames = 64 * a.ll_sub;
     HIPCHK(h, launch_loglik_fp32(a, h->stream));
     return GMMHIP_OK;
 }
@@ -574,7 +604,7 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
         return fail(h, GMMHIP_ERR_ARG, "accumulate: null output for a requested statistic");
     DevGuard g(h->dev);
     HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, h->N));
-    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N)));
+    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N, 1)));
     bool mixed = false;
     if ((rc = prepare_mixed(h, &mixed))) return rc;
     h->last_used_mixed = mixed;
@@ -584,7 +614,10 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
     const int tile = mixed ? 32 : 16;
     const int Mtx = (h->M + tile - 1) / tile;
     const int ngroups = (Mtx + 3) / 4;
-    const uint64_t chunk_frames = pick_chunk_frames(h, ngroups);
+    const uint64_t chunk_frames = pick_chunk_frames(h, ngroups, mixed);
+    const int ll_sub = mixed ? 2 : choose_ll_sub(h, GMMHIP_TUNE_ESTEP);
+    h->used_ll_frames = 64 * ll_sub;
+    h->used_chunk = (int)chunk_frames;
     const int nchunks = (int)((h->N + chunk_frames - 1) / chunk_frames);
     const size_t Mp = (size_t)Mtx * tile;
     const size_t E = Mp * KPx;
@@ -624,13 +657,14 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
         a.chunk_frames = chunk_frames;
         a.nchunks = nchunks;
         a.partials = h->d_part;
+        a.ll_sub = ll_sub;
         HIPCHK(h, launch_loglik_fp32(a, h->stream));
         if (prof) HIPCHK(h, hipEventRecord(h->ev[1], h->stream));
         HIPCHK(h, launch_stats_fp32(a, h->stream));
     }
     if (prof) HIPCHK(h, hipEventRecord(h->ev[2], h->stream));
     HIPCHK(h, launch_reduce_partials(h->d_part, nchunks, E, h->d_red, h->stream));
-    HIPCHK(h, launch_sum_doubles(h->d_blockLL, ll_blocks(h->N), h->d_tot, h->stream));
+    HIPCHK(h, launch_sum_doubles(h->d_blockLL, ll_blocks(h->N, ll_sub), h->d_tot, h->stream));
     if (prof) HIPCHK(h, hipEventRecord(h->ev[3], h->stream));
     HIPCHK(h, hipMemcpyAsync(h->h_res, h->d_red, E * sizeof(double), hipMemcpyDeviceToHost,
                              h->stream));
@@ -662,6 +696,24 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
     }
     *tot_ll += tot;
     *nframes += h->N;
+    return GMMHIP_OK;
+}
+
+int gmmhip_tune_load(const char *path) { return gmmhip_tune::load(path); }
+
+int gmmhip_tune_lookup(int path, int precision, int D, int M, int *ll_frames, int *chunk_frames) {
+    if (D <= 0 || M <= 0) return GMMHIP_ERR_ARG;
+    gmmhip_tune::Entry e;
+    if (!gmmhip_tune::lookup(path, precision, D, M, &e)) return GMMHIP_ERR_STATE;
+    if (ll_frames) *ll_frames = e.ll_frames;
+    if (chunk_frames) *chunk_frames = e.chunk_frames;
+    return GMMHIP_OK;
+}
+
+int gmmhip_tiles_used(gmmhip_t h, int *ll_frames, int *chunk_frames) {
+    if (!h) return GMMHIP_ERR_ARG;
+    if (ll_frames) *ll_frames = h->used_ll_frames;
+    if (chunk_frames) *chunk_frames = h->used_chunk;
     return GMMHIP_OK;
 }
 

@@ -65,6 +65,8 @@ typedef struct gmmhip_opts {
     int chunk_frames;    /* frames per FP32 partial accumulator, 0 = auto */
     int verbosity;       /* 0 silent */
     int profile;         /* 1: time every kernel with HIP events */
+    int ll_frames;       /* FP32 log-likelihood frame tile: 64, 128, 256;
+                            0 = tuning table, else 128 */
 } gmmhip_opts;
 
 /* Fill *o with the defaults above. */
@@ -195,6 +197,26 @@ typedef struct gmmhip_batch_out {
 int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on_device,
                        const uint64_t *offsets, int U, int D, const gmmhip_batch_cfg *cfg,
                        gmmhip_batch_out *out);
+
+/* ---- Tuning table -----------------------------------------------------
+ * Tile choices per (path, precision, D, M) picked by a timed sweep
+ * (python -m gmm_amd.autotune) and stored as text in
+ * gmm_amd/tuning/gfx950.tune (or $GMMHIP_TUNE_FILE): the FP32
+ * log-likelihood frame tile and the most frames per statistics work item.
+ * Used when opts.ll_frames / opts.chunk_frames (cfg->chunk_frames for
+ * batches) are 0.  No counterpart in the reference, whose launch shapes are
+ * compile-time constants (trainer/CU_GMMStatsEstimator_CUDA.cu:606-643). */
+enum { GMMHIP_TUNE_ESTEP = 0, GMMHIP_TUNE_BATCH = 1 };
+/* Reload the table from path (NULL: the default); returns the number of
+ * entries, or -1 if the file cannot be read (the table is then empty). */
+int gmmhip_tune_load(const char *path);
+/* The table's choice for (path, precision, D, M), nearest M (then D) when
+ * there is no exact entry; GMMHIP_ERR_STATE when it has none for the path
+ * and precision. */
+int gmmhip_tune_lookup(int path, int precision, int D, int M, int *ll_frames,
+                       int *chunk_frames);
+/* Tiles the handle's last accumulate / loglikes / batch call used. */
+int gmmhip_tiles_used(gmmhip_t h, int *ll_frames, int *chunk_frames);
 
 /* ---- Serving loop -------------------------------------------------------
  * A server keeps one UBM resident in HBM and answers requests of three kinds

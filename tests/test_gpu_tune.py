@@ -1,0 +1,76 @@
+"""Tile choices (log-likelihood frame tile, statistics work-item size) do
+not change results: per-frame log-likelihoods are bit-identical for every
+frame tile (each frame's log-sum-exp runs over the same component order);
+with the work-item size fixed the statistics are bit-identical too and the
+total log-likelihood differs only in the FP64 grouping of block sums
+(1e-12 relative).  The tuning table's entry is what a handle uses when the
+options leave the tiles to it."""
+import numpy as np
+import pytest
+
+import _oracle as orc
+import gmm_amd
+from gmm_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def acc():
+    if gmm_amd.device_count() < 1:
+        pytest.skip("no GPU")
+    a = gmm_amd.Accumulator(0)
+    yield a
+    a.close()
+
+
+@pytest.mark.parametrize("D,M,N", [(20, 100, 5000), (60, 300, 20000), (39, 64, 1000)])
+def test_ll_frame_tiles_identical(acc, D, M, N):
+    rng = np.random.default_rng(D + M)
+    w = rng.dirichlet(np.ones(M))
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    var = rng.uniform(0.5, 2, (M, D)).astype(np.float32)
+    x = synth.sample_frames_np(w, mu, var, N, 3)
+    acc.set_model(w.astype(np.float32), mu, var)
+    acc.set_frames(x)
+    res = {}
+    for ll in (64, 128, 256):
+        acc.set_opts(ll_frames=ll, chunk_frames=1024)
+        lls = acc.loglikes()
+        st = acc.accumulate()
+        assert acc.tiles_used() == (ll, 1024)
+        res[ll] = (lls, st)
+    acc.set_opts(ll_frames=0, chunk_frames=0)
+    for ll in (64, 256):
+        assert np.array_equal(res[ll][0], res[128][0])
+        for a, b in zip(res[ll][1][:3], res[128][1][:3]):
+            assert np.array_equal(a, b)
+        assert res[ll][1][3] == pytest.approx(res[128][1][3], rel=1e-12)
+    ref = orc.frame_loglik(x, w.astype(np.float32), mu, var)
+    assert np.allclose(res[64][0], ref, rtol=1e-5, atol=1e-5)
+
+
+def test_table_entry_is_used(acc, tmp_path):
+    D, M, N = 24, 80, 30000
+    p = tmp_path / "t.tune"
+    p.write_text("estep fp32 %d %d 256 2048 1.0\nbatch fp32 %d %d 64 256 1.0\n" % (D, M, D, M))
+    rng = np.random.default_rng(1)
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    var = rng.uniform(0.5, 2, (M, D)).astype(np.float32)
+    x = synth.sample_frames_np(w.astype(np.float64), mu, var, N, 4)
+    try:
+        assert gmm_amd.tune_load(str(p)) == 2
+        acc.set_opts(ll_frames=0, chunk_frames=0)
+        acc.set_model(w, mu, var)
+        acc.set_frames(x)
+        acc.accumulate()
+        # the table's value bounds the work item; N/64 groups-worth of
+        # chunks would be smaller, so the bound binds only for large N
+        ll, cf = acc.tiles_used()
+        assert ll == 256 and cf <= 2048
+        off = np.array([0, 1000, 5000, N], np.uint64)
+        acc.batch_adapt(x, off, mllr=False)
+        assert acc.tiles_used() == (64, 256)
+    finally:
+        gmm_amd.tune_load(None)
