@@ -3,12 +3,17 @@
 
 A step is one E-step pass (posteriors + zeroth/first/second-order
 statistics + total log-likelihood, the reference's accumulateStatsMT) over
-the whole resident batch of synthetic frames.  Default workload: UBM-scale
-plain EM, D=60, M=512, N=10,000,000 frames (seeded true mixture), model
-initialised from 512 random frames with the global variance.  The headline
-is the mixed-precision form (split fp16 operands, FP32 accumulate; total
-log-likelihood within 1e-6 relative of FP64), timed beside the plain FP32
-form on the same frames (`fp32_compare`).
+the whole resident batch of synthetic frames.  Default workload: the
+mixed-precision E-step configuration, D=60, M=2048, N=20,000,000 frames
+from a seeded 2048-component mixture with frame offsets ~N(5, 1) per dim;
+the model is initialised from 2048 random frames with the global variance.
+The headline is the mixed-precision form (split fp16 operands, FP32
+accumulate, FP64 reduction), timed beside the plain FP32 form on the same
+frames (`fp32_compare`), with each form's error against an FP64 reference
+of the same E-step over all frames (`accuracy_vs_fp64`).
+
+Other workloads: --workload ubm_plain (D=60, M=512, N=1e7), plain_em_small,
+batch_map (batched MAP + MLLR of 4,096 utterances), serve (serving loop).
 
 Prints ONE JSON line (rank 0).  Frames are generated directly in HBM, so the
 timed region starts with inputs resident.
@@ -47,6 +52,73 @@ def cpu_baseline(x_sample, w, mu, var, budget_s=12.0, max_frames=10_000_000):
             "kind": "port",
             "sample": "first %d frames of the workload, %.1f s, reference SSE CPU path restated "
                       "(oracle/sse_port.c), %d threads" % (n, dt, used)}
+
+
+def fp64_reference(x, w, mu, var, min_gamma=1e-4, chunk=262144):
+    """FP64 E-step over all frames on the GPU (torch float64; a checker,
+    not the product): total log-likelihood and n, F, S with the 1e-4
+    posterior cut, for the error report."""
+    import torch
+    dev = x.device
+    M, D = mu.shape
+    mu64 = torch.as_tensor(mu, dtype=torch.float64, device=dev)
+    var64 = torch.as_tensor(var, dtype=torch.float64, device=dev)
+    ivar = 1.0 / var64
+    gconst = (torch.log(torch.as_tensor(w, dtype=torch.float64, device=dev))
+              - 0.5 * (D * np.log(2 * np.pi) + torch.log(var64).sum(1))
+              - 0.5 * (mu64 * mu64 * ivar).sum(1))
+    B1 = (mu64 * ivar).t().contiguous()
+    B2 = (-0.5 * ivar).t().contiguous()
+    tot = torch.zeros((), dtype=torch.float64, device=dev)
+    n = torch.zeros(M, dtype=torch.float64, device=dev)
+    F = torch.zeros((M, D), dtype=torch.float64, device=dev)
+    S = torch.zeros((M, D), dtype=torch.float64, device=dev)
+    for a in range(0, x.shape[0], chunk):
+        xc = x[a:a + chunk].double()
+        x2 = xc * xc
+        L = torch.mm(xc, B1) + torch.mm(x2, B2) + gconst
+        lse = torch.logsumexp(L, dim=1)
+        tot += lse.sum()
+        g = torch.exp(L - lse[:, None])
+        g = torch.where(g > min_gamma, g, torch.zeros_like(g))
+        gt = g.t().contiguous()
+        n += g.sum(0)
+        F += torch.mm(gt, xc)
+        S += torch.mm(gt, x2)
+        del L, g, gt
+    return float(tot), n.cpu().numpy(), F.cpu().numpy(), S.cpu().numpy()
+
+
+def accuracy(res, ref):
+    """Errors of one form's statistics against the FP64 reference: total
+    log-likelihood (relative), occupancies n (max abs; n is the sum of the
+    posteriors), and the M-step means F/n and variances S/n - mean^2 (max
+    abs over components with n >= 1)."""
+    n, F, S, tot = res[0], res[1], res[2], res[3]
+    rt, rn, rF, rS = ref
+    ok = rn >= 1.0
+    mu, rmu = F[ok] / n[ok, None], rF[ok] / rn[ok, None]
+    v, rv = S[ok] / n[ok, None] - mu * mu, rS[ok] / rn[ok, None] - rmu * rmu
+    return {"tot_loglike_rel_err": abs(tot - rt) / abs(rt),
+            "occupancy_max_abs_err": float(np.max(np.abs(n - rn))),
+            "occupancy_max_rel_err": float(np.max(np.abs(n[ok] - rn[ok]) / rn[ok])),
+            "mean_max_abs_err": float(np.max(np.abs(mu - rmu))),
+            "var_max_abs_err": float(np.max(np.abs(v - rv))),
+            "var_max_rel_err": float(np.max(np.abs(v - rv) / rv))}
+
+
+def traffic_of(kernel, workload):
+    """HBM bytes per launch of a kernel on a workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/traffic.json (scaled
+    by the copy-kernel calibration stated there), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        t = json.load(open(p))
+        return t["kernels"][workload][kernel]["bytes_per_launch"]
+    except (KeyError, ValueError):
+        return None
 
 
 def bench_batch(args):
@@ -105,6 +177,28 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_copy(args):
+    """A plain 256 MiB device-to-device copy (torch, the runtime's copy
+    kernel): the known-size launch that scales rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters to bytes."""
+    import torch
+    n = 64 * 1024 * 1024
+    x = torch.randn(n, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"metric": "device copy GB/s", "value": 2 * n * 4 * args.steps / el / 1e9,
+                      "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+                      "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+                      "data": "synthetic", "config": {"workload": "copy", "bytes": n * 4}}),
+          flush=True)
+
+
 def bench_serve(args):
     """Serving loop: UBM D=60, M=2048 resident in HBM; open-loop Poisson
     arrivals (60% score / 30% MAP / 10% MLLR, lengths lognormal(median 300,
@@ -148,11 +242,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ubm_plain")
+    ap.add_argument("--workload", default="mixed_estep")
     ap.add_argument("--frames", type=int, default=0, help="override N")
     ap.add_argument("--precision", default="fp16x3", choices=["fp32", "fp16x3", "bf16x3"])
     ap.add_argument("--no-compare", action="store_true", help="skip the FP32 comparison leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 reference")
     ap.add_argument("--rates", default="2000,5000,10000", help="serve: arrival rates (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="serve: seconds per rate")
     args = ap.parse_args()
@@ -166,6 +261,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "copy":
+        return bench_copy(args)
 
     import torch
     import gmm_amd
@@ -208,6 +305,9 @@ def main():
                    "stats_kernel_ms": kt32["stats"][0] / max(kt32["stats"][1], 1),
                    "tot_loglike": r32[3]}
     res, el, kt = run(args.precision)
+    ref = None
+    if not args.no_accuracy:
+        ref = fp64_reference(x, w, mu, var)
     value = N * M * args.steps / el
 
     flop_fc = 8 * D + 4
@@ -217,9 +317,11 @@ def main():
     st_avg = st_ms / max(st_n, 1) / 1e3
     achieved = flop_fc * N * M / st_avg
     peak = PEAK["fp32"] if args.precision == "fp32" else PEAK["fp16"]
+    kname = "stats_kernel" if args.precision == "fp32" else "stats_mixed_kernel"
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "kernel": "stats_kernel" if args.precision == "fp32" else "stats_mixed_kernel",
+                "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": traffic_of(kname, args.workload),
+                "kernel": kname,
                 "avg_launch_ms": st_avg * 1e3,
                 "pipeline_ms": {"loglik": ll_ms / max(ll_n, 1), "stats": st_ms / max(st_n, 1),
                                 "reduce": rd_ms / max(rd_n, 1)},
@@ -252,6 +354,10 @@ def main():
         compare["tot_loglike_rel_diff"] = abs(res[3] - compare["tot_loglike"]) / abs(
             compare["tot_loglike"])
         line["fp32_compare"] = compare
+    if ref is not None:
+        line["accuracy_vs_fp64"] = {args.precision: accuracy(res, ref)}
+        if compare is not None:
+            line["accuracy_vs_fp64"]["fp32"] = accuracy(r32, ref)
     if not args.no_cpu_baseline:
         ns = min(N, 10_000_000)
         xs = x[:ns].cpu().numpy()
