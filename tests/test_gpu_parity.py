@@ -120,7 +120,7 @@ def test_plain_em_small_after_one_update(acc):
 
 @pytest.mark.parametrize("N,D,M", [
     (1, 3, 1), (2, 1, 2), (17, 5, 3), (129, 20, 17), (1000, 60, 33),
-    (4999, 13, 100), (300, 127, 16), (2050, 39, 64), (777, 60, 513),
+    (4999, 13, 100), (300, 127, 16), (2050, 39, 64), (777, 60, 513), (1500, 60, 2048),
 ])
 def test_shapes(acc, N, D, M):
     x, w, mu, var = rand_case(1000 + N + D + M, N, D, M)
