@@ -340,11 +340,93 @@ hipError_t launch_mllr_check(const double *n, int U, int M, int D, int *status, 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident solve for E = D+1 <= 64.  One wave per (u, i): lane r
+// holds row r of [G | k] (G symmetric, unpacked from the packed upper
+// triangle staged in LDS, plus the prior) in registers.  Gauss-Jordan
+// elimination without pivoting, as the reference inverts G_i
+// (MATRIX::_Invert_Gen, Matrix4_LMa.cpp:2232; stable for SPD G, whose pivots
+// are the ratios of its leading principal minors and so all positive): at
+// step j the pivot row is broadcast element by element with v_readlane,
+// every other row subtracts its multiple of it, and every row shifts left
+// one column, so the current column is always register 0 and every register
+// index is a compile-time constant.  After E steps lane r holds k'_r and its
+// pivot g'_rr, and w_r = k'_r / g'_rr.  A pivot <= 0 (G not positive
+// definite) marks the utterance's status.
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+template <int EMAX>
+__global__ __launch_bounds__(64) void mllr_solve_gj_kernel(MllrSolveArgs a) {
+    __shared__ double sG[EMAX * (EMAX + 1) / 2];
+    const int E = a.D + 1;
+    const int NP = E * (E + 1) / 2;
+    const uint64_t ui = blockIdx.x;
+    const int u = (int)(ui / a.D), i = (int)(ui % a.D);
+    const int r = threadIdx.x;
+    if (a.status[u] != 0) {
+        if (r < E) a.W[ui * E + r] = 0.0;
+        return;
+    }
+    const double *gp = a.Gp + ui * a.ldg;
+    for (int p = r; p < NP; p += 64) sG[p] = gp[p];
+    __syncthreads();
+    const double *gi = a.Ginit ? a.Ginit + (size_t)i * E * E + (size_t)r * E : nullptr;
+    double g[EMAX];
+#pragma unroll
+    for (int c = 0; c < EMAX; c++) {
+        double v = 0.0;
+        if (r < E && c < E) {
+            const int lo = r < c ? r : c, hi = r < c ? c : r;
+            // packed index of (lo, hi), lo <= hi: lo*E - lo(lo-1)/2 + (hi - lo)
+            v = sG[lo * E - (lo * (lo - 1)) / 2 + (hi - lo)];
+            if (gi) v += gi[c];
+        }
+        g[c] = v;
+    }
+    double kr = 0.0;
+    if (r < E) {
+        kr = a.Kp[ui * a.ldk + r];
+        if (a.kinit) kr += a.kinit[(size_t)i * E + r];
+    }
+    double diag = 1.0;
+    bool bad = false;
+    for (int j = 0; j < E; j++) {
+        const double piv = readlane_f64(g[0], j);
+        const double kj = readlane_f64(kr, j);
+        bad = bad || !(piv > 0.0);
+        const double f = (r == j) ? 0.0 : g[0] / piv;
+        if (r == j) diag = piv;
+        kr = fma(-f, kj, kr);
+        // columns to the right of the pivot, in blocks of 8 behind uniform
+        // branches; registers past the active columns are dead after the step
+        const int rem = E - 1 - j;
+#pragma unroll
+        for (int c0 = 0; c0 < EMAX - 1; c0 += 8) {
+            if (c0 < rem) {
+#pragma unroll
+                for (int c = c0; c < c0 + 8 && c < EMAX - 1; c++)
+                    g[c] = fma(-f, readlane_f64(g[c + 1], j), g[c + 1]);
+            }
+        }
+    }
+    if (r < E) a.W[ui * E + r] = kr / diag;
+    if (bad && r == 0) atomicOr(&a.status[u], 8);
+}
+
 hipError_t launch_mllr_solve(const MllrSolveArgs &a, hipStream_t s) {
     const int E = a.D + 1;
-    const size_t shm = ((size_t)E * (E + 1) + E) * sizeof(double);
-    hipLaunchKernelGGL(mllr_solve_kernel, dim3((unsigned)((uint64_t)a.U * a.D)), dim3(64), shm, s,
-                       a);
+    const dim3 grid((unsigned)((uint64_t)a.U * a.D));
+    if (E <= 64) {
+        hipLaunchKernelGGL(mllr_solve_gj_kernel<64>, grid, dim3(64), 0, s, a);
+    } else {
+        const size_t shm = ((size_t)E * (E + 1) + E) * sizeof(double);
+        hipLaunchKernelGGL(mllr_solve_kernel, grid, dim3(64), shm, s, a);
+    }
     return hipGetLastError();
 }
 
