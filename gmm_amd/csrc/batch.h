@@ -27,15 +27,33 @@ struct MllrSolveArgs {
     double *W;            // [(u,i)][E]
 };
 
+// MLLR statistics of many utterances as one GEMM per kind:
+//   G[(u,i)][p] = sum_m n[u][m] ivar[m][i] P[m][p]   (B = P,  Cn = (D+1)(D+2)/2)
+//   k[(u,i)][e] = sum_m F[u][m][i] ivar[m][i] Xi[m][e] (B = Xi, Cn = D+1)
+// with components of zero occupancy left out; the A operand is formed while
+// it is staged, so it never exists in HBM.
+// Components the batched MLLR statistics support (the occupied-component
+// list of an utterance lives in LDS).
+constexpr int MLLR_MAX_COMPONENTS = 4096;
+
+struct MllrGkArgs {
+    const double *n;     // [U][M]
+    const double *F;     // [U][M][D]
+    const double *ivar;  // [M][D]
+    const double *B;     // [M][ldb]
+    int ldb, Cn;
+    int U, M, D;
+    double *C;           // [U*D][ldc]
+    int ldc;
+    int kmode;           // 0: G, 1: k
+};
+
 hipError_t launch_fill_u32(unsigned int *p, unsigned int v, int n, hipStream_t s);
+hipError_t launch_mllr_gk(const MllrGkArgs &a, hipStream_t s);
 hipError_t launch_seg_reduce(const SegReduceArgs &a, hipStream_t s);
 hipError_t launch_map_means(const double *n, const double *F, const float *mu, int U, int M, int D,
                             int tau, float *out, hipStream_t s);
-hipError_t launch_mllr_operands(const double *n, const double *F, const float *var, int U, int M,
-                                int D, int ldk, double *A1, double *A2, hipStream_t s);
 hipError_t launch_mllr_check(const double *n, int U, int M, int D, int *status, hipStream_t s);
-hipError_t launch_gemm_f64(const double *A, int lda, const double *B, int ldb, double *C, int ldc,
-                           int R, int Cn, int K, hipStream_t s);
 hipError_t launch_mllr_solve(const MllrSolveArgs &a, hipStream_t s);
 // out[u] = sum of v[begin[u] .. end[u]) in FP64; the summation order is
 // fixed relative to begin[u] (one block per segment)

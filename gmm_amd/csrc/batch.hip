@@ -89,29 +89,6 @@ __global__ __launch_bounds__(256) void map_means_kernel(const double *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// GEMM operands of the MLLR statistics: A1[(u,i)][m] = n_um / var_mi (zero
-// for n_um <= 0), A2[(u,i)][m] = F_umi / var_mi.  One thread per (u, i, m).
-__global__ __launch_bounds__(256) void mllr_operands_kernel(const double *__restrict__ n,
-                                                            const double *__restrict__ F,
-                                                            const float *__restrict__ var, int U,
-                                                            int M, int D, int ldk,
-                                                            double *__restrict__ A1,
-                                                            double *__restrict__ A2) {
-    const uint64_t idx = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint64_t total = (uint64_t)U * D * M;
-    if (idx >= total) return;
-    const int m = (int)(idx % M);
-    const uint64_t ui = idx / M;
-    const int i = (int)(ui % D);
-    const int u = (int)(ui / D);
-    const double nm = n[(size_t)u * M + m];
-    const double iv = 1.0 / (double)var[(size_t)m * D + i];
-    const bool act = nm > 0.0;
-    A1[ui * ldk + m] = act ? nm * iv : 0.0;
-    A2[ui * ldk + m] = act ? F[((size_t)u * M + m) * D + i] * iv : 0.0;
-}
-
-// ---------------------------------------------------------------------------
 // CheckInvertibility (OL_ModelAdapt.cpp:1851-1878) per utterance: status 0, or
 // 7 when fewer than D+1 components are active or the occupancy is below D+1.
 __global__ __launch_bounds__(256) void mllr_check_kernel(const double *__restrict__ n, int U, int M,
@@ -145,72 +122,133 @@ __global__ __launch_bounds__(256) void mllr_check_kernel(const double *__restric
 }
 
 // ---------------------------------------------------------------------------
-// FP64 GEMM C[R][Cn] = A[R][K] B[K][Cn] (row-major, leading dimensions lda,
-// ldb, ldc) on v_mfma_f64_16x16x4f64 (operands as the f32 16x16x4 form:
-// lane (q, c) gives A[c][q] and B[q][c]; the result of lane (q, c) register r
-// is row q + 4r, column c).  Block tile 64 x 64, four waves of
-// 32 x 32, K staged through LDS 16 at a time; out-of-range rows, columns and
-// K are zero-filled.
-__global__ __launch_bounds__(256) void gemm_f64_kernel(const double *__restrict__ A, int lda,
-                                                       const double *__restrict__ B, int ldb,
-                                                       double *__restrict__ C, int ldc, int R,
-                                                       int Cn, int K) {
-    __shared__ double As[64][17];
-    __shared__ double Bs[16][65];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+// MLLR G / k statistics of every utterance (ComputeKiGi_MLLR,
+// OL_ModelAdapt.cpp:1525-1581):
+//   C[(u,i)][col] = sum over components m with n[u][m] > 0 of
+//                   A_u[i][m] B[m][col],
+//   A_u[i][m] = n[u][m] ivar[m][i] (G) or F[u][m][i] ivar[m][i] (k).
+// Block = (utterance u, 64-row tile, 128-column tile).  The block first compacts the
+// components u occupies into an ordered list in LDS (a wave ballot and
+// prefix count per 256 components), so the K loop runs over those only —
+// a few hundred of 2048 for a 300-frame utterance — and the result depends
+// on u's statistics alone.  Then a 64 x 128 tile (rows i, four waves of
+// 32 x 64, 8 MFMA tiles each) accumulates on v_mfma_f64_16x16x4f64 (lane
+// (q, c) gives A[c][q] and B[q][c]; result register r of lane (q, c) is row
+// q + 4r, column c), K staged 16 components at a time through k-major LDS
+// tiles with the next chunk prefetched into registers.  The column tiles of
+// one utterance run on one XCD.
+template <int KMODE>
+__global__ __launch_bounds__(256) void mllr_gk_kernel(MllrGkArgs a) {
+    constexpr int LDA = 80, LDB = 144;  // row strides (doubles) of the k-major tiles
+    constexpr int MAXM = MLLR_MAX_COMPONENTS;
+    __shared__ double At[16 * LDA];
+    __shared__ double Bt[16 * LDB];
+    __shared__ int act[MAXM];
+    __shared__ int wcount[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int q = lane >> 4, c = lane & 15;
-    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    f64x4 acc[2][2];
+    const int ncol = (a.Cn + 127) / 128, nrt = (a.D + 63) / 64;
+    const int b = blockIdx.x, xcd = b & 7, idx = b >> 3;
+    const int u = (idx / (ncol * nrt)) * 8 + xcd;
+    const int rt = (idx % (ncol * nrt)) / ncol, ct = idx % ncol;
+    if (u >= a.U) return;
+    const int c0 = ct * 128, i0 = rt * 64;
+    const double *nu = a.n + (size_t)u * a.M;
+
+    // ordered list of the occupied components
+    int K = 0;
+    for (int m0 = 0; m0 < a.M; m0 += 256) {
+        const int m = m0 + tid;
+        const bool on = m < a.M && nu[m] > 0.0;
+        const unsigned long long bal = __ballot(on);
+        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wcount[wave] = __popcll(bal);
+        __syncthreads();
+        int off = K;
+        for (int w = 0; w < wave; w++) off += wcount[w];
+        if (on) act[off + before] = m;
+        K += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        __syncthreads();
+    }
+
+    // staging: A tile 16 x 64 (4 per thread), B tile 16 x 128 (8 per thread)
+    const int ai = tid & 63, ak = tid >> 6;     // A: row i0 + ai, K rows ak + 4j
+    const int bc = tid & 127, bk = tid >> 7;    // B: column, K rows bk + 2j
+    const bool iok = i0 + ai < a.D;
+    const bool cok = c0 + bc < a.Cn;
+    const double *fu = a.F + (size_t)u * a.M * a.D + i0 + ai;
+    const double *ip = a.ivar + i0 + ai;
+    const double *bp = a.B + c0 + bc;
+    double pa[4], pb[8];
+#define GMMHIP_GK_LOAD(K0_)                                                      \
+    _Pragma("unroll") for (int j = 0; j < 4; j++) {                              \
+        const int kk = (K0_) + ak + 4 * j;                                       \
+        double v = 0.0;                                                          \
+        if (kk < K && iok) {                                                     \
+            const int m = act[kk];                                               \
+            const double x = KMODE ? fu[(size_t)m * a.D] : nu[m];                \
+            v = x * ip[(size_t)m * a.D];                                         \
+        }                                                                        \
+        pa[j] = v;                                                               \
+    }                                                                            \
+    _Pragma("unroll") for (int j = 0; j < 8; j++) {                              \
+        const int kk = (K0_) + bk + 2 * j;                                       \
+        pb[j] = (kk < K && cok) ? bp[(size_t)act[kk] * a.ldb] : 0.0;             \
+    }
+    GMMHIP_GK_LOAD(0)
+
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 64;
+    f64x4 acc[2][4];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = (f64x4){0.0, 0.0, 0.0, 0.0};
+        for (int y = 0; y < 4; y++) acc[x][y] = (f64x4){0.0, 0.0, 0.0, 0.0};
 
     for (int k0 = 0; k0 < K; k0 += 16) {
         __syncthreads();
-        for (int i = tid; i < 64 * 16; i += 256) {
-            const int rr = i >> 4, kk = i & 15;
-            const int gr = r0 + rr, gk = k0 + kk;
-            As[rr][kk] = (gr < R && gk < K) ? A[(size_t)gr * lda + gk] : 0.0;
-        }
-        for (int i = tid; i < 16 * 64; i += 256) {
-            const int kk = i >> 6, cc = i & 63;
-            const int gk = k0 + kk, gc = c0 + cc;
-            Bs[kk][cc] = (gk < K && gc < Cn) ? B[(size_t)gk * ldb + gc] : 0.0;
-        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) At[(ak + 4 * j) * LDA + ai] = pa[j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) Bt[(bk + 2 * j) * LDB + bc] = pb[j];
         __syncthreads();
+        if (k0 + 16 < K) {
+            GMMHIP_GK_LOAD(k0 + 16)
+        }
 #pragma unroll
         for (int kk = 0; kk < 16; kk += 4) {
+            double av[2], bv[4];
 #pragma unroll
-            for (int a = 0; a < 2; a++) {
-                const double av = As[wr + 16 * a + c][kk + q];
+            for (int x = 0; x < 2; x++) av[x] = At[(kk + q) * LDA + wr + 16 * x + c];
 #pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    const double bv = Bs[kk + q][wc + 16 * b + c];
-                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[a][b], 0, 0, 0);
-                }
-            }
+            for (int y = 0; y < 4; y++) bv[y] = Bt[(kk + q) * LDB + wc + 16 * y + c];
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++)
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
         }
     }
+#undef GMMHIP_GK_LOAD
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+        for (int y = 0; y < 4; y++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                // v_mfma_f64_16x16x4f64 result: lane (q, c), register r holds
-                // row q + 4r of the 16 x 16 tile
-                const int gr = r0 + wr + 16 * a + q + 4 * r, gc = c0 + wc + 16 * b + c;
-                if (gr < R && gc < Cn) C[(size_t)gr * ldc + gc] = acc[a][b][r];
+                const int i = i0 + wr + 16 * x + q + 4 * r, col = c0 + wc + 16 * y + c;
+                if (i < a.D && col < a.Cn) a.C[((size_t)u * a.D + i) * a.ldc + col] = acc[x][y][r];
             }
 }
 
-hipError_t launch_gemm_f64(const double *A, int lda, const double *B, int ldb, double *C, int ldc,
-                           int R, int Cn, int K, hipStream_t s) {
-    const dim3 grid((Cn + 63) / 64, (R + 63) / 64);
-    hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, R, Cn, K);
+hipError_t launch_mllr_gk(const MllrGkArgs &a, hipStream_t s) {
+    if (a.U <= 0) return hipSuccess;
+    if (a.M > MLLR_MAX_COMPONENTS) return hipErrorInvalidValue;
+    const int ncol = (a.Cn + 127) / 128, nrt = (a.D + 63) / 64;
+    const dim3 grid((unsigned)((a.U + 7) / 8 * 8 * ncol * nrt));
+    if (a.kmode)
+        hipLaunchKernelGGL(mllr_gk_kernel<1>, grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(mllr_gk_kernel<0>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -324,14 +362,6 @@ hipError_t launch_map_means(const double *n, const double *F, const float *mu, i
     const uint64_t total = (uint64_t)U * M * D;
     hipLaunchKernelGGL(map_means_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, n,
                        F, mu, U, M, D, tau, out);
-    return hipGetLastError();
-}
-
-hipError_t launch_mllr_operands(const double *n, const double *F, const float *var, int U, int M,
-                                int D, int ldk, double *A1, double *A2, hipStream_t s) {
-    const uint64_t total = (uint64_t)U * D * M;
-    hipLaunchKernelGGL(mllr_operands_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       s, n, F, var, U, M, D, ldk, A1, A2);
     return hipGetLastError();
 }
 

@@ -7,8 +7,9 @@
 //   stats_queue_kernel: a persistent grid pulls (chunk, component group)
 //     items from a device counter; chunks never straddle utterances,
 //   seg_reduce_kernel: per-utterance FP64 sums of the chunk partials,
-//   map_means_kernel, and for MLLR mllr_check / mllr_operands / two FP64
-//     GEMMs against UBM-only matrices / mllr_solve.
+//   map_means_kernel, and for MLLR mllr_check / two FP64 GEMMs (G, k)
+//     against UBM-only matrices with the A operand formed from n, F and
+//     1/var while staged / mllr_solve.
 // With cfg->graph the whole pipeline is captured once into a hipGraph and
 // replayed while the batch shape and buffers stay the same.
 #include "batch.h"
@@ -26,13 +27,15 @@ struct gmmhip_batch_state {
     size_t n_cap = 0, F_cap = 0;
     float *d_means = nullptr;
     size_t means_cap = 0;
-    double *d_A1 = nullptr, *d_A2 = nullptr, *d_G = nullptr, *d_K = nullptr, *d_W = nullptr;
-    size_t A1_cap = 0, A2_cap = 0, G_cap = 0, K_cap = 0, W_cap = 0;
+    double *d_G = nullptr, *d_K = nullptr, *d_W = nullptr;
+    size_t G_cap = 0, K_cap = 0, W_cap = 0;
     int *d_status = nullptr;
     size_t status_cap = 0;
-    // UBM-only MLLR operands: P[m][packed (a<=b)] = xi_a xi_b, Xi[m][a], prior
+    // UBM-only MLLR operands: P[m][packed (a<=b)] = xi_a xi_b, Xi[m][a], prior,
+    // 1/var in FP64
     double *d_P = nullptr, *d_Xi = nullptr, *d_Ginit = nullptr, *d_kinit = nullptr;
-    size_t P_cap = 0, Xi_cap = 0, Ginit_cap = 0, kinit_cap = 0;
+    double *d_ivar = nullptr;
+    size_t P_cap = 0, Xi_cap = 0, Ginit_cap = 0, kinit_cap = 0, ivar_cap = 0;
     uint64_t mllr_model = ~0ull;
     // graph cache
     hipGraph_t graph = nullptr;
@@ -46,10 +49,9 @@ void batch_state_free(gmmhip_batch_state *b) {
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     for (void *p : {(void *)b->d_lo, (void *)b->d_hi, (void *)b->d_first, (void *)b->d_queue,
-                    (void *)b->d_n, (void *)b->d_F, (void *)b->d_means, (void *)b->d_A1,
-                    (void *)b->d_A2, (void *)b->d_G, (void *)b->d_K, (void *)b->d_W,
-                    (void *)b->d_status, (void *)b->d_P, (void *)b->d_Xi, (void *)b->d_Ginit,
-                    (void *)b->d_kinit})
+                    (void *)b->d_n, (void *)b->d_F, (void *)b->d_means, (void *)b->d_G,
+                    (void *)b->d_K, (void *)b->d_W, (void *)b->d_status, (void *)b->d_P,
+                    (void *)b->d_Xi, (void *)b->d_Ginit, (void *)b->d_kinit, (void *)b->d_ivar})
         if (p) hipFree(p);
     delete b;
 }
@@ -61,7 +63,8 @@ namespace {
 int build_mllr_operands(gmmhip_ctx *h, gmmhip_batch_state *b) {
     if (b->mllr_model == h->model_version) return GMMHIP_OK;
     const int M = h->M, D = h->D, E = D + 1, NP = E * (E + 1) / 2;
-    std::vector<double> P((size_t)M * NP), Xi((size_t)M * E);
+    std::vector<double> P((size_t)M * NP), Xi((size_t)M * E), iv((size_t)M * D);
+    for (size_t j = 0; j < iv.size(); j++) iv[j] = 1.0 / (double)h->m_var[j];
     std::vector<double> Gi((size_t)D * E * E, 0.0), ki((size_t)D * E, 0.0);
     std::vector<double> xi(E);
     const double minv = h->opt.min_var;
@@ -94,6 +97,8 @@ int build_mllr_operands(gmmhip_ctx *h, gmmhip_batch_state *b) {
     HIPCHK(h, ensure(&b->d_Xi, &b->Xi_cap, Xi.size()));
     HIPCHK(h, ensure(&b->d_Ginit, &b->Ginit_cap, Gi.size()));
     HIPCHK(h, ensure(&b->d_kinit, &b->kinit_cap, ki.size()));
+    HIPCHK(h, ensure(&b->d_ivar, &b->ivar_cap, iv.size()));
+    HIPCHK(h, hipMemcpyAsync(b->d_ivar, iv.data(), iv.size() * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(b->d_P, P.data(), P.size() * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(b->d_Xi, Xi.data(), Xi.size() * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(b->d_Ginit, Gi.data(), Gi.size() * 8, hipMemcpyHostToDevice, h->stream));
@@ -124,7 +129,7 @@ int model_operands(gmmhip_ctx *h, bool mllr, ModelOperands *o) {
     }
     o->mu = h->d_mu_f32;
     o->var = h->d_var_f32;
-    o->P = o->Xi = o->Ginit = o->kinit = nullptr;
+    o->P = o->Xi = o->Ginit = o->kinit = o->ivar = nullptr;
     if (mllr) {
         const int rc = build_mllr_operands(h, b);
         if (rc) return rc;
@@ -132,6 +137,7 @@ int model_operands(gmmhip_ctx *h, bool mllr, ModelOperands *o) {
         o->Xi = b->d_Xi;
         o->Ginit = b->d_Ginit;
         o->kinit = b->d_kinit;
+        o->ivar = b->d_ivar;
     }
     return GMMHIP_OK;
 }
@@ -225,10 +231,27 @@ int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
         const int E = p.D + 1, NP = E * (E + 1) / 2;
         const int R = p.U * p.D;
         HIPCHK(h, launch_mllr_check(b->d_n, p.U, p.M, p.D, b->d_status, s));
-        HIPCHK(h, launch_mllr_operands(b->d_n, b->d_F, h->d_var_f32, p.U, p.M, p.D, p.M, b->d_A1,
-                                       b->d_A2, s));
-        HIPCHK(h, launch_gemm_f64(b->d_A1, p.M, b->d_P, NP, b->d_G, NP, R, NP, p.M, s));
-        HIPCHK(h, launch_gemm_f64(b->d_A2, p.M, b->d_Xi, E, b->d_K, E, R, E, p.M, s));
+        MllrGkArgs g{};
+        g.n = b->d_n;
+        g.F = b->d_F;
+        g.ivar = b->d_ivar;
+        g.U = p.U;
+        g.M = p.M;
+        g.D = p.D;
+        g.B = b->d_P;
+        g.ldb = NP;
+        g.Cn = NP;
+        g.C = b->d_G;
+        g.ldc = NP;
+        g.kmode = 0;
+        HIPCHK(h, launch_mllr_gk(g, s));
+        g.B = b->d_Xi;
+        g.ldb = E;
+        g.Cn = E;
+        g.C = b->d_K;
+        g.ldc = E;
+        g.kmode = 1;
+        HIPCHK(h, launch_mllr_gk(g, s));
         MllrSolveArgs m{};
         m.Gp = b->d_G;
         m.Kp = b->d_K;
@@ -261,6 +284,9 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
             return fail(h, GMMHIP_ERR_ARG, "batch_adapt: offsets must be non-decreasing");
     if (out->mllr_W && !out->mllr_status)
         return fail(h, GMMHIP_ERR_ARG, "batch_adapt: mllr_W needs mllr_status");
+    if (out->mllr_W && h->M > MLLR_MAX_COMPONENTS)
+        return fail(h, GMMHIP_ERR_ARG, "batch_adapt: MLLR supports at most %d components",
+                    MLLR_MAX_COMPONENTS);
     const uint64_t N = offsets[U];
     if (N == 0) return fail(h, GMMHIP_ERR_ARG, "batch_adapt: no frames");
     DevGuard g(h->dev);
@@ -327,8 +353,6 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
     }
     if (p.want_means) HIPCHK(h, ensure(&b->d_means, &b->means_cap, (size_t)U * M * D));
     if (p.want_mllr) {
-        HIPCHK(h, ensure(&b->d_A1, &b->A1_cap, (size_t)U * D * M));
-        HIPCHK(h, ensure(&b->d_A2, &b->A2_cap, (size_t)U * D * M));
         HIPCHK(h, ensure(&b->d_G, &b->G_cap, (size_t)U * D * NP));
         HIPCHK(h, ensure(&b->d_K, &b->K_cap, (size_t)U * D * E));
         HIPCHK(h, ensure(&b->d_W, &b->W_cap, (size_t)U * D * E));

@@ -16,7 +16,7 @@
 //   mixed form: prep_frames (z = (x - o) * s split hi/lo), loglik_mixed,
 //               stats_mixed over the MAP/MLLR chunks;
 //   seg_reduce (n, F per request), seg_sum (LL per request),
-//   map_means (MAP requests), mllr_check / mllr_operands / two FP64 GEMMs /
+//   map_means (MAP requests), mllr_check / the two FP64 G, k GEMMs /
 //   mllr_solve (MLLR requests),
 //   one D2H copy of [LL | MLLR status | n | W | means | per-frame LL].
 // Chunks start at each request's first frame, the mixed form's centring o
@@ -71,7 +71,7 @@ struct Dims {
 struct Layout {
     size_t beg, end, lo, hi, first, X, in_end;
     size_t ll, status, n, W, means, LLnat, out_end, out_copy;
-    size_t queue, LL2, blockLL, part, F, A1, A2, G, K;
+    size_t queue, LL2, blockLL, part, F, G, K;
     size_t zah, zal, zbh, zbl, bad, nrange, total;
 };
 
@@ -106,8 +106,6 @@ Layout layout_of(const Shape &s, const Dims &d) {
     L.blockLL = take(pad128(s.N) / 128 * 8);
     L.part = take((size_t)s.nchunks * Mp * KP * 4);
     L.F = take((size_t)Us * d.M * d.D * 8);
-    L.A1 = take((size_t)s.Umllr * d.D * d.M * 8);
-    L.A2 = take((size_t)s.Umllr * d.D * d.M * 8);
     L.G = take((size_t)s.Umllr * d.D * d.NP * 8);
     L.K = take((size_t)s.Umllr * d.D * d.E * 8);
     const size_t zb = s.mixed ? Nz * KP * 2 : 0;
@@ -395,13 +393,29 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
         const double *n = (const double *)(A + L.n) + (size_t)sh.Umap * d.M;
         const double *F = (const double *)(A + L.F) + (size_t)sh.Umap * d.M * d.D;
         int *status = (int *)(A + L.status);
-        double *A1 = (double *)(A + L.A1), *A2 = (double *)(A + L.A2);
         double *G = (double *)(A + L.G), *K = (double *)(A + L.K);
-        const int R = sh.Umllr * d.D;
         SRV_HIPCHK(launch_mllr_check(n, sh.Umllr, d.M, d.D, status, st));
-        SRV_HIPCHK(launch_mllr_operands(n, F, s->mo.var, sh.Umllr, d.M, d.D, d.M, A1, A2, st));
-        SRV_HIPCHK(launch_gemm_f64(A1, d.M, s->mo.P, d.NP, G, d.NP, R, d.NP, d.M, st));
-        SRV_HIPCHK(launch_gemm_f64(A2, d.M, s->mo.Xi, d.E, K, d.E, R, d.E, d.M, st));
+        MllrGkArgs g{};
+        g.n = n;
+        g.F = F;
+        g.ivar = s->mo.ivar;
+        g.U = sh.Umllr;
+        g.M = d.M;
+        g.D = d.D;
+        g.B = s->mo.P;
+        g.ldb = d.NP;
+        g.Cn = d.NP;
+        g.C = G;
+        g.ldc = d.NP;
+        g.kmode = 0;
+        SRV_HIPCHK(launch_mllr_gk(g, st));
+        g.B = s->mo.Xi;
+        g.ldb = d.E;
+        g.Cn = d.E;
+        g.C = K;
+        g.ldc = d.E;
+        g.kmode = 1;
+        SRV_HIPCHK(launch_mllr_gk(g, st));
         MllrSolveArgs m{};
         m.Gp = G;
         m.Kp = K;
@@ -664,6 +678,7 @@ extern "C" int gmmhip_server_create(int device, const gmmhip_server_cfg *cfg, in
     if (c.slot_bytes == 0) c.slot_bytes = 1ull << 30;
     if (c.max_batch <= 0) c.max_batch = 1024;
     if (c.chunk_frames < 0 || c.tau < 0) return GMMHIP_ERR_ARG;
+    if (M > MLLR_MAX_COMPONENTS) return GMMHIP_ERR_ARG;  // MLLR requests
     gmmhip_server *s = new gmmhip_server();
     s->cfg = c;
     s->dev = device;

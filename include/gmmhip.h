@@ -183,6 +183,7 @@ typedef struct gmmhip_batch_cfg {
     int outputs_on_device;  /* 1: the output pointers are device memory */
 } gmmhip_batch_cfg;
 
+/* MLLR outputs need M <= 4096 (GMMHIP_ERR_ARG otherwise). */
 typedef struct gmmhip_batch_out {
     float *means;           /* [U][M][D] MAP-adapted means, or NULL */
     double *occ;            /* [U][M] occupancies n, or NULL */
@@ -291,7 +292,8 @@ typedef struct gmmhip_server_stats {
                                   split coefficients overflow FP16) */
 } gmmhip_server_stats;
 
-/* M active components (weights w[M], means mu[M][D], variances var[M][D]). */
+/* M active components (weights w[M], means mu[M][D], variances var[M][D]),
+ * M <= 4096. */
 int gmmhip_server_create(int device, const gmmhip_server_cfg *cfg, int M, int D, const float *w,
                          const float *mu, const float *var, gmmhip_server_t *out);
 /* Queue a request (non-blocking).  Sets r->status = GMMHIP_REQ_PENDING. */

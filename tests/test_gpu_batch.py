@@ -135,3 +135,19 @@ def test_batch_matches_single_accumulate(acc):
     acc.set_opts(chunk_frames=0)
     assert np.allclose(res["occ"][0], n, rtol=1e-12, atol=1e-12)
     assert np.allclose(res["first"][0], F, rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("D,M", [(70, 160), (20, 600)])
+def test_batch_mllr_wide_and_many_components(acc, D, M):
+    """D = 70: two 64-row tiles in the G/k kernel and the E > 64 solve;
+    M = 600: occupied-component lists of several hundred entries."""
+    x, off, w, mu, var = make_batch(7 + D, 5, M, D, lo=1500, hi=2500)
+    acc.set_model(w, mu, var)
+    res = acc.batch_adapt(x, off, tau=16, prior=True, means=False)
+    ref = per_utt_oracle(x, off, w, mu, var)
+    for u in range(len(off) - 1):
+        assert res["status"][u] == 0, u
+        Wg = orc.mllr_transform(mu, var, res["occ"][u], res["first"][u], ubm_mu=mu, ubm_var=var)
+        assert np.allclose(res["W"][u], Wg, rtol=1e-8, atol=1e-8), u
+        Wo = orc.mllr_transform(mu, var, ref[u][0], ref[u][1], ubm_mu=mu, ubm_var=var)
+        assert np.allclose(res["W"][u], Wo, rtol=1e-3, atol=1e-3), u
