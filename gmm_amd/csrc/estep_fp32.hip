@@ -157,13 +157,13 @@ __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
             for (int s = 0; s < SUB; s++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
+                    // branch-free: a value more than 32 above the reference
+                    // moves it (sm * 2^-d + 1), otherwise sm += 2^d
                     const float d = acc[s][r] - mx[s][r];
-                    if (d > 32.0f) {
-                        sm[s][r] = fmaf(sm[s][r], fast_exp2(-d), 1.0f);
-                        mx[s][r] = acc[s][r];
-                    } else {
-                        sm[s][r] += fast_exp2(d);
-                    }
+                    const bool up = d > 32.0f;
+                    const float e = fast_exp2(up ? -d : d);
+                    sm[s][r] = up ? fmaf(sm[s][r], e, 1.0f) : sm[s][r] + e;
+                    mx[s][r] = up ? acc[s][r] : mx[s][r];
                 }
         }
     }
@@ -222,12 +222,24 @@ __global__ __launch_bounds__(256) void loglik_kernel(EstepArgs a) {
 // fetched from HBM once per XCD L2.  The next 64 frames are prefetched into
 // registers while the current ones are contracted.
 //
-// Statistics column mapping: MFMA jt of the g^T Z contraction covers columns
-// c*JT + jt (c = lane%16), so a lane's B operands are consecutive in the LDS
-// row.  Columns are [1, x', x'^2, 0...]: n, F', S'.
+// Statistics column mapping: MFMA jt of the g^T Z contraction covers column
+// stats_col(c, jt) of lane c = lane%16.  Columns are [1, x', x'^2, 0...]:
+// n, F', S'.
 // Two waves per SIMD (<= 256 VGPRs) up to KP = 160 (D <= 79); larger
 // depths keep their accumulators in registers at one wave per SIMD.
 #define GMMHIP_STATS_BOUNDS __launch_bounds__(256, (KP <= 160 ? 2 : 1))
+
+// Statistics column of MFMA jt in lane c (c = lane % 16).  When KP is a
+// multiple of 64 the lane's columns come in runs of 4 spaced 64 apart, so
+// each 16-byte LDS read of the g^T Z contraction covers 16 consecutive
+// 16-byte pieces across the 16 lanes (distinct banks); otherwise a lane owns
+// JT consecutive columns.
+template <int KP>
+__device__ __forceinline__ int stats_col(int c, int jt) {
+    constexpr int JT = KP / 16;
+    if constexpr (JT % 4 == 0) return (jt >> 2) * 64 + 4 * c + (jt & 3);
+    else return c * JT + jt;
+}
 
 // Zr rows are read as 16-byte vectors: keep the tile 16-byte aligned.
 template <int KP>
@@ -362,9 +374,9 @@ __device__ __forceinline__ void stats_item(const EstepArgs &a, StatsShared<KP> &
                 const int row = 16 * s + 4 * q + r;
                 float g = fast_exp2(acc1[s][r] - Ls[row]);
                 g = (g > a.min_gamma) ? g : 0.0f;
-                const float *zr = &Zr[row * S + c * JT];
+                const float *zr = &Zr[row * S];
 #pragma unroll
-                for (int j = 0; j < JT; j++) acc2[j] = mfma4(g, zr[j], acc2[j]);
+                for (int j = 0; j < JT; j++) acc2[j] = mfma4(g, zr[stats_col<KP>(c, j)], acc2[j]);
             }
         }
     }
@@ -377,7 +389,7 @@ __device__ __forceinline__ void stats_item(const EstepArgs &a, StatsShared<KP> &
         for (int r = 0; r < 4; r++)
 #pragma unroll
             for (int j = 0; j < JT; j++)
-                out[(size_t)(4 * q + r) * KP + c * JT + j] = acc2[j][r];
+                out[(size_t)(4 * q + r) * KP + stats_col<KP>(c, j)] = acc2[j][r];
     }
 }
 
@@ -394,9 +406,9 @@ __device__ __forceinline__ void stats_init_shared(const EstepArgs &a, StatsShare
 // fetched from HBM once per XCD L2.  The next 64 frames are prefetched into
 // registers while the current ones are contracted.
 //
-// Statistics column mapping: MFMA jt of the g^T Z contraction covers columns
-// c*JT + jt (c = lane%16), so a lane's B operands are consecutive in the LDS
-// row.  Columns are [1, x', x'^2, 0...]: n, F', S'.
+// Statistics column mapping: MFMA jt of the g^T Z contraction covers column
+// stats_col(c, jt) of lane c = lane%16.  Columns are [1, x', x'^2, 0...]:
+// n, F', S'.
 template <int KP>
 __global__ GMMHIP_STATS_BOUNDS void stats_kernel(EstepArgs a) {
     __shared__ StatsShared<KP> sh;

@@ -13,11 +13,12 @@
 // layouts (prep_frames_kernel):
 //   ZA[n][k]                   row-major: the log-likelihood contraction
 //                              reads 8 consecutive k of one frame per lane;
-//   ZB[blk][u][ct][c][h][j]    per 32-frame block: the statistics
+//   ZB[blk][u][ct][h][c][j]    per 32-frame block: the statistics
 //                              contraction over frames reads, for column
 //                              32ct+c, the 8 frames 16u + 8(j/4) + 4h + j%4
 //                              that lane (h, c) holds after the first
-//                              contraction.
+//                              contraction (the 32 lanes of a half-wave read
+//                              consecutive 16-byte pieces).
 // MFMA 32x32x16 lane layout (lane = 32h + c): A[row c][k 8h+j],
 // B[k 8h+j][col c], C reg r -> row 8(r/4) + 4h + r%4, col c.
 #include "estep.h"
@@ -136,7 +137,9 @@ __global__ __launch_bounds__(256) void prep_frames_kernel(PrepArgs p) {
     const int u = f >> 4, r16 = f & 15;
     const int h = (r16 >> 2) & 1, j = ((r16 >> 3) << 2) | (r16 & 3);
     const int ct = k >> 5, c = k & 31, CT = KP >> 5;
-    const uint64_t zb = blk * 32 * (uint64_t)KP + ((((uint64_t)u * CT + ct) * 32 + c) * 2 + h) * 8 + j;
+    // [blk][u][ct][h][c][j]: the 32 lanes of each half-wave read consecutive
+    // 16-byte pieces
+    const uint64_t zb = blk * 32 * (uint64_t)KP + ((((uint64_t)u * CT + ct) * 2 + h) * 32 + c) * 8 + j;
     p.zb_hi_raw[zb] = hb;
     p.zb_lo_raw[zb] = lb;
 }
@@ -213,14 +216,14 @@ __global__ __launch_bounds__(256) void loglik_mixed_kernel(MixedArgs a) {
         } else {
 #pragma unroll
             for (int r = 0; r < 16; r++) {
+                // branch-free: a value more than 32 above the reference moves
+                // it (sm * 2^-d + 1), otherwise sm += 2^d
                 const float v = cvalid ? accA[r] + accB[r] : -INFINITY;
                 const float d = v - mx[r];
-                if (d > 32.0f) {
-                    sm[r] = fmaf(sm[r], exp2f_fast(-d), 1.0f);
-                    mx[r] = v;
-                } else {
-                    sm[r] += exp2f_fast(d);
-                }
+                const bool up = d > 32.0f;
+                const float e = exp2f_fast(up ? -d : d);
+                sm[r] = up ? fmaf(sm[r], e, 1.0f) : sm[r] + e;
+                mx[r] = up ? v : mx[r];
             }
         }
     }
@@ -395,8 +398,8 @@ __global__ __launch_bounds__(256, (KP <= 192 ? 2 : 1)) void stats_mixed_kernel(M
         for (int u = 0; u < 2; u++) {
 #pragma unroll
             for (int ct = 0; ct < CT; ct++) {
-                const u32x4 yh4 = ZBl[0][((u * CT + ct) * 32 + c) * 2 + h];
-                const u32x4 yl4 = ZBl[1][((u * CT + ct) * 32 + c) * 2 + h];
+                const u32x4 yh4 = ZBl[0][((u * CT + ct) * 2 + h) * 32 + c];
+                const u32x4 yl4 = ZBl[1][((u * CT + ct) * 2 + h) * 32 + c];
                 const V8 yh = __builtin_bit_cast(V8, yh4);
                 const V8 yl = __builtin_bit_cast(V8, yl4);
                 acc2[ct] = mfma32(ghi[u], yh, acc2[ct]);
@@ -416,6 +419,415 @@ __global__ __launch_bounds__(256, (KP <= 192 ? 2 : 1)) void stats_mixed_kernel(M
             const int row = 8 * (r >> 2) + 4 * h + (r & 3);
             out[(size_t)row * KP + 32 * ct + c] = acc2[ct][r];
         }
+}
+
+// Two FP32 values split into packed 16-bit hi and lo words (hi = T(v),
+// lo = T(v - hi), round to nearest), as the A operand words of the frame
+// contraction.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<_Float16> {
+    typedef f16x2 type;
+};
+template <>
+struct Vec2<__bf16> {
+    typedef bf16x2 type;
+};
+template <typename T>
+__device__ __forceinline__ void split_pair(float a, float b, unsigned int &hi, unsigned int &lo) {
+    typedef typename Vec2<T>::type T2;
+    const T2 h = __builtin_convertvector((f32x2){a, b}, T2);
+    const f32x2 hb = __builtin_convertvector(h, f32x2);
+    const T2 l = __builtin_convertvector((f32x2){a - hb.x, b - hb.y}, T2);
+    hi = __builtin_bit_cast(unsigned int, h);
+    lo = __builtin_bit_cast(unsigned int, l);
+}
+
+// ---------------------------------------------------------------------------
+// Direct-to-LDS staging (gfx950 global_load_lds_dwordx4: lane L of a wave
+// writes its 16 bytes at the wave-uniform LDS base + 16 L) into two LDS
+// buffers, for depths whose feature row is a power-of-two number of 16-byte
+// pieces (KP = 32, 64, 128, 256).  The copy of step s+1 runs while step s is
+// contracted, with one barrier per step and no staging registers.
+//
+// ZA rows are stored swizzled: piece w of frame row c sits at slot
+// c*U + (w ^ za_swz(c)), so the 16 lanes that an LDS cycle serves (same w,
+// consecutive c) hit 16 distinct bank groups.
+template <int U>
+__device__ __forceinline__ int za_swz(int c) {
+    if constexpr (U >= 16) return c & 15;
+    else return (c / (16 / U)) & (U - 1);
+}
+
+__device__ __forceinline__ void lds_dma16(const void *src, void *lds_wave_base) {
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)lds_wave_base,
+                                     16, 0, 0);
+}
+
+// Statistics with direct-to-LDS staging: the four waves copy, per 32-frame
+// step, ZA hi, ZA lo, ZB hi and ZB lo (one part each, U pieces x 32 rows).
+template <int KP, typename T>
+__global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kernel(MixedArgs a) {
+    typedef typename Vec8<T>::type V8;
+    constexpr int KS = KP / 16;
+    constexpr int CT = KP / 32;
+    constexpr int U = KP / 8;               // 16-byte pieces per feature row
+    constexpr int PART = 32 * U;            // pieces per part and step
+    constexpr int NDMA = PART / 64;         // copies per wave and step
+    static_assert((U & (U - 1)) == 0 && PART % 64 == 0, "power-of-two feature rows");
+    __shared__ u32x4 Zs[2][4][PART];
+    __shared__ float Ls[2][32];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, c = lane & 31;
+
+    const int b = blockIdx.x;
+    const int xcd = b & 7, idx = b >> 3;
+    const int group = idx % a.ngroups;
+    const int chunk = (idx / a.ngroups) * 8 + xcd;
+    if (chunk >= a.nchunks) return;
+    const int tile = group * 4 + wave;
+    const bool active = tile < a.Mt32;
+    const bool cvalid = (32 * tile + c) < a.M;
+    // posterior threshold: padded components never pass
+    const float thr = cvalid ? a.min_gamma : INFINITY;
+
+    V8 bhi[KS], blo[KS];
+    {
+        const u32x4 *P16 = reinterpret_cast<const u32x4 *>(a.P16);
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            u32x4 x = (u32x4){0u, 0u, 0u, 0u}, y = (u32x4){0u, 0u, 0u, 0u};
+            if (active) {
+                x = P16[((size_t)tile * KS * 2 + ks * 2 + 0) * 64 + lane];
+                y = P16[((size_t)tile * KS * 2 + ks * 2 + 1) * 64 + lane];
+            }
+            bhi[ks] = __builtin_bit_cast(V8, x);
+            blo[ks] = __builtin_bit_cast(V8, y);
+        }
+    }
+    f32x16 acc2[CT];
+#pragma unroll
+    for (int j = 0; j < CT; j++) acc2[j] = (f32x16){};
+
+    uint64_t f0, f1;
+    if (a.chunk_lo) {
+        f0 = a.chunk_lo[chunk];
+        f1 = a.chunk_hi[chunk];
+    } else {
+        f0 = (uint64_t)chunk * a.chunk_frames;
+        f1 = f0 + a.chunk_frames;
+        if (f1 > a.N) f1 = a.N;
+    }
+
+    // this wave's part and the source piece of each of its copies within a
+    // 32-frame block (ZA parts de-swizzle, ZB parts copy straight)
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(
+        wave == 0 ? a.za_hi : wave == 1 ? a.za_lo : wave == 2 ? a.zb_hi : a.zb_lo);
+    int soff[NDMA];
+#pragma unroll
+    for (int j = 0; j < NDMA; j++) {
+        const int p = j * 64 + lane;
+        if (wave < 2) {
+            const int row = p / U, v = p % U;
+            soff[j] = row * U + (v ^ za_swz<U>(row));
+        } else {
+            soff[j] = p;
+        }
+    }
+    // Step k covers frames f0 + 32k.  Step s's posteriors (VALU) interleave
+    // with the first contraction of step s+1 (MFMA), then step s's frame
+    // contraction runs.  So iteration s reads ZA of step s+1 and ZB of step
+    // s: ZA is copied two steps ahead and ZB one step ahead, each into the
+    // buffer its previous occupant has left.
+    const int nsteps = (int)((f1 - f0 + 31) / 32);
+#define GMMHIP_DMA_PART(K_, BUF_)                                                  \
+    _Pragma("unroll") for (int j = 0; j < NDMA; j++)                               \
+        lds_dma16(src + (f0 + 32 * (uint64_t)(K_)) * U + soff[j], &Zs[BUF_][wave][j * 64]);
+#define GMMHIP_CONTRACT1(ACC_, BUF_)                                               \
+    _Pragma("unroll") for (int ks = 0; ks < KS; ks++) {                            \
+        const int slot = c * U + ((2 * ks + h) ^ za_swz<U>(c));                    \
+        const V8 zh = __builtin_bit_cast(V8, Zs[BUF_][0][slot]);                   \
+        const V8 zl = __builtin_bit_cast(V8, Zs[BUF_][1][slot]);                   \
+        ACC_ = mfma32(zh, bhi[ks], ACC_);                                          \
+        ACC_ = mfma32(zh, blo[ks], ACC_);                                          \
+        ACC_ = mfma32(zl, bhi[ks], ACC_);                                          \
+    }
+    GMMHIP_DMA_PART(0, 0)
+    if (wave < 2 && nsteps > 1) {
+        GMMHIP_DMA_PART(1, 1)
+    }
+    float pl = INFINITY;
+    if (tid < 32) Ls[0][tid] = (f0 + tid < f1) ? a.LL2[f0 + tid] : INFINITY;
+    if (tid < 32) pl = (f0 + 32 + tid < f1) ? a.LL2[f0 + 32 + tid] : INFINITY;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // one FP32 accumulator takes hi*hi, hi*lo and lo*hi
+    f32x16 acc = {};
+    if (active) {
+        GMMHIP_CONTRACT1(acc, 0)
+    }
+
+    for (int s = 0; s < nsteps; s++) {
+        const int cur = s & 1, nxt = cur ^ 1;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid < 32) {
+            Ls[nxt][tid] = pl;
+            const uint64_t fb = f0 + 32 * (uint64_t)(s + 2) + tid;
+            pl = (fb < f1) ? a.LL2[fb] : INFINITY;
+        }
+        if (wave < 2) {
+            if (s + 2 < nsteps) {
+                GMMHIP_DMA_PART(s + 2, cur)
+            }
+        } else if (s + 1 < nsteps) {
+            GMMHIP_DMA_PART(s + 1, nxt)
+        }
+        if (!active) continue;
+
+        // first contraction of step s+1 (on the last step it runs on stale
+        // pieces and is discarded) interleaved with step s's posteriors,
+        // split into hi/lo A operands of the frame contraction: register
+        // r = 8u + j holds frame 16u + 8(j/4) + 4h + j%4; rows r and r+1
+        // (r even) are consecutive frames and one packed word
+        f32x16 nacc = {};
+        GMMHIP_CONTRACT1(nacc, nxt)
+        u32x4 gh4[2], gl4[2];
+#pragma unroll
+        for (int r = 0; r < 16; r += 2) {
+            const int row = 8 * (r >> 2) + 4 * h + (r & 3);
+            float g0 = exp2f_fast(acc[r] - Ls[cur][row]);
+            float g1 = exp2f_fast(acc[r + 1] - Ls[cur][row + 1]);
+            g0 = (g0 > thr) ? g0 : 0.0f;
+            g1 = (g1 > thr) ? g1 : 0.0f;
+            unsigned int wh, wl;
+            split_pair<T>(g0, g1, wh, wl);
+            gh4[r >> 3][(r & 7) >> 1] = wh;
+            gl4[r >> 3][(r & 7) >> 1] = wl;
+        }
+#pragma unroll
+        for (int i = 0; i < KS; i++) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // two LDS reads
+            __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);  // three MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 10, 0); // ten VALU
+        }
+        V8 ghi[2], glo[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            ghi[u] = __builtin_bit_cast(V8, gh4[u]);
+            glo[u] = __builtin_bit_cast(V8, gl4[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                const int slot = ((u * CT + ct) * 2 + h) * 32 + c;
+                const V8 yh = __builtin_bit_cast(V8, Zs[cur][2][slot]);
+                const V8 yl = __builtin_bit_cast(V8, Zs[cur][3][slot]);
+                acc2[ct] = mfma32(ghi[u], yh, acc2[ct]);
+                acc2[ct] = mfma32(ghi[u], yl, acc2[ct]);
+                acc2[ct] = mfma32(glo[u], yh, acc2[ct]);
+            }
+        }
+        acc = nacc;
+    }
+#undef GMMHIP_DMA_PART
+#undef GMMHIP_CONTRACT1
+
+    if (!active) return;
+    const size_t Mp = (size_t)a.Mt32 * 32;
+    float *out = a.partials + ((size_t)chunk * Mp + (size_t)tile * 32) * KP;
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int row = 8 * (r >> 2) + 4 * h + (r & 3);
+            out[(size_t)row * KP + 32 * ct + c] = acc2[ct][r];
+        }
+}
+
+// Online log-sum-exp over one 32-component tile for the 16 frame rows of a
+// lane: first = the block's first tile (initialises the reference mx and the
+// sum sm); a padded component (valid = false, uniform per lane and tile)
+// contributes nothing.  Branch-free: a value more than 32 above the
+// reference moves it (sm 2^-d + 1), otherwise sm += 2^d.
+__device__ __forceinline__ void lse_tile_update(float (&mx)[16], float (&sm)[16],
+                                                const f32x16 &acc, bool valid, bool first) {
+    if (first) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            mx[r] = valid ? acc[r] : -INFINITY;
+            sm[r] = valid ? 1.0f : 0.0f;
+        }
+    } else if (valid) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const float d = acc[r] - mx[r];
+            const bool up = d > 32.0f;
+            const float e = exp2f_fast(up ? -d : d);
+            sm[r] = up ? fmaf(sm[r], e, 1.0f) : sm[r] + e;
+            mx[r] = up ? acc[r] : mx[r];
+        }
+    }
+}
+
+// Log-likelihood with direct-to-LDS staging of the component tiles (two
+// LDS buffers, one barrier per tile); same arithmetic as loglik_mixed_kernel.
+template <int KP, typename T>
+__global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
+    typedef typename Vec8<T>::type V8;
+    constexpr int KS = KP / 16;
+    constexpr int TILE16 = KS * 2 * 64;      // 16-byte pieces per component tile
+    constexpr int PF = TILE16 / 256;
+    static_assert(TILE16 % 256 == 0, "component tile must split evenly over 256 threads");
+    __shared__ u32x4 Pl[2][TILE16];
+    __shared__ double red[8];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, c = lane & 31;
+    const uint64_t base = (uint64_t)blockIdx.x * 128;
+
+    V8 ahi[KS], alo[KS];
+    {
+        const uint64_t fr = base + 32 * wave + c;  // < Npad
+        const u32x4 *zh = reinterpret_cast<const u32x4 *>(a.za_hi) + fr * (KP / 8);
+        const u32x4 *zl = reinterpret_cast<const u32x4 *>(a.za_lo) + fr * (KP / 8);
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const u32x4 x = zh[2 * ks + h], y = zl[2 * ks + h];
+            ahi[ks] = __builtin_bit_cast(V8, x);
+            alo[ks] = __builtin_bit_cast(V8, y);
+        }
+    }
+
+    // mx = -inf, sm = 0: the general update below starts every lane's sum
+    // at its first value (d = +inf takes the reset branch, sm = 0 * 0 + 1)
+    float mx[16], sm[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        mx[r] = -INFINITY;
+        sm[r] = 0.0f;
+    }
+    const u32x4 *P16 = reinterpret_cast<const u32x4 *>(a.P16);
+    // wave w copies pieces [w PF 64, (w+1) PF 64) of each component tile
+#define GMMHIP_DMA_TILE(T_, BUF_)                                                    \
+    _Pragma("unroll") for (int j = 0; j < PF; j++) {                                 \
+        const int o = (wave * PF + j) * 64;                                          \
+        lds_dma16(P16 + (size_t)(T_) * TILE16 + o + lane, &Pl[BUF_][o]);             \
+    }
+#define GMMHIP_TILE_MFMA(ACC_)                                                       \
+    _Pragma("unroll") for (int ks = 0; ks < KS; ks++) {                              \
+        const V8 bh = __builtin_bit_cast(V8, bt[2 * ks]);                            \
+        const V8 bl = __builtin_bit_cast(V8, bt[2 * ks + 1]);                        \
+        ACC_ = mfma32(ahi[ks], bh, ACC_);                                            \
+        ACC_ = mfma32(ahi[ks], bl, ACC_);                                            \
+        ACC_ = mfma32(alo[ks], bh, ACC_);                                            \
+    }
+    u32x4 bt[2 * KS];
+    // tile 0
+    GMMHIP_DMA_TILE(0, 0)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (a.Mt32 > 1) {
+        GMMHIP_DMA_TILE(1, 1)
+    }
+#pragma unroll
+    for (int j = 0; j < 2 * KS; j++) bt[j] = Pl[0][j * 64 + lane];
+    // one FP32 accumulator takes hi*hi, hi*lo and lo*hi
+    f32x16 acc = {};
+    GMMHIP_TILE_MFMA(acc)
+    // tile t's MFMAs interleave with the log-sum-exp update of tile t-1
+    // (full tiles: every tile but the last has 32 valid components)
+    for (int t = 1; t < a.Mt32; t++) {
+        const int buf = t & 1;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (t + 1 < a.Mt32) {
+            GMMHIP_DMA_TILE(t + 1, buf ^ 1)
+        }
+#pragma unroll
+        for (int j = 0; j < 2 * KS; j++) bt[j] = Pl[buf][j * 64 + lane];
+        __builtin_amdgcn_sched_barrier(0);
+        f32x16 nacc = {};
+        GMMHIP_TILE_MFMA(nacc)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const float d = acc[r] - mx[r];
+            const bool up = d > 32.0f;
+            const float e = exp2f_fast(up ? -d : d);
+            sm[r] = up ? fmaf(sm[r], e, 1.0f) : sm[r] + e;
+            mx[r] = up ? acc[r] : mx[r];
+        }
+#pragma unroll
+        for (int i = 0; i < 3 * KS; i++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);  // six VALU
+        }
+        acc = nacc;
+    }
+    // last tile: padded components (>= M) are masked per lane
+    {
+        const bool valid = (32 * (a.Mt32 - 1) + c) < a.M;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            // (a padded lane that saw no component keeps mx = -inf, sm = 0)
+            const float d = acc[r] - mx[r];
+            const bool up = valid && d > 32.0f;
+            const float e = valid ? exp2f_fast(up ? -d : d) : 0.0f;
+            sm[r] = up ? fmaf(sm[r], e, 1.0f) : sm[r] + e;
+            mx[r] = up ? acc[r] : mx[r];
+        }
+    }
+#undef GMMHIP_TILE_MFMA
+
+    // combine the 32 component lanes of each frame row (xor within 32 lanes)
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const float m2 = __shfl_xor(mx[r], o);
+            const float s2 = __shfl_xor(sm[r], o);
+            const float hi = fmaxf(mx[r], m2);
+            // lanes that only saw padded components carry (-inf, 0)
+            const float e1 = (mx[r] == -INFINITY) ? 0.0f : sm[r] * exp2f_fast(mx[r] - hi);
+            const float e2 = (m2 == -INFINITY) ? 0.0f : s2 * exp2f_fast(m2 - hi);
+            sm[r] = e1 + e2;
+            mx[r] = hi;
+        }
+    }
+    double local = 0.0;
+    if (c == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int row = 8 * (r >> 2) + 4 * h + (r & 3);
+            const uint64_t fr = base + 32 * wave + row;
+            float l2 = mx[r] + log2f_fast(sm[r]);
+            float nat = l2 * kLn2m;
+            if (!(nat > a.min_ll) || isinf(nat) || (fr < a.N && a.bad[fr])) {
+                nat = a.min_ll;
+                l2 = INFINITY;
+            }
+            if (fr < a.N) {
+                a.LL2[fr] = l2;
+                if (a.LLnat) a.LLnat[fr] = nat;
+                local += (double)nat;
+            }
+        }
+        red[wave * 2 + h] = local;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int i = 0; i < 8; i++) s += red[i];
+        a.blockLL[blockIdx.x] = s;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -440,12 +852,12 @@ hipError_t launch_prep_frames(const PrepArgs &p, int prec, hipStream_t s) {
 hipError_t launch_loglik_mixed(const MixedArgs &a, int prec, hipStream_t s) {
     const dim3 grid((unsigned)((a.Npad + 127) / 128));
     switch (a.KP) {
-#define X(K)                                                                                   \
-    case K:                                                                                    \
-        if (prec == 2)                                                                         \
-            hipLaunchKernelGGL((loglik_mixed_kernel<K, _Float16>), grid, dim3(256), 0, s, a); \
-        else                                                                                   \
-            hipLaunchKernelGGL((loglik_mixed_kernel<K, __bf16>), grid, dim3(256), 0, s, a);   \
+#define X(K)                                                                                      \
+    case K:                                                                                       \
+        if (prec == 2)                                                                            \
+            hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, _Float16>), grid, dim3(256), 0, s, a); \
+        else                                                                                      \
+            hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, __bf16>), grid, dim3(256), 0, s, a);   \
         break;
         GMMHIP_KP32_CASES(X)
 #undef X
@@ -454,6 +866,8 @@ hipError_t launch_loglik_mixed(const MixedArgs &a, int prec, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Feature rows of a power-of-two number of 16-byte pieces take the
+// direct-to-LDS statistics kernel; the others the register-staged one.
 hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s) {
     const int nchunks_pad = (a.nchunks + 7) / 8 * 8;
     const dim3 grid((unsigned)(nchunks_pad * a.ngroups));
@@ -465,8 +879,16 @@ hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s) {
         else                                                                                  \
             hipLaunchKernelGGL((stats_mixed_kernel<K, __bf16>), grid, dim3(256), 0, s, a);   \
         break;
-        GMMHIP_KP32_CASES(X)
+#define XD(K)                                                                                     \
+    case K:                                                                                       \
+        if (prec == 2)                                                                            \
+            hipLaunchKernelGGL((stats_mixed_dma_kernel<K, _Float16>), grid, dim3(256), 0, s, a);  \
+        else                                                                                      \
+            hipLaunchKernelGGL((stats_mixed_dma_kernel<K, __bf16>), grid, dim3(256), 0, s, a);    \
+        break;
+        XD(32) XD(64) X(96) XD(128) X(160) X(192) X(224) XD(256)
 #undef X
+#undef XD
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
