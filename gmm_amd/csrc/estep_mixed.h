@@ -9,6 +9,9 @@ namespace gmmhip {
 // Contraction depth of the mixed form: [1, z(D), z^2(D)] padded to 32.
 inline int kp_mixed_for_dim(int D) { return (2 * D + 1 + 31) / 32 * 32; }
 
+// largest feature dimension of the mixed forms (2D+1 <= 256)
+constexpr int PREP_MAX_D = 127;
+
 struct PrepArgs {
     const float *X;        // [N][D] contiguous
     uint64_t N, Npad;      // Npad = ceil32(N)
@@ -17,7 +20,7 @@ struct PrepArgs {
     const float *scale;    // [D] 1/std
     uint16_t *za_hi_raw, *za_lo_raw;  // [Npad][KP]
     uint16_t *zb_hi_raw, *zb_lo_raw;  // [Npad/32][32*KP]
-    uint8_t *bad;          // [N] 1: non-finite frame, 2: out of the 16-bit range
+    uint8_t *bad;          // [N] written for every frame: 1 non-finite, 2 out of the 16-bit range, 0 ok
     unsigned int *nrange;  // count of (frame, dim) values flagged 2
     float zlimit;          // |z| above this is out of range (fp16: 240)
 };

@@ -95,53 +95,111 @@ __global__ __launch_bounds__(256) void colstats_kernel(const float *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Split frame features into hi/lo pieces in both layouts.  One thread per
-// (frame, k); frames past N (padding to a multiple of 32) are zero rows.
+// Split frame features into hi/lo pieces in both layouts.  One block per
+// 32-frame block: the block's centred features are staged in LDS once, then
+// every store is one 16-byte piece (8 features of a frame for ZA, 8 frames
+// of a feature for ZB) and consecutive threads write consecutive pieces.
+// Frames past N (padding to a multiple of 32) are zero rows.  bad[n] is
+// written for every frame: 1 if the frame holds a non-finite value, else 2
+// if a centred value lies outside the 16-bit range, else 0.
+template <typename T>
+__device__ __forceinline__ unsigned int split_bits(float v, unsigned int &lo) {
+    const T hi = (T)v;
+    const T l = (T)(v - (float)hi);
+    lo = __builtin_bit_cast(uint16_t, l);
+    return __builtin_bit_cast(uint16_t, hi);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void prep_frames_kernel(PrepArgs p) {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const int KP = p.KP;
-    const uint64_t total = p.Npad * (uint64_t)KP;
-    if (i >= total) return;
-    const uint64_t n = i / KP;
-    const int k = (int)(i - n * KP);
-    const int D = p.D;
-    float v = 0.0f;
-    if (n < p.N) {
-        if (k == 0) {
-            v = 1.0f;
-        } else if (k <= 2 * D) {
-            const int d = (k <= D) ? k - 1 : k - 1 - D;
-            const float x = p.X[n * D + d];
-            float z = (x - p.off[d]) * p.scale[d];
-            if (!isfinite(x)) {
-                // non-finite frame: its log-likelihood is clamped, no statistics
-                z = 0.0f;
-                p.bad[n] = 1;
-            } else if (fabsf(z) > p.zlimit) {
-                // outside the 16-bit range: the call runs in FP32 instead
-                z = 0.0f;
-                p.bad[n] = 2;
-                if (k <= D) atomicAdd(p.nrange, 1u);
-            }
-            v = (k <= D) ? z : z * z;
+    __shared__ float zs[32 * PREP_MAX_D];
+    __shared__ unsigned int flags[32];
+    __shared__ unsigned int cnt;
+    const int tid = threadIdx.x;
+    const int D = p.D, KP = p.KP;
+    const uint64_t blk = blockIdx.x;
+    const uint64_t n0 = blk * 32;
+    const int nf = (n0 >= p.N) ? 0 : (int)((p.N - n0) < 32 ? (p.N - n0) : 32);
+    if (tid < 32) flags[tid] = 0u;
+    if (tid == 0) cnt = 0u;
+    __syncthreads();
+    // the block's frames are 32 x D consecutive floats
+    for (int e = tid; e < nf * D; e += 256) {
+        const int f = e / D, d = e - f * D;
+        const float x = p.X[n0 * D + e];
+        float z = (x - p.off[d]) * p.scale[d];
+        if (!isfinite(x)) {
+            // non-finite frame: its log-likelihood is clamped, no statistics
+            z = 0.0f;
+            atomicOr(&flags[f], 1u);
+        } else if (fabsf(z) > p.zlimit) {
+            // outside the 16-bit range: the call runs in FP32 instead
+            z = 0.0f;
+            atomicOr(&flags[f], 2u);
+            atomicAdd(&cnt, 1u);
         }
+        zs[e] = z;
     }
-    const T hi = (T)v;
-    const T lo = (T)(v - (float)hi);
-    const uint16_t hb = __builtin_bit_cast(uint16_t, hi), lb = __builtin_bit_cast(uint16_t, lo);
-    p.za_hi_raw[i] = hb;
-    p.za_lo_raw[i] = lb;
-    const uint64_t blk = n >> 5;
-    const int f = (int)(n & 31);
-    const int u = f >> 4, r16 = f & 15;
-    const int h = (r16 >> 2) & 1, j = ((r16 >> 3) << 2) | (r16 & 3);
-    const int ct = k >> 5, c = k & 31, CT = KP >> 5;
-    // [blk][u][ct][h][c][j]: the 32 lanes of each half-wave read consecutive
-    // 16-byte pieces
-    const uint64_t zb = blk * 32 * (uint64_t)KP + ((((uint64_t)u * CT + ct) * 2 + h) * 32 + c) * 8 + j;
-    p.zb_hi_raw[zb] = hb;
-    p.zb_lo_raw[zb] = lb;
+    __syncthreads();
+    if (tid < nf) {
+        const unsigned int fl = flags[tid];
+        p.bad[n0 + tid] = (fl & 1u) ? 1 : (fl ? 2 : 0);
+    }
+    if (tid == 0 && cnt) atomicAdd(p.nrange, cnt);
+
+    // feature k of frame f: 1, z_1..z_D, z_1^2..z_D^2, zero padding
+    auto val = [&](int f, int k) -> float {
+        if (f >= nf) return 0.0f;
+        if (k == 0) return 1.0f;
+        if (k <= D) return zs[f * D + k - 1];
+        if (k <= 2 * D) {
+            const float z = zs[f * D + k - 1 - D];
+            return z * z;
+        }
+        return 0.0f;
+    };
+    const int U = KP / 8, CT = KP / 32;
+    const int pieces = 32 * U;  // per layout and part
+    u32x4 *zah = reinterpret_cast<u32x4 *>(p.za_hi_raw) + blk * pieces;
+    u32x4 *zal = reinterpret_cast<u32x4 *>(p.za_lo_raw) + blk * pieces;
+    u32x4 *zbh = reinterpret_cast<u32x4 *>(p.zb_hi_raw) + blk * pieces;
+    u32x4 *zbl = reinterpret_cast<u32x4 *>(p.zb_lo_raw) + blk * pieces;
+    // ZA[n][k]: piece q of frame f holds k = 8q .. 8q+7
+    for (int e = tid; e < pieces; e += 256) {
+        const int f = e / U, k0 = (e - f * U) * 8;
+        u32x4 H, L;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            unsigned int l0, l1;
+            const unsigned int h0 = split_bits<T>(val(f, k0 + 2 * i), l0);
+            const unsigned int h1 = split_bits<T>(val(f, k0 + 2 * i + 1), l1);
+            H[i] = h0 | (h1 << 16);
+            L[i] = l0 | (l1 << 16);
+        }
+        zah[e] = H;
+        zal[e] = L;
+    }
+    // ZB[blk][u][ct][h][c][j]: slot ((u*CT + ct)*2 + h)*32 + c holds feature
+    // 32ct + c of frames 16u + 8(j/4) + 4h + j%4, j = 0..7
+    for (int e = tid; e < pieces; e += 256) {
+        const int c = e & 31, h = (e >> 5) & 1, rest = e >> 6;
+        const int ct = rest % CT, u = rest / CT;
+        const int k = 32 * ct + c;
+        u32x4 H, L;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int j0 = 2 * i, j1 = 2 * i + 1;
+            const int f0 = 16 * u + 8 * (j0 >> 2) + 4 * h + (j0 & 3);
+            const int f1 = 16 * u + 8 * (j1 >> 2) + 4 * h + (j1 & 3);
+            unsigned int l0, l1;
+            const unsigned int h0 = split_bits<T>(val(f0, k), l0);
+            const unsigned int h1 = split_bits<T>(val(f1, k), l1);
+            H[i] = h0 | (h1 << 16);
+            L[i] = l0 | (l1 << 16);
+        }
+        zbh[e] = H;
+        zbl[e] = L;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -840,8 +898,9 @@ hipError_t launch_colstats(const float *X, uint64_t N, int D, int blocks, uint64
 }
 
 hipError_t launch_prep_frames(const PrepArgs &p, int prec, hipStream_t s) {
-    const uint64_t total = p.Npad * (uint64_t)p.KP;
-    const dim3 grid((unsigned)((total + 255) / 256));
+    if (p.D < 1 || p.D > PREP_MAX_D || p.KP % 32 || p.KP < 2 * p.D + 1 || p.Npad % 32)
+        return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(p.Npad / 32));
     if (prec == 2)
         hipLaunchKernelGGL(prep_frames_kernel<_Float16>, grid, dim3(256), 0, s, p);
     else

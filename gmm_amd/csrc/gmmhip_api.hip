@@ -206,7 +206,6 @@ int prepare_mixed(gmmhip_ctx *h, bool *use_mixed) {
         }
         HIPCHK(h, ensure(&h->d_bad, &h->bad_cap, (size_t)N));
         if (!h->d_nrange) HIPCHK(h, hipMalloc((void **)&h->d_nrange, sizeof(unsigned int)));
-        HIPCHK(h, hipMemsetAsync(h->d_bad, 0, N, h->stream));
         HIPCHK(h, hipMemsetAsync(h->d_nrange, 0, sizeof(unsigned int), h->stream));
         PrepArgs p{};
         p.X = h->d_x;

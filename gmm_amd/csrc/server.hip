@@ -318,7 +318,6 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
         const uint64_t Nz = pad128(sh.N);
         uint8_t *bad = (uint8_t *)(A + L.bad);
         unsigned int *nrange = (unsigned int *)(A + L.nrange);
-        SRV_HIPCHK(launch_fill_u32((unsigned int *)bad, 0u, (int)((sh.N + 3) / 4), st));
         SRV_HIPCHK(launch_fill_u32(nrange, 0u, 1, st));
         PrepArgs p{};
         p.X = (const float *)(A + L.X);
