@@ -374,6 +374,9 @@ __device__ __forceinline__ void stats_item(const EstepArgs &a, StatsShared<KP> &
                 const int row = 16 * s + 4 * q + r;
                 float g = fast_exp2(acc1[s][r] - Ls[row]);
                 g = (g > a.min_gamma) ? g : 0.0f;
+                // a 4-frame x 16-component block with every posterior under
+                // the cut adds exact zeros: skip its products
+                if (__builtin_amdgcn_ballot_w64(g != 0.0f) == 0) continue;
                 const float *zr = &Zr[row * S];
 #pragma unroll
                 for (int j = 0; j < JT; j++) acc2[j] = mfma4(g, zr[stats_col<KP>(c, j)], acc2[j]);

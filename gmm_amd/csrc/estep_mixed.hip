@@ -454,6 +454,10 @@ __global__ __launch_bounds__(256, (KP <= 192 ? 2 : 1)) void stats_mixed_kernel(M
         }
 #pragma unroll
         for (int u = 0; u < 2; u++) {
+            // a 16-frame half with every posterior under the cut adds
+            // nothing: skip its products
+            const u32x4 gw = __builtin_bit_cast(u32x4, ghi[u]);
+            if (__builtin_amdgcn_ballot_w64((gw[0] | gw[1] | gw[2] | gw[3]) != 0u) == 0) continue;
 #pragma unroll
             for (int ct = 0; ct < CT; ct++) {
                 const u32x4 yh4 = ZBl[0][((u * CT + ct) * 2 + h) * 32 + c];
@@ -526,8 +530,18 @@ __device__ __forceinline__ void lds_dma16(const void *src, void *lds_wave_base) 
                                      16, 0, 0);
 }
 
-// Statistics with direct-to-LDS staging: the four waves copy, per 32-frame
-// step, ZA hi, ZA lo, ZB hi and ZB lo (one part each, U pieces x 32 rows).
+// Statistics with direct-to-LDS staging.  A block of four waves owns four
+// component tiles (32 components each) and walks its frame chunk in 32-frame
+// steps; per step each wave copies one of the four staged parts — ZA hi, ZA
+// lo, ZB hi, ZB lo (U pieces x 32 rows each) — and the last wave also the
+// step's 32 per-frame log-likelihoods.  Iteration s contracts ZA of step s+1
+// (the first contraction, interleaved with step s's posteriors) and ZB of
+// step s (the frame contraction), so ZA is copied two steps ahead and ZB one
+// step ahead, each into the slot its previous occupant left at the start of
+// the iteration.  Each slot is a __shared__ variable of its own and the loop
+// is unrolled by two so that every access names its slot: the compiler can
+// then see that the copies in flight do not alias the reads of the
+// iteration and does not wait for them.
 template <int KP, typename T>
 __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kernel(MixedArgs a) {
     typedef typename Vec8<T>::type V8;
@@ -537,11 +551,12 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
     constexpr int PART = 32 * U;            // pieces per part and step
     constexpr int NDMA = PART / 64;         // copies per wave and step
     static_assert((U & (U - 1)) == 0 && PART % 64 == 0, "power-of-two feature rows");
-    __shared__ u32x4 Zs[2][4][PART];
-    __shared__ float Ls[2][32];
+    __shared__ u32x4 ZA0[2][PART], ZA1[2][PART], ZB0[2][PART], ZB1[2][PART];
+    __shared__ float Ls0[32], Ls1[32];
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int h = lane >> 5, c = lane & 31;
 
     const int b = blockIdx.x;
@@ -583,118 +598,127 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
         if (f1 > a.N) f1 = a.N;
     }
 
-    // this wave's part and the source piece of each of its copies within a
-    // 32-frame block (ZA parts de-swizzle, ZB parts copy straight)
+    // this wave's part (0 ZA hi, 1 ZA lo, 2 ZB hi, 3 ZB lo) and the source
+    // piece of each of its copies within a 32-frame block (ZA parts
+    // de-swizzle, ZB parts copy straight); LL2 holds at least ceil32(f1)
+    // entries, those past f1 are replaced by +inf below
+    const int part = wave;
+    const bool za_wave = part < 2;
+    const bool ll_wave = part == 3;
+    const int hl = part & 1;
     const u32x4 *src = reinterpret_cast<const u32x4 *>(
-        wave == 0 ? a.za_hi : wave == 1 ? a.za_lo : wave == 2 ? a.zb_hi : a.zb_lo);
+        part == 0 ? a.za_hi : part == 1 ? a.za_lo : part == 2 ? a.zb_hi : a.zb_lo);
     int soff[NDMA];
 #pragma unroll
     for (int j = 0; j < NDMA; j++) {
         const int p = j * 64 + lane;
-        if (wave < 2) {
+        if (za_wave) {
             const int row = p / U, v = p % U;
             soff[j] = row * U + (v ^ za_swz<U>(row));
         } else {
             soff[j] = p;
         }
     }
-    // Step k covers frames f0 + 32k.  Step s's posteriors (VALU) interleave
-    // with the first contraction of step s+1 (MFMA), then step s's frame
-    // contraction runs.  So iteration s reads ZA of step s+1 and ZB of step
-    // s: ZA is copied two steps ahead and ZB one step ahead, each into the
-    // buffer its previous occupant has left.
     const int nsteps = (int)((f1 - f0 + 31) / 32);
-#define GMMHIP_DMA_PART(K_, BUF_)                                                  \
-    _Pragma("unroll") for (int j = 0; j < NDMA; j++)                               \
-        lds_dma16(src + (f0 + 32 * (uint64_t)(K_)) * U + soff[j], &Zs[BUF_][wave][j * 64]);
-#define GMMHIP_CONTRACT1(ACC_, BUF_)                                               \
+#define GMMHIP_DMA_PART(K_, ZD_, LD_)                                              \
+    {                                                                              \
+        const u32x4 *sstep = src + (f0 + 32 * (uint64_t)(K_)) * U;                 \
+        _Pragma("unroll") for (int j = 0; j < NDMA; j++)                           \
+            lds_dma16(sstep + soff[j], &ZD_[hl][j * 64]);                          \
+        if (ll_wave && lane < 32)                                                  \
+            __builtin_amdgcn_global_load_lds(                                      \
+                a.LL2 + f0 + 32 * (uint64_t)(K_) + lane,                           \
+                (__attribute__((address_space(3))) void *)&LD_[0], 4, 0, 0);       \
+    }
+#define GMMHIP_CONTRACT1(ACC_, ZR_)                                                \
     _Pragma("unroll") for (int ks = 0; ks < KS; ks++) {                            \
         const int slot = c * U + ((2 * ks + h) ^ za_swz<U>(c));                    \
-        const V8 zh = __builtin_bit_cast(V8, Zs[BUF_][0][slot]);                   \
-        const V8 zl = __builtin_bit_cast(V8, Zs[BUF_][1][slot]);                   \
+        const V8 zh = __builtin_bit_cast(V8, ZR_[0][slot]);                        \
+        const V8 zl = __builtin_bit_cast(V8, ZR_[1][slot]);                        \
         ACC_ = mfma32(zh, bhi[ks], ACC_);                                          \
         ACC_ = mfma32(zh, blo[ks], ACC_);                                          \
         ACC_ = mfma32(zl, bhi[ks], ACC_);                                          \
     }
-    GMMHIP_DMA_PART(0, 0)
-    if (wave < 2 && nsteps > 1) {
-        GMMHIP_DMA_PART(1, 1)
+    // prologue: ZA of steps 0 and 1, ZB and LL2 of step 0
+    if (za_wave) {
+        GMMHIP_DMA_PART(0, ZA0, Ls0)
+        if (nsteps > 1) GMMHIP_DMA_PART(1, ZA1, Ls1)
+    } else {
+        GMMHIP_DMA_PART(0, ZB0, Ls0)
     }
-    float pl = INFINITY;
-    if (tid < 32) Ls[0][tid] = (f0 + tid < f1) ? a.LL2[f0 + tid] : INFINITY;
-    if (tid < 32) pl = (f0 + 32 + tid < f1) ? a.LL2[f0 + 32 + tid] : INFINITY;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     // one FP32 accumulator takes hi*hi, hi*lo and lo*hi
     f32x16 acc = {};
     if (active) {
-        GMMHIP_CONTRACT1(acc, 0)
+        GMMHIP_CONTRACT1(acc, ZA0)
     }
 
-    for (int s = 0; s < nsteps; s++) {
-        const int cur = s & 1, nxt = cur ^ 1;
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (tid < 32) {
-            Ls[nxt][tid] = pl;
-            const uint64_t fb = f0 + 32 * (uint64_t)(s + 2) + tid;
-            pl = (fb < f1) ? a.LL2[fb] : INFINITY;
-        }
-        if (wave < 2) {
-            if (s + 2 < nsteps) {
-                GMMHIP_DMA_PART(s + 2, cur)
-            }
-        } else if (s + 1 < nsteps) {
-            GMMHIP_DMA_PART(s + 1, nxt)
-        }
-        if (!active) continue;
-
-        // first contraction of step s+1 (on the last step it runs on stale
-        // pieces and is discarded) interleaved with step s's posteriors,
-        // split into hi/lo A operands of the frame contraction: register
-        // r = 8u + j holds frame 16u + 8(j/4) + 4h + j%4; rows r and r+1
-        // (r even) are consecutive frames and one packed word
-        f32x16 nacc = {};
-        GMMHIP_CONTRACT1(nacc, nxt)
-        u32x4 gh4[2], gl4[2];
-#pragma unroll
-        for (int r = 0; r < 16; r += 2) {
-            const int row = 8 * (r >> 2) + 4 * h + (r & 3);
-            float g0 = exp2f_fast(acc[r] - Ls[cur][row]);
-            float g1 = exp2f_fast(acc[r + 1] - Ls[cur][row + 1]);
-            g0 = (g0 > thr) ? g0 : 0.0f;
-            g1 = (g1 > thr) ? g1 : 0.0f;
-            unsigned int wh, wl;
-            split_pair<T>(g0, g1, wh, wl);
-            gh4[r >> 3][(r & 7) >> 1] = wh;
-            gl4[r >> 3][(r & 7) >> 1] = wl;
-        }
-#pragma unroll
-        for (int i = 0; i < KS; i++) {
-            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // two LDS reads
-            __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);  // three MFMA
-            __builtin_amdgcn_sched_group_barrier(0x002, 10, 0); // ten VALU
-        }
-        V8 ghi[2], glo[2];
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-            ghi[u] = __builtin_bit_cast(V8, gh4[u]);
-            glo[u] = __builtin_bit_cast(V8, gl4[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-#pragma unroll
-            for (int ct = 0; ct < CT; ct++) {
-                const int slot = ((u * CT + ct) * 2 + h) * 32 + c;
-                const V8 yh = __builtin_bit_cast(V8, Zs[cur][2][slot]);
-                const V8 yl = __builtin_bit_cast(V8, Zs[cur][3][slot]);
-                acc2[ct] = mfma32(ghi[u], yh, acc2[ct]);
-                acc2[ct] = mfma32(ghi[u], yl, acc2[ct]);
-                acc2[ct] = mfma32(glo[u], yh, acc2[ct]);
-            }
-        }
-        acc = nacc;
+    // iteration s with parity P: reads ZA of step s+1 (ZAn), ZB and LL2 of
+    // step s (ZBc, Lc); copies ZA of step s+2 into ZAc and ZB / LL2 of step
+    // s+1 into ZBn / Ln
+#define GMMHIP_STATS_STEP(S_, ZAc, ZAn, ZBc, ZBn, Lc, Ln)                         \
+    {                                                                              \
+        const int s_ = (S_);                                                       \
+        __builtin_amdgcn_s_waitcnt(0);                                             \
+        if (ll_wave && s_ == nsteps - 1 && lane < 32 && f0 + 32 * (uint64_t)s_ + lane >= f1) \
+            Lc[lane] = INFINITY; /* frames past the chunk end: posterior 0 */      \
+        __syncthreads();                                                           \
+        if (za_wave) {                                                             \
+            if (s_ + 2 < nsteps) GMMHIP_DMA_PART(s_ + 2, ZAc, Lc)                   \
+        } else if (s_ + 1 < nsteps) {                                              \
+            GMMHIP_DMA_PART(s_ + 1, ZBn, Ln)                                       \
+        }                                                                          \
+        if (active) {                                                              \
+            f32x16 nacc = {};                                                      \
+            GMMHIP_CONTRACT1(nacc, ZAn)                                            \
+            u32x4 gh4[2], gl4[2];                                                  \
+            _Pragma("unroll") for (int r = 0; r < 16; r += 2) {                    \
+                const int row = 8 * (r >> 2) + 4 * h + (r & 3);                    \
+                float g0 = exp2f_fast(acc[r] - Lc[row]);                           \
+                float g1 = exp2f_fast(acc[r + 1] - Lc[row + 1]);                   \
+                g0 = (g0 > thr) ? g0 : 0.0f;                                       \
+                g1 = (g1 > thr) ? g1 : 0.0f;                                       \
+                unsigned int wh, wl;                                               \
+                split_pair<T>(g0, g1, wh, wl);                                     \
+                gh4[r >> 3][(r & 7) >> 1] = wh;                                    \
+                gl4[r >> 3][(r & 7) >> 1] = wl;                                    \
+            }                                                                      \
+            _Pragma("unroll") for (int i = 0; i < KS; i++) {                       \
+                __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);                 \
+                __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);                 \
+                __builtin_amdgcn_sched_group_barrier(0x002, 10, 0);                \
+            }                                                                      \
+            _Pragma("unroll") for (int u = 0; u < 2; u++) {                        \
+                const unsigned int any = gh4[u][0] | gh4[u][1] | gh4[u][2] | gh4[u][3]; \
+                if (__builtin_amdgcn_ballot_w64(any != 0u) == 0) continue;         \
+                const V8 ghi = __builtin_bit_cast(V8, gh4[u]);                     \
+                const V8 glo = __builtin_bit_cast(V8, gl4[u]);                     \
+                _Pragma("unroll") for (int ct = 0; ct < CT; ct++) {                \
+                    const int slot = ((u * CT + ct) * 2 + h) * 32 + c;             \
+                    const V8 yh = __builtin_bit_cast(V8, ZBc[0][slot]);            \
+                    const V8 yl = __builtin_bit_cast(V8, ZBc[1][slot]);            \
+                    acc2[ct] = mfma32(ghi, yh, acc2[ct]);                          \
+                    acc2[ct] = mfma32(ghi, yl, acc2[ct]);                          \
+                    acc2[ct] = mfma32(glo, yh, acc2[ct]);                          \
+                }                                                                  \
+            }                                                                      \
+            acc = nacc;                                                            \
+        }                                                                          \
     }
+    // Step s's posteriors (register r = 8u + j holds frame 16u + 8(j/4) +
+    // 4h + j%4; rows r and r+1, r even, are consecutive frames and one
+    // packed word) interleave with the first contraction of step s+1 (on
+    // the last step it runs on stale pieces and is discarded); a 16-frame
+    // half whose posteriors are all under the cut (exact zeros: hi is zero
+    // only when g is) adds nothing to the frame contraction and is skipped.
+    int s = 0;
+    for (; s + 1 < nsteps; s += 2) {
+        GMMHIP_STATS_STEP(s, ZA0, ZA1, ZB0, ZB1, Ls0, Ls1)
+        GMMHIP_STATS_STEP(s + 1, ZA1, ZA0, ZB1, ZB0, Ls1, Ls0)
+    }
+    if (s < nsteps) GMMHIP_STATS_STEP(s, ZA0, ZA1, ZB0, ZB1, Ls0, Ls1)
+#undef GMMHIP_STATS_STEP
 #undef GMMHIP_DMA_PART
 #undef GMMHIP_CONTRACT1
 
@@ -710,30 +734,23 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
         }
 }
 
-// Online log-sum-exp over one 32-component tile for the 16 frame rows of a
-// lane: first = the block's first tile (initialises the reference mx and the
-// sum sm); a padded component (valid = false, uniform per lane and tile)
-// contributes nothing.  Branch-free: a value more than 32 above the
-// reference moves it (sm 2^-d + 1), otherwise sm += 2^d.
-__device__ __forceinline__ void lse_tile_update(float (&mx)[16], float (&sm)[16],
-                                                const f32x16 &acc, bool valid, bool first) {
-    if (first) {
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            mx[r] = valid ? acc[r] : -INFINITY;
-            sm[r] = valid ? 1.0f : 0.0f;
-        }
-    } else if (valid) {
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const float d = acc[r] - mx[r];
-            const bool up = d > 32.0f;
-            const float e = exp2f_fast(up ? -d : d);
-            sm[r] = up ? fmaf(sm[r], e, 1.0f) : sm[r] + e;
-            mx[r] = up ? acc[r] : mx[r];
-        }
-    }
+// Maximum over the 32 lanes of a half-wave (lanes 32h .. 32h+31): DPP
+// within each 16-lane row, then one exchange between the two rows.
+__device__ __forceinline__ float row_max32(float v) {
+    v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                     0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false)));  // quad [1,0,3,2]
+    v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                     0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false)));  // quad [2,3,0,1]
+    v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                     0, __builtin_bit_cast(int, v), 0x141, 0xF, 0xF, false))); // row half-mirror
+    v = fmaxf(v, __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                     0, __builtin_bit_cast(int, v), 0x140, 0xF, 0xF, false))); // row mirror
+    return fmaxf(v, __shfl_xor(v, 16));
 }
+
+// Headroom of the log-sum-exp reference (log2 units): values up to this
+// far above a row's reference are summed without moving it.
+static constexpr float LSE_HEADROOM = 64.0f;
 
 // Log-likelihood with direct-to-LDS staging of the component tiles (two
 // LDS buffers, one barrier per tile); same arithmetic as loglik_mixed_kernel.
@@ -744,7 +761,9 @@ __global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
     constexpr int TILE16 = KS * 2 * 64;      // 16-byte pieces per component tile
     constexpr int PF = TILE16 / 256;
     static_assert(TILE16 % 256 == 0, "component tile must split evenly over 256 threads");
-    __shared__ u32x4 Pl[2][TILE16];
+    // one __shared__ variable per buffer: the compiler then sees that a copy
+    // in flight does not alias the reads of the other buffer
+    __shared__ u32x4 Pl0[TILE16], Pl1[TILE16];
     __shared__ double red[8];
 
     const int tid = threadIdx.x;
@@ -775,10 +794,10 @@ __global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
     }
     const u32x4 *P16 = reinterpret_cast<const u32x4 *>(a.P16);
     // wave w copies pieces [w PF 64, (w+1) PF 64) of each component tile
-#define GMMHIP_DMA_TILE(T_, BUF_)                                                    \
+#define GMMHIP_DMA_TILE(T_, PW_)                                                     \
     _Pragma("unroll") for (int j = 0; j < PF; j++) {                                 \
         const int o = (wave * PF + j) * 64;                                          \
-        lds_dma16(P16 + (size_t)(T_) * TILE16 + o + lane, &Pl[BUF_][o]);             \
+        lds_dma16(P16 + (size_t)(T_) * TILE16 + o + lane, &PW_[o]);                  \
     }
 #define GMMHIP_TILE_MFMA(ACC_)                                                       \
     _Pragma("unroll") for (int ks = 0; ks < KS; ks++) {                              \
@@ -790,46 +809,79 @@ __global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
     }
     u32x4 bt[2 * KS];
     // tile 0
-    GMMHIP_DMA_TILE(0, 0)
+    GMMHIP_DMA_TILE(0, Pl0)
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (a.Mt32 > 1) {
-        GMMHIP_DMA_TILE(1, 1)
+        GMMHIP_DMA_TILE(1, Pl1)
     }
 #pragma unroll
-    for (int j = 0; j < 2 * KS; j++) bt[j] = Pl[0][j * 64 + lane];
+    for (int j = 0; j < 2 * KS; j++) bt[j] = Pl0[j * 64 + lane];
     // one FP32 accumulator takes hi*hi, hi*lo and lo*hi
     f32x16 acc = {};
     GMMHIP_TILE_MFMA(acc)
-    // tile t's MFMAs interleave with the log-sum-exp update of tile t-1
-    // (full tiles: every tile but the last has 32 valid components)
-    for (int t = 1; t < a.Mt32; t++) {
-        const int buf = t & 1;
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (t + 1 < a.Mt32) {
-            GMMHIP_DMA_TILE(t + 1, buf ^ 1)
-        }
-#pragma unroll
-        for (int j = 0; j < 2 * KS; j++) bt[j] = Pl[buf][j * 64 + lane];
-        __builtin_amdgcn_sched_barrier(0);
-        f32x16 nacc = {};
-        GMMHIP_TILE_MFMA(nacc)
+    if (a.Mt32 > 1) {
+        // every tile but the last has 32 valid components: start each frame
+        // row's reference at the row maximum of tile 0 (all 32 lanes agree)
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const float d = acc[r] - mx[r];
-            const bool up = d > 32.0f;
-            const float e = exp2f_fast(up ? -d : d);
-            sm[r] = up ? fmaf(sm[r], e, 1.0f) : sm[r] + e;
-            mx[r] = up ? acc[r] : mx[r];
+            mx[r] = row_max32(acc[r]);
+            sm[r] = 0.0f;
         }
-#pragma unroll
-        for (int i = 0; i < 3 * KS; i++) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
-            __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);  // six VALU
-        }
-        acc = nacc;
     }
+    // Tile t's MFMAs interleave with the log-sum-exp update of tile t-1.
+    // Fast path: no value of the wave's tile lies more than LSE_HEADROOM
+    // above its row reference, and sm += 2^d (sm stays below 2^(11+64)).
+    // Otherwise the whole tile takes the general update, which moves the
+    // reference of every value that exceeded it.
+#define GMMHIP_LL_STEP(ACC_, NACC_, T_, PR_, PW_)                                    \
+    {                                                                                \
+        __builtin_amdgcn_s_waitcnt(0);                                               \
+        __syncthreads();                                                             \
+        if ((T_) + 1 < a.Mt32) {                                                     \
+            GMMHIP_DMA_TILE((T_) + 1, PW_)                                           \
+        }                                                                            \
+        _Pragma("unroll") for (int j = 0; j < 2 * KS; j++) bt[j] = PR_[j * 64 + lane]; \
+        __builtin_amdgcn_sched_barrier(0);                                           \
+        NACC_ = (f32x16){};                                                          \
+        GMMHIP_TILE_MFMA(NACC_)                                                      \
+        float d[16], e[16];                                                          \
+        _Pragma("unroll") for (int r = 0; r < 16; r++) {                             \
+            d[r] = ACC_[r] - mx[r];                                                  \
+            e[r] = exp2f_fast(d[r]);                                                 \
+        }                                                                            \
+        float dm = fmaxf(fmaxf(d[0], d[1]), d[2]);                                   \
+        _Pragma("unroll") for (int r = 3; r < 16; r += 2) dm = fmaxf(fmaxf(dm, d[r]), d[r + 1 < 16 ? r + 1 : r]); \
+        _Pragma("unroll") for (int i = 0; i < 3 * KS; i++) {                         \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                       \
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);                       \
+        }                                                                            \
+        if (__builtin_amdgcn_ballot_w64(dm > LSE_HEADROOM) == 0) {                   \
+            _Pragma("unroll") for (int r = 0; r < 16; r++) sm[r] += e[r];            \
+        } else {                                                                     \
+            /* a real branch (sm opaque here, so the common path's sums are   */ \
+            /* not hoisted above it into separate registers)                     */ \
+            _Pragma("unroll") for (int r = 0; r < 16; r++) asm volatile("" : "+v"(sm[r])); \
+            _Pragma("unroll") for (int r = 0; r < 16; r++) {                         \
+                const bool up = d[r] > LSE_HEADROOM;                                 \
+                const float ee = up ? exp2f_fast(-d[r]) : e[r];                      \
+                sm[r] = up ? fmaf(sm[r], ee, 1.0f) : sm[r] + ee;                     \
+                mx[r] = up ? ACC_[r] : mx[r];                                        \
+            }                                                                        \
+        }                                                                            \
+    }
+    f32x16 acc2;
+    int t = 1;
+    // tile t sits in Pl1 for odd t, Pl0 for even t
+    for (; t + 1 < a.Mt32; t += 2) {
+        GMMHIP_LL_STEP(acc, acc2, t, Pl1, Pl0)
+        GMMHIP_LL_STEP(acc2, acc, t + 1, Pl0, Pl1)
+    }
+    if (t < a.Mt32) {
+        GMMHIP_LL_STEP(acc, acc2, t, Pl1, Pl0)
+        acc = acc2;
+    }
+#undef GMMHIP_LL_STEP
     // last tile: padded components (>= M) are masked per lane
     {
         const bool valid = (32 * (a.Mt32 - 1) + c) < a.M;
@@ -837,7 +889,7 @@ __global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
         for (int r = 0; r < 16; r++) {
             // (a padded lane that saw no component keeps mx = -inf, sm = 0)
             const float d = acc[r] - mx[r];
-            const bool up = valid && d > 32.0f;
+            const bool up = valid && d > LSE_HEADROOM;
             const float e = valid ? exp2f_fast(up ? -d : d) : 0.0f;
             sm[r] = up ? fmaf(sm[r], e, 1.0f) : sm[r] + e;
             mx[r] = up ? acc[r] : mx[r];

@@ -547,7 +547,8 @@ int gmmhip_loglikes_device(gmmhip_t h, float *d_out) {
     if (rc) return rc;
     if (!d_out) return fail(h, GMMHIP_ERR_ARG, "loglikes_device: null output");
     DevGuard g(h->dev);
-    HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, h->N));
+    // padded: the mixed statistics kernel stages 32-frame steps of LL2
+    HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, pad128(h->N)));
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N, 1)));
     bool mixed = false;
     if ((rc = prepare_mixed(h, &mixed))) return rc;
@@ -602,7 +603,8 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
         ((flags & GMMHIP_ACC_VAR) && !S))
         return fail(h, GMMHIP_ERR_ARG, "accumulate: null output for a requested statistic");
     DevGuard g(h->dev);
-    HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, h->N));
+    // padded: the mixed statistics kernel stages 32-frame steps of LL2
+    HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, pad128(h->N)));
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N, 1)));
     bool mixed = false;
     if ((rc = prepare_mixed(h, &mixed))) return rc;

@@ -102,7 +102,7 @@ Layout layout_of(const Shape &s, const Dims &d) {
     L.out_end = p;
     L.out_copy = (s.frame_ll ? L.out_end : L.LLnat) - L.ll;
     L.queue = take(4);
-    L.LL2 = take(s.N * 4);
+    L.LL2 = take(pad128(s.N) * 4);  // the mixed statistics kernel stages 32-frame steps
     L.blockLL = take(pad128(s.N) / 128 * 8);
     L.part = take((size_t)s.nchunks * Mp * KP * 4);
     L.F = take((size_t)Us * d.M * d.D * 8);
