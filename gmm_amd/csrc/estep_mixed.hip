@@ -503,8 +503,19 @@ template <typename T>
 __device__ __forceinline__ void split_pair(float a, float b, unsigned int &hi, unsigned int &lo) {
     typedef typename Vec2<T>::type T2;
     const T2 h = __builtin_convertvector((f32x2){a, b}, T2);
-    const f32x2 hb = __builtin_convertvector(h, f32x2);
-    const T2 l = __builtin_convertvector((f32x2){a - hb.x, b - hb.y}, T2);
+    f32x2 r;
+    if constexpr (sizeof(T) == 2 && __is_same(T, _Float16)) {
+        // a - hi (exact) in one mixed-precision FMA per value, reading the
+        // half straight from the packed pair: -hi * 1.0 + a
+        const unsigned int hp = __builtin_bit_cast(unsigned int, h);
+        asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r.x) : "v"(hp), "v"(a));
+        asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+            : "=v"(r.y) : "v"(hp), "v"(b));
+    } else {
+        const f32x2 hb = __builtin_convertvector(h, f32x2);
+        r = (f32x2){a - hb.x, b - hb.y};
+    }
+    const T2 l = __builtin_convertvector(r, T2);
     hi = __builtin_bit_cast(unsigned int, h);
     lo = __builtin_bit_cast(unsigned int, l);
 }
@@ -528,6 +539,20 @@ __device__ __forceinline__ int za_swz(int c) {
 __device__ __forceinline__ void lds_dma16(const void *src, void *lds_wave_base) {
     __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)lds_wave_base,
                                      16, 0, 0);
+}
+
+// The same copy through a buffer resource over [base, base + bytes): the
+// base stays in scalar registers and each lane supplies a 32-bit byte
+// offset, so a copy costs no 64-bit address arithmetic.  Lane L still lands
+// at lds_wave_base + 16 L.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t dma_rsrc(const void *base, unsigned int bytes) {
+    // dword 3 of a gfx9 buffer descriptor: 32-bit raw data format
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ void lds_dma16(__amdgpu_buffer_rsrc_t r, unsigned int voff,
+                                          void *lds_wave_base) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        r, (__attribute__((address_space(3))) void *)lds_wave_base, 16, voff, 0, 0, 0);
 }
 
 // Statistics with direct-to-LDS staging.  A block of four waves owns four
@@ -608,26 +633,29 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
     const int hl = part & 1;
     const u32x4 *src = reinterpret_cast<const u32x4 *>(
         part == 0 ? a.za_hi : part == 1 ? a.za_lo : part == 2 ? a.zb_hi : a.zb_lo);
-    int soff[NDMA];
+    // (byte offsets within the step's 32-frame block of the part)
+    unsigned int soff[NDMA];
 #pragma unroll
     for (int j = 0; j < NDMA; j++) {
         const int p = j * 64 + lane;
         if (za_wave) {
             const int row = p / U, v = p % U;
-            soff[j] = row * U + (v ^ za_swz<U>(row));
+            soff[j] = (unsigned int)(row * U + (v ^ za_swz<U>(row))) * 16u;
         } else {
-            soff[j] = p;
+            soff[j] = (unsigned int)p * 16u;
         }
     }
+    const unsigned int ulane = (unsigned int)lane;
     const int nsteps = (int)((f1 - f0 + 31) / 32);
 #define GMMHIP_DMA_PART(K_, ZD_, LD_)                                              \
     {                                                                              \
-        const u32x4 *sstep = src + (f0 + 32 * (uint64_t)(K_)) * U;                 \
+        const __amdgpu_buffer_rsrc_t rs =                                          \
+            dma_rsrc(src + (f0 + 32 * (uint64_t)(K_)) * U, PART * 16);             \
         _Pragma("unroll") for (int j = 0; j < NDMA; j++)                           \
-            lds_dma16(sstep + soff[j], &ZD_[hl][j * 64]);                          \
+            lds_dma16(rs, soff[j], &ZD_[hl][j * 64]);                              \
         if (ll_wave && lane < 32)                                                  \
             __builtin_amdgcn_global_load_lds(                                      \
-                a.LL2 + f0 + 32 * (uint64_t)(K_) + lane,                           \
+                a.LL2 + f0 + 32 * (uint64_t)(K_) + ulane,                          \
                 (__attribute__((address_space(3))) void *)&LD_[0], 4, 0, 0);       \
     }
 #define GMMHIP_CONTRACT1(ACC_, ZR_)                                                \
@@ -657,7 +685,7 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
     // iteration s with parity P: reads ZA of step s+1 (ZAn), ZB and LL2 of
     // step s (ZBc, Lc); copies ZA of step s+2 into ZAc and ZB / LL2 of step
     // s+1 into ZBn / Ln
-#define GMMHIP_STATS_STEP(S_, ZAc, ZAn, ZBc, ZBn, Lc, Ln)                         \
+#define GMMHIP_STATS_STEP(S_, ZAc, ZAn, ZBc, ZBn, Lc, Ln, ACC_, NACC_)           \
     {                                                                              \
         const int s_ = (S_);                                                       \
         __builtin_amdgcn_s_waitcnt(0);                                             \
@@ -670,13 +698,13 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
             GMMHIP_DMA_PART(s_ + 1, ZBn, Ln)                                       \
         }                                                                          \
         if (active) {                                                              \
-            f32x16 nacc = {};                                                      \
-            GMMHIP_CONTRACT1(nacc, ZAn)                                            \
+            NACC_ = (f32x16){};                                                    \
+            GMMHIP_CONTRACT1(NACC_, ZAn)                                           \
             u32x4 gh4[2], gl4[2];                                                  \
             _Pragma("unroll") for (int r = 0; r < 16; r += 2) {                    \
                 const int row = 8 * (r >> 2) + 4 * h + (r & 3);                    \
-                float g0 = exp2f_fast(acc[r] - Lc[row]);                           \
-                float g1 = exp2f_fast(acc[r + 1] - Lc[row + 1]);                   \
+                float g0 = exp2f_fast(ACC_[r] - Lc[row]);                          \
+                float g1 = exp2f_fast(ACC_[r + 1] - Lc[row + 1]);                  \
                 g0 = (g0 > thr) ? g0 : 0.0f;                                       \
                 g1 = (g1 > thr) ? g1 : 0.0f;                                       \
                 unsigned int wh, wl;                                               \
@@ -703,7 +731,6 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
                     acc2[ct] = mfma32(glo, yh, acc2[ct]);                          \
                 }                                                                  \
             }                                                                      \
-            acc = nacc;                                                            \
         }                                                                          \
     }
     // Step s's posteriors (register r = 8u + j holds frame 16u + 8(j/4) +
@@ -712,12 +739,14 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
     // the last step it runs on stale pieces and is discarded); a 16-frame
     // half whose posteriors are all under the cut (exact zeros: hi is zero
     // only when g is) adds nothing to the frame contraction and is skipped.
+    // (the two unrolled halves swap the roles of acc and acc_n)
+    f32x16 acc_n;
     int s = 0;
     for (; s + 1 < nsteps; s += 2) {
-        GMMHIP_STATS_STEP(s, ZA0, ZA1, ZB0, ZB1, Ls0, Ls1)
-        GMMHIP_STATS_STEP(s + 1, ZA1, ZA0, ZB1, ZB0, Ls1, Ls0)
+        GMMHIP_STATS_STEP(s, ZA0, ZA1, ZB0, ZB1, Ls0, Ls1, acc, acc_n)
+        GMMHIP_STATS_STEP(s + 1, ZA1, ZA0, ZB1, ZB0, Ls1, Ls0, acc_n, acc)
     }
-    if (s < nsteps) GMMHIP_STATS_STEP(s, ZA0, ZA1, ZB0, ZB1, Ls0, Ls1)
+    if (s < nsteps) GMMHIP_STATS_STEP(s, ZA0, ZA1, ZB0, ZB1, Ls0, Ls1, acc, acc_n)
 #undef GMMHIP_STATS_STEP
 #undef GMMHIP_DMA_PART
 #undef GMMHIP_CONTRACT1
@@ -797,7 +826,8 @@ __global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
 #define GMMHIP_DMA_TILE(T_, PW_)                                                     \
     _Pragma("unroll") for (int j = 0; j < PF; j++) {                                 \
         const int o = (wave * PF + j) * 64;                                          \
-        lds_dma16(P16 + (size_t)(T_) * TILE16 + o + lane, &PW_[o]);                  \
+        lds_dma16(dma_rsrc(P16 + (size_t)(T_) * TILE16, TILE16 * 16),                \
+                  (unsigned int)(o + lane) * 16u, &PW_[o]);                          \
     }
 #define GMMHIP_TILE_MFMA(ACC_)                                                       \
     _Pragma("unroll") for (int ks = 0; ks < KS; ks++) {                              \
