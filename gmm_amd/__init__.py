@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(PKG, "_build", "libgmmhip.so")
 EXPORTS = [
     "gmmhip_default_opts", "gmmhip_device_count", "gmmhip_create",
     "gmmhip_destroy", "gmmhip_set_opts", "gmmhip_last_error",
-    "gmmhip_set_model", "gmmhip_set_frames", "gmmhip_set_frames_contig",
+    "gmmhip_set_model", "gmmhip_set_model_full", "gmmhip_set_frames", "gmmhip_set_frames_contig",
     "gmmhip_set_frames_device", "gmmhip_accumulate", "gmmhip_loglikes",
     "gmmhip_loglikes_device", "gmmhip_stream", "gmmhip_sync",
     "gmmhip_kernel_times", "gmmhip_reset_kernel_times", "gmmhip_batch_adapt",
@@ -226,6 +226,19 @@ class Accumulator:
         M, D = mu.shape
         self._check(lib().gmmhip_set_model(self._h, M, D, _fp(w), _fp(mu), _fp(var)))
         self.M, self.D = M, D
+        self.full = False
+
+    def set_model_full(self, w, mu, cov):
+        """Full-covariance model: cov[M, D, D] symmetric positive definite.
+        accumulate() then returns S[M, D(D+1)/2], the packed upper triangle
+        of sum g x x^T (row k holds columns k..D-1)."""
+        w = np.ascontiguousarray(w, np.float32)
+        mu = np.ascontiguousarray(mu, np.float32)
+        cov = np.ascontiguousarray(cov, np.float32)
+        M, D = mu.shape
+        self._check(lib().gmmhip_set_model_full(self._h, M, D, _fp(w), _fp(mu), _fp(cov)))
+        self.M, self.D = M, D
+        self.full = True
 
     def set_frames(self, x):
         x = np.ascontiguousarray(x, np.float32)
@@ -246,11 +259,14 @@ class Accumulator:
         self.N = N
 
     def accumulate(self, flags=ACC_MEAN | ACC_VAR, into=None):
-        """Returns (n[M], F[M,D], S[M,D], tot_ll, nframes); adds into `into` if given."""
+        """Returns (n[M], F[M,D], S, tot_ll, nframes), S[M,D] for a diagonal
+        model, S[M,D(D+1)/2] (packed upper) for a full one; adds into `into`
+        if given."""
         if into is None:
             n = np.zeros(self.M)
             F = np.zeros((self.M, self.D))
-            S = np.zeros((self.M, self.D))
+            S = np.zeros((self.M, self.D * (self.D + 1) // 2 if getattr(self, "full", False)
+                          else self.D))
             tot = ctypes.c_double(0.0)
             nf = ctypes.c_uint64(0)
         else:

@@ -277,6 +277,8 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
     if (!frames || !offsets || U <= 0 || !cfg || !out)
         return fail(h, GMMHIP_ERR_ARG, "batch_adapt: null argument");
     if (h->M <= 0) return fail(h, GMMHIP_ERR_STATE, "no model inserted");
+    if (h->full_model)
+        return fail(h, GMMHIP_ERR_ARG, "batch_adapt: needs a diagonal model (full covariance inserted)");
     if (D != h->D) return fail(h, GMMHIP_ERR_ARG, "batch_adapt: D=%d but the model has D=%d", D, h->D);
     if (offsets[0] != 0) return fail(h, GMMHIP_ERR_ARG, "batch_adapt: offsets[0] must be 0");
     for (int u = 0; u < U; u++)

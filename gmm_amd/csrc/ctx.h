@@ -19,6 +19,9 @@
 using namespace gmmhip;
 
 struct gmmhip_batch_state;  // batched per-utterance buffers (batch_api.hip)
+namespace gmmhip {
+struct FullState;           // full-covariance model and buffers (fullcov.hip)
+}
 
 struct gmmhip_ctx {
     int dev = 0;
@@ -81,6 +84,11 @@ struct gmmhip_ctx {
     int p16_prec = 0;
     bool p16_overflow = false;
     bool last_used_mixed = false;
+
+    // full-covariance model (fullcov.hip); the diagonal fields above are
+    // unused while full_model is set
+    gmmhip::FullState *full = nullptr;
+    bool full_model = false;
 
     // batched per-utterance state (batch_api.hip)
     gmmhip_batch_state *batch = nullptr;
@@ -153,6 +161,14 @@ struct DevGuard {
 
 // Frees the batched state of a handle (batch_api.hip).
 void batch_state_free(gmmhip_batch_state *b);
+
+// Full-covariance path (fullcov.hip): state release, per-frame
+// log-likelihoods into device memory, statistics accumulation with the
+// packed upper-triangle second-order statistics.
+void full_state_free(gmmhip::FullState *fs);
+int full_loglikes_device(gmmhip_ctx *h, float *d_out);
+int full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S, double *tot_ll,
+                    uint64_t *nframes);
 
 // Device copies of the active model for the per-utterance kernels: FP32
 // means / variances and, with mllr, the UBM-only MLLR operands

@@ -1,0 +1,431 @@
+// Full-covariance E-step and statistics (the reference's gammasKernelFull +
+// logLikeKernel + gammasNormKernel + accStatsFullKernel,
+// trainer/CU_GMMStatsEstimator_CUDAkernels.cuh:836-917, :922-995,
+// :1037-1110, :459-576; host side insertModel's full branch,
+// trainer/OL_GMMStatsEstimator.cpp:156-229, :291-299).
+//
+// With centred frames x' = x - c and the feature map
+//     phi(x') = [1, x'_1..x'_D, x'_i x'_j (i <= j, packed upper order)]
+// of K = 1 + D + D(D+1)/2 entries, both halves of the E-step are dense
+// contractions over K:
+//     l[t][m]   = phi(x'_t) . theta_m
+//       theta_m = [log w + gconst - mu'^T P mu' / 2, (P mu')_d, -P_ii/2 (i == j), -P_ij (i < j)]
+//     stats_m   = sum_t g_tm phi(x'_t) = [n, sum g x', sum g x'_i x'_j]
+// so a frame chunk runs: feature expansion (HBM-bound), L = Phi Theta^T and
+// the statistics Gamma^T Phi as plain FP32 GEMMs (rocBLAS, deterministic:
+// atomics off), and between them one fused pass per frame computing the
+// log-sum-exp, the clamp rules and the cut posteriors in place.  Chunk
+// statistics are added into FP64 accumulators in a fixed order.
+#include "ctx.h"
+
+#include <rocblas/rocblas.h>
+
+namespace gmmhip {
+
+struct FullState {
+    rocblas_handle blas = nullptr;
+    int K = 0, KP = 0;           // features, padded to a multiple of 8
+    float *d_theta = nullptr;    // [M][KP]
+    size_t theta_cap = 0;
+    int *d_pairs = nullptr;      // [K] (i << 16) | j for feature k (i, j < D; see fc_pairs)
+    size_t pairs_cap = 0;
+    float *d_cen = nullptr;      // [D]
+    size_t cen_cap = 0;
+    float *d_phi = nullptr;      // [nc][KP]
+    size_t phi_cap = 0;
+    float *d_L = nullptr;        // [nc][M]: log-likelihoods, then posteriors
+    size_t L_cap = 0;
+    uint8_t *d_bad = nullptr;    // [nc]
+    size_t bad_cap = 0;
+    float *d_part = nullptr;     // [M][KP] FP32 statistics of one chunk
+    size_t part_cap = 0;
+    double *d_acc = nullptr;     // [M][KP] FP64 statistics over chunks
+    size_t acc_cap = 0;
+    std::vector<float> h_cen;
+};
+
+// Feature k of phi: 0 -> 1; 1 + d -> x'_d; else the (i, j) pair (i <= j) in
+// packed upper order, matching the reference's full second-order statistics
+// (trainer/OL_GMMStatsEstimator.cpp:1358-1364).
+static std::vector<int> fc_pairs(int D) {
+    std::vector<int> p;
+    p.push_back(-1);
+    for (int d = 0; d < D; d++) p.push_back(-2 - d);
+    for (int i = 0; i < D; i++)
+        for (int j = i; j < D; j++) p.push_back((i << 16) | j);
+    return p;
+}
+
+// phi rows of frames [n0, n0 + nc): one block of 256 threads per 8 frames;
+// the frames are centred in LDS and every thread writes consecutive
+// features.  A frame holding a non-finite (or overflowing, |x'| > 1e18)
+// value gets a zero row and bad = 1: its log-likelihood is clamped and it
+// contributes no statistics, as the reference (non-finite component
+// log-likelihoods -> _minLogLike -> no posterior above the cut).
+__global__ __launch_bounds__(256) void fc_features_kernel(const float *__restrict__ X, uint64_t n0,
+                                                          int nc, int D, int K, int KP,
+                                                          const float *__restrict__ cen,
+                                                          const int *__restrict__ pairs,
+                                                          float *__restrict__ phi,
+                                                          uint8_t *__restrict__ bad) {
+    __shared__ float xs[8][128];
+    __shared__ int badf[8];
+    const int f0 = blockIdx.x * 8;
+    const int tid = threadIdx.x;
+    if (tid < 8) badf[tid] = 0;
+    __syncthreads();
+    for (int e = tid; e < 8 * D; e += 256) {
+        const int f = e / D, d = e - f * D;
+        float v = 0.0f;
+        if (f0 + f < nc) {
+            v = X[(n0 + f0 + f) * (uint64_t)D + d] - cen[d];
+            if (!(fabsf(v) <= 1e18f)) {
+                badf[f] = 1;
+                v = 0.0f;
+            }
+        }
+        xs[f][d] = v;
+    }
+    __syncthreads();
+    for (int f = 0; f < 8; f++) {
+        if (f0 + f >= nc) break;
+        float *row = phi + (size_t)(f0 + f) * KP;
+        const bool fb = badf[f] != 0;
+        for (int k = tid; k < KP; k += 256) {
+            float v = 0.0f;
+            if (k < K && !fb) {
+                const int p = pairs[k];
+                if (p == -1) v = 1.0f;
+                else if (p < 0) v = xs[f][-2 - p];
+                else v = xs[f][p >> 16] * xs[f][p & 0xffff];
+            }
+            row[k] = v;
+        }
+        if (tid == 0) bad[f0 + f] = fb ? 1 : 0;
+    }
+}
+
+// One wave per frame: log-sum-exp of the frame's M component
+// log-likelihoods (exact, max-shifted), the clamp rules of logLikeKernel
+// (LL <= min_ll or non-finite -> min_ll, posterior 0), posteriors
+// g = exp(l - LL) cut at g <= min_gamma, written in place.  LLnat[n0 + t]
+// (optional) receives the frame's log-likelihood; blockLL[block] the FP64
+// sum of the block's four frames.
+__global__ __launch_bounds__(256) void fc_lse_kernel(float *__restrict__ L, int nc, int M,
+                                                     const uint8_t *__restrict__ bad,
+                                                     float min_ll, float min_gamma,
+                                                     float *__restrict__ LLnat, uint64_t n0,
+                                                     double *__restrict__ blockLL) {
+    __shared__ double red[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = blockIdx.x * 4 + wave;
+    double lld = 0.0;
+    if (t < nc) {
+        float *row = L + (size_t)t * M;
+        float mx = -INFINITY;
+        bool nonfinite = false;
+        for (int m = lane; m < M; m += 64) {
+            const float v = row[m];
+            nonfinite |= !isfinite(v);
+            mx = fmaxf(mx, v);
+        }
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        float s = 0.0f;
+        for (int m = lane; m < M; m += 64) s += expf(row[m] - mx);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        const bool anynf = __builtin_amdgcn_ballot_w64(nonfinite) != 0;
+        float ll = mx + logf(s);
+        const bool clamp = bad[t] || anynf || !(ll > min_ll) || !isfinite(ll);
+        if (clamp) ll = min_ll;
+        for (int m = lane; m < M; m += 64) {
+            float g = 0.0f;
+            if (!clamp) {
+                g = (M == 1) ? 1.0f : expf(row[m] - ll);
+                g = (g > min_gamma) ? g : 0.0f;
+            }
+            row[m] = g;
+        }
+        if (lane == 0 && LLnat) LLnat[n0 + t] = ll;
+        lld = (double)ll;
+    }
+    if (lane == 0) red[wave] = lld;
+    __syncthreads();
+    if (threadIdx.x == 0) blockLL[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// acc[e] += part[e] (FP64), one element per thread: the fixed-order
+// cross-chunk reduction of the FP32 chunk statistics.
+__global__ __launch_bounds__(256) void fc_acc_kernel(const float *__restrict__ part, size_t E,
+                                                     int first, double *__restrict__ acc) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= E) return;
+    acc[e] = (first ? 0.0 : acc[e]) + (double)part[e];
+}
+
+}  // namespace gmmhip
+
+using namespace gmmhip_detail;
+
+namespace {
+
+#define BLASCHK(h, expr)                                                                \
+    do {                                                                                \
+        rocblas_status s_ = (expr);                                                     \
+        if (s_ != rocblas_status_success)                                               \
+            return fail((h), s_ == rocblas_status_memory_error ? GMMHIP_ERR_OOM         \
+                                                               : GMMHIP_ERR_HIP,        \
+                        "%s failed: %s", #expr, rocblas_status_to_string(s_));          \
+    } while (0)
+
+int full_state(gmmhip_ctx *h, gmmhip::FullState **out) {
+    if (!h->full) {
+        h->full = new gmmhip::FullState();
+        DevGuard g(h->dev);
+        if (rocblas_create_handle(&h->full->blas) != rocblas_status_success)
+            return fail(h, GMMHIP_ERR_HIP, "rocblas_create_handle failed");
+        rocblas_set_stream(h->full->blas, h->stream);
+        // deterministic kernels: no atomics (the default, stated explicitly)
+        rocblas_set_atomics_mode(h->full->blas, rocblas_atomics_not_allowed);
+    }
+    *out = h->full;
+    return GMMHIP_OK;
+}
+
+// Frames per chunk: the feature rows and the log-likelihood rows of a chunk
+// take at most ~512 MB;
+// opts.chunk_frames (at least 256) overrides.
+uint64_t full_chunk(const gmmhip_ctx *h, int KP) {
+    uint64_t nc;
+    if (h->opt.chunk_frames > 0) {
+        nc = std::max<uint64_t>(256, (uint64_t)h->opt.chunk_frames);
+    } else {
+        const uint64_t per = 4ull * (uint64_t)(KP + h->M) + 1;
+        nc = std::max<uint64_t>(1024, (512ull << 20) / per / 256 * 256);
+    }
+    return std::min<uint64_t>(nc, h->N);
+}
+
+// One pass over the resident frames: per chunk features, L = Phi Theta^T,
+// fused log-sum-exp / posteriors, and (stats) Gamma^T Phi into the FP64
+// accumulators.  LLnat (device, optional) receives per-frame
+// log-likelihoods; *tot the FP64 total.
+int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot) {
+    gmmhip::FullState *fs = h->full;
+    const int M = h->M, D = h->D, K = fs->K, KP = fs->KP;
+    const uint64_t nc_max = full_chunk(h, KP);
+    HIPCHK(h, ensure(&fs->d_phi, &fs->phi_cap, (size_t)nc_max * KP));
+    HIPCHK(h, ensure(&fs->d_L, &fs->L_cap, (size_t)nc_max * M));
+    HIPCHK(h, ensure(&fs->d_bad, &fs->bad_cap, (size_t)nc_max));
+    const uint64_t nblk_total = (h->N + nc_max - 1) / nc_max * ((nc_max + 3) / 4);
+    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, (size_t)nblk_total));
+    if (stats) {
+        HIPCHK(h, ensure(&fs->d_part, &fs->part_cap, (size_t)M * KP));
+        HIPCHK(h, ensure(&fs->d_acc, &fs->acc_cap, (size_t)M * KP));
+    }
+    const float one = 1.0f, zero = 0.0f;
+    uint64_t nblk = 0;
+    int first = 1;
+    for (uint64_t n0 = 0; n0 < h->N; n0 += nc_max) {
+        const int nc = (int)std::min<uint64_t>(nc_max, h->N - n0);
+        hipLaunchKernelGGL(gmmhip::fc_features_kernel, dim3((nc + 7) / 8), dim3(256), 0, h->stream,
+                           h->d_x, n0, nc, D, K, KP, fs->d_cen, fs->d_pairs, fs->d_phi, fs->d_bad);
+        HIPCHK(h, hipGetLastError());
+        // L (column-major M x nc) = Theta^T (M x KP) * Phi (KP x nc)
+        BLASCHK(h, rocblas_sgemm(fs->blas, rocblas_operation_transpose, rocblas_operation_none, M,
+                                 nc, KP, &one, fs->d_theta, KP, fs->d_phi, KP, &zero, fs->d_L, M));
+        const int blocks = (nc + 3) / 4;
+        hipLaunchKernelGGL(gmmhip::fc_lse_kernel, dim3(blocks), dim3(256), 0, h->stream, fs->d_L, nc,
+                           M, fs->d_bad, h->opt.min_loglike, h->opt.min_gamma, LLnat, n0,
+                           h->d_blockLL + nblk);
+        HIPCHK(h, hipGetLastError());
+        nblk += blocks;
+        if (stats) {
+            // part (column-major KP x M) = Phi (KP x nc) * Gamma^T (nc x M)
+            BLASCHK(h, rocblas_sgemm(fs->blas, rocblas_operation_none, rocblas_operation_transpose,
+                                     KP, M, nc, &one, fs->d_phi, KP, fs->d_L, M, &zero, fs->d_part,
+                                     KP));
+            const size_t E = (size_t)M * KP;
+            hipLaunchKernelGGL(gmmhip::fc_acc_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0,
+                               h->stream, fs->d_part, E, first, fs->d_acc);
+            HIPCHK(h, hipGetLastError());
+            first = 0;
+        }
+    }
+    HIPCHK(h, launch_sum_doubles(h->d_blockLL, nblk, h->d_tot, h->stream));
+    HIPCHK(h, hipMemcpyAsync(tot, h->d_tot, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    return GMMHIP_OK;
+}
+
+}  // namespace
+
+namespace gmmhip_detail {
+
+void full_state_free(gmmhip::FullState *fs) {
+    if (!fs) return;
+    if (fs->blas) rocblas_destroy_handle(fs->blas);
+    hipFree(fs->d_theta);
+    hipFree(fs->d_pairs);
+    hipFree(fs->d_cen);
+    hipFree(fs->d_phi);
+    hipFree(fs->d_L);
+    hipFree(fs->d_bad);
+    hipFree(fs->d_part);
+    hipFree(fs->d_acc);
+    delete fs;
+}
+
+int full_loglikes_device(gmmhip_ctx *h, float *d_out) {
+    double tot = 0.0;
+    int rc = full_pass(h, false, d_out, &tot);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return GMMHIP_OK;
+}
+
+int full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S, double *tot_ll,
+                    uint64_t *nframes) {
+    gmmhip::FullState *fs = h->full;
+    const int M = h->M, D = h->D, KP = fs->KP;
+    double tot = 0.0;
+    int rc = full_pass(h, true, nullptr, &tot);
+    if (rc) return rc;
+    std::vector<double> acc((size_t)M * KP);
+    HIPCHK(h, hipMemcpyAsync(acc.data(), fs->d_acc, acc.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (std::isnan(tot))
+        return fail(h, GMMHIP_ERR_NAN, "accumulateStatsMT(): overall log-likelihood = NaN!");
+    // Undo the centring in FP64: x = x' + c ->
+    //   F_i = F'_i + n c_i,  S_ij = S'_ij + c_i F'_j + c_j F'_i + n c_i c_j.
+    const size_t DP = (size_t)D * (D + 1) / 2;
+    for (int m = 0; m < M; m++) {
+        const double *r = acc.data() + (size_t)m * KP;
+        const double nm = r[0];
+        const double *f1 = r + 1, *s1 = r + 1 + D;
+        n[m] += nm;
+        if (flags & GMMHIP_ACC_MEAN)
+            for (int d = 0; d < D; d++) F[(size_t)m * D + d] += f1[d] + nm * (double)fs->h_cen[d];
+        if (flags & GMMHIP_ACC_VAR) {
+            size_t p = 0;
+            for (int i = 0; i < D; i++) {
+                const double ci = fs->h_cen[i];
+                for (int j = i; j < D; j++, p++) {
+                    const double cj = fs->h_cen[j];
+                    S[(size_t)m * DP + p] += s1[p] + ci * f1[j] + cj * f1[i] + nm * ci * cj;
+                }
+            }
+        }
+    }
+    *tot_ll += tot;
+    *nframes += h->N;
+    return GMMHIP_OK;
+}
+
+}  // namespace gmmhip_detail
+
+// Full-covariance counterpart of gmmhip_set_model: the inverse covariance
+// and log-determinant by an FP64 Cholesky (getCovInverse / getLogDet use
+// FP32 LAPACK spotrf/spotri), then the feature-space operand theta in FP32.
+int gmmhip_set_model_full(gmmhip_t h, int M, int D, const float *w, const float *mu,
+                          const float *cov) {
+    if (!h) return GMMHIP_ERR_ARG;
+    if (M <= 0 || D <= 0 || D > 127 || !w || !mu || !cov)
+        return fail(h, GMMHIP_ERR_ARG, "set_model_full: bad arguments (M=%d, D=%d)", M, D);
+    for (int m = 0; m < M; m++)
+        if (!(w[m] > 0.0f))
+            return fail(h, GMMHIP_ERR_ARG,
+                        "set_model_full: component %d has weight %g; insert only the active "
+                        "(weight > 0) components in RearrangeMixtures order", m, (double)w[m]);
+    gmmhip::FullState *fs = nullptr;
+    int rc = full_state(h, &fs);
+    if (rc) return rc;
+    const int K = 1 + D + D * (D + 1) / 2;
+    const int KP = (K + 7) / 8 * 8;
+    // centre: the weighted mean of the component means
+    std::vector<double> c(D, 0.0);
+    double ws = 0.0;
+    for (int m = 0; m < M; m++) ws += w[m];
+    for (int m = 0; m < M; m++)
+        for (int d = 0; d < D; d++) c[d] += (double)w[m] * mu[(size_t)m * D + d] / ws;
+    std::vector<float> cf(D);
+    for (int d = 0; d < D; d++) cf[d] = (float)c[d];
+
+    const double minus_log_2pi = -1.837877066409345;
+    std::vector<float> theta((size_t)M * KP, 0.0f);
+    std::vector<double> L((size_t)D * D), Li((size_t)D * D), P((size_t)D * D), mc(D), Pm(D);
+    for (int m = 0; m < M; m++) {
+        const float *cm = cov + (size_t)m * D * D;
+        for (int i = 0; i < D * D; i++) L[i] = cm[i];
+        // lower Cholesky, row-major
+        for (int j = 0; j < D; j++) {
+            double s = L[(size_t)j * D + j];
+            for (int k = 0; k < j; k++) s -= L[(size_t)j * D + k] * L[(size_t)j * D + k];
+            if (!(s > 0.0))
+                return fail(h, GMMHIP_ERR_VAR,
+                            "getCovInverse(): cholesky factorization failed! (component %d)", m);
+            const double ljj = std::sqrt(s);
+            L[(size_t)j * D + j] = ljj;
+            for (int i = j + 1; i < D; i++) {
+                double t = L[(size_t)i * D + j];
+                for (int k = 0; k < j; k++) t -= L[(size_t)i * D + k] * L[(size_t)j * D + k];
+                L[(size_t)i * D + j] = t / ljj;
+            }
+        }
+        double logdet = 0.0;
+        for (int i = 0; i < D; i++) logdet += std::log(L[(size_t)i * D + i]);
+        logdet *= 2.0;
+        std::fill(Li.begin(), Li.end(), 0.0);
+        for (int j = 0; j < D; j++) {
+            Li[(size_t)j * D + j] = 1.0 / L[(size_t)j * D + j];
+            for (int i = j + 1; i < D; i++) {
+                double s = 0.0;
+                for (int k = j; k < i; k++) s -= L[(size_t)i * D + k] * Li[(size_t)k * D + j];
+                Li[(size_t)i * D + j] = s / L[(size_t)i * D + i];
+            }
+        }
+        for (int i = 0; i < D; i++)
+            for (int j = i; j < D; j++) {
+                double s = 0.0;
+                for (int k = std::max(i, j); k < D; k++) s += Li[(size_t)k * D + i] * Li[(size_t)k * D + j];
+                P[(size_t)i * D + j] = P[(size_t)j * D + i] = s;
+            }
+        for (int d = 0; d < D; d++) mc[d] = (double)mu[(size_t)m * D + d] - (double)cf[d];
+        double q = 0.0;
+        for (int i = 0; i < D; i++) {
+            double s = 0.0;
+            for (int j = 0; j < D; j++) s += P[(size_t)i * D + j] * mc[j];
+            Pm[i] = s;
+            q += mc[i] * s;
+        }
+        float *th = theta.data() + (size_t)m * KP;
+        th[0] = (float)(std::log((double)w[m]) + minus_log_2pi * D / 2.0 - logdet / 2.0 - 0.5 * q);
+        for (int d = 0; d < D; d++) th[1 + d] = (float)Pm[d];
+        int k = 1 + D;
+        for (int i = 0; i < D; i++)
+            for (int j = i; j < D; j++, k++)
+                th[k] = (float)((i == j ? -0.5 : -1.0) * P[(size_t)i * D + j]);
+    }
+    const std::vector<int> pairs = fc_pairs(D);
+    DevGuard g(h->dev);
+    HIPCHK(h, ensure(&fs->d_theta, &fs->theta_cap, theta.size()));
+    HIPCHK(h, ensure(&fs->d_pairs, &fs->pairs_cap, pairs.size()));
+    HIPCHK(h, ensure(&fs->d_cen, &fs->cen_cap, (size_t)D));
+    HIPCHK(h, hipMemcpyAsync(fs->d_theta, theta.data(), theta.size() * sizeof(float),
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(fs->d_pairs, pairs.data(), pairs.size() * sizeof(int),
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(fs->d_cen, cf.data(), D * sizeof(float), hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    fs->K = K;
+    fs->KP = KP;
+    fs->h_cen = cf;
+    h->M = M;
+    h->D = D;
+    h->full_model = true;
+    h->m_w.assign(w, w + M);
+    h->m_mu.assign(mu, mu + (size_t)M * D);
+    h->m_var.clear();
+    h->model_version++;
+    return GMMHIP_OK;
+}

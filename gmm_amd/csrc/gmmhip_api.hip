@@ -35,6 +35,8 @@ uint64_t pick_chunk_frames(const gmmhip_ctx *h, int ngroups, bool mixed) {
 int check_ready(gmmhip_ctx *h) {
     if (!h) return GMMHIP_ERR_ARG;
     if (h->M <= 0) return fail(h, GMMHIP_ERR_STATE, "no model inserted");
+    if (h->full_model && h->opt.precision != GMMHIP_PREC_FP32)
+        return fail(h, GMMHIP_ERR_ARG, "full-covariance models run in FP32 only");
     if (!h->d_x || h->N == 0) return fail(h, GMMHIP_ERR_STATE, "no frames inserted");
     if (h->xD != h->D)
         return fail(h, GMMHIP_ERR_ARG, "frame dimension %d != model dimension %d", h->xD, h->D);
@@ -341,6 +343,7 @@ int gmmhip_destroy(gmmhip_t h) {
     hipFree(h->d_zscale);
     hipFree(h->d_P16);
     batch_state_free(h->batch);
+    full_state_free(h->full);
     hipFree(h->d_mu_f32);
     hipFree(h->d_var_f32);
     if (h->h_res) hipHostFree(h->h_res);
@@ -469,6 +472,7 @@ int gmmhip_set_model(gmmhip_t h, int M, int D, const float *w, const float *mu,
     h->m_mu.assign(mu, mu + (size_t)M * D);
     h->m_var.assign(var, var + (size_t)M * D);
     h->m_gconst = gconst;
+    h->full_model = false;
     h->model_version++;
     return GMMHIP_OK;
 }
@@ -547,6 +551,7 @@ int gmmhip_loglikes_device(gmmhip_t h, float *d_out) {
     if (rc) return rc;
     if (!d_out) return fail(h, GMMHIP_ERR_ARG, "loglikes_device: null output");
     DevGuard g(h->dev);
+    if (h->full_model) return full_loglikes_device(h, d_out);
     // padded: the mixed statistics kernel stages 32-frame steps of LL2
     HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, pad128(h->N)));
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N, 1)));
@@ -603,6 +608,7 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
         ((flags & GMMHIP_ACC_VAR) && !S))
         return fail(h, GMMHIP_ERR_ARG, "accumulate: null output for a requested statistic");
     DevGuard g(h->dev);
+    if (h->full_model) return full_accumulate(h, flags, n, F, S, tot_ll, nframes);
     // padded: the mixed statistics kernel stages 32-frame steps of LL2
     HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, pad128(h->N)));
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N, 1)));

@@ -104,6 +104,22 @@ const char *gmmhip_last_error(gmmhip_t h);
 int gmmhip_set_model(gmmhip_t h, int M, int D, const float *w,
                      const float *mu, const float *var);
 
+/* Insert a full-covariance model: M active components (weights > 0, in
+ * RearrangeMixtures order), D <= 127 dims, row-major mu [M*D] and
+ * symmetric cov [M*D*D].  Replaces insertModel's full-covariance branch
+ * (trainer/OL_GMMStatsEstimator.cpp:243-299: getCovInverse / getLogDet by
+ * Cholesky, :156-229) + uploadModel with alignedDimVar = D*D
+ * (trainer/CU_GMMStatsEstimator_CUDA.cu:284-318).  Afterwards
+ * gmmhip_accumulate exports S as the packed upper triangle of sum g x x^T,
+ * D(D+1)/2 per component in the reference's GMMStats order (row k holds
+ * columns k..D-1, trainer/OL_GMMStatsEstimator.cpp:1358-1364), and
+ * gmmhip_loglikes works as for a diagonal model.  FP32 only (opts.precision
+ * must be FP32); not usable with gmmhip_batch_adapt.  A diagonal model
+ * inserted later replaces it.  Returns GMMHIP_ERR_VAR when a covariance is
+ * not positive definite ("cholesky factorization failed"). */
+int gmmhip_set_model_full(gmmhip_t h, int M, int D, const float *w,
+                          const float *mu, const float *cov);
+
 /* Upload N frames of D floats given as row pointers (the reference's
  * float** vectors; rows may be non-contiguous) into device memory, where they
  * stay resident until replaced.  Replaces alignVectors + uploadParam

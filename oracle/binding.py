@@ -208,3 +208,71 @@ def sse_accumulate(x, w, mu, var, nthreads):
     if used < 0:
         raise ValueError("variance below 1e-6")
     return n, F, S, tot.value, used
+
+
+# ---------------------------------------------------------------- full covariance
+
+def prepare_model_full(w, cov):
+    """(prec[M,D,D], gconst[M]) of a full-covariance model; raises on a
+    matrix that is not positive definite."""
+    w, cov = _f32(w), _f32(cov)
+    M, D = cov.shape[0], cov.shape[1]
+    prec = np.zeros((M, D, D))
+    gconst = np.zeros(M)
+    rc = lib().orc_prepare_model_full(M, D, _p(w), _p(cov), _p(prec), _p(gconst))
+    if rc != 0:
+        raise ValueError("covariance of component %d is not positive definite" % (-rc - 1))
+    return prec, gconst
+
+
+def frame_loglik_full(x, w, mu, cov, want_gamma=False):
+    x, mu = _f32(x), _f32(mu)
+    N, D = x.shape
+    M = mu.shape[0]
+    prec, gconst = prepare_model_full(w, cov)
+    LL = np.zeros(N)
+    gamma = np.zeros((N, M)) if want_gamma else None
+    lib().orc_frame_loglik_full(N, D, M, _p(x), _p(mu), _p(prec), _p(gconst),
+                                _p(LL), _p(gamma))
+    return (LL, gamma) if want_gamma else LL
+
+
+def accumulate_full(x, w, mu, cov, min_gamma=MIN_GAMMA):
+    """(n[M], F[M,D], S[M,D(D+1)/2] packed upper, tot_ll)."""
+    x, mu = _f32(x), _f32(mu)
+    N, D = x.shape
+    M = mu.shape[0]
+    prec, gconst = prepare_model_full(w, cov)
+    n = np.zeros(M)
+    F = np.zeros((M, D))
+    S = np.zeros((M, D * (D + 1) // 2))
+    tot = ctypes.c_double(0.0)
+    lib().orc_accumulate_full(N, D, M, _p(x), _p(mu), _p(prec), _p(gconst),
+                              ctypes.c_double(min_gamma), _p(n), _p(F), _p(S),
+                              ctypes.byref(tot))
+    return n, F, S, tot.value
+
+
+def update_em_full(w, mu, cov, n, F, S, tot_samples):
+    w, mu, cov = _f32(w).copy(), _f32(mu).copy(), _f32(cov).copy()
+    M, D = mu.shape
+    n, F, S = _f64(n), _f64(F), _f64(S)
+    left = lib().orc_update_em_full(M, D, _p(w), _p(mu), _p(cov), _p(n), _p(F), _p(S),
+                                    ctypes.c_double(tot_samples))
+    return w, mu, cov, left
+
+
+def eig_leading(cov):
+    cov = _f32(cov)
+    D = cov.shape[0]
+    lam = ctypes.c_double(0.0)
+    v = np.zeros(D)
+    lib().orc_eig_leading(D, _p(cov), ctypes.byref(lam), _p(v))
+    return lam.value, v
+
+
+def packed_upper_index(D):
+    """(rows, cols) of the packed upper-triangle order of the full
+    second-order statistics."""
+    r, c = np.triu_indices(D)
+    return r, c

@@ -463,3 +463,326 @@ void orc_mllr_apply(int M, int D, const double *W, const float *mu_in,
         }
     }
 }
+
+/* ------------------------------------------------------------------------
+ * Full covariance (survey scope row (f)2).  Paths relative to GMM/lib_bmssr/.
+ *
+ * How each function is pinned (tests/test_oracle_fullcov.py):
+ *   orc_prepare_model_full / orc_frame_loglik_full -- example/data.logL
+ *       through example/myModel.gmm written as full matrices with zero
+ *       off-diagonals; scipy multivariate_normal + logsumexp brute force;
+ *       affine invariance (x -> Ax + b shifts every log-likelihood by
+ *       -log|det A|).
+ *   orc_accumulate_full -- numpy posterior-weighted outer products on tiny
+ *       inputs; the diagonal of S equals orc_accumulate's S on a diagonal
+ *       model.
+ *   orc_update_em_full -- one component with all-ones posteriors gives the
+ *       sample mean and covariance; example/myModelFC.gmm reproduced by the
+ *       from-scratch driver (to the level stated in the test).
+ *   orc_eig_leading -- numpy.linalg.eigh on random SPD matrices.
+ * ------------------------------------------------------------------------ */
+
+/* In-place lower Cholesky factor of a row-major SPD matrix in double: the
+ * FP64 counterpart of spotrf('L') in getCovInverse / getLogDet
+ * (trainer/OL_GMMStatsEstimator.cpp:156-229).  Returns 0, or 1 when a pivot
+ * is not positive (the reference throws there). */
+static int chol_lower(int D, double *A)
+{
+    for (int j = 0; j < D; j++) {
+        double s = A[(size_t)j * D + j];
+        for (int k = 0; k < j; k++)
+            s -= A[(size_t)j * D + k] * A[(size_t)j * D + k];
+        if (!(s > 0.0))
+            return 1;
+        const double ljj = sqrt(s);
+        A[(size_t)j * D + j] = ljj;
+        for (int i = j + 1; i < D; i++) {
+            double t = A[(size_t)i * D + j];
+            for (int k = 0; k < j; k++)
+                t -= A[(size_t)i * D + k] * A[(size_t)j * D + k];
+            A[(size_t)i * D + j] = t / ljj;
+        }
+        for (int k = j + 1; k < D; k++)
+            A[(size_t)j * D + k] = 0.0;
+    }
+    return 0;
+}
+
+/* insertModel, full-covariance branch (OL_GMMStatsEstimator.cpp:291-299):
+ * icov = cov^-1 (getCovInverse, spotrf + spotri, :156-189), logGconst =
+ * MINUS_LOG_2PI*D/2 - logdet/2 with logdet = 2 sum log L_ii (getLogDet,
+ * :194-229), plus log w.  prec[m*D*D] is the full symmetric inverse.
+ * Returns 0, or -(1+m) when component m is not positive definite. */
+int orc_prepare_model_full(int M, int D, const float *w, const float *cov,
+                           double *prec, double *gconst)
+{
+    double *L = (double *)malloc(sizeof(double) * (size_t)D * D);
+    double *Li = (double *)malloc(sizeof(double) * (size_t)D * D);
+    for (int m = 0; m < M; m++) {
+        const float *c = cov + (size_t)m * D * D;
+        for (int i = 0; i < D * D; i++)
+            L[i] = c[i];
+        if (chol_lower(D, L)) {
+            free(L);
+            free(Li);
+            return -(1 + m);
+        }
+        double logdet = 0.0;
+        for (int i = 0; i < D; i++)
+            logdet += log(L[(size_t)i * D + i]);
+        logdet *= 2.0;
+        /* Li = L^-1 (lower), by forward substitution on the identity */
+        memset(Li, 0, sizeof(double) * (size_t)D * D);
+        for (int j = 0; j < D; j++) {
+            Li[(size_t)j * D + j] = 1.0 / L[(size_t)j * D + j];
+            for (int i = j + 1; i < D; i++) {
+                double s = 0.0;
+                for (int k = j; k < i; k++)
+                    s -= L[(size_t)i * D + k] * Li[(size_t)k * D + j];
+                Li[(size_t)i * D + j] = s / L[(size_t)i * D + i];
+            }
+        }
+        /* prec = Li^T Li */
+        double *P = prec + (size_t)m * D * D;
+        for (int i = 0; i < D; i++)
+            for (int j = i; j < D; j++) {
+                double s = 0.0;
+                for (int k = (i > j ? i : j); k < D; k++)
+                    s += Li[(size_t)k * D + i] * Li[(size_t)k * D + j];
+                P[(size_t)i * D + j] = s;
+                P[(size_t)j * D + i] = s;
+            }
+        double logw = (w[m] > 0.0f) ? log((double)w[m]) : orc_min_loglike();
+        gconst[m] = ORC_MINUS_LOG_2PI * D / 2.0 - logdet / 2.0 + logw;
+    }
+    free(L);
+    free(Li);
+    return 0;
+}
+
+/* computeMixLogLikeFullCov (OL_GMMStatsEstimator.cpp:593-621, SSE form
+ * OL_GMMStatsEstimator_SSE.cpp:258-317): gconst - 0.5 (x-mu)^T icov (x-mu),
+ * non-finite -> _minLogLike; log-weight included in gconst. */
+static double comp_ll_full(const float *x, int D, const float *mu_m,
+                           const double *P, double gconst_m)
+{
+    double q = 0.0;
+    for (int i = 0; i < D; i++) {
+        const double ti = (double)x[i] - (double)mu_m[i];
+        double r = 0.0;
+        for (int j = 0; j < D; j++)
+            r += P[(size_t)i * D + j] * ((double)x[j] - (double)mu_m[j]);
+        q += ti * r;
+    }
+    double l = gconst_m - 0.5 * q;
+    if (!isfinite(l))
+        return orc_min_loglike();
+    return l;
+}
+
+/* Per-frame log-likelihood and posteriors with full covariances: the same
+ * combination rules as orc_frame_loglik (exact log-sum-exp, the GPU path's
+ * semantics; clamp at _minLogLike; gamma before the 1e-4 cut). */
+void orc_frame_loglik_full(int N, int D, int M, const float *x, const float *mu,
+                           const double *prec, const double *gconst, double *LL,
+                           double *gamma)
+{
+    const double minLL = orc_min_loglike();
+    double *l = (double *)malloc(sizeof(double) * (M > 0 ? M : 1));
+    for (int n = 0; n < N; n++) {
+        const float *xn = x + (size_t)n * D;
+        double mx = minLL;
+        for (int m = 0; m < M; m++) {
+            l[m] = comp_ll_full(xn, D, mu + (size_t)m * D, prec + (size_t)m * D * D,
+                                gconst[m]);
+            if (l[m] > mx)
+                mx = l[m];
+        }
+        double ll;
+        if (mx <= minLL) {
+            ll = minLL;
+        } else {
+            double s = 0.0;
+            for (int m = 0; m < M; m++)
+                s += exp(l[m] - mx);
+            ll = mx + log(s);
+        }
+        if (!isfinite(ll) || ll < minLL)
+            ll = minLL;
+        LL[n] = ll;
+        if (gamma) {
+            for (int m = 0; m < M; m++) {
+                double g;
+                if (ll <= minLL)
+                    g = 0.0;
+                else if (M == 1)
+                    g = 1.0;
+                else
+                    g = (l[m] > minLL) ? exp(l[m] - ll) : 0.0;
+                gamma[(size_t)n * M + m] = g;
+            }
+        }
+    }
+    free(l);
+}
+
+/* Packed upper-triangle index of (k, kk), kk >= k, in the reference's full
+ * second-order statistics (OL_GMMStatsEstimator.cpp:1358-1364: shift =
+ * k(k+1)/2, index k*D + kk - shift). */
+static size_t packed_upper(int D, int k, int kk)
+{
+    return (size_t)k * D + kk - (size_t)k * (k + 1) / 2;
+}
+
+/* accumulateStats with _fFullVar (OL_GMMStatsEstimator.cpp:1105-1193,
+ * :1334-1378): for posterior > min_gamma, n += g, F += g x, S[k<=kk] +=
+ * g x_k x_kk (packed upper, D(D+1)/2 per component).  Outputs overwritten. */
+void orc_accumulate_full(int N, int D, int M, const float *x, const float *mu,
+                         const double *prec, const double *gconst, double min_gamma,
+                         double *n_out, double *F, double *S, double *tot_ll)
+{
+    const size_t DP = (size_t)D * (D + 1) / 2;
+    double *g = (double *)malloc(sizeof(double) * (M > 0 ? M : 1));
+    double LL;
+    memset(n_out, 0, sizeof(double) * M);
+    memset(F, 0, sizeof(double) * (size_t)M * D);
+    memset(S, 0, sizeof(double) * (size_t)M * DP);
+    double tot = 0.0;
+    for (int n = 0; n < N; n++) {
+        const float *xn = x + (size_t)n * D;
+        orc_frame_loglik_full(1, D, M, xn, mu, prec, gconst, &LL, g);
+        tot += LL;
+        for (int m = 0; m < M; m++) {
+            if (!(g[m] > min_gamma))
+                continue;
+            n_out[m] += g[m];
+            double *Fm = F + (size_t)m * D, *Sm = S + (size_t)m * DP;
+            for (int k = 0; k < D; k++) {
+                const double xk = xn[k];
+                Fm[k] += g[m] * xk;
+                for (int kk = k; kk < D; kk++)
+                    Sm[packed_upper(D, k, kk)] += g[m] * xk * (double)xn[kk];
+            }
+        }
+    }
+    *tot_ll = tot;
+    free(g);
+}
+
+/* UpdateModelEM, full-covariance branch, non-robust (OL_ModelAdapt.cpp:
+ * 823-949): w = n/N_tot; mean = (float)(F/n); cov[k][kk] = cov[kk][k] =
+ * (float)(S/n - mean_k * mean_kk) with the product of the float means taken
+ * in float (:917-923); no variance floor.  Components with n < 2 are reset
+ * (GMMMixture::Reset, OL_GMModel.cpp:73-83: zero mean, identity covariance)
+ * and get weight 0; weights are renormalised over the survivors. */
+int orc_update_em_full(int M, int D, float *w, float *mu, float *cov,
+                       const double *n, const double *F, const double *S,
+                       double tot_samples)
+{
+    const double MIN_IND_SAMPLES = 2.0; /* OL_ModelAdapt.h:54 */
+    const size_t DP = (size_t)D * (D + 1) / 2;
+    int nzero = 0, wchange = 0;
+    double gcorr = 0.0;
+    float *mean = (float *)malloc(sizeof(float) * (D > 0 ? D : 1));
+    for (int j = 0; j < M; j++) {
+        float *c = cov + (size_t)j * D * D;
+        if (w[j] == 0.0f) {
+            nzero++;
+            continue;
+        }
+        const double np = n[j];
+        if (np < MIN_IND_SAMPLES) {
+            memset(mu + (size_t)j * D, 0, sizeof(float) * D);
+            memset(c, 0, sizeof(float) * (size_t)D * D);
+            for (int d = 0; d < D; d++)
+                c[(size_t)d * D + d] = 1.0f;
+            w[j] = 0.0f;
+            wchange = 1;
+            continue;
+        }
+        w[j] = (float)(np / tot_samples);
+        wchange = 1;
+        gcorr += w[j];
+        for (int k = 0; k < D; k++)
+            mean[k] = (float)(F[(size_t)j * D + k] / np);
+        const double *Sj = S + (size_t)j * DP;
+        for (int k = 0; k < D; k++)
+            for (int kk = k; kk < D; kk++) {
+                const float mm = mean[k] * mean[kk];
+                const float v = (float)(Sj[packed_upper(D, k, kk)] / np - (double)mm);
+                c[(size_t)k * D + kk] = v;
+                c[(size_t)kk * D + k] = v;
+            }
+        memcpy(mu + (size_t)j * D, mean, sizeof(float) * D);
+    }
+    if (wchange && gcorr > 0.0)
+        for (int j = 0; j < M; j++)
+            w[j] = (float)(w[j] / gcorr);
+    free(mean);
+    return M - nzero;
+}
+
+/* Leading eigenpair of a symmetric matrix (the ssyevr('V','I','U', il = iu
+ * = D) call of getEigValVecs, OL_ModelAdapt.cpp:72-160, used by the
+ * full-covariance splitGaussian, :614-660), by cyclic Jacobi rotations in
+ * double.  The eigenvector has unit length; its sign, which LAPACK leaves
+ * open, is fixed so that the entry of largest magnitude (lowest index on a
+ * tie) is negative -- the choice that reproduces example/myModelFC.gmm. */
+void orc_eig_leading(int D, const float *cov, double *lam, double *v)
+{
+    double *A = (double *)malloc(sizeof(double) * (size_t)D * D);
+    double *V = (double *)malloc(sizeof(double) * (size_t)D * D);
+    for (int i = 0; i < D * D; i++)
+        A[i] = cov[i];
+    for (int i = 0; i < D; i++)
+        for (int j = 0; j < D; j++)
+            V[(size_t)i * D + j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0;
+        for (int p = 0; p < D; p++)
+            for (int q = p + 1; q < D; q++)
+                off += A[(size_t)p * D + q] * A[(size_t)p * D + q];
+        if (off < 1e-30)
+            break;
+        for (int p = 0; p < D; p++)
+            for (int q = p + 1; q < D; q++) {
+                const double apq = A[(size_t)p * D + q];
+                if (fabs(apq) < 1e-300)
+                    continue;
+                const double app = A[(size_t)p * D + p], aqq = A[(size_t)q * D + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) /
+                                 (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < D; k++) {
+                    const double akp = A[(size_t)k * D + p], akq = A[(size_t)k * D + q];
+                    A[(size_t)k * D + p] = cs * akp - sn * akq;
+                    A[(size_t)k * D + q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < D; k++) {
+                    const double apk = A[(size_t)p * D + k], aqk = A[(size_t)q * D + k];
+                    A[(size_t)p * D + k] = cs * apk - sn * aqk;
+                    A[(size_t)q * D + k] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < D; k++) {
+                    const double vkp = V[(size_t)k * D + p], vkq = V[(size_t)k * D + q];
+                    V[(size_t)k * D + p] = cs * vkp - sn * vkq;
+                    V[(size_t)k * D + q] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < D; i++)
+        if (A[(size_t)i * D + i] > A[(size_t)best * D + best])
+            best = i;
+    *lam = A[(size_t)best * D + best];
+    int imax = 0;
+    for (int i = 1; i < D; i++)
+        if (fabs(V[(size_t)i * D + best]) > fabs(V[(size_t)imax * D + best]))
+            imax = i;
+    const double sgn = (V[(size_t)imax * D + best] > 0.0) ? -1.0 : 1.0;
+    for (int i = 0; i < D; i++)
+        v[i] = sgn * V[(size_t)i * D + best];
+    free(A);
+    free(V);
+}

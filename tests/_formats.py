@@ -53,6 +53,30 @@ def read_gmm_txt(path):
     return w, mu, var, norm
 
 
+def read_gmm_txt_full(path):
+    """Returns (w[M], mu[M,D], cov[M,D,D], norm[D]) for a full-covariance
+    model: weight line "w F", then the mean, then D rows of 1 + D numbers
+    (the diagonal element, then the row), OL_GMModel.cpp:511-533 / :700-740."""
+    toks = open(path).read().split()
+    p = 0
+    D = int(toks[p]); p += 1
+    norm = np.array([float(t) for t in toks[p:p + D]], np.float32); p += D
+    M = int(toks[p]); p += 1
+    w = np.zeros(M, np.float32)
+    mu = np.zeros((M, D), np.float32)
+    cov = np.zeros((M, D, D), np.float32)
+    for m in range(M):
+        w[m] = float(toks[p]); p += 1
+        if toks[p] != "F":
+            raise ValueError("diagonal model")
+        p += 1
+        mu[m] = [float(t) for t in toks[p:p + D]]; p += D
+        for i in range(D):
+            p += 1  # the diagonal element, repeated
+            cov[m, i] = [float(t) for t in toks[p:p + D]]; p += D
+    return w, mu, cov, norm
+
+
 def read_savemat_txt(path):
     with open(path) as f:
         toks = f.read().split()

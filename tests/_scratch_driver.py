@@ -68,3 +68,63 @@ def train_from_scratch(x, nummix, it_em):
             nonzero += 1
     w, mu, var, _ = _rearrange(w, mu, var)
     return w, mu, var
+
+
+def _rearrange_full(w, mu, cov):
+    perm, nz = orc.rearrange_perm(w)
+    return w[perm].copy(), mu[perm].copy(), cov[perm].copy(), nz
+
+
+def train_from_scratch_full(x, nummix, it_em):
+    """trainGMM --type 0 --full-cov: the same driver with the full-covariance
+    statistics / update, GMMMixture::Reset's identity covariance for empty
+    slots (OL_GMModel.cpp:73-83) and the full-covariance splitGaussian
+    (OL_ModelAdapt.cpp:630-646): the new component copies the covariance and
+    the two means move by -+ sqrt(lambda) v along the leading eigenpair."""
+    x = np.ascontiguousarray(x, np.float32)
+    N, D = x.shape
+    xd = x.astype(np.float64)
+    r, c = np.triu_indices(D)
+    n = np.array([float(N)])
+    F = xd.sum(0)[None]
+    S = (xd.T @ xd)[r, c][None]
+    eye = np.eye(D, dtype=np.float32)
+    w0, mu0, cov0, left = orc.update_em_full(np.ones(1), np.zeros((1, D)), eye[None],
+                                             n, F, S, N)
+    w = np.zeros(nummix, np.float32)
+    mu = np.zeros((nummix, D), np.float32)
+    cov = np.tile(eye, (nummix, 1, 1))
+    w[0], mu[0], cov[0] = w0[0], mu0[0], cov0[0]
+    nonzero = 1
+    for it in range(nummix):
+        for _ in range(it_em):
+            w, mu, cov, _ = _rearrange_full(w, mu, cov)
+            act = w > 0
+            na, Fa, Sa, _tot = orc.accumulate_full(x, w[act], mu[act], cov[act])
+            n = np.zeros(nummix)
+            F = np.zeros((nummix, D))
+            S = np.zeros((nummix, D * (D + 1) // 2))
+            n[act], F[act], S[act] = na, Fa, Sa
+            if nonzero > 1:
+                w, mu, cov, nonzero = orc.update_em_full(w, mu, cov, n, F, S, N)
+            if nonzero < 2:
+                break
+        if nonzero == 0 or nonzero == nummix:
+            break
+        if it < nummix - 1:
+            imax, wm = 0, 0.0
+            for m in range(nummix - 1, -1, -1):
+                if wm < w[m]:
+                    wm, imax = w[m], m
+            znew = max(m for m in range(nummix) if w[m] == 0)
+            cov[znew] = cov[imax]
+            half = np.float32(w[imax] / np.float32(2.0))
+            w[znew] = half
+            w[imax] = half
+            lam, v = orc.eig_leading(cov[imax])
+            shift = (np.sqrt(lam) * v).astype(np.float32)
+            mu[znew] = mu[imax] + shift
+            mu[imax] = mu[imax] - shift
+            nonzero += 1
+    w, mu, cov, _ = _rearrange_full(w, mu, cov)
+    return w, mu, cov

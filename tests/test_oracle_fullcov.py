@@ -1,0 +1,195 @@
+"""Pins the oracle's full-covariance functions (oracle/gmm_oracle.c,
+orc_*_full, orc_eig_leading) before they check the HIP full-covariance path:
+the diagonal golden vector through a full-matrix model, the reference's
+full-covariance fixture, brute force and invariants."""
+import os
+
+import numpy as np
+import pytest
+from scipy.special import logsumexp
+from scipy.stats import multivariate_normal
+
+import _formats as fm
+import _oracle as orc
+import _scratch_driver as sd
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rand_full_model(rng, M, D, mean_sd=2.0):
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, mean_sd, (M, D)).astype(np.float32)
+    A = rng.normal(size=(M, D, D))
+    cov = (A @ A.transpose(0, 2, 1) / D + 0.3 * np.eye(D)).astype(np.float32)
+    cov = 0.5 * (cov + cov.transpose(0, 2, 1))
+    return w, mu, cov
+
+
+def diag_as_full(var):
+    M, D = var.shape
+    cov = np.zeros((M, D, D), np.float32)
+    cov[:, np.arange(D), np.arange(D)] = var
+    return cov
+
+
+def test_diagonal_model_as_full_matrices_reproduces_data_logl(golden):
+    """example/myModel.gmm written with zero off-diagonals gives
+    example/data.logL (the diagonal golden vector) to print precision."""
+    LL = orc.frame_loglik_full(golden["x"], golden["w"], golden["mu"], diag_as_full(golden["var"]))
+    ref = golden["ll"]
+    tol = 0.5 * 10.0 ** (np.floor(np.log10(np.abs(ref))) - 5) + 1e-6 * np.abs(ref)
+    assert np.all(np.abs(LL - ref) <= tol), np.abs(LL - ref).max()
+
+
+def test_planted_scaled_covariance_fails():
+    """A covariance off by a factor (wrong log-det and quadratic form) is
+    caught by the brute-force check that the true one passes."""
+    rng = np.random.default_rng(3)
+    w, mu, cov = rand_full_model(rng, 2, 4)
+    x = rng.normal(size=(50, 4)).astype(np.float32)
+    ref = logsumexp(np.stack([np.log(np.float64(w[m])) + multivariate_normal(
+        mu[m].astype(np.float64), cov[m].astype(np.float64)).logpdf(x.astype(np.float64))
+        for m in range(2)], 1), axis=1)
+    good = orc.frame_loglik_full(x, w, mu, cov)
+    assert np.abs(good - ref).max() < 1e-9
+    bad = orc.frame_loglik_full(x, w, mu, 2.0 * cov)  # log-det and quadratic both change
+    assert np.abs(bad - ref).max() > 0.1
+
+
+@pytest.mark.parametrize("M,D,N", [(1, 3, 20), (3, 5, 40), (4, 12, 30)])
+def test_loglik_brute_force(M, D, N):
+    rng = np.random.default_rng(M * 100 + D)
+    w, mu, cov = rand_full_model(rng, M, D)
+    x = rng.normal(0, 2, (N, D)).astype(np.float32)
+    LL, g = orc.frame_loglik_full(x, w, mu, cov, want_gamma=True)
+    l = np.stack([np.log(np.float64(w[m])) + multivariate_normal(
+        mu[m].astype(np.float64), cov[m].astype(np.float64)).logpdf(x.astype(np.float64))
+        for m in range(M)], 1).reshape(N, M)
+    ref = logsumexp(l, axis=1)
+    assert np.abs(LL - ref).max() < 1e-9
+    assert np.abs(g - np.exp(l - ref[:, None])).max() < 1e-12
+
+
+def test_accumulate_brute_force_packed_upper():
+    rng = np.random.default_rng(5)
+    M, D, N = 3, 6, 80
+    w, mu, cov = rand_full_model(rng, M, D)
+    x = rng.normal(0, 2, (N, D)).astype(np.float32)
+    _, g = orc.frame_loglik_full(x, w, mu, cov, want_gamma=True)
+    g = np.where(g > orc.MIN_GAMMA, g, 0.0)
+    n, F, S, tot = orc.accumulate_full(x, w, mu, cov)
+    xd = x.astype(np.float64)
+    r, c = np.triu_indices(D)
+    assert np.allclose(n, g.sum(0), rtol=1e-13)
+    assert np.allclose(F, g.T @ xd, rtol=1e-12, atol=1e-12)
+    assert np.allclose(S, np.einsum("nm,nk,nl->mkl", g, xd, xd)[:, r, c], rtol=1e-12, atol=1e-12)
+    assert tot == pytest.approx(orc.frame_loglik_full(x, w, mu, cov).sum(), rel=1e-13)
+
+
+def test_full_stats_diagonal_matches_diagonal_oracle():
+    """On a diagonal model, n, F and the diagonal of S equal the diagonal
+    oracle's statistics."""
+    rng = np.random.default_rng(8)
+    M, D, N = 4, 5, 200
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    var = rng.uniform(0.3, 3, (M, D)).astype(np.float32)
+    x = rng.normal(0, 2, (N, D)).astype(np.float32)
+    n1, F1, S1, t1 = orc.accumulate(x, w, mu, var)
+    n2, F2, S2, t2 = orc.accumulate_full(x, w, mu, diag_as_full(var))
+    r, c = np.triu_indices(D)
+    diag_pos = np.where(r == c)[0]
+    assert np.allclose(n1, n2, rtol=1e-12)
+    assert np.allclose(F1, F2, rtol=1e-12, atol=1e-12)
+    assert np.allclose(S1, S2[:, diag_pos], rtol=1e-12, atol=1e-12)
+    assert t1 == pytest.approx(t2, rel=1e-12)
+
+
+def test_affine_invariance():
+    """x -> A x + b with mu -> A mu + b and cov -> A cov A^T shifts every
+    log-likelihood by -log|det A| and leaves the posteriors unchanged."""
+    rng = np.random.default_rng(11)
+    M, D, N = 3, 4, 60
+    w, mu, cov = rand_full_model(rng, M, D)
+    x = rng.normal(0, 2, (N, D)).astype(np.float32)
+    A = np.diag([2.0, 0.5, 4.0, 1.0]).astype(np.float32)  # exact in float
+    b = np.array([1.0, -2.0, 0.5, 3.0], np.float32)
+    x2 = (x @ A.T + b).astype(np.float32)
+    mu2 = (mu @ A.T + b).astype(np.float32)
+    cov2 = np.einsum("ij,mjk,lk->mil", A, cov, A).astype(np.float32)
+    L1, g1 = orc.frame_loglik_full(x, w, mu, cov, want_gamma=True)
+    L2, g2 = orc.frame_loglik_full(x2, w, mu2, cov2, want_gamma=True)
+    assert np.allclose(L2, L1 - np.log(abs(np.linalg.det(A.astype(np.float64)))), atol=1e-9)
+    assert np.allclose(g1, g2, atol=1e-10)
+
+
+def test_update_em_one_component_is_sample_covariance():
+    rng = np.random.default_rng(13)
+    N, D = 500, 5
+    x = rng.normal(0, 1.5, (N, D)).astype(np.float32)
+    xd = x.astype(np.float64)
+    r, c = np.triu_indices(D)
+    n = np.array([float(N)])
+    F = xd.sum(0)[None]
+    S = (xd.T @ xd)[r, c][None]
+    w, mu, cov, left = orc.update_em_full(np.ones(1), np.zeros((1, D)), np.eye(D)[None], n, F, S, N)
+    assert left == 1 and w[0] == 1.0
+    m = (xd.mean(0)).astype(np.float32)
+    assert np.array_equal(mu[0], m)
+    ref = xd.T @ xd / N - np.outer(m, m).astype(np.float32)
+    assert np.allclose(cov[0], ref, rtol=1e-6, atol=1e-7)
+    assert np.array_equal(cov[0], cov[0].T)
+
+
+def test_update_em_resets_starved_component():
+    D = 3
+    w = np.array([0.5, 0.5], np.float32)
+    mu = np.ones((2, D), np.float32)
+    cov = np.tile(2 * np.eye(D, dtype=np.float32), (2, 1, 1))
+    n = np.array([10.0, 1.0])
+    F = np.ones((2, D)) * n[:, None]
+    r, c = np.triu_indices(D)
+    S = np.tile((3 * np.eye(D))[r, c], (2, 1)) * n[:, None]
+    w2, mu2, cov2, left = orc.update_em_full(w, mu, cov, n, F, S, 11.0)
+    # the count returned excludes only components that entered with weight
+    # 0, as the reference's (OL_ModelAdapt.cpp:870-873, :948)
+    assert left == 2 and w2[1] == 0 and w2[0] == 1.0
+    assert np.array_equal(cov2[1], np.eye(D, dtype=np.float32)) and not mu2[1].any()
+
+
+@pytest.mark.parametrize("D", [2, 5, 20])
+def test_eig_leading_matches_numpy(D):
+    rng = np.random.default_rng(D)
+    A = rng.normal(size=(D, D))
+    C = (A @ A.T / D + np.eye(D)).astype(np.float32)
+    lam, v = orc.eig_leading(C)
+    ev, V = np.linalg.eigh(C.astype(np.float64))
+    assert lam == pytest.approx(ev[-1], rel=1e-12)
+    assert np.abs(np.abs(v) - np.abs(V[:, -1])).max() < 1e-9
+    assert np.linalg.norm(v) == pytest.approx(1.0, rel=1e-12)
+    assert v[np.argmax(np.abs(v))] < 0
+
+
+def test_mymodelfc_reproduced(golden):
+    """example/myModelFC.gmm (trainModel.bat:6: --type 0 --nummix 4 --it-EM 8
+    --full-cov) is reproduced by the full-covariance split driver: same
+    components in the same order, weights to 3e-4, means to 6e-3 and
+    covariances to 4e-3.  That is as close as the restatement gets: at this
+    point the reference's EM still moves the means by 1e-2 per iteration,
+    and the residual is unchanged under float-precision E-step, float
+    statistics products or perturbed split vectors, so it is not rounding;
+    its cause is not known.  A wrong split sign or direction misses by 0.5
+    or more in the means (checked below)."""
+    rw, rmu, rcov, _ = fm.read_gmm_txt_full(os.path.join(HERE, "golden", "myModelFC.gmm"))
+    w, mu, cov = sd.train_from_scratch_full(golden["x"], 4, 8)
+    assert np.abs(w - rw).max() < 3e-4
+    assert np.abs(mu - rmu).max() < 6e-3
+    assert np.abs(cov - rcov).max() < 4e-3
+
+
+def test_mymodelfc_split_sign_matters(golden, monkeypatch):
+    rw, rmu, rcov, _ = fm.read_gmm_txt_full(os.path.join(HERE, "golden", "myModelFC.gmm"))
+    real = orc.eig_leading
+    monkeypatch.setattr(orc, "eig_leading", lambda c: (lambda lv: (lv[0], -lv[1]))(real(c)))
+    w, mu, cov = sd.train_from_scratch_full(golden["x"], 4, 8)
+    assert np.abs(mu - rmu).max() > 0.3
