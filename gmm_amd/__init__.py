@@ -34,7 +34,7 @@ KT_LOGLIK, KT_STATS, KT_REDUCE, KT_COUNT = 0, 1, 2, 3
 # host-layer entry points (include/gmmhost.h), bound in gmm_amd.host
 EXTRA_EXPORTS = [
     "gmmhost_model_load", "gmmhost_model_save", "gmmhost_model_create",
-    "gmmhost_model_dims", "gmmhost_model_get", "gmmhost_model_rearrange",
+    "gmmhost_model_create_full", "gmmhost_model_full", "gmmhost_model_dims", "gmmhost_model_get", "gmmhost_model_rearrange",
     "gmmhost_model_free", "gmmhost_update_em", "gmmhost_update_map",
     "gmmhost_mllr_transform", "gmmhost_mllr_apply", "gmmhost_last_error",
 ]

@@ -53,8 +53,6 @@ public:
     // Rearranges the model (zero-weight components last), computes its
     // constants and uploads it.  first=false adds it to the current group.
     void insertModel(GMModel &model, bool first = true) {
-        if (model.GetFullCovStatus())
-            throw std::runtime_error("insertModel(): full-covariance models are not supported");
         if (!first && !group_.empty() && model.GetDimension() != group_[0].D)
             throw std::runtime_error("Models (GMMs) have to have same dimensions");
         if (first) group_.clear();
@@ -65,9 +63,15 @@ public:
         p.M = p.Malloc - (int)nzero;
         p.w.assign(model.weights().begin(), model.weights().begin() + p.M);
         p.mu.assign(model.means().begin(), model.means().begin() + (size_t)p.M * p.D);
-        p.var.assign(model.vars().begin(), model.vars().begin() + (size_t)p.M * p.D);
-        for (float v : p.var)
-            if (v < _min_var) throw std::runtime_error("insertModel(): model contains very low variances!");
+        p.full = model.GetFullCovStatus();
+        const size_t dv = model.GetDimVar();
+        p.var.assign(model.vars().begin(), model.vars().begin() + (size_t)p.M * dv);
+        // the diagonal form rejects variances below min_var
+        // (OL_GMMStatsEstimator.cpp:279-285); the full form rejects matrices
+        // that are not positive definite when they are uploaded
+        if (!p.full)
+            for (float v : p.var)
+                if (v < _min_var) throw std::runtime_error("insertModel(): model contains very low variances!");
         group_.push_back(std::move(p));
         select(group_.size() - 1);
     }
@@ -79,9 +83,9 @@ public:
 
     void setAccFlags(bool mixProbsOnly, bool meanStats, bool varStats, bool varFull = false,
                      bool auxStats = false) {
-        if (varFull || auxStats)
-            throw std::runtime_error("setAccFlags(): full-covariance and robust statistics are not supported");
+        if (auxStats) throw std::runtime_error("setAccFlags(): robust (aux) statistics are not supported");
         flagsSet_ = true;
+        fFullVar_ = varFull;
         fMixProbOnly_ = mixProbsOnly;
         fMean_ = meanStats && !mixProbsOnly;
         fVar_ = varStats && !mixProbsOnly;
@@ -99,18 +103,23 @@ public:
             fVar_ = gmms.isAllocated() ? gmms.hasVars() : false;
         }
         if (!gmms.isAllocated()) {
-            gmms.alloc(p.D, p.Malloc, fMean_, fVar_);
+            gmms.alloc(p.D, p.Malloc, fMean_, fVar_, p.full);
             gmms.reset();
         }
         if ((int)dim != p.D || gmms.getDim() != dim)
             throw std::logic_error("accumulateStatsMT(): Inconsistent feature & statistic dimensions!");
         if ((fMean_ && !gmms.hasMeans()) || (fVar_ && !gmms.hasVars()))
             throw std::logic_error("accumulateStatsMT(): statistics not allocated for the requested flags");
+        // a full-covariance model produces full second-order statistics
+        // (the diagonal ones would be their diagonal; not offered)
+        if (fVar_ && gmms.hasFullCov() != p.full)
+            throw std::logic_error("accumulateStatsMT(): full/diagonal statistics do not match the model");
         if (NSamples == 0) return;
         upload(vectors, NSamples, dim);
         syncOpts();
         const int flags = (fMean_ ? GMMHIP_ACC_MEAN : 0) | (fVar_ ? GMMHIP_ACC_VAR : 0);
-        std::vector<double> n(p.M, 0.0), F(fMean_ ? (size_t)p.M * p.D : 0), S(fVar_ ? (size_t)p.M * p.D : 0);
+        const size_t vs = gmms.hasVars() ? gmms.getVarSize() : (size_t)p.D;
+        std::vector<double> n(p.M, 0.0), F(fMean_ ? (size_t)p.M * p.D : 0), S(fVar_ ? (size_t)p.M * vs : 0);
         double tot = 0.0;
         uint64_t nf = 0;
         check(gmmhip_accumulate(h_, flags, n.data(), fMean_ ? F.data() : nullptr,
@@ -121,7 +130,7 @@ public:
         for (int m = 0; m < p.M; m++) {
             gn[m] += (TS)n[m];
             for (int d = 0; d < p.D && fMean_; d++) gF[(size_t)m * p.D + d] += (TS)F[(size_t)m * p.D + d];
-            for (int d = 0; d < p.D && fVar_; d++) gS[(size_t)m * p.D + d] += (TS)S[(size_t)m * p.D + d];
+            for (size_t d = 0; d < vs && fVar_; d++) gS[(size_t)m * vs + d] += (TS)S[(size_t)m * vs + d];
         }
         gmms._totalAccSamples += nf;
         gmms._totLogLike += (TS)tot;
@@ -148,14 +157,15 @@ public:
 private:
     struct Packed {
         int D = 0, M = 0, Malloc = 0;
-        std::vector<float> w, mu, var;
+        bool full = false;
+        std::vector<float> w, mu, var;  // var: [M][D] or [M][D][D]
     };
     gmmhip_t h_ = nullptr;
     int dev_ = 0;
     std::string type_ = "GPU HIP";
     std::vector<Packed> group_;
     size_t cur_ = 0;
-    bool flagsSet_ = false, fMixProbOnly_ = false, fMean_ = true, fVar_ = false;
+    bool flagsSet_ = false, fMixProbOnly_ = false, fMean_ = true, fVar_ = false, fFullVar_ = false;
     bool sameData_ = false;
     bool haveFrames_ = false;
 
@@ -166,7 +176,10 @@ private:
     void select(size_t n) {
         const Packed &p = group_[n];
         syncOpts();
-        check(gmmhip_set_model(h_, p.M, p.D, p.w.data(), p.mu.data(), p.var.data()));
+        if (p.full)
+            check(gmmhip_set_model_full(h_, p.M, p.D, p.w.data(), p.mu.data(), p.var.data()));
+        else
+            check(gmmhip_set_model(h_, p.M, p.D, p.w.data(), p.mu.data(), p.var.data()));
         cur_ = n;
     }
     void syncOpts() {

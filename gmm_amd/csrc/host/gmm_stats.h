@@ -1,6 +1,9 @@
 // Sufficient-statistics container, the counterpart of GMMStats<T>
 // (model/OL_GMMStats.h:44-148 of GMMTeam/GMM): per component the occupancy
-// n = sum g, first order F = sum g x and diagonal second order S = sum g x^2,
+// n = sum g, first order F = sum g x and second order S = sum g x^2
+// (diagonal) or the packed upper triangle of sum g x x^T (full: D(D+1)/2
+// per component, row k holding columns k..D-1, OL_GMMStatsEstimator.cpp:
+// 1358-1364),
 // plus the number of accumulated frames and the total log-likelihood.
 // Estimators ADD into it (the caller resets), operator+= is the reduction
 // (OL_GMMStats.cpp:99-142), and save()/load() keep the reference's TXT/BIN
@@ -21,20 +24,23 @@ template <typename T>
 class GMMStats {
 public:
     GMMStats() = default;
-    GMMStats(unsigned int dim, unsigned int nummix, bool allocMeans = true, bool allocVars = false) {
-        alloc(dim, nummix, allocMeans, allocVars);
+    GMMStats(unsigned int dim, unsigned int nummix, bool allocMeans = true, bool allocVars = false,
+             bool allocFullVars = false) {
+        alloc(dim, nummix, allocMeans, allocVars, allocFullVars);
     }
 
     // alloc does not initialise values -> reset()
     void alloc(unsigned int dim, unsigned int nummix, bool allocMeans = true,
-               bool allocVars = false) {
+               bool allocVars = false, bool allocFullVars = false) {
         dim_ = dim;
         M_ = nummix;
         allM_ = allocMeans;
         allV_ = allocVars;
+        allFV_ = allocVars && allocFullVars;
+        varsize_ = allFV_ ? dim * (dim + 1) / 2 : dim;
         n_.assign(nummix, T(0));
         F_.assign(allocMeans ? (size_t)nummix * dim : 0, T(0));
-        S_.assign(allocVars ? (size_t)nummix * dim : 0, T(0));
+        S_.assign(allocVars ? (size_t)nummix * varsize_ : 0, T(0));
         allocated_ = true;
     }
     void reset() {
@@ -47,14 +53,15 @@ public:
     bool isAllocated() const { return allocated_; }
     bool hasMeans() const { return allM_; }
     bool hasVars() const { return allV_; }
-    bool hasFullCov() const { return false; }
+    bool hasFullCov() const { return allFV_; }
+    unsigned int getVarSize() const { return varsize_; }
     unsigned int getMixNum() const { return M_; }
     unsigned int getDim() const { return dim_; }
     uint64_t getTotAccSamples() const { return _totalAccSamples; }
     T getTotLogLike() const { return _totLogLike; }
     T getProb(unsigned int m) const { return n_[m]; }
     const T *getMean(unsigned int m) const { return allM_ ? &F_[(size_t)m * dim_] : nullptr; }
-    const T *getVar(unsigned int m) const { return allV_ ? &S_[(size_t)m * dim_] : nullptr; }
+    const T *getVar(unsigned int m) const { return allV_ ? &S_[(size_t)m * varsize_] : nullptr; }
 
     T *probData() { return n_.data(); }
     T *meanData() { return allM_ ? F_.data() : nullptr; }
@@ -66,7 +73,8 @@ public:
             *this = r;
             return *this;
         }
-        if (r.M_ != M_ || r.dim_ != dim_) throw std::logic_error("GMMStats::operator+=: shape mismatch");
+        if (r.M_ != M_ || r.dim_ != dim_ || (allV_ && r.allV_ && r.varsize_ != varsize_))
+            throw std::logic_error("GMMStats::operator+=: shape mismatch");
         for (size_t i = 0; i < n_.size(); i++) n_[i] += r.n_[i];
         if (allM_ && r.allM_)
             for (size_t i = 0; i < F_.size(); i++) F_[i] += r.F_[i];
@@ -86,8 +94,8 @@ public:
         if (!f) throw std::runtime_error(std::string("GMMStats::save: cannot open ") + filename);
         if (txt) {
             f.precision(17);
-            f << (int)saveMProbs << " " << (int)sM << " " << (int)sV << " 0 " << dim_ << " "
-              << dim_ << " " << M_ << "\n\n";
+            f << (int)saveMProbs << " " << (int)sM << " " << (int)sV << " " << (int)(sV && allFV_)
+              << " " << dim_ << " " << varsize_ << " " << M_ << "\n\n";
             for (unsigned m = 0; m < M_; m++) {
                 if (saveMProbs) f << n_[m] << "\n";
                 if (sM) {
@@ -95,19 +103,21 @@ public:
                     f << "\n";
                 }
                 if (sV) {
-                    for (unsigned d = 0; d < dim_; d++) f << " " << S_[(size_t)m * dim_ + d];
+                    for (unsigned d = 0; d < varsize_; d++) f << " " << S_[(size_t)m * varsize_ + d];
                     f << "\n";
                 }
                 f << "\n";
             }
         } else {
-            const int32_t hdr[7] = {(int32_t)saveMProbs, (int32_t)sM, (int32_t)sV, 0,
-                                    (int32_t)dim_, (int32_t)dim_, (int32_t)M_};
+            const int32_t hdr[7] = {(int32_t)saveMProbs, (int32_t)sM, (int32_t)sV,
+                                    (int32_t)(sV && allFV_), (int32_t)dim_, (int32_t)varsize_,
+                                    (int32_t)M_};
             f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
             for (unsigned m = 0; m < M_; m++) {
                 if (saveMProbs) f.write(reinterpret_cast<const char *>(&n_[m]), sizeof(T));
                 if (sM) f.write(reinterpret_cast<const char *>(&F_[(size_t)m * dim_]), sizeof(T) * dim_);
-                if (sV) f.write(reinterpret_cast<const char *>(&S_[(size_t)m * dim_]), sizeof(T) * dim_);
+                if (sV)
+                    f.write(reinterpret_cast<const char *>(&S_[(size_t)m * varsize_]), sizeof(T) * varsize_);
             }
         }
         if (!f) throw std::runtime_error(std::string("GMMStats::save: write failed ") + filename);
@@ -122,8 +132,10 @@ public:
         } else {
             f.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
         }
-        if (!f || hdr[3] != 0 || hdr[4] != hdr[5]) throw std::runtime_error("GMMStats::load: bad header");
-        alloc(hdr[4], hdr[6], hdr[1] != 0, hdr[2] != 0);
+        const bool fv = hdr[3] != 0;
+        if (!f || hdr[4] <= 0 || hdr[5] != (fv ? hdr[4] * (hdr[4] + 1) / 2 : hdr[4]))
+            throw std::runtime_error("GMMStats::load: bad header");
+        alloc(hdr[4], hdr[6], hdr[1] != 0, hdr[2] != 0, fv);
         reset();
         for (unsigned m = 0; m < M_; m++) {
             if (txt) {
@@ -131,11 +143,12 @@ public:
                 if (allM_)
                     for (unsigned d = 0; d < dim_; d++) f >> F_[(size_t)m * dim_ + d];
                 if (allV_)
-                    for (unsigned d = 0; d < dim_; d++) f >> S_[(size_t)m * dim_ + d];
+                    for (unsigned d = 0; d < varsize_; d++) f >> S_[(size_t)m * varsize_ + d];
             } else {
                 if (hdr[0]) f.read(reinterpret_cast<char *>(&n_[m]), sizeof(T));
                 if (allM_) f.read(reinterpret_cast<char *>(&F_[(size_t)m * dim_]), sizeof(T) * dim_);
-                if (allV_) f.read(reinterpret_cast<char *>(&S_[(size_t)m * dim_]), sizeof(T) * dim_);
+                if (allV_)
+                    f.read(reinterpret_cast<char *>(&S_[(size_t)m * varsize_]), sizeof(T) * varsize_);
             }
         }
         if (!f) throw std::runtime_error(std::string("GMMStats::load: truncated ") + filename);
@@ -145,8 +158,8 @@ public:
     T _totLogLike = T(0);
 
 private:
-    bool allocated_ = false, allM_ = false, allV_ = false;
-    unsigned int dim_ = 0, M_ = 0;
+    bool allocated_ = false, allM_ = false, allV_ = false, allFV_ = false;
+    unsigned int dim_ = 0, M_ = 0, varsize_ = 0;
     std::vector<T> n_, F_, S_;
 };
 

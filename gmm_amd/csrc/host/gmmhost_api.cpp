@@ -20,12 +20,12 @@ int fail(const std::string &msg, int code = 1) {
 
 // Statistics wrapper for the update rules (GMMStats<double> view).
 gmm::GMMStats<double> make_stats(int M, int D, const double *n, const double *F, const double *S,
-                                 uint64_t tot) {
-    gmm::GMMStats<double> st(D, M, F != nullptr, S != nullptr);
+                                 uint64_t tot, bool full = false) {
+    gmm::GMMStats<double> st(D, M, F != nullptr, S != nullptr, full);
     st.reset();
     std::memcpy(st.probData(), n, sizeof(double) * M);
     if (F) std::memcpy(st.meanData(), F, sizeof(double) * M * D);
-    if (S) std::memcpy(st.varData(), S, sizeof(double) * M * D);
+    if (S) std::memcpy(st.varData(), S, sizeof(double) * M * st.getVarSize());
     st._totalAccSamples = tot;
     return st;
 }
@@ -66,6 +66,23 @@ int gmmhost_model_create(int M, int D, const float *w, const float *mu, const fl
     return 0;
 }
 
+int gmmhost_model_create_full(int M, int D, const float *w, const float *mu, const float *cov,
+                              gmmhost_model_t *out) {
+    if (M <= 0 || D <= 0 || !w || !mu || !cov || !out) return fail("bad arguments");
+    gmmhost_model *h = new gmmhost_model();
+    h->m = gmm::GMModel(D, M, true);
+    for (int k = 0; k < M; k++)
+        h->m.SetMixParam(k, mu + (size_t)k * D, cov + (size_t)k * D * D, w[k]);
+    *out = h;
+    return 0;
+}
+
+int gmmhost_model_full(gmmhost_model_t m, int *full) {
+    if (!m || !full) return fail("null argument");
+    *full = m->m.GetFullCovStatus() ? 1 : 0;
+    return 0;
+}
+
 int gmmhost_model_dims(gmmhost_model_t m, int *M, int *D) {
     if (!m || !M || !D) return fail("null argument");
     *M = m->m.GetNumberOfMixtures();
@@ -96,7 +113,7 @@ int gmmhost_update_em(gmmhost_model_t m, const double *n, const double *F, const
     if (!m || !n) return fail("null argument");
     try {
         const int M = m->m.GetNumberOfMixtures(), D = m->m.GetDimension();
-        gmm::GMMStats<double> st = make_stats(M, D, n, F, S, tot_samples);
+        gmm::GMMStats<double> st = make_stats(M, D, n, F, S, tot_samples, m->m.GetFullCovStatus());
         RuleHost ad;
         gmm::CModelAdapt::OPTIONS o;
         o._min_var = min_var;

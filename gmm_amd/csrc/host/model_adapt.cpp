@@ -50,7 +50,9 @@ void CModelAdapt::AccumulateStats(GMModel *model, GMMStats<double> &stats, bool 
         std::cout << "[#frames accumulated = " << stats.getTotAccSamples() << "]" << std::endl;
 }
 
-// UpdateModelEM (OL_ModelAdapt.cpp:823-949), diagonal, non-robust.
+// UpdateModelEM (OL_ModelAdapt.cpp:823-949), non-robust; the full-covariance
+// branch (:906-925) sets cov[k][kk] = cov[kk][k] = (float)(S/n - mean_k *
+// mean_kk) with the product of the float means taken in float and no floor.
 unsigned int CModelAdapt::UpdateModelEM(GMModel *model, GMMStats<double> &stats, bool upweight,
                                         bool upmean, bool upvar) {
     const int D = model->GetDimension();
@@ -83,7 +85,19 @@ unsigned int CModelAdapt::UpdateModelEM(GMModel *model, GMMStats<double> &stats,
             if (!F || (upvar && !S))
                 throw std::logic_error("UpdateModelEM(): statistics not allocated");
             for (int k = 0; k < D; k++) mean[k] = (float)(F[k] / np);
-            if (upvar) {
+            if (upvar && model->GetFullCovStatus()) {
+                if (!stats.hasFullCov())
+                    throw std::logic_error("UpdateModelEM(): full-covariance model needs full statistics");
+                float *cov = model->GetMixtureVar(j);
+                size_t p = 0;
+                for (int k = 0; k < D; k++)
+                    for (int kk = k; kk < D; kk++, p++) {
+                        const float mm = mean[k] * mean[kk];
+                        const float v = (float)(S[p] / np - (double)mm);
+                        cov[(size_t)k * D + kk] = v;
+                        cov[(size_t)kk * D + k] = v;
+                    }
+            } else if (upvar) {
                 float *var = model->GetMixtureVar(j);
                 for (int k = 0; k < D; k++) {
                     float v = (float)(S[k] / np - std::pow((double)mean[k], 2));
@@ -368,6 +382,9 @@ MLLRTransform CModelAdapt::IterW_MLLR(GMModel *UBM, int iters, bool wantQ) {
 void CModelAdapt::AdaptModel(GMModel *UBM) {
     if (!UBM) throw std::logic_error("AdaptModel(): UBM not given!");
     if (!data_) throw std::logic_error("AdaptModel(): Insert adaptation data first!");
+    if (UBM->GetFullCovStatus())
+        throw std::runtime_error("AdaptModel(): MAP/MLLR adaptation of full-covariance models is not "
+                                 "supported by this build");
     UBM_ = UBM;
     nummix_ = UBM->GetNumberOfMixtures();
     const int D = UBM->GetDimension();
@@ -397,34 +414,89 @@ void CModelAdapt::AdaptModel(GMModel *UBM) {
     }
 }
 
+// Leading eigenpair of a symmetric matrix (getEigValVecs' ssyevr('V', 'I',
+// 'U', il = iu = D), OL_ModelAdapt.cpp:72-160) by cyclic Jacobi rotations
+// in double.  LAPACK leaves the eigenvector's sign open; it is fixed so that
+// the entry of largest magnitude (first on a tie) is negative, the choice
+// under which the reference's example/myModelFC.gmm is reproduced.
+static void leading_eigenpair(int D, const float *cov, double &lam, std::vector<double> &v) {
+    std::vector<double> A((size_t)D * D), V((size_t)D * D, 0.0);
+    for (int i = 0; i < D * D; i++) A[i] = cov[i];
+    for (int i = 0; i < D; i++) V[(size_t)i * D + i] = 1.0;
+    auto a = [&](int i, int j) -> double & { return A[(size_t)i * D + j]; };
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0;
+        for (int p = 0; p < D; p++)
+            for (int q = p + 1; q < D; q++) off += a(p, q) * a(p, q);
+        if (off < 1e-30) break;
+        for (int p = 0; p < D; p++)
+            for (int q = p + 1; q < D; q++) {
+                if (std::fabs(a(p, q)) < 1e-300) continue;
+                const double theta = (a(q, q) - a(p, p)) / (2.0 * a(p, q));
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < D; k++) {
+                    const double kp = a(k, p), kq = a(k, q);
+                    a(k, p) = c * kp - s * kq;
+                    a(k, q) = s * kp + c * kq;
+                }
+                for (int k = 0; k < D; k++) {
+                    const double pk = a(p, k), qk = a(q, k);
+                    a(p, k) = c * pk - s * qk;
+                    a(q, k) = s * pk + c * qk;
+                }
+                for (int k = 0; k < D; k++) {
+                    double &vp = V[(size_t)k * D + p], &vq = V[(size_t)k * D + q];
+                    const double tp = vp, tq = vq;
+                    vp = c * tp - s * tq;
+                    vq = s * tp + c * tq;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < D; i++)
+        if (a(i, i) > a(best, best)) best = i;
+    lam = a(best, best);
+    int imax = 0;
+    for (int i = 1; i < D; i++)
+        if (std::fabs(V[(size_t)i * D + best]) > std::fabs(V[(size_t)imax * D + best])) imax = i;
+    const double sg = V[(size_t)imax * D + best] > 0.0 ? -1.0 : 1.0;
+    v.resize(D);
+    for (int i = 0; i < D; i++) v[i] = sg * V[(size_t)i * D + best];
+}
+
 // TrainFromScratch (OL_ModelAdapt.cpp:673-821) with splitGaussian
-// (:614-660, diagonal: shift +-0.5*sqrt(var)) and splitMultipleGaussians with
-// split-X = 0 (one split per round: heaviest component, ties to the higher
-// index, into the highest-index zero-weight slot, GetGaussStats :965-991).
-void CModelAdapt::TrainFromScratch(unsigned int nummix) {
+// (:614-660; diagonal: shift +-0.5*sqrt(var); full covariance: shift
+// +-sqrt(lambda) v along the leading eigenpair) and splitMultipleGaussians
+// with split-X = 0 (one split per round: heaviest component, ties to the
+// higher index, into the highest-index zero-weight slot, GetGaussStats
+// :965-991).
+void CModelAdapt::TrainFromScratch(unsigned int nummix, bool fullcov) {
     if (!data_ || N_ <= 3) throw std::runtime_error("TrainFromScratch(): Not enough data to model training.");
     const int D = (int)D_;
     // Global statistics: a single component with a broad variance has
     // posterior 1 for every finite frame.  The first pass finds the mean; the
     // second is centred on it, so E[x^2] - mu^2 does not cancel.
-    GMModel g(D, 1);
-    std::vector<float> z(D, 0.0f), v(D, 1e4f);
+    GMModel g(D, 1, fullcov);
+    std::vector<float> z(D, 0.0f), v((size_t)g.GetDimVar(), fullcov ? 0.0f : 1e4f);
+    if (fullcov)
+        for (int d = 0; d < D; d++) v[(size_t)d * D + d] = 1e4f;
     g.SetMixParam(0, z.data(), v.data(), 1.0f);
-    GMMStats<double> gs(D, 1, true, true);
+    GMMStats<double> gs(D, 1, true, true, fullcov);
     gs.reset();
     AccumulateStats(&g, gs, false, true, false);
     for (int d = 0; d < D; d++) z[d] = (float)(gs.getMean(0)[d] / gs.getProb(0));
     g.SetMixParam(0, z.data(), v.data(), 1.0f);
     gs.reset();
     AccumulateStats(&g, gs, false, true, true);
-    GMModel init(D, 1);
+    GMModel init(D, 1, fullcov);
     init.SetMixWeight(0, 1.0f);
     if (UpdateModelEM(&init, gs) == 0)
         throw std::runtime_error("trainFromScratch(): amount of data too low to estimate a model");
-    model_.reset(new GMModel(D, (int)nummix));
+    model_.reset(new GMModel(D, (int)nummix, fullcov));
     model_->SetMixParam(0, init.GetMixtureMean(0), init.GetMixtureVar(0), init.GetMixtureWeight(0));
     nummix_ = (int)nummix;
-    stats_.alloc(D, nummix, true, true);
+    stats_.alloc(D, nummix, true, true, fullcov);
     const unsigned int maxIter = nummix < 2 ? 0 : nummix;
     unsigned int nonzero = 1;
     GMModel *model = model_.get();
@@ -454,16 +526,27 @@ void CModelAdapt::TrainFromScratch(unsigned int nummix) {
             for (int m = (int)nummix - 1; m >= 0 && znew < 0; m--)
                 if (model->GetMixtureWeight(m) == 0) znew = m;
             if (znew < 0) break;
-            std::memcpy(model->GetMixtureVar(znew), model->GetMixtureVar(imax), sizeof(float) * D);
+            std::memcpy(model->GetMixtureVar(znew), model->GetMixtureVar(imax),
+                        sizeof(float) * model->GetDimVar());
             const float half = model->GetMixtureWeight(imax) / 2.0f;
             model->SetMixWeight(znew, half);
             model->SetMixWeight(imax, half);
             float *mi = model->GetMixtureMean(imax), *mn = model->GetMixtureMean(znew);
             const float *vi = model->GetMixtureVar(imax);
+            std::vector<float> sh(D);
+            if (fullcov) {
+                double lam = 0.0;
+                std::vector<double> ev;
+                leading_eigenpair(D, vi, lam, ev);
+                if (!(lam >= 0.0))
+                    throw std::runtime_error("splitGaussian(): covariance matrix not positive definite!");
+                for (int d = 0; d < D; d++) sh[d] = (float)(std::sqrt(lam) * ev[d]);
+            } else {
+                for (int d = 0; d < D; d++) sh[d] = (float)(std::sqrt(vi[d]) * 0.5);
+            }
             for (int d = 0; d < D; d++) {
-                const float sh = (float)(std::sqrt(vi[d]) * 0.5);
-                mn[d] = mi[d] + sh;
-                mi[d] -= sh;
+                mn[d] = mi[d] + sh[d];
+                mi[d] -= sh[d];
             }
             nonzero += 1;
         }

@@ -1,8 +1,9 @@
 // Trainer / adaptation driver over the HIP estimator: the subset of
 // CModelAdapt (trainer/OL_ModelAdapt.h:140-180 of GMMTeam/GMM) on the
-// diagonal-GMM hot path -- EM from a UBM or from scratch (Gaussian
-// splitting), MAP adaptation, global MLLR (+ MLLR then MAP) -- with the
-// reference's update rules.  All statistics come from GMMStatsEstimator_HIP.
+// GMM hot path -- EM from a UBM or from scratch (Gaussian splitting; diagonal
+// or full covariance), MAP adaptation, global MLLR (+ MLLR then MAP) of
+// diagonal models -- with the reference's update rules.  All statistics come
+// from GMMStatsEstimator_HIP.
 #pragma once
 
 #include <memory>
@@ -59,8 +60,9 @@ public:
     // MAP / MLLR / MLLR_MAP adaptation of UBM (AdaptModel, :1163-1247).
     void AdaptModel(GMModel *UBM);
     // EM from scratch with binary splitting up to nummix components
-    // (TrainFromScratch, :673-821, diagonal branch, no log-lik threshold).
-    void TrainFromScratch(unsigned int nummix);
+    // (TrainFromScratch, :673-821, no log-lik threshold), diagonal or full
+    // covariance.
+    void TrainFromScratch(unsigned int nummix, bool fullcov = false);
 
     GMModel *GetClientModel() { return model_.get(); }
     const MLLRTransform *GetTransform() const { return W_.get(); }

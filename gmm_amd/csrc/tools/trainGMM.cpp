@@ -1,7 +1,8 @@
 // trainGMM -- training and adaptation tool over the gfx950 accumulator.
 // Option names follow the reference tool (projects/Model_ADAPT/Model_ADAPT.cpp
 // :278-401, trainer/OL_ModelAdapt.cpp:2032-2135); the hot-path subset is
-// supported: --type 0 (from scratch), 1 (MLLR), 2 (MAP/EM), 4 (MLLR+MAP).
+// supported: --type 0 (from scratch; --full-cov for full covariances),
+// 1 (MLLR), 2 (MAP/EM), 4 (MLLR+MAP).
 // --SSE/--CUDA are accepted and ignored: statistics always run on the GPU.
 #include <cstdio>
 #include <iostream>
@@ -50,12 +51,13 @@ int main(int argc, char **argv) {
     a.option("NB-GPU", 0, false, "8");
     a.option("numThrd", 0, false, "1");
     a.option("precision", 0, false, "fp32");
+    a.option("full-cov", 0, true, "0");
     a.option("verbosity", 'v', true, "0");
     try {
         a.parse(argc, argv);
         if (a.b("help") || a.all("in-file").empty() || !a.has("outFile")) {
             std::cout << "usage: trainGMM -i FEATS [-i ...] --data-T {0,1,2} -o OUT.gmm [-T] "
-                         "--type {0,1,2,4} [--in-UBM UBM.gmm [-L]] [--nummix K] [--it-EM n] "
+                         "--type {0,1,2,4} [--in-UBM UBM.gmm [-L]] [--nummix K [--full-cov]] [--it-EM n] "
                          "[--it-MAP n] [--tau r] [--mean] [--var] [--weight] [--mllr-init] "
                          "[--it-mllr n] [--apply-trn] [--save-trn] [--precision fp32|fp16x3|bf16x3]\n";
             return a.b("help") ? 0 : 1;
@@ -97,7 +99,7 @@ int main(int argc, char **argv) {
         ad.InsertData(feats.rows.data(), feats.N, feats.D);
         if (o._adaptType == FROM_SCRATCH) {
             if (a.i("nummix") < 1) throw std::runtime_error("--type 0 needs --nummix");
-            ad.TrainFromScratch((unsigned)a.i("nummix"));
+            ad.TrainFromScratch((unsigned)a.i("nummix"), a.b("full-cov"));
         } else {
             if (!a.has("in-UBM")) throw std::runtime_error("--in-UBM is required for adaptation");
             GMModel ubm;

@@ -8,6 +8,7 @@ from . import lib
 
 HOST_EXPORTS = [
     "gmmhost_model_load", "gmmhost_model_save", "gmmhost_model_create",
+    "gmmhost_model_create_full", "gmmhost_model_full",
     "gmmhost_model_dims", "gmmhost_model_get", "gmmhost_model_rearrange",
     "gmmhost_model_free", "gmmhost_update_em", "gmmhost_update_map",
     "gmmhost_mllr_transform", "gmmhost_mllr_apply", "gmmhost_last_error",
@@ -28,6 +29,9 @@ def _L():
         L.gmmhost_model_save.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
         L.gmmhost_model_create.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _fp, _fp,
                                            ctypes.POINTER(vp)]
+        L.gmmhost_model_create_full.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _fp, _fp,
+                                                ctypes.POINTER(vp)]
+        L.gmmhost_model_full.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.gmmhost_model_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int)]
         L.gmmhost_model_get.argtypes = [vp, _fp, _fp, _fp]
@@ -68,7 +72,7 @@ def _pd(a):
 
 
 class Model:
-    """GMModel handle (diagonal)."""
+    """GMModel handle (diagonal or full covariance)."""
 
     def __init__(self, handle):
         self._h = handle
@@ -88,6 +92,20 @@ class Model:
                                        ctypes.byref(h)))
         return cls(h)
 
+    @classmethod
+    def create_full(cls, w, mu, cov):
+        w, mu, cov = _f(w), _f(mu), _f(cov)
+        h = ctypes.c_void_p()
+        _chk(_L().gmmhost_model_create_full(mu.shape[0], mu.shape[1], w.ctypes.data_as(_fp),
+                                            mu.ctypes.data_as(_fp), cov.ctypes.data_as(_fp),
+                                            ctypes.byref(h)))
+        return cls(h)
+
+    def full(self):
+        f = ctypes.c_int()
+        _chk(_L().gmmhost_model_full(self._h, ctypes.byref(f)))
+        return bool(f.value)
+
     def __del__(self):
         if getattr(self, "_h", None):
             _L().gmmhost_model_free(self._h)
@@ -102,10 +120,11 @@ class Model:
         return M.value, D.value
 
     def get(self):
+        """(w[M], mu[M,D], var[M,D]) or, for a full model, (w, mu, cov[M,D,D])."""
         M, D = self.dims()
         w = np.zeros(M, np.float32)
         mu = np.zeros((M, D), np.float32)
-        var = np.zeros((M, D), np.float32)
+        var = np.zeros((M, D, D) if self.full() else (M, D), np.float32)
         _chk(_L().gmmhost_model_get(self._h, w.ctypes.data_as(_fp), mu.ctypes.data_as(_fp),
                                     var.ctypes.data_as(_fp)))
         return w, mu, var

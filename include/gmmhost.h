@@ -26,7 +26,14 @@ int gmmhost_model_save(gmmhost_model_t m, const char *path, int txt);
  * SetMixParam, :120-156). */
 int gmmhost_model_create(int M, int D, const float *w, const float *mu, const float *var,
                          gmmhost_model_t *out);
+/* A full-covariance model from w[M], mu[M*D], cov[M*D*D] (GMModel(dim,
+ * nummix, full_cov = true) + SetMixParam). */
+int gmmhost_model_create_full(int M, int D, const float *w, const float *mu, const float *cov,
+                              gmmhost_model_t *out);
 int gmmhost_model_dims(gmmhost_model_t m, int *M, int *D);
+/* *full = 1 for a full-covariance model (GetFullCovStatus, OL_GMModel.h:100). */
+int gmmhost_model_full(gmmhost_model_t m, int *full);
+/* var receives M*D variances, or M*D*D for a full-covariance model. */
 int gmmhost_model_get(gmmhost_model_t m, float *w, float *mu, float *var);
 /* RearrangeMixtures (:857-885): zero-weight components to the end,
  * reversed; *nzero receives their count. */
@@ -34,9 +41,10 @@ int gmmhost_model_rearrange(gmmhost_model_t m, int *nzero);
 void gmmhost_model_free(gmmhost_model_t m);
 
 /* UpdateModelEM (trainer/OL_ModelAdapt.cpp:823-949) on model m with
- * statistics n[M], F[M*D], S[M*D] over tot_samples frames; returns the
- * reference's count of components with non-zero weight on entry through
- * *left. */
+ * statistics n[M], F[M*D], S[M*D] over tot_samples frames (a full-covariance
+ * model takes S[M*D(D+1)/2], the packed upper triangle of sum g x x^T);
+ * returns the reference's count of components with non-zero weight on entry
+ * through *left. */
 int gmmhost_update_em(gmmhost_model_t m, const double *n, const double *F, const double *S,
                       uint64_t tot_samples, float min_var, int upweight, int upmean, int upvar,
                       int *left);

@@ -67,6 +67,38 @@ def test_train_from_scratch_reproduces_example(tmp_path, golden):
     assert (np.abs(var - golden["var"]) / golden["var"]).max() < 5e-5
 
 
+def test_train_from_scratch_full_cov(tmp_path, golden):
+    """trainModel.bat:6 -- --full-cov: the same run with full covariances
+    gives the components of myModelFC.gmm in the same order, to the level at
+    which the FP64 oracle's own split driver reproduces that file, and agrees
+    more closely with that driver."""
+    import _scratch_driver as sd
+    out = tmp_path / "myModelFC.gmm"
+    run("trainGMM", "-i", os.path.join(GOLD, "data.bin"), "--data-T", 2, "--type", 0,
+        "--nummix", 4, "--it-EM", 8, "-o", out, "-T", "--full-cov")
+    w, mu, cov, _ = fm.read_gmm_txt_full(out)
+    rw, rmu, rcov, _ = fm.read_gmm_txt_full(os.path.join(GOLD, "myModelFC.gmm"))
+    assert np.abs(w - rw).max() < 3e-4
+    assert np.abs(mu - rmu).max() < 6e-3
+    assert np.abs(cov - rcov).max() < 4e-3
+    ow, omu, ocov = sd.train_from_scratch_full(golden["x"], 4, 8)
+    assert np.abs(w - ow).max() < 2e-5
+    assert np.abs(mu - omu).max() < 2e-4
+    assert np.abs(cov - ocov).max() < 2e-4
+
+
+def test_getloglike_full_cov_model(golden):
+    """getLogLike.bat:8 -- mean log-likelihood of data.bin under
+    myModelFC.gmm, against the oracle."""
+    path = os.path.join(GOLD, "myModelFC.gmm")
+    outp = run("getLogLike", "-i", os.path.join(GOLD, "data.bin"), "--data-T", 2, "--in-GMM",
+               path, "-L")
+    name, val = outp.strip().split("\t")
+    w, mu, cov, _ = fm.read_gmm_txt_full(path)
+    ref = orc.frame_loglik_full(golden["x"], w, mu, cov).mean()
+    assert float(val) == pytest.approx(ref, rel=1e-5)
+
+
 def _ubm_case(tmp_path, seed=41, M=64, D=12, N=20000):
     rng = np.random.default_rng(seed)
     w = rng.dirichlet(np.ones(M))

@@ -142,3 +142,48 @@ def test_mllr_underdetermined_rejected():
 def test_missing_file_reports_error(tmp_path):
     with pytest.raises(host.HostError):
         host.Model.load(tmp_path / "nope.gmm", True)
+
+
+# ---------------------------------------------------------------- full covariance
+def test_read_reference_fullcov_model(tmp_path):
+    """example/myModelFC.gmm through the library's reader equals the
+    independent parser; TXT and BIN (negative dimension) round trips."""
+    path = os.path.join(GOLD, "myModelFC.gmm")
+    m = host.Model.load(path, True)
+    assert m.full()
+    w, mu, cov = m.get()
+    rw, rmu, rcov, _ = fm.read_gmm_txt_full(path)
+    assert np.array_equal(w, rw) and np.array_equal(mu, rmu) and np.array_equal(cov, rcov)
+    for txt in (True, False):
+        p = tmp_path / ("fc.gmm" if txt else "fc.bin")
+        m.save(p, txt)
+        r = host.Model.load(p, txt)
+        assert r.full()
+        for u, v in zip((w, mu, cov), r.get()):
+            if txt:
+                assert np.allclose(u, v, rtol=6e-7, atol=0)
+            else:
+                assert np.array_equal(u, v)
+    with open(tmp_path / "fc.bin", "rb") as f:
+        hdr = f.read(11)
+    assert hdr[:9] == b"GMM-DB-VN" and np.frombuffer(hdr[9:11], "<i2")[0] == -20
+    w2, mu2, cov2, _ = fm.read_gmm_txt_full(tmp_path / "fc.gmm")
+    assert np.allclose(cov2, rcov, rtol=6e-7, atol=0)
+
+
+def test_update_em_full_matches_oracle():
+    rng = np.random.default_rng(17)
+    M, D, N = 3, 4, 400
+    w = np.array([0.3, 0.3, 0.4], np.float32)
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    A = rng.normal(size=(M, D, D))
+    cov = (A @ A.transpose(0, 2, 1) / D + np.eye(D)).astype(np.float32)
+    x = rng.normal(0, 2, (N, D)).astype(np.float32)
+    n, F, S, _ = orc.accumulate_full(x, w, mu, cov)
+    m = host.Model.create_full(w, mu, cov)
+    left = m.update_em(n, F, S, N)
+    gw, gmu, gcov = m.get()
+    ow, omu, ocov, oleft = orc.update_em_full(w, mu, cov, n, F, S, N)
+    assert left == oleft
+    assert np.array_equal(gw, ow) and np.array_equal(gmu, omu)
+    assert np.array_equal(gcov, ocov)
