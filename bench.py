@@ -13,7 +13,8 @@ frames (`fp32_compare`), with each form's error against an FP64 reference
 of the same E-step over all frames (`accuracy_vs_fp64`).
 
 Other workloads: --workload ubm_plain (D=60, M=512, N=1e7), plain_em_small,
-batch_map (batched MAP + MLLR of 4,096 utterances), serve (serving loop).
+batch_map (batched MAP + MLLR of 4,096 utterances), serve (serving loop),
+fullcov_estep (full-covariance E-step, D=60, M=512, N=1e6).
 
 Prints ONE JSON line (rank 0).  Frames are generated directly in HBM, so the
 timed region starts with inputs resident.
@@ -199,6 +200,93 @@ def bench_copy(args):
           flush=True)
 
 
+def bench_fullcov(args):
+    """Full-covariance E-step (the reference's gammasKernelFull +
+    accStatsFullKernel path): D=60, M=512, N=1,000,000 frames drawn from a
+    seeded 512-component full-covariance mixture; the model is initialised
+    from 512 random frames with the global covariance (the first EM
+    iteration).  One step = gmmhip_accumulate over the resident frames:
+    feature expansion, the two FP32 GEMMs (log-likelihoods, gamma^T Phi),
+    the fused log-sum-exp/posterior pass and the FP64 chunk reduction.
+    Algorithmic work 4K FLOP per frame x component with K = 1 + D + D(D+1)/2
+    features."""
+    import torch
+    import gmm_amd
+    D, M = 60, 512
+    N = args.frames or 1_000_000
+    seed = 20261208
+    rng = np.random.default_rng(seed)
+    wt = rng.dirichlet(np.full(M, 0.5))
+    mut = rng.normal(0, 2.0, (M, D))
+    A = rng.normal(size=(M, D, D)) / np.sqrt(D)
+    covt = A @ A.transpose(0, 2, 1) + np.diag(rng.uniform(0.3, 1.5, D))
+    Lt = np.linalg.cholesky(covt)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    x = torch.empty((N, D), dtype=torch.float32, device="cuda")
+    Lg = torch.as_tensor(Lt, dtype=torch.float32, device="cuda")
+    mug = torch.as_tensor(mut, dtype=torch.float32, device="cuda")
+    wg = torch.as_tensor(wt, dtype=torch.float64, device="cuda")
+    for a in range(0, N, 250_000):
+        b = min(N, a + 250_000)
+        k = torch.multinomial(wg, b - a, replacement=True, generator=g)
+        z = torch.randn((b - a, D, 1), generator=g, device="cuda")
+        x[a:b] = mug[k] + torch.bmm(Lg[k].contiguous(), z)[:, :, 0]
+    torch.cuda.synchronize()
+    idx = np.sort(rng.choice(N, size=M, replace=False))
+    mu = x[torch.as_tensor(idx, device="cuda")].cpu().numpy()
+    xd = x.double()
+    cg = torch.cov(xd.t()).cpu().numpy().astype(np.float32)
+    cov = np.tile(0.5 * (cg + cg.T), (M, 1, 1)).astype(np.float32)
+    w = np.full(M, 1.0 / M, np.float32)
+    del xd
+    acc = gmm_amd.Accumulator(0)
+    acc.set_model_full(w, mu, cov)
+    acc.set_frames_device(x.data_ptr(), N, D)
+    for _ in range(args.warmup):
+        acc.accumulate()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = acc.accumulate()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    K = 1 + D + D * (D + 1) // 2
+    flop = 4.0 * K * N * M
+    step = el / args.steps
+    line = {
+        "metric": "full-covariance E-step frame*comp/s (fp32)",
+        "value": N * M * args.steps / el, "unit": "frame*comp/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": step * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded full-covariance mixture, frames generated in HBM)",
+        "config": {"workload": "fullcov_estep", "D": D, "M": M, "N": N, "K": K,
+                   "tot_loglike_per_frame": res[3] / N},
+        "roofline": {"bound": "mfma", "achieved": flop / step / 1e12, "peak": PEAK["fp32"] / 1e12,
+                     "unit": "TFLOP/s", "frac": flop / step / PEAK["fp32"], "traffic": None,
+                     "kernel": "whole E-step (features + two FP32 GEMMs + log-sum-exp + FP64 "
+                               "reduction)", "avg_launch_ms": step * 1e3},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import binding as orc
+        xs = x[:64].cpu().numpy()
+        t0 = time.perf_counter()
+        orc.accumulate_full(xs, w, mu, cov)
+        dt = time.perf_counter() - t0
+        n = int(min(N, max(64, 64 * 10.0 / max(dt, 1e-3))))
+        xs = x[:n].cpu().numpy()
+        t0 = time.perf_counter()
+        orc.accumulate_full(xs, w, mu, cov)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {
+            "value": n * M / dt, "unit": "frame*comp/s", "cores": 1, "kind": "port",
+            "sample": "first %d frames of the workload, %.1f s, the reference's full-covariance "
+                      "CPU E-step restated in FP64 (oracle/gmm_oracle.c, orc_accumulate_full), "
+                      "1 thread" % (n, dt)}
+    acc.close()
+    print(json.dumps(line), flush=True)
+
+
 def bench_serve(args):
     """Serving loop: UBM D=60, M=2048 resident in HBM; open-loop Poisson
     arrivals (60% score / 30% MAP / 10% MLLR, lengths lognormal(median 300,
@@ -263,6 +351,8 @@ def main():
         return bench_serve(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "fullcov_estep":
+        return bench_fullcov(args)
 
     import torch
     import gmm_amd

@@ -14,8 +14,11 @@
 // so a frame chunk runs: feature expansion (HBM-bound), L = Phi Theta^T and
 // the statistics Gamma^T Phi as plain FP32 GEMMs (rocBLAS, deterministic:
 // atomics off), and between them one fused pass per frame computing the
-// log-sum-exp, the clamp rules and the cut posteriors in place.  Chunk
-// statistics are added into FP64 accumulators in a fixed order.
+// log-sum-exp, the clamp rules and the cut posteriors in place.  The
+// statistics GEMM has a small output (K x M) and a long contraction (the
+// chunk's frames), so it runs as S independent GEMMs over frame sub-chunks
+// (one strided-batched call) to fill the chip; the S partials and the chunks
+// are summed into FP64 accumulators in a fixed order.
 #include "ctx.h"
 
 #include <rocblas/rocblas.h>
@@ -37,7 +40,7 @@ struct FullState {
     size_t L_cap = 0;
     uint8_t *d_bad = nullptr;    // [nc]
     size_t bad_cap = 0;
-    float *d_part = nullptr;     // [M][KP] FP32 statistics of one chunk
+    float *d_part = nullptr;     // [S][M][KP] FP32 statistics of one chunk's sub-chunks
     size_t part_cap = 0;
     double *d_acc = nullptr;     // [M][KP] FP64 statistics over chunks
     size_t acc_cap = 0;
@@ -56,14 +59,15 @@ static std::vector<int> fc_pairs(int D) {
     return p;
 }
 
-// phi rows of frames [n0, n0 + nc): one block of 256 threads per 8 frames;
+// phi rows of frames [n0, n0 + nc) (rows up to ncp zero): one block of 256
+// threads per 8 frames;
 // the frames are centred in LDS and every thread writes consecutive
 // features.  A frame holding a non-finite (or overflowing, |x'| > 1e18)
 // value gets a zero row and bad = 1: its log-likelihood is clamped and it
 // contributes no statistics, as the reference (non-finite component
 // log-likelihoods -> _minLogLike -> no posterior above the cut).
 __global__ __launch_bounds__(256) void fc_features_kernel(const float *__restrict__ X, uint64_t n0,
-                                                          int nc, int D, int K, int KP,
+                                                          int nc, int ncp, int D, int K, int KP,
                                                           const float *__restrict__ cen,
                                                           const int *__restrict__ pairs,
                                                           float *__restrict__ phi,
@@ -87,10 +91,11 @@ __global__ __launch_bounds__(256) void fc_features_kernel(const float *__restric
         xs[f][d] = v;
     }
     __syncthreads();
+    // rows [nc, ncp) pad the chunk to whole sub-chunks: zero features
     for (int f = 0; f < 8; f++) {
-        if (f0 + f >= nc) break;
+        if (f0 + f >= ncp) break;
         float *row = phi + (size_t)(f0 + f) * KP;
-        const bool fb = badf[f] != 0;
+        const bool fb = badf[f] != 0 || f0 + f >= nc;
         for (int k = tid; k < KP; k += 256) {
             float v = 0.0f;
             if (k < K && !fb) {
@@ -111,7 +116,7 @@ __global__ __launch_bounds__(256) void fc_features_kernel(const float *__restric
 // g = exp(l - LL) cut at g <= min_gamma, written in place.  LLnat[n0 + t]
 // (optional) receives the frame's log-likelihood; blockLL[block] the FP64
 // sum of the block's four frames.
-__global__ __launch_bounds__(256) void fc_lse_kernel(float *__restrict__ L, int nc, int M,
+__global__ __launch_bounds__(256) void fc_lse_kernel(float *__restrict__ L, int nc, int ncp, int M,
                                                      const uint8_t *__restrict__ bad,
                                                      float min_ll, float min_gamma,
                                                      float *__restrict__ LLnat, uint64_t n0,
@@ -120,6 +125,10 @@ __global__ __launch_bounds__(256) void fc_lse_kernel(float *__restrict__ L, int 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = blockIdx.x * 4 + wave;
     double lld = 0.0;
+    if (t >= nc && t < ncp) {
+        // padding frames: posterior 0
+        for (int m = lane; m < M; m += 64) L[(size_t)t * M + m] = 0.0f;
+    }
     if (t < nc) {
         float *row = L + (size_t)t * M;
         float mx = -INFINITY;
@@ -153,13 +162,15 @@ __global__ __launch_bounds__(256) void fc_lse_kernel(float *__restrict__ L, int 
     if (threadIdx.x == 0) blockLL[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// acc[e] += part[e] (FP64), one element per thread: the fixed-order
-// cross-chunk reduction of the FP32 chunk statistics.
+// acc[e] += sum_s part[s][e] (FP64, s in order), one element per thread:
+// the fixed-order reduction of the FP32 sub-chunk statistics.
 __global__ __launch_bounds__(256) void fc_acc_kernel(const float *__restrict__ part, size_t E,
-                                                     int first, double *__restrict__ acc) {
+                                                     int S, int first, double *__restrict__ acc) {
     const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= E) return;
-    acc[e] = (first ? 0.0 : acc[e]) + (double)part[e];
+    double v = first ? 0.0 : acc[e];
+    for (int s = 0; s < S; s++) v += (double)part[(size_t)s * E + e];
+    acc[e] = v;
 }
 
 }  // namespace gmmhip
@@ -213,13 +224,19 @@ int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot) {
     gmmhip::FullState *fs = h->full;
     const int M = h->M, D = h->D, K = fs->K, KP = fs->KP;
     const uint64_t nc_max = full_chunk(h, KP);
-    HIPCHK(h, ensure(&fs->d_phi, &fs->phi_cap, (size_t)nc_max * KP));
-    HIPCHK(h, ensure(&fs->d_L, &fs->L_cap, (size_t)nc_max * M));
-    HIPCHK(h, ensure(&fs->d_bad, &fs->bad_cap, (size_t)nc_max));
-    const uint64_t nblk_total = (h->N + nc_max - 1) / nc_max * ((nc_max + 3) / 4);
+    // sub-chunks of the statistics GEMM: enough (256 x 128)-tiles of the
+    // K x M output to cover the CUs, each sub-chunk >= 256 frames
+    const int tiles = ((KP + 255) / 256) * ((M + 127) / 128);
+    int S = std::max(1, std::min(32, (2 * (h->num_cus > 0 ? h->num_cus : 256) + tiles - 1) / tiles));
+    S = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)S, nc_max / 256));
+    const uint64_t ncp_max = (nc_max + S - 1) / S * S;
+    HIPCHK(h, ensure(&fs->d_phi, &fs->phi_cap, (size_t)ncp_max * KP));
+    HIPCHK(h, ensure(&fs->d_L, &fs->L_cap, (size_t)ncp_max * M));
+    HIPCHK(h, ensure(&fs->d_bad, &fs->bad_cap, (size_t)ncp_max));
+    const uint64_t nblk_total = (h->N + nc_max - 1) / nc_max * ((ncp_max + 3) / 4);
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, (size_t)nblk_total));
     if (stats) {
-        HIPCHK(h, ensure(&fs->d_part, &fs->part_cap, (size_t)M * KP));
+        HIPCHK(h, ensure(&fs->d_part, &fs->part_cap, (size_t)S * M * KP));
         HIPCHK(h, ensure(&fs->d_acc, &fs->acc_cap, (size_t)M * KP));
     }
     const float one = 1.0f, zero = 0.0f;
@@ -227,26 +244,32 @@ int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot) {
     int first = 1;
     for (uint64_t n0 = 0; n0 < h->N; n0 += nc_max) {
         const int nc = (int)std::min<uint64_t>(nc_max, h->N - n0);
-        hipLaunchKernelGGL(gmmhip::fc_features_kernel, dim3((nc + 7) / 8), dim3(256), 0, h->stream,
-                           h->d_x, n0, nc, D, K, KP, fs->d_cen, fs->d_pairs, fs->d_phi, fs->d_bad);
+        const int sub = (nc + S - 1) / S;  // frames per sub-chunk
+        const int ncp = sub * S;
+        hipLaunchKernelGGL(gmmhip::fc_features_kernel, dim3((ncp + 7) / 8), dim3(256), 0, h->stream,
+                           h->d_x, n0, nc, ncp, D, K, KP, fs->d_cen, fs->d_pairs, fs->d_phi,
+                           fs->d_bad);
         HIPCHK(h, hipGetLastError());
-        // L (column-major M x nc) = Theta^T (M x KP) * Phi (KP x nc)
+        // L (column-major M x nc, frame-major rows) = Theta^T (M x KP) * Phi (KP x nc)
         BLASCHK(h, rocblas_sgemm(fs->blas, rocblas_operation_transpose, rocblas_operation_none, M,
                                  nc, KP, &one, fs->d_theta, KP, fs->d_phi, KP, &zero, fs->d_L, M));
-        const int blocks = (nc + 3) / 4;
+        const int blocks = (ncp + 3) / 4;
         hipLaunchKernelGGL(gmmhip::fc_lse_kernel, dim3(blocks), dim3(256), 0, h->stream, fs->d_L, nc,
-                           M, fs->d_bad, h->opt.min_loglike, h->opt.min_gamma, LLnat, n0,
+                           ncp, M, fs->d_bad, h->opt.min_loglike, h->opt.min_gamma, LLnat, n0,
                            h->d_blockLL + nblk);
         HIPCHK(h, hipGetLastError());
         nblk += blocks;
         if (stats) {
-            // part (column-major KP x M) = Phi (KP x nc) * Gamma^T (nc x M)
-            BLASCHK(h, rocblas_sgemm(fs->blas, rocblas_operation_none, rocblas_operation_transpose,
-                                     KP, M, nc, &one, fs->d_phi, KP, fs->d_L, M, &zero, fs->d_part,
-                                     KP));
+            // part[s] (column-major KP x M) = Phi_s (KP x sub) * Gamma_s^T (sub x M), frames
+            // [s sub, (s+1) sub) of the chunk
+            BLASCHK(h, rocblas_sgemm_strided_batched(
+                           fs->blas, rocblas_operation_none, rocblas_operation_transpose, KP, M, sub,
+                           &one, fs->d_phi, KP, (rocblas_stride)sub * KP, fs->d_L, M,
+                           (rocblas_stride)sub * M, &zero, fs->d_part, KP, (rocblas_stride)KP * M,
+                           S));
             const size_t E = (size_t)M * KP;
             hipLaunchKernelGGL(gmmhip::fc_acc_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0,
-                               h->stream, fs->d_part, E, first, fs->d_acc);
+                               h->stream, fs->d_part, E, S, first, fs->d_acc);
             HIPCHK(h, hipGetLastError());
             first = 0;
         }

@@ -129,7 +129,10 @@ def test_full_determinism_chunking_and_accumulate_into(acc):
     acc.set_opts(chunk_frames=0)
     assert np.allclose(c[0], a[0], rtol=1e-5, atol=1e-4)
     assert np.allclose(c[2], a[2], rtol=1e-4, atol=1e-2)
-    assert c[3] == pytest.approx(a[3], rel=1e-12)
+    # the GEMM kernel (and so the FP32 summation order over the features) is
+    # chosen per chunk shape: per-frame log-likelihoods may differ in the
+    # last bits between chunk sizes, never between runs of one size
+    assert c[3] == pytest.approx(a[3], rel=1e-8)
 
 
 def test_full_errors(acc):
