@@ -203,133 +203,6 @@ __global__ __launch_bounds__(256) void prep_frames_kernel(PrepArgs p) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-frame log-sum-exp: block = 4 waves x 32 frames; a wave keeps the split
-// A fragments of its frames in registers and streams 32-component tiles of
-// the split model through LDS.  Two accumulators (hi*hi, cross terms) keep
-// two independent MFMA chains per wave.
-template <int KP, typename T>
-__global__ __launch_bounds__(256) void loglik_mixed_kernel(MixedArgs a) {
-    typedef typename Vec8<T>::type V8;
-    constexpr int KS = KP / 16;
-    constexpr int TILE16 = KS * 2 * 64;      // 16-byte pieces per component tile
-    constexpr int PF = TILE16 / 256;
-    static_assert(TILE16 % 256 == 0, "component tile must split evenly over 256 threads");
-    __shared__ u32x4 Pl[TILE16];
-    __shared__ double red[8];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int h = lane >> 5, c = lane & 31;
-    const uint64_t base = (uint64_t)blockIdx.x * 128;
-
-    V8 ahi[KS], alo[KS];
-    {
-        const uint64_t fr = base + 32 * wave + c;  // < Npad
-        const u32x4 *zh = reinterpret_cast<const u32x4 *>(a.za_hi) + fr * (KP / 8);
-        const u32x4 *zl = reinterpret_cast<const u32x4 *>(a.za_lo) + fr * (KP / 8);
-#pragma unroll
-        for (int ks = 0; ks < KS; ks++) {
-            const u32x4 x = zh[2 * ks + h], y = zl[2 * ks + h];
-            ahi[ks] = __builtin_bit_cast(V8, x);
-            alo[ks] = __builtin_bit_cast(V8, y);
-        }
-    }
-
-    float mx[16], sm[16];
-    const u32x4 *P16 = reinterpret_cast<const u32x4 *>(a.P16);
-    u32x4 pf[PF];
-#define GMMHIP_PREFETCH_TILE(T_)                                        \
-    _Pragma("unroll") for (int j = 0; j < PF; j++) {                    \
-        pf[j] = P16[(size_t)(T_) * TILE16 + tid + 256 * j];             \
-    }
-    GMMHIP_PREFETCH_TILE(0)
-    for (int t = 0; t < a.Mt32; t++) {
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < PF; j++) Pl[tid + 256 * j] = pf[j];
-        __syncthreads();
-        if (t + 1 < a.Mt32) {
-            GMMHIP_PREFETCH_TILE(t + 1)
-        }
-
-        f32x16 accA = {}, accB = {};
-#pragma unroll
-        for (int ks = 0; ks < KS; ks++) {
-            const u32x4 bh4 = Pl[(ks * 2 + 0) * 64 + lane];
-            const u32x4 bl4 = Pl[(ks * 2 + 1) * 64 + lane];
-            const V8 bh = __builtin_bit_cast(V8, bh4);
-            const V8 bl = __builtin_bit_cast(V8, bl4);
-            accA = mfma32(ahi[ks], bh, accA);
-            accB = mfma32(ahi[ks], bl, accB);
-            accB = mfma32(alo[ks], bh, accB);
-        }
-        // padded components (>= M) are masked per lane
-        const bool cvalid = (32 * t + c) < a.M;
-        if (t == 0) {
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                mx[r] = cvalid ? accA[r] + accB[r] : -INFINITY;
-                sm[r] = cvalid ? 1.0f : 0.0f;
-            }
-        } else {
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                // branch-free: a value more than 32 above the reference moves
-                // it (sm * 2^-d + 1), otherwise sm += 2^d
-                const float v = cvalid ? accA[r] + accB[r] : -INFINITY;
-                const float d = v - mx[r];
-                const bool up = d > 32.0f;
-                const float e = exp2f_fast(up ? -d : d);
-                sm[r] = up ? fmaf(sm[r], e, 1.0f) : sm[r] + e;
-                mx[r] = up ? v : mx[r];
-            }
-        }
-    }
-
-    // combine the 32 component lanes of each frame row (xor within 32 lanes)
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const float m2 = __shfl_xor(mx[r], o);
-            const float s2 = __shfl_xor(sm[r], o);
-            const float hi = fmaxf(mx[r], m2);
-            // lanes that only saw padded components carry (-inf, 0)
-            const float e1 = (mx[r] == -INFINITY) ? 0.0f : sm[r] * exp2f_fast(mx[r] - hi);
-            const float e2 = (m2 == -INFINITY) ? 0.0f : s2 * exp2f_fast(m2 - hi);
-            sm[r] = e1 + e2;
-            mx[r] = hi;
-        }
-    }
-    double local = 0.0;
-    if (c == 0) {
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int row = 8 * (r >> 2) + 4 * h + (r & 3);
-            const uint64_t fr = base + 32 * wave + row;
-            float l2 = mx[r] + log2f_fast(sm[r]);
-            float nat = l2 * kLn2m;
-            if (!(nat > a.min_ll) || isinf(nat) || (fr < a.N && a.bad[fr])) {
-                nat = a.min_ll;
-                l2 = INFINITY;
-            }
-            if (fr < a.N) {
-                a.LL2[fr] = l2;
-                if (a.LLnat) a.LLnat[fr] = nat;
-                local += (double)nat;
-            }
-        }
-        red[wave * 2 + h] = local;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double s = 0.0;
-        for (int i = 0; i < 8; i++) s += red[i];
-        a.blockLL[blockIdx.x] = s;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Statistics: block = (frame chunk, group of 128 components); wave w owns the
 // 32-component tile 4*group + w and its 32 x KP accumulators.  Per 32 frames
 // the block stages the ZA rows and the ZB block of hi/lo pieces in LDS
@@ -781,8 +654,11 @@ __device__ __forceinline__ float row_max32(float v) {
 // far above a row's reference are summed without moving it.
 static constexpr float LSE_HEADROOM = 64.0f;
 
-// Log-likelihood with direct-to-LDS staging of the component tiles (two
-// LDS buffers, one barrier per tile); same arithmetic as loglik_mixed_kernel.
+// Per-frame log-sum-exp: block = 4 waves x 32 frames; a wave keeps the split
+// A fragments of its frames in registers and streams 32-component tiles of
+// the split model through LDS by direct-to-LDS copies (two LDS buffers, one
+// barrier per tile).  hi*hi + hi*lo + lo*hi accumulate into one chain per
+// tile; the log-sum-exp of tile t runs on the VALU between tile t+1's MFMAs.
 template <int KP, typename T>
 __global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
     typedef typename Vec8<T>::type V8;

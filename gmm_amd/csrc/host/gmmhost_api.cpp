@@ -140,7 +140,7 @@ int gmmhost_update_map(gmmhost_model_t m, gmmhost_model_t ubm, const double *n, 
         o._AdaptMeanBool = upmean != 0;
         o._AdaptVarBool = upvar != 0;
         ad.SetOptions(o);
-        ad.Stats() = make_stats(M, D, n, F, S, tot_samples);
+        ad.Stats() = make_stats(M, D, n, F, S, tot_samples, m->m.GetFullCovStatus());
         ad.UpdateModelMAP(&m->m, &ubm->m);
     } catch (std::exception &e) {
         return fail(e.what());

@@ -114,7 +114,10 @@ unsigned int CModelAdapt::UpdateModelEM(GMModel *model, GMMStats<double> &stats,
     return (unsigned int)(nummix - (int)nzero);
 }
 
-// UpdateModelMAP (OL_ModelAdapt.cpp:993-1106), diagonal, non-robust.
+// UpdateModelMAP (OL_ModelAdapt.cpp:993-1106), non-robust.  Full
+// covariance (:1079-1085): cov[k][kk] = cov[kk][k] = (float)(alfa S/n +
+// (1-alfa)(ubm cov[k][kk] + ubm mu_k ubm mu_kk) - mu_k mu_kk), products of
+// float means taken in float, no floor.
 void CModelAdapt::UpdateModelMAP(GMModel *model, GMModel *UBM) {
     if (opt_._tau == 0) {
         if (UpdateModelEM(model, stats_, opt_._AdaptWeightBool, opt_._AdaptMeanBool,
@@ -127,7 +130,10 @@ void CModelAdapt::UpdateModelMAP(GMModel *model, GMModel *UBM) {
     }
     const int nummix = model->GetNumberOfMixtures();
     const int D = model->GetDimension();
-    std::vector<float> am(D), av(D);
+    const bool full = stats_.hasVars() && stats_.hasFullCov();
+    if (opt_._AdaptVarBool && (full != UBM->GetFullCovStatus() || full != model->GetFullCovStatus()))
+        throw std::logic_error("UpdateModelMAP(): full/diagonal statistics do not match the models");
+    std::vector<float> am(D), av((size_t)model->GetDimVar());
     double mixcorr = 0.0;
     for (int j = 0; j < nummix; j++) {
         if (UBM->GetMixtureWeight(j) == 0) continue;
@@ -147,13 +153,24 @@ void CModelAdapt::UpdateModelMAP(GMModel *model, GMModel *UBM) {
             const double *F = stats_.getMean(j);
             const double *S = stats_.getVar(j);
             for (int k = 0; k < D; k++) am[k] = (float)(alfa * F[k] / np + (1 - alfa) * um[k]);
-            if (opt_._AdaptVarBool) {
+            if (opt_._AdaptVarBool && !full) {
                 for (int k = 0; k < D; k++) {
                     av[k] = (float)(alfa * S[k] / np + (1 - alfa) * (uv[k] + um[k] * um[k]) -
                                     am[k] * am[k]);
                     if (av[k] < opt_._min_var) av[k] = opt_._min_var;
                 }
                 std::memcpy(model->GetMixtureVar(j), av.data(), sizeof(float) * D);
+            } else if (opt_._AdaptVarBool) {
+                size_t p = 0;  // packed upper index of (k, kk)
+                for (int k = 0; k < D; k++)
+                    for (int kk = k; kk < D; kk++, p++) {
+                        const float v = (float)(alfa * S[p] / np +
+                                                (1 - alfa) * (uv[(size_t)k * D + kk] + um[k] * um[kk]) -
+                                                am[k] * am[kk]);
+                        av[(size_t)k * D + kk] = v;
+                        av[(size_t)kk * D + k] = v;
+                    }
+                std::memcpy(model->GetMixtureVar(j), av.data(), sizeof(float) * av.size());
             }
             std::memcpy(model->GetMixtureMean(j), am.data(), sizeof(float) * D);
         }
@@ -224,8 +241,11 @@ void CModelAdapt::prepareKiGiInitMatrices(const GMModel &UBM, float min_var, std
     const int M = UBM.GetNumberOfMixtures(), D = UBM.GetDimension(), E = D + 1;
     G.assign((size_t)D * E * E, 0.0);
     k.assign((size_t)D * E, 0.0);
+    std::vector<float> vd(D);
     for (int m = 0; m < M; m++) {
-        const float *var = UBM.GetMixtureVar(m), *mean = UBM.GetMixtureMean(m);
+        // a full-covariance UBM contributes its diagonal (GetMixtureVarDiag, :1502)
+        std::memcpy(vd.data(), UBM.GetMixtureVarDiag(m), sizeof(float) * D);
+        const float *var = vd.data(), *mean = UBM.GetMixtureMean(m);
         for (int i = 0; i < D; i++) {
             const double cG = (var[i] < min_var) ? 1.0 / min_var : 1.0 / var[i];
             const double ck = mean[i] * cG;
@@ -258,7 +278,7 @@ void CModelAdapt::ComputeKiGi(const GMModel &model, int i, const double *Ginit, 
         const double np = stats_.getProb(m);
         if (!(np > 0)) continue;
         const float *mm = model.GetMixtureMean(m);
-        const double vi = model.GetMixtureVar(m)[i];
+        const double vi = model.GetMixtureVar(m)[model.GetFullCovStatus() ? (size_t)i * D + i : i];
         const double cm = stats_.getMean(m)[i] / vi, cv = np / vi;
         for (int a = 0; a < D; a++) {
             k[a] += cm * mm[a];
@@ -382,9 +402,9 @@ MLLRTransform CModelAdapt::IterW_MLLR(GMModel *UBM, int iters, bool wantQ) {
 void CModelAdapt::AdaptModel(GMModel *UBM) {
     if (!UBM) throw std::logic_error("AdaptModel(): UBM not given!");
     if (!data_) throw std::logic_error("AdaptModel(): Insert adaptation data first!");
-    if (UBM->GetFullCovStatus())
-        throw std::runtime_error("AdaptModel(): MAP/MLLR adaptation of full-covariance models is not "
-                                 "supported by this build");
+    // statistics as the reference allocates them (OL_ModelAdapt.cpp:281-298):
+    // full second-order statistics for a full UBM, none for MLLR
+    const bool fullV = UBM->GetFullCovStatus();
     UBM_ = UBM;
     nummix_ = UBM->GetNumberOfMixtures();
     const int D = UBM->GetDimension();
@@ -393,7 +413,7 @@ void CModelAdapt::AdaptModel(GMModel *UBM) {
     W_.reset();
     switch (opt_._adaptType) {
     case MAP:
-        stats_.alloc(D, nummix_, opt_._AdaptMeanBool || opt_._AdaptVarBool, opt_._AdaptVarBool);
+        stats_.alloc(D, nummix_, opt_._AdaptMeanBool || opt_._AdaptVarBool, opt_._AdaptVarBool, fullV);
         model_ = AdaptModelMAP(UBM);
         break;
     case MLLR: {
@@ -403,7 +423,7 @@ void CModelAdapt::AdaptModel(GMModel *UBM) {
         break;
     }
     case MLLR_MAP: {
-        stats_.alloc(D, nummix_, true, opt_._AdaptVarBool);
+        stats_.alloc(D, nummix_, true, opt_._AdaptVarBool, fullV);
         MLLRTransform W = IterW_MLLR(UBM, opt_._iterMLLR, false);
         std::unique_ptr<GMModel> mm = AdaptModelMLLR(*UBM, W);
         model_ = AdaptModelMAP(mm.get());

@@ -49,12 +49,14 @@ int gmmhost_update_em(gmmhost_model_t m, const double *n, const double *F, const
                       uint64_t tot_samples, float min_var, int upweight, int upmean, int upvar,
                       int *left);
 /* UpdateModelMAP (OL_ModelAdapt.cpp:993-1106): m starts as (a copy of) the
- * UBM ubm and is adapted with relevance factor tau. */
+ * UBM ubm and is adapted with relevance factor tau (full-covariance models:
+ * packed S as for gmmhost_update_em). */
 int gmmhost_update_map(gmmhost_model_t m, gmmhost_model_t ubm, const double *n, const double *F,
                        const double *S, uint64_t tot_samples, int tau, float min_var,
                        int upweight, int upmean, int upvar);
 /* Global MLLR transform W[D*(D+1)] (row-major [A|b]) of model m from
- * statistics n[M], F[M*D] (ComputeW_MLLR + ComputeKiGi_MLLR, :1417-1581),
+ * statistics n[M], F[M*D] (ComputeW_MLLR + ComputeKiGi_MLLR, :1417-1581;
+ * a full-covariance model contributes the diagonal of its covariances),
  * optionally with the UBM prior of prepareKiGiInitMatrices (:1484-1523,
  * ubm may be NULL).  Returns 7 (GMMHIP_ERR_MLLR) when CheckInvertibility
  * (:1851-1878) rejects the statistics. */

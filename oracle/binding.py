@@ -262,6 +262,18 @@ def update_em_full(w, mu, cov, n, F, S, tot_samples):
     return w, mu, cov, left
 
 
+def update_map_full(ubm_w, ubm_mu, ubm_cov, n, F, S, tot_samples, tau,
+                    upweight=False, upmean=True, upvar=False):
+    ubm_w, ubm_mu, ubm_cov = _f32(ubm_w), _f32(ubm_mu), _f32(ubm_cov)
+    w, mu, cov = ubm_w.copy(), ubm_mu.copy(), ubm_cov.copy()
+    M, D = mu.shape
+    n, F, S = _f64(n), _f64(F), _f64(S)
+    lib().orc_update_map_full(M, D, _p(w), _p(mu), _p(cov), _p(ubm_w), _p(ubm_mu), _p(ubm_cov),
+                              _p(n), _p(F), _p(S), ctypes.c_double(tot_samples), int(tau),
+                              int(upweight), int(upmean), int(upvar))
+    return w, mu, cov
+
+
 def eig_leading(cov):
     cov = _f32(cov)
     D = cov.shape[0]

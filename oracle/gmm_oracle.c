@@ -465,7 +465,7 @@ void orc_mllr_apply(int M, int D, const double *W, const float *mu_in,
 }
 
 /* ------------------------------------------------------------------------
- * Full covariance (survey scope row (f)2).  Paths relative to GMM/lib_bmssr/.
+ * Full covariance.  Paths relative to GMM/lib_bmssr/.
  *
  * How each function is pinned (tests/test_oracle_fullcov.py):
  *   orc_prepare_model_full / orc_frame_loglik_full -- example/data.logL
@@ -479,6 +479,10 @@ void orc_mllr_apply(int M, int D, const double *W, const float *mu_in,
  *   orc_update_em_full -- one component with all-ones posteriors gives the
  *       sample mean and covariance; example/myModelFC.gmm reproduced by the
  *       from-scratch driver (to the level stated in the test).
+ *   orc_update_map_full -- diagonal models written as full matrices give
+ *       orc_update_map's result; tau -> infinity keeps the UBM; a tiny tau
+ *       gives orc_update_em_full's means and covariances; one component
+ *       against the closed-form interpolation of the moments.
  *   orc_eig_leading -- numpy.linalg.eigh on random SPD matrices.
  * ------------------------------------------------------------------------ */
 
@@ -720,6 +724,59 @@ int orc_update_em_full(int M, int D, float *w, float *mu, float *cov,
             w[j] = (float)(w[j] / gcorr);
     free(mean);
     return M - nzero;
+}
+
+/* UpdateModelMAP for tau > 0, full-covariance branch, non-robust
+ * (OL_ModelAdapt.cpp:993-1106, :1079-1085): as orc_update_map, with
+ * cov[k][kk] = cov[kk][k] = (float)(alfa S_kk'/n + (1-alfa)(ubm_cov[k][kk] +
+ * ubm_mu_k ubm_mu_kk) - mu_k mu_kk), the products of float means taken in
+ * float, no variance floor.  S is packed upper per component. */
+void orc_update_map_full(int M, int D, float *w, float *mu, float *cov,
+                         const float *ubm_w, const float *ubm_mu, const float *ubm_cov,
+                         const double *n, const double *F, const double *S,
+                         double tot_samples, int tau, int upweight, int upmean, int upvar)
+{
+    const size_t DP = (size_t)D * (D + 1) / 2;
+    double mixcorr = 0.0;
+    float *amean = (float *)malloc(sizeof(float) * (D > 0 ? D : 1));
+    for (int j = 0; j < M; j++) {
+        if (ubm_w[j] == 0.0f)
+            continue;
+        const double np = n[j];
+        if (!(np > 0.0))
+            continue;
+        const float *um = ubm_mu + (size_t)j * D, *uc = ubm_cov + (size_t)j * D * D;
+        const double alfa = np / (np + tau);
+        if (upweight && tot_samples > 0) {
+            const float aw = (float)(alfa * (np / tot_samples) + (1 - alfa) * ubm_w[j]);
+            w[j] = aw;
+            mixcorr += aw;
+        }
+        if (upmean || upvar) {
+            for (int k = 0; k < D; k++)
+                amean[k] = (float)(alfa * F[(size_t)j * D + k] / np + (1 - alfa) * um[k]);
+            if (upvar) {
+                float *c = cov + (size_t)j * D * D;
+                const double *Sj = S + (size_t)j * DP;
+                for (int k = 0; k < D; k++)
+                    for (int kk = k; kk < D; kk++) {
+                        /* the UBM moment is summed in float, as the reference's
+                         * float operands make it */
+                        const float ubm_m2 = uc[(size_t)k * D + kk] + um[k] * um[kk];
+                        const float mm = amean[k] * amean[kk];
+                        const float v = (float)(alfa * Sj[packed_upper(D, k, kk)] / np +
+                                                (1 - alfa) * (double)ubm_m2 - (double)mm);
+                        c[(size_t)k * D + kk] = v;
+                        c[(size_t)kk * D + k] = v;
+                    }
+            }
+            memcpy(mu + (size_t)j * D, amean, sizeof(float) * D);
+        }
+    }
+    if (upweight && mixcorr != 0.0)
+        for (int j = 0; j < M; j++)
+            w[j] = w[j] / (float)mixcorr;
+    free(amean);
 }
 
 /* Leading eigenpair of a symmetric matrix (the ssyevr('V','I','U', il = iu

@@ -155,3 +155,56 @@ def test_train_mllr_from_ubm(tmp_path):
     assert np.allclose(Wg, W, rtol=1e-4, atol=1e-4)
     gw, gmu, gvar, _ = fm.read_gmm_txt(out)
     assert np.allclose(gmu, orc.mllr_apply(W, mu), rtol=1e-4, atol=1e-4)
+
+
+def test_train_map_full_cov_ubm(tmp_path, golden):
+    """MAP (weights, means and full covariances, tau = 16, two iterations)
+    of data.bin from the reference's full-covariance myModelFC.gmm, against
+    the oracle's full-covariance E-step and MAP rule applied the same way."""
+    ubm_p = os.path.join(GOLD, "myModelFC.gmm")
+    out = tmp_path / "mapfc.gmm"
+    run("trainGMM", "-i", os.path.join(GOLD, "data.bin"), "--data-T", 2, "--in-UBM", ubm_p, "-L",
+        "--type", 2, "--it-MAP", 2, "--tau", 16, "--mean", "--var", "--weight", "-o", out, "-T")
+    uw, umu, ucov, _ = fm.read_gmm_txt_full(ubm_p)
+    x = golden["x"]
+    w, mu, cov = uw, umu, ucov
+    for _ in range(2):
+        n, F, S, _ = orc.accumulate_full(x, w, mu, cov)
+        w, mu, cov = orc.update_map_full(uw, umu, ucov, n, F, S, x.shape[0], 16, upweight=True,
+                                         upmean=True, upvar=True)
+    gw, gmu, gcov, _ = fm.read_gmm_txt_full(out)
+    assert np.allclose(gw, w, rtol=1e-5, atol=1e-7)
+    assert np.abs(gmu - mu).max() < 1e-4 * max(1.0, np.abs(mu).max())
+    assert np.abs(gcov - cov).max() < 1e-4 * max(1.0, np.abs(cov).max())
+
+
+def test_train_mllr_full_cov_ubm(tmp_path):
+    """Global MLLR from a full-covariance UBM: full-covariance posteriors,
+    the diagonal of the covariances in the row statistics."""
+    rng = np.random.default_rng(47)
+    M, D, N = 40, 6, 30000
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    A = rng.normal(size=(M, D, D))
+    cov = (0.3 * A @ A.transpose(0, 2, 1) / D + 0.5 * np.eye(D)).astype(np.float32)
+    cov = 0.5 * (cov + cov.transpose(0, 2, 1))
+    ubm_p = tmp_path / "ubmfc.gmm"
+    host.Model.create_full(w, mu, cov).save(ubm_p, True)
+    uw, umu, ucov, _ = fm.read_gmm_txt_full(ubm_p)
+    k = rng.choice(M, size=N, p=uw.astype(np.float64) / uw.sum())
+    L = np.linalg.cholesky(ucov.astype(np.float64))
+    x = (1.1 * umu[k] + 0.4 + np.einsum("nij,nj->ni", L[k], rng.normal(size=(N, D)))).astype(np.float32)
+    data_p = tmp_path / "data.raw"
+    fm.write_raw(data_p, x)
+    out = tmp_path / "mllrfc.gmm"
+    run("trainGMM", "-i", data_p, "--data-T", 2, "--in-UBM", ubm_p, "-L", "--type", 1,
+        "--apply-trn", "--save-trn", "-o", out, "-T")
+    n, F, _, _ = orc.accumulate_full(x, uw, umu, ucov)
+    var = np.ascontiguousarray(ucov[:, np.arange(D), np.arange(D)])
+    W = orc.mllr_transform(umu, var, n, F)
+    lines = (tmp_path / "mllrfc.mat").read_text().split("\n")
+    Wg = np.array([[float(v) for v in ln.split()] for ln in lines[2:2 + D]])
+    assert np.allclose(Wg, W, rtol=1e-4, atol=1e-4)
+    gw, gmu, gcov, _ = fm.read_gmm_txt_full(out)
+    assert np.allclose(gmu, orc.mllr_apply(W, umu), rtol=1e-4, atol=1e-4)
+    assert np.array_equal(gcov, ucov)

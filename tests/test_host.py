@@ -187,3 +187,47 @@ def test_update_em_full_matches_oracle():
     assert left == oleft
     assert np.array_equal(gw, ow) and np.array_equal(gmu, omu)
     assert np.array_equal(gcov, ocov)
+
+
+@pytest.mark.parametrize("flags", [(True, True, True), (False, True, False), (False, False, True)])
+def test_update_map_full_matches_oracle(flags):
+    """UpdateModelMAP's full-covariance branch (OL_ModelAdapt.cpp:1079-1085)
+    against the oracle, bit for bit."""
+    rng = np.random.default_rng(19)
+    M, D, N = 3, 5, 400
+    w = np.array([0.3, 0.3, 0.4], np.float32)
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    A = rng.normal(size=(M, D, D))
+    cov = (A @ A.transpose(0, 2, 1) / D + np.eye(D)).astype(np.float32)
+    cov = 0.5 * (cov + cov.transpose(0, 2, 1))
+    x = (rng.normal(0, 2, (N, D)) + 0.3).astype(np.float32)
+    n, F, S, _ = orc.accumulate_full(x, w, mu, cov)
+    ubm = host.Model.create_full(w, mu, cov)
+    m = host.Model.create_full(w, mu, cov)
+    m.update_map(ubm, n, F, S, N, 16, upweight=flags[0], upmean=flags[1], upvar=flags[2])
+    hw, hmu, hcov = m.get()
+    rw, rmu, rcov = orc.update_map_full(w, mu, cov, n, F, S, N, 16, upweight=flags[0],
+                                        upmean=flags[1], upvar=flags[2])
+    assert np.array_equal(hw, rw) and np.array_equal(hmu, rmu) and np.array_equal(hcov, rcov)
+
+
+def test_mllr_transform_full_model_uses_diagonal():
+    """A full-covariance model enters the MLLR statistics through the
+    diagonal of its covariances (GetMixtureVarDiag, OL_ModelAdapt.cpp:1502,
+    :1561): same transform as the diagonal model of those variances."""
+    rng = np.random.default_rng(6)
+    M, D = 30, 5
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    A = rng.normal(size=(M, D, D))
+    cov = (A @ A.transpose(0, 2, 1) / D + np.eye(D)).astype(np.float32)
+    var = np.ascontiguousarray(cov[:, np.arange(D), np.arange(D)])
+    n = rng.uniform(2, 20, M)
+    F = n[:, None] * (mu.astype(np.float64) + 0.3) + rng.normal(0, 0.1, (M, D))
+    mf = host.Model.create_full(w, mu, cov)
+    md = host.Model.create(w, mu, var)
+    Wf = mf.mllr_transform(n, F, ubm=mf)
+    Wd = md.mllr_transform(n, F, ubm=md)
+    assert np.array_equal(Wf, Wd)
+    R = orc.mllr_transform(mu, var, n, F, ubm_mu=mu, ubm_var=var)
+    assert np.allclose(Wf, R, rtol=1e-9, atol=1e-9)

@@ -193,3 +193,82 @@ def test_mymodelfc_split_sign_matters(golden, monkeypatch):
     monkeypatch.setattr(orc, "eig_leading", lambda c: (lambda lv: (lv[0], -lv[1]))(real(c)))
     w, mu, cov = sd.train_from_scratch_full(golden["x"], 4, 8)
     assert np.abs(mu - rmu).max() > 0.3
+
+
+def _full_stats(seed, M=3, D=5, N=400):
+    rng = np.random.default_rng(seed)
+    w, mu, cov = rand_full_model(rng, M, D)
+    x = (rng.normal(0, 2, (N, D)) + 0.5).astype(np.float32)
+    n, F, S, _ = orc.accumulate_full(x, w, mu, cov)
+    return x, w, mu, cov, n, F, S
+
+
+def test_update_map_full_diagonal_equals_diagonal_map():
+    """A diagonal UBM written as full matrices: weights, means and the
+    diagonal of every adapted covariance are bit-identical to the diagonal
+    MAP rule fed the diagonal of the same packed statistics (pins the
+    packed-upper indexing and the float rounding points)."""
+    rng = np.random.default_rng(21)
+    M, D, N = 4, 6, 500
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    var = rng.uniform(0.5, 3, (M, D)).astype(np.float32)
+    x = rng.normal(0, 2.5, (N, D)).astype(np.float32)
+    n, F, S, _ = orc.accumulate_full(x, w, mu, diag_as_full(var))
+    r, c = np.triu_indices(D)
+    Sd = S[:, r == c]
+    for tau in (1, 16, 300):
+        fw, fmu, fcov = orc.update_map_full(w, mu, diag_as_full(var), n, F, S, N, tau,
+                                            upweight=True, upmean=True, upvar=True)
+        dw, dmu, dvar = orc.update_map(w, mu, var, n, F, Sd, N, tau, min_var=1e-6,
+                                       upweight=True, upmean=True, upvar=True)
+        assert dvar.min() > 1e-6  # no floor involved
+        assert np.array_equal(fw, dw) and np.array_equal(fmu, dmu)
+        assert np.array_equal(fcov[:, np.arange(D), np.arange(D)], dvar)
+
+
+def test_update_map_full_second_moment_interpolation():
+    """One component, every posterior 1: the adapted second moment
+    cov + mu mu^T interpolates between the data's X^T X / N and the UBM's
+    cov_0 + mu_0 mu_0^T with alpha = N / (N + tau), and the mean between the
+    sample mean and mu_0."""
+    rng = np.random.default_rng(22)
+    N, D, tau = 300, 4, 50
+    x = rng.normal(1.0, 1.5, (N, D)).astype(np.float32)
+    xd = x.astype(np.float64)
+    r, c = np.triu_indices(D)
+    n = np.array([float(N)])
+    F = xd.sum(0)[None]
+    S = (xd.T @ xd)[r, c][None]
+    w0 = np.ones(1, np.float32)
+    mu0 = rng.normal(0, 1, (1, D)).astype(np.float32)
+    A = rng.normal(size=(D, D))
+    cov0 = (A @ A.T / D + np.eye(D)).astype(np.float32)[None]
+    cov0 = 0.5 * (cov0 + cov0.transpose(0, 2, 1))
+    w, mu, cov = orc.update_map_full(w0, mu0, cov0, n, F, S, N, tau, upmean=True, upvar=True)
+    a = N / (N + tau)
+    m = a * xd.mean(0) + (1 - a) * mu0[0].astype(np.float64)
+    assert np.allclose(mu[0], m, rtol=1e-6, atol=1e-6)
+    second = a * xd.T @ xd / N + (1 - a) * (cov0[0].astype(np.float64) + np.outer(mu0[0], mu0[0]))
+    assert np.allclose(cov[0] + np.outer(mu[0], mu[0]), second, rtol=2e-6, atol=2e-6)
+    assert np.array_equal(cov[0], cov[0].T)
+    # a planted mistake: the UBM prior without its mean term
+    bad = a * xd.T @ xd / N + (1 - a) * cov0[0] - np.outer(m, m)
+    assert np.abs(cov[0] - bad).max() > 1e-2
+
+
+def test_update_map_full_limits():
+    """tau -> infinity keeps the UBM; tau = 1 with large counts gives the EM
+    update's means and covariances."""
+    x, w, mu, cov, n, F, S = _full_stats(23)
+    N = x.shape[0]
+    bw, bmu, bcov = orc.update_map_full(w, mu, cov, n, F, S, N, 10 ** 9, upweight=True,
+                                        upmean=True, upvar=True)
+    assert np.allclose(bmu, mu, atol=1e-5) and np.allclose(bcov, cov, atol=1e-5)
+    assert np.allclose(bw, w, atol=1e-6)
+    big = 1e7  # scale the counts: alpha = 1 - 1e-7-ish
+    ew, emu, ecov, _ = orc.update_em_full(w, mu, cov, n * big, F * big, S * big, N * big)
+    aw, amu, acov = orc.update_map_full(w, mu, cov, n * big, F * big, S * big, N * big, 1,
+                                        upweight=True, upmean=True, upvar=True)
+    assert np.allclose(amu, emu, atol=1e-5) and np.allclose(acov, ecov, atol=1e-5)
+    assert np.allclose(aw, ew, atol=1e-6)
