@@ -21,7 +21,7 @@ EXPORTS = [
     "gmmhip_default_opts", "gmmhip_device_count", "gmmhip_create",
     "gmmhip_destroy", "gmmhip_set_opts", "gmmhip_last_error",
     "gmmhip_set_model", "gmmhip_set_model_full", "gmmhip_set_frames", "gmmhip_set_frames_contig",
-    "gmmhip_set_frames_device", "gmmhip_accumulate", "gmmhip_loglikes",
+    "gmmhip_set_frames_device", "gmmhip_accumulate", "gmmhip_accumulate_aux", "gmmhip_loglikes",
     "gmmhip_loglikes_device", "gmmhip_stream", "gmmhip_sync",
     "gmmhip_kernel_times", "gmmhip_reset_kernel_times", "gmmhip_batch_adapt",
     "gmmhip_server_create", "gmmhip_server_submit", "gmmhip_server_wait",
@@ -35,7 +35,8 @@ KT_LOGLIK, KT_STATS, KT_REDUCE, KT_COUNT = 0, 1, 2, 3
 EXTRA_EXPORTS = [
     "gmmhost_model_load", "gmmhost_model_save", "gmmhost_model_create",
     "gmmhost_model_create_full", "gmmhost_model_full", "gmmhost_model_dims", "gmmhost_model_get", "gmmhost_model_rearrange",
-    "gmmhost_model_free", "gmmhost_update_em", "gmmhost_update_map",
+    "gmmhost_model_free", "gmmhost_update_em", "gmmhost_update_em_robust",
+    "gmmhost_make_robust_stats", "gmmhost_update_map",
     "gmmhost_mllr_transform", "gmmhost_mllr_apply", "gmmhost_last_error",
 ]
 
@@ -118,6 +119,7 @@ def lib():
         L.gmmhip_set_frames_device.argtypes = [c_vp, c_vp, c_u64, ctypes.c_int, c_u64]
         L.gmmhip_accumulate.argtypes = [c_vp, ctypes.c_int, c_dp, c_dp, c_dp, c_dp,
                                         ctypes.POINTER(c_u64)]
+        L.gmmhip_accumulate_aux.argtypes = [c_vp, c_dp]
         L.gmmhip_loglikes.argtypes = [c_vp, c_fp]
         L.gmmhip_loglikes_device.argtypes = [c_vp, c_vp]
         L.gmmhip_stream.restype = c_vp
@@ -276,6 +278,13 @@ class Accumulator:
         self._check(lib().gmmhip_accumulate(self._h, flags, _dp(n), _dp(F), _dp(S),
                                             ctypes.byref(tot), ctypes.byref(nf)))
         return n, F, S, tot.value, nf.value
+
+    def accumulate_aux(self, into=None):
+        """Robust auxiliary statistics [M, D+2] = [aux (D) | aux2 | aux3] of
+        the resident frames; adds into `into` if given."""
+        aux = np.zeros((self.M, self.D + 2)) if into is None else into
+        self._check(lib().gmmhip_accumulate_aux(self._h, _dp(aux)))
+        return aux
 
     def loglikes(self):
         out = np.zeros(self.N, np.float32)

@@ -21,6 +21,7 @@ using namespace gmmhip;
 struct gmmhip_batch_state;  // batched per-utterance buffers (batch_api.hip)
 namespace gmmhip {
 struct FullState;           // full-covariance model and buffers (fullcov.hip)
+struct AuxState;            // robust auxiliary statistics buffers (auxstats.hip)
 }
 
 struct gmmhip_ctx {
@@ -89,6 +90,7 @@ struct gmmhip_ctx {
     // unused while full_model is set
     gmmhip::FullState *full = nullptr;
     bool full_model = false;
+    gmmhip::AuxState *aux = nullptr;
 
     // batched per-utterance state (batch_api.hip)
     gmmhip_batch_state *batch = nullptr;
@@ -166,6 +168,7 @@ void batch_state_free(gmmhip_batch_state *b);
 // log-likelihoods into device memory, statistics accumulation with the
 // packed upper-triangle second-order statistics.
 void full_state_free(gmmhip::FullState *fs);
+void aux_state_free(gmmhip::AuxState *a);
 int full_loglikes_device(gmmhip_ctx *h, float *d_out);
 int full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S, double *tot_ll,
                     uint64_t *nframes);

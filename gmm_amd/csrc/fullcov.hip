@@ -19,45 +19,9 @@
 // chunk's frames), so it runs as S independent GEMMs over frame sub-chunks
 // (one strided-batched call) to fill the chip; the S partials and the chunks
 // are summed into FP64 accumulators in a fixed order.
-#include "ctx.h"
-
-#include <rocblas/rocblas.h>
+#include "gemm_path.h"
 
 namespace gmmhip {
-
-struct FullState {
-    rocblas_handle blas = nullptr;
-    int K = 0, KP = 0;           // features, padded to a multiple of 8
-    float *d_theta = nullptr;    // [M][KP]
-    size_t theta_cap = 0;
-    int *d_pairs = nullptr;      // [K] (i << 16) | j for feature k (i, j < D; see fc_pairs)
-    size_t pairs_cap = 0;
-    float *d_cen = nullptr;      // [D]
-    size_t cen_cap = 0;
-    float *d_phi = nullptr;      // [nc][KP]
-    size_t phi_cap = 0;
-    float *d_L = nullptr;        // [nc][M]: log-likelihoods, then posteriors
-    size_t L_cap = 0;
-    uint8_t *d_bad = nullptr;    // [nc]
-    size_t bad_cap = 0;
-    float *d_part = nullptr;     // [S][M][KP] FP32 statistics of one chunk's sub-chunks
-    size_t part_cap = 0;
-    double *d_acc = nullptr;     // [M][KP] FP64 statistics over chunks
-    size_t acc_cap = 0;
-    std::vector<float> h_cen;
-};
-
-// Feature k of phi: 0 -> 1; 1 + d -> x'_d; else the (i, j) pair (i <= j) in
-// packed upper order, matching the reference's full second-order statistics
-// (trainer/OL_GMMStatsEstimator.cpp:1358-1364).
-static std::vector<int> fc_pairs(int D) {
-    std::vector<int> p;
-    p.push_back(-1);
-    for (int d = 0; d < D; d++) p.push_back(-2 - d);
-    for (int i = 0; i < D; i++)
-        for (int j = i; j < D; j++) p.push_back((i << 16) | j);
-    return p;
-}
 
 // phi rows of frames [n0, n0 + nc) (rows up to ncp zero): one block of 256
 // threads per 8 frames;
@@ -113,12 +77,12 @@ __global__ __launch_bounds__(256) void fc_features_kernel(const float *__restric
 // One wave per frame: log-sum-exp of the frame's M component
 // log-likelihoods (exact, max-shifted), the clamp rules of logLikeKernel
 // (LL <= min_ll or non-finite -> min_ll, posterior 0), posteriors
-// g = exp(l - LL) cut at g <= min_gamma, written in place.  LLnat[n0 + t]
+// g = exp(l - LL) (with cut: set to 0 when g <= min_gamma), written in place.  LLnat[n0 + t]
 // (optional) receives the frame's log-likelihood; blockLL[block] the FP64
 // sum of the block's four frames.
 __global__ __launch_bounds__(256) void fc_lse_kernel(float *__restrict__ L, int nc, int ncp, int M,
                                                      const uint8_t *__restrict__ bad,
-                                                     float min_ll, float min_gamma,
+                                                     float min_ll, float min_gamma, int cut,
                                                      float *__restrict__ LLnat, uint64_t n0,
                                                      double *__restrict__ blockLL) {
     __shared__ double red[4];
@@ -150,7 +114,7 @@ __global__ __launch_bounds__(256) void fc_lse_kernel(float *__restrict__ L, int 
             float g = 0.0f;
             if (!clamp) {
                 g = (M == 1) ? 1.0f : expf(row[m] - ll);
-                g = (g > min_gamma) ? g : 0.0f;
+                if (cut) g = (g > min_gamma) ? g : 0.0f;
             }
             row[m] = g;
         }
@@ -177,18 +141,21 @@ __global__ __launch_bounds__(256) void fc_acc_kernel(const float *__restrict__ p
 
 using namespace gmmhip_detail;
 
-namespace {
+namespace gmmhip_detail {
 
-#define BLASCHK(h, expr)                                                                \
-    do {                                                                                \
-        rocblas_status s_ = (expr);                                                     \
-        if (s_ != rocblas_status_success)                                               \
-            return fail((h), s_ == rocblas_status_memory_error ? GMMHIP_ERR_OOM         \
-                                                               : GMMHIP_ERR_HIP,        \
-                        "%s failed: %s", #expr, rocblas_status_to_string(s_));          \
-    } while (0)
+std::vector<int> feature_codes(int D, bool full) {
+    // feature order of the reference's full second-order statistics
+    // (trainer/OL_GMMStatsEstimator.cpp:1358-1364) for full models
+    std::vector<int> p;
+    p.push_back(-1);
+    for (int d = 0; d < D; d++) p.push_back(-2 - d);
+    for (int i = 0; i < D; i++)
+        for (int j = i; j < D; j++)
+            if (full || j == i) p.push_back((i << 16) | j);
+    return p;
+}
 
-int full_state(gmmhip_ctx *h, gmmhip::FullState **out) {
+int gemm_state(gmmhip_ctx *h, gmmhip::FullState **out) {
     if (!h->full) {
         h->full = new gmmhip::FullState();
         DevGuard g(h->dev);
@@ -201,6 +168,33 @@ int full_state(gmmhip_ctx *h, gmmhip::FullState **out) {
     *out = h->full;
     return GMMHIP_OK;
 }
+
+hipError_t launch_fc_features(const float *X, uint64_t n0, int nc, int ncp, int D, int K, int KP,
+                              const float *cen, const int *codes, float *phi, uint8_t *bad,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(gmmhip::fc_features_kernel, dim3((ncp + 7) / 8), dim3(256), 0, s, X, n0, nc,
+                       ncp, D, K, KP, cen, codes, phi, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_fc_lse(float *L, int nc, int ncp, int M, const uint8_t *bad, float min_ll,
+                         float min_gamma, int cut, float *LLnat, uint64_t n0, double *blockLL,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(gmmhip::fc_lse_kernel, dim3((ncp + 3) / 4), dim3(256), 0, s, L, nc, ncp, M,
+                       bad, min_ll, min_gamma, cut, LLnat, n0, blockLL);
+    return hipGetLastError();
+}
+
+hipError_t launch_fc_acc(const float *part, size_t E, int S, int first, double *acc,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(gmmhip::fc_acc_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, s,
+                       part, E, S, first, acc);
+    return hipGetLastError();
+}
+
+}  // namespace gmmhip_detail
+
+namespace {
 
 // Frames per chunk: the feature rows and the log-likelihood rows of a chunk
 // take at most ~512 MB;
@@ -246,18 +240,14 @@ int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot) {
         const int nc = (int)std::min<uint64_t>(nc_max, h->N - n0);
         const int sub = (nc + S - 1) / S;  // frames per sub-chunk
         const int ncp = sub * S;
-        hipLaunchKernelGGL(gmmhip::fc_features_kernel, dim3((ncp + 7) / 8), dim3(256), 0, h->stream,
-                           h->d_x, n0, nc, ncp, D, K, KP, fs->d_cen, fs->d_pairs, fs->d_phi,
-                           fs->d_bad);
-        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, launch_fc_features(h->d_x, n0, nc, ncp, D, K, KP, fs->d_cen, fs->d_pairs,
+                                     fs->d_phi, fs->d_bad, h->stream));
         // L (column-major M x nc, frame-major rows) = Theta^T (M x KP) * Phi (KP x nc)
         BLASCHK(h, rocblas_sgemm(fs->blas, rocblas_operation_transpose, rocblas_operation_none, M,
                                  nc, KP, &one, fs->d_theta, KP, fs->d_phi, KP, &zero, fs->d_L, M));
         const int blocks = (ncp + 3) / 4;
-        hipLaunchKernelGGL(gmmhip::fc_lse_kernel, dim3(blocks), dim3(256), 0, h->stream, fs->d_L, nc,
-                           ncp, M, fs->d_bad, h->opt.min_loglike, h->opt.min_gamma, LLnat, n0,
-                           h->d_blockLL + nblk);
-        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, launch_fc_lse(fs->d_L, nc, ncp, M, fs->d_bad, h->opt.min_loglike,
+                                h->opt.min_gamma, 1, LLnat, n0, h->d_blockLL + nblk, h->stream));
         nblk += blocks;
         if (stats) {
             // part[s] (column-major KP x M) = Phi_s (KP x sub) * Gamma_s^T (sub x M), frames
@@ -268,9 +258,7 @@ int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot) {
                            (rocblas_stride)sub * M, &zero, fs->d_part, KP, (rocblas_stride)KP * M,
                            S));
             const size_t E = (size_t)M * KP;
-            hipLaunchKernelGGL(gmmhip::fc_acc_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0,
-                               h->stream, fs->d_part, E, S, first, fs->d_acc);
-            HIPCHK(h, hipGetLastError());
+            HIPCHK(h, launch_fc_acc(fs->d_part, E, S, first, fs->d_acc, h->stream));
             first = 0;
         }
     }
@@ -360,7 +348,7 @@ int gmmhip_set_model_full(gmmhip_t h, int M, int D, const float *w, const float 
                         "set_model_full: component %d has weight %g; insert only the active "
                         "(weight > 0) components in RearrangeMixtures order", m, (double)w[m]);
     gmmhip::FullState *fs = nullptr;
-    int rc = full_state(h, &fs);
+    int rc = gemm_state(h, &fs);
     if (rc) return rc;
     const int K = 1 + D + D * (D + 1) / 2;
     const int KP = (K + 7) / 8 * 8;
@@ -428,7 +416,7 @@ int gmmhip_set_model_full(gmmhip_t h, int M, int D, const float *w, const float 
             for (int j = i; j < D; j++, k++)
                 th[k] = (float)((i == j ? -0.5 : -1.0) * P[(size_t)i * D + j]);
     }
-    const std::vector<int> pairs = fc_pairs(D);
+    const std::vector<int> pairs = feature_codes(D, true);
     DevGuard g(h->dev);
     HIPCHK(h, ensure(&fs->d_theta, &fs->theta_cap, theta.size()));
     HIPCHK(h, ensure(&fs->d_pairs, &fs->pairs_cap, pairs.size()));

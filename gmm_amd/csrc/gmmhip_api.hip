@@ -343,6 +343,7 @@ int gmmhip_destroy(gmmhip_t h) {
     hipFree(h->d_zscale);
     hipFree(h->d_P16);
     batch_state_free(h->batch);
+    aux_state_free(h->aux);
     full_state_free(h->full);
     hipFree(h->d_mu_f32);
     hipFree(h->d_var_f32);

@@ -81,14 +81,16 @@ public:
         select(n);
     }
 
+    // auxStats: the robust auxiliary statistics (accumulateAuxStats) are
+    // added too, into statistics allocated with them.
     void setAccFlags(bool mixProbsOnly, bool meanStats, bool varStats, bool varFull = false,
                      bool auxStats = false) {
-        if (auxStats) throw std::runtime_error("setAccFlags(): robust (aux) statistics are not supported");
         flagsSet_ = true;
         fFullVar_ = varFull;
         fMixProbOnly_ = mixProbsOnly;
         fMean_ = meanStats && !mixProbsOnly;
         fVar_ = varStats && !mixProbsOnly;
+        fAux_ = auxStats && !mixProbsOnly;
     }
     void resetAccFlags() { flagsSet_ = false; }
     void setSameDataFlag() { sameData_ = true; }
@@ -101,14 +103,15 @@ public:
             fMixProbOnly_ = false;
             fMean_ = gmms.isAllocated() ? gmms.hasMeans() : true;
             fVar_ = gmms.isAllocated() ? gmms.hasVars() : false;
+            fAux_ = gmms.isAllocated() ? gmms.hasAux() : false;
         }
         if (!gmms.isAllocated()) {
-            gmms.alloc(p.D, p.Malloc, fMean_, fVar_, p.full);
+            gmms.alloc(p.D, p.Malloc, fMean_, fVar_, p.full, fAux_);
             gmms.reset();
         }
         if ((int)dim != p.D || gmms.getDim() != dim)
             throw std::logic_error("accumulateStatsMT(): Inconsistent feature & statistic dimensions!");
-        if ((fMean_ && !gmms.hasMeans()) || (fVar_ && !gmms.hasVars()))
+        if ((fMean_ && !gmms.hasMeans()) || (fVar_ && !gmms.hasVars()) || (fAux_ && !gmms.hasAux()))
             throw std::logic_error("accumulateStatsMT(): statistics not allocated for the requested flags");
         // a full-covariance model produces full second-order statistics
         // (the diagonal ones would be their diagonal; not offered)
@@ -131,6 +134,13 @@ public:
             gn[m] += (TS)n[m];
             for (int d = 0; d < p.D && fMean_; d++) gF[(size_t)m * p.D + d] += (TS)F[(size_t)m * p.D + d];
             for (size_t d = 0; d < vs && fVar_; d++) gS[(size_t)m * vs + d] += (TS)S[(size_t)m * vs + d];
+        }
+        if (fAux_) {
+            const size_t W = (size_t)p.D + 2;
+            std::vector<double> aux((size_t)p.M * W, 0.0);
+            check(gmmhip_accumulate_aux(h_, aux.data()));
+            TS *gA = gmms.auxData();
+            for (size_t i = 0; i < aux.size(); i++) gA[i] += (TS)aux[i];
         }
         gmms._totalAccSamples += nf;
         gmms._totLogLike += (TS)tot;
@@ -165,7 +175,8 @@ private:
     std::string type_ = "GPU HIP";
     std::vector<Packed> group_;
     size_t cur_ = 0;
-    bool flagsSet_ = false, fMixProbOnly_ = false, fMean_ = true, fVar_ = false, fFullVar_ = false;
+    bool flagsSet_ = false, fMixProbOnly_ = false, fMean_ = true, fVar_ = false, fFullVar_ = false,
+         fAux_ = false;
     bool sameData_ = false;
     bool haveFrames_ = false;
 

@@ -4,6 +4,9 @@
 // (diagonal) or the packed upper triangle of sum g x x^T (full: D(D+1)/2
 // per component, row k holding columns k..D-1, OL_GMMStatsEstimator.cpp:
 // 1358-1364),
+// the robust auxiliary statistics when allocated (per component
+// [aux (D) | aux2 | aux3]: sum h |x_t - x_t-1|, sum sqrt g, sum h over frame
+// pairs; OL_GMMStats.h:53-55),
 // plus the number of accumulated frames and the total log-likelihood.
 // Estimators ADD into it (the caller resets), operator+= is the reduction
 // (OL_GMMStats.cpp:99-142), and save()/load() keep the reference's TXT/BIN
@@ -25,28 +28,31 @@ class GMMStats {
 public:
     GMMStats() = default;
     GMMStats(unsigned int dim, unsigned int nummix, bool allocMeans = true, bool allocVars = false,
-             bool allocFullVars = false) {
-        alloc(dim, nummix, allocMeans, allocVars, allocFullVars);
+             bool allocFullVars = false, bool allocAux = false) {
+        alloc(dim, nummix, allocMeans, allocVars, allocFullVars, allocAux);
     }
 
     // alloc does not initialise values -> reset()
     void alloc(unsigned int dim, unsigned int nummix, bool allocMeans = true,
-               bool allocVars = false, bool allocFullVars = false) {
+               bool allocVars = false, bool allocFullVars = false, bool allocAux = false) {
         dim_ = dim;
         M_ = nummix;
         allM_ = allocMeans;
         allV_ = allocVars;
         allFV_ = allocVars && allocFullVars;
+        allA_ = allocAux;
         varsize_ = allFV_ ? dim * (dim + 1) / 2 : dim;
         n_.assign(nummix, T(0));
         F_.assign(allocMeans ? (size_t)nummix * dim : 0, T(0));
         S_.assign(allocVars ? (size_t)nummix * varsize_ : 0, T(0));
+        A_.assign(allocAux ? (size_t)nummix * (dim + 2) : 0, T(0));
         allocated_ = true;
     }
     void reset() {
         std::fill(n_.begin(), n_.end(), T(0));
         std::fill(F_.begin(), F_.end(), T(0));
         std::fill(S_.begin(), S_.end(), T(0));
+        std::fill(A_.begin(), A_.end(), T(0));
         _totalAccSamples = 0;
         _totLogLike = T(0);
     }
@@ -54,6 +60,7 @@ public:
     bool hasMeans() const { return allM_; }
     bool hasVars() const { return allV_; }
     bool hasFullCov() const { return allFV_; }
+    bool hasAux() const { return allA_; }
     unsigned int getVarSize() const { return varsize_; }
     unsigned int getMixNum() const { return M_; }
     unsigned int getDim() const { return dim_; }
@@ -62,6 +69,12 @@ public:
     T getProb(unsigned int m) const { return n_[m]; }
     const T *getMean(unsigned int m) const { return allM_ ? &F_[(size_t)m * dim_] : nullptr; }
     const T *getVar(unsigned int m) const { return allV_ ? &S_[(size_t)m * varsize_] : nullptr; }
+
+    // robust statistics of component m: aux[D], then aux2, aux3
+    const T *getAux(unsigned int m) const { return allA_ ? &A_[(size_t)m * (dim_ + 2)] : nullptr; }
+    T getAux2(unsigned int m) const { return A_[(size_t)m * (dim_ + 2) + dim_]; }
+    T getAux3(unsigned int m) const { return A_[(size_t)m * (dim_ + 2) + dim_ + 1]; }
+    T *auxData() { return allA_ ? A_.data() : nullptr; }
 
     T *probData() { return n_.data(); }
     T *meanData() { return allM_ ? F_.data() : nullptr; }
@@ -80,6 +93,8 @@ public:
             for (size_t i = 0; i < F_.size(); i++) F_[i] += r.F_[i];
         if (allV_ && r.allV_)
             for (size_t i = 0; i < S_.size(); i++) S_[i] += r.S_[i];
+        if (allA_ && r.allA_)
+            for (size_t i = 0; i < A_.size(); i++) A_[i] += r.A_[i];
         _totalAccSamples += r._totalAccSamples;
         _totLogLike += r._totLogLike;
         return *this;
@@ -154,13 +169,28 @@ public:
         if (!f) throw std::runtime_error(std::string("GMMStats::load: truncated ") + filename);
     }
 
+    // saveAux (OL_GMMStats.cpp:778-812): text, "dim nummix", a blank line,
+    // then per component aux2, aux3 and the row of aux.
+    void saveAux(const char *filename) const {
+        std::ofstream f(filename);
+        if (!f) throw std::runtime_error(std::string("saveAux(): Unable to open file ") + filename);
+        f << dim_ << " " << M_ << "\n\n";
+        if (allA_)
+            for (unsigned m = 0; m < M_; m++) {
+                f << getAux2(m) << "\n" << getAux3(m) << "\n";
+                for (unsigned d = 0; d < dim_; d++) f << " " << A_[(size_t)m * (dim_ + 2) + d];
+                f << "\n";
+            }
+        if (!f) throw std::runtime_error(std::string("saveAux(): Exception writing file ") + filename);
+    }
+
     uint64_t _totalAccSamples = 0;
     T _totLogLike = T(0);
 
 private:
-    bool allocated_ = false, allM_ = false, allV_ = false, allFV_ = false;
+    bool allocated_ = false, allM_ = false, allV_ = false, allFV_ = false, allA_ = false;
     unsigned int dim_ = 0, M_ = 0, varsize_ = 0;
-    std::vector<T> n_, F_, S_;
+    std::vector<T> n_, F_, S_, A_;
 };
 
 }  // namespace gmm

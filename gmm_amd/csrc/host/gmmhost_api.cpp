@@ -20,12 +20,13 @@ int fail(const std::string &msg, int code = 1) {
 
 // Statistics wrapper for the update rules (GMMStats<double> view).
 gmm::GMMStats<double> make_stats(int M, int D, const double *n, const double *F, const double *S,
-                                 uint64_t tot, bool full = false) {
-    gmm::GMMStats<double> st(D, M, F != nullptr, S != nullptr, full);
+                                 uint64_t tot, bool full = false, const double *aux = nullptr) {
+    gmm::GMMStats<double> st(D, M, F != nullptr, S != nullptr, full, aux != nullptr);
     st.reset();
     std::memcpy(st.probData(), n, sizeof(double) * M);
     if (F) std::memcpy(st.meanData(), F, sizeof(double) * M * D);
     if (S) std::memcpy(st.varData(), S, sizeof(double) * M * st.getVarSize());
+    if (aux) std::memcpy(st.auxData(), aux, sizeof(double) * M * (D + 2));
     st._totalAccSamples = tot;
     return st;
 }
@@ -120,6 +121,41 @@ int gmmhost_update_em(gmmhost_model_t m, const double *n, const double *F, const
         ad.SetOptions(o);
         const unsigned l = ad.UpdateModelEM(&m->m, st, upweight != 0, upmean != 0, upvar != 0);
         if (left) *left = (int)l;
+    } catch (std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int gmmhost_update_em_robust(gmmhost_model_t m, const double *n, const double *F, const double *S,
+                             const double *aux, uint64_t tot_samples, float min_var, int upweight,
+                             int upmean, int upvar, int *left) {
+    if (!m || !n || !F || !S || !aux) return fail("null argument");
+    try {
+        const int M = m->m.GetNumberOfMixtures(), D = m->m.GetDimension();
+        gmm::GMMStats<double> st =
+            make_stats(M, D, n, F, S, tot_samples, m->m.GetFullCovStatus(), aux);
+        RuleHost ad;
+        gmm::CModelAdapt::OPTIONS o;
+        o._min_var = min_var;
+        o._robustCov = true;
+        ad.SetOptions(o);
+        const unsigned l = ad.UpdateModelEM(&m->m, st, upweight != 0, upmean != 0, upvar != 0);
+        if (left) *left = (int)l;
+    } catch (std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
+int gmmhost_make_robust_stats(int M, int D, int full, const double *n, const double *F, double *S,
+                              const double *aux, double *nind) {
+    if (M <= 0 || D <= 0 || !n || !F || !S || !aux) return fail("bad argument");
+    try {
+        gmm::GMMStats<double> st = make_stats(M, D, n, F, S, 0, full != 0, aux);
+        const std::vector<double> ni = gmm::CModelAdapt::MakeRobustStats(st);
+        std::memcpy(S, st.varData(), sizeof(double) * M * st.getVarSize());
+        if (nind) std::memcpy(nind, ni.data(), sizeof(double) * M);
     } catch (std::exception &e) {
         return fail(e.what());
     }

@@ -38,11 +38,11 @@ void CModelAdapt::initEstimator() {
 // AccumulateStats (OL_ModelAdapt.cpp:411-466): insert the model (which
 // rearranges it), set the flags and add the statistics of the inserted data.
 void CModelAdapt::AccumulateStats(GMModel *model, GMMStats<double> &stats, bool mixProbsOnly,
-                                  bool meanStats, bool varStats) {
+                                  bool meanStats, bool varStats, bool auxStats) {
     if (!data_ || N_ == 0) throw std::logic_error("AccumulateStats(): Insert data first!");
     initEstimator();
     if (model) est_->insertModel(*model);
-    est_->setAccFlags(mixProbsOnly, meanStats, varStats);
+    est_->setAccFlags(mixProbsOnly, meanStats, varStats, stats.hasFullCov(), auxStats);
     if (!dataChanged_) est_->setSameDataFlag();
     est_->accumulateStatsMT(data_, N_, D_, stats);
     dataChanged_ = false;
@@ -50,13 +50,101 @@ void CModelAdapt::AccumulateStats(GMModel *model, GMMStats<double> &stats, bool 
         std::cout << "[#frames accumulated = " << stats.getTotAccSamples() << "]" << std::endl;
 }
 
-// UpdateModelEM (OL_ModelAdapt.cpp:823-949), non-robust; the full-covariance
-// branch (:906-925) sets cov[k][kk] = cov[kk][k] = (float)(S/n - mean_k *
-// mean_kk) with the product of the float means taken in float and no floor.
+// GetNIndFrames (OL_ModelAdapt.cpp:468-535): per component
+//   MII = (1/D) sum_d aux[d] / aux3 / std_d   (std: global deviation from the
+//         summed statistics)
+//   nind = 1 + (aux2 N_glob / sum aux2 - 1) (MII / 1.12)^3,
+// 0 for a component with n = 0 or aux3 = 0.
+std::vector<double> CModelAdapt::GetNIndFrames(const GMMStats<double> &stats) {
+    const unsigned D = stats.getDim(), M = stats.getMixNum();
+    if (!stats.hasAux() || !stats.hasMeans() || !stats.hasVars())
+        throw std::logic_error("GetNIndFrames(): robust estimation needs auxiliary, first- and "
+                               "second-order statistics");
+    const bool full = stats.hasFullCov();
+    std::vector<double> gstd(D, 0.0), gmean(D, 0.0), nind(M, 0.0);
+    double gprob = 0.0, sumNg = 0.0;
+    for (unsigned m = 0; m < M; m++) {
+        gprob += stats.getProb(m);
+        const double *F = stats.getMean(m), *S = stats.getVar(m);
+        for (unsigned d = 0; d < D; d++) {
+            gmean[d] += F[d];
+            gstd[d] += S[full ? (size_t)d * D + d - (size_t)d * (d + 1) / 2 : d];
+        }
+        sumNg += stats.getAux2(m);
+    }
+    for (unsigned d = 0; d < D; d++) {
+        gmean[d] /= gprob;
+        gstd[d] = std::sqrt(gstd[d] / gprob - gmean[d] * gmean[d]);
+        if (gstd[d] == 0)
+            throw std::runtime_error("GetNIndFrames(): global variance equal zero!");
+    }
+    for (unsigned m = 0; m < M; m++) {
+        const double *a = stats.getAux(m);
+        const double a3 = stats.getAux3(m);
+        if (stats.getProb(m) == 0 || a3 == 0) continue;
+        double v = 0.0;
+        for (unsigned d = 0; d < D; d++) v += a[d] / a3 / gstd[d];
+        v /= D;
+        v = std::pow(v / 1.12, 3);
+        nind[m] = 1 + (stats.getAux2(m) * gprob / sumNg - 1) * v;
+    }
+    return nind;
+}
+
+// getDiagCovMultiplier / getOffDiagCovMultiplier (OL_ModelAdapt.cpp:537-548).
+float CModelAdapt::getDiagCovMultiplier(double nind) {
+    if (nind < 2) return (float)std::pow(std::pow(1.0 + 1.0 / (2.0 - 1.25), 3.5), 3 - nind);
+    return (float)std::pow(1.0 + 1.0 / (nind - 1.25), 3.5);
+}
+
+float CModelAdapt::getOffDiagCovMultiplier(int D, double nind) {
+    if (nind < 2) return 0.0f;
+    return (float)(1 - std::pow(((double)D / ((double)D - 1 + nind)), 1.4));
+}
+
+// MakeRobustStats (OL_ModelAdapt.cpp:550-588): S <- dc S + (1 - dc) F_k F_kk / n
+// on the diagonal (dc = diagonal multiplier x nind / (nind - 1), float) and
+// with the off-diagonal multiplier on the off-diagonal entries; (1 - dc) is
+// a float difference, as in the reference.  Returns the independent-frame
+// counts.
+std::vector<double> CModelAdapt::MakeRobustStats(GMMStats<double> &stats) {
+    std::vector<double> nind = GetNIndFrames(stats);
+    const int D = (int)stats.getDim();
+    const size_t VS = stats.getVarSize();
+    for (unsigned j = 0; j < stats.getMixNum(); j++) {
+        float dc = getDiagCovMultiplier(nind[j]);
+        dc = (float)((double)dc * (nind[j] / (nind[j] - 1)));
+        const double *F = stats.getMean(j);
+        const double n = stats.getProb(j);
+        double *S = stats.varData() + (size_t)j * VS;
+        if (stats.hasFullCov()) {
+            const float odc = getOffDiagCovMultiplier(D, nind[j]);
+            size_t p = 0;
+            for (int k = 0; k < D; k++)
+                for (int kk = k; kk < D; kk++, p++) {
+                    if (kk == k)
+                        S[p] = dc * S[p] + (double)(1.0f - dc) * std::pow(F[k], 2.0) / n;
+                    else
+                        S[p] = odc * S[p] + (double)(1.0f - odc) * F[k] * F[kk] / n;
+                }
+        } else {
+            for (int k = 0; k < D; k++) S[k] = dc * S[k] + (double)(1.0f - dc) * std::pow(F[k], 2.0) / n;
+        }
+    }
+    return nind;
+}
+
+// UpdateModelEM (OL_ModelAdapt.cpp:823-949); with _robustCov the statistics
+// are first made robust and the independent-frame counts replace the
+// occupancies in the minimum-data test.  The full-covariance branch
+// (:906-925) sets cov[k][kk] = cov[kk][k] = (float)(S/n - mean_k * mean_kk)
+// with the product of the float means taken in float and no floor.
 unsigned int CModelAdapt::UpdateModelEM(GMModel *model, GMMStats<double> &stats, bool upweight,
                                         bool upmean, bool upvar) {
     const int D = model->GetDimension();
     const int nummix = std::min<int>(model->GetNumberOfMixtures(), (int)stats.getMixNum());
+    std::vector<double> nind;
+    if (opt_._robustCov) nind = MakeRobustStats(stats);
     std::vector<float> mean(D);
     bool wchange = false;
     unsigned int nzero = 0;
@@ -68,7 +156,8 @@ unsigned int CModelAdapt::UpdateModelEM(GMModel *model, GMMStats<double> &stats,
         }
         const double np = stats.getProb(j);
         if (opt_._saveMixOcc) model->SetMixOcc(j, (float)np);
-        if (np < MIN_IND_SAMPLES) {
+        const double ni = opt_._robustCov ? nind[j] : np;
+        if (ni < MIN_IND_SAMPLES) {
             model->ResetMixture(j);
             model->SetMixWeight(j, 0.0f);
             wchange = true;
@@ -88,6 +177,12 @@ unsigned int CModelAdapt::UpdateModelEM(GMModel *model, GMMStats<double> &stats,
             if (upvar && model->GetFullCovStatus()) {
                 if (!stats.hasFullCov())
                     throw std::logic_error("UpdateModelEM(): full-covariance model needs full statistics");
+                if (!opt_._robustCov && ni < D) {
+                    std::cerr << " [WARNING: covariance matrix for Gaussian with idx=" << j
+                              << " is likely to be ill conditioned (not enough data!)]" << std::endl;
+                    std::cerr << " [         try to turn on --robust when working with "
+                                 "full-covariance matrices]" << std::endl;
+                }
                 float *cov = model->GetMixtureVar(j);
                 size_t p = 0;
                 for (int k = 0; k < D; k++)
@@ -128,6 +223,7 @@ void CModelAdapt::UpdateModelMAP(GMModel *model, GMModel *UBM) {
         }
         return;
     }
+    if (opt_._robustCov) MakeRobustStats(stats_);  // :1037-1040
     const int nummix = model->GetNumberOfMixtures();
     const int D = model->GetDimension();
     const bool full = stats_.hasVars() && stats_.hasFullCov();
@@ -184,7 +280,8 @@ void CModelAdapt::IterEM(GMModel *model, int iters) {
     const bool weightOnly = opt_._AdaptWeightBool && !opt_._AdaptMeanBool && !opt_._AdaptVarBool;
     for (int i = 0; i < iters; i++) {
         stats_.reset();
-        AccumulateStats(model, stats_, weightOnly, opt_._AdaptMeanBool, opt_._AdaptVarBool);
+        AccumulateStats(model, stats_, weightOnly, opt_._AdaptMeanBool, opt_._AdaptVarBool,
+                        opt_._robustCov);
         GMModel tmp(*model);
         UpdateModelMAP(model, &tmp);
     }
@@ -194,7 +291,8 @@ void CModelAdapt::IterMAP(GMModel *model, int iters) {
     const bool weightOnly = opt_._AdaptWeightBool && !opt_._AdaptMeanBool && !opt_._AdaptVarBool;
     for (int i = 0; i < iters; i++) {
         stats_.reset();
-        AccumulateStats(model, stats_, weightOnly, opt_._AdaptMeanBool, opt_._AdaptVarBool);
+        AccumulateStats(model, stats_, weightOnly, opt_._AdaptMeanBool, opt_._AdaptVarBool,
+                        opt_._robustCov);
         UpdateModelMAP(model, UBM_);
     }
 }
@@ -413,7 +511,8 @@ void CModelAdapt::AdaptModel(GMModel *UBM) {
     W_.reset();
     switch (opt_._adaptType) {
     case MAP:
-        stats_.alloc(D, nummix_, opt_._AdaptMeanBool || opt_._AdaptVarBool, opt_._AdaptVarBool, fullV);
+        stats_.alloc(D, nummix_, opt_._AdaptMeanBool || opt_._AdaptVarBool, opt_._AdaptVarBool, fullV,
+                     opt_._robustCov);
         model_ = AdaptModelMAP(UBM);
         break;
     case MLLR: {
@@ -423,7 +522,7 @@ void CModelAdapt::AdaptModel(GMModel *UBM) {
         break;
     }
     case MLLR_MAP: {
-        stats_.alloc(D, nummix_, true, opt_._AdaptVarBool, fullV);
+        stats_.alloc(D, nummix_, true, opt_._AdaptVarBool, fullV, opt_._robustCov);
         MLLRTransform W = IterW_MLLR(UBM, opt_._iterMLLR, false);
         std::unique_ptr<GMModel> mm = AdaptModelMLLR(*UBM, W);
         model_ = AdaptModelMAP(mm.get());
@@ -502,13 +601,14 @@ void CModelAdapt::TrainFromScratch(unsigned int nummix, bool fullcov) {
     if (fullcov)
         for (int d = 0; d < D; d++) v[(size_t)d * D + d] = 1e4f;
     g.SetMixParam(0, z.data(), v.data(), 1.0f);
-    GMMStats<double> gs(D, 1, true, true, fullcov);
+    const bool robust = opt_._robustCov;
+    GMMStats<double> gs(D, 1, true, true, fullcov, robust);
     gs.reset();
     AccumulateStats(&g, gs, false, true, false);
     for (int d = 0; d < D; d++) z[d] = (float)(gs.getMean(0)[d] / gs.getProb(0));
     g.SetMixParam(0, z.data(), v.data(), 1.0f);
     gs.reset();
-    AccumulateStats(&g, gs, false, true, true);
+    AccumulateStats(&g, gs, false, true, true, robust);
     GMModel init(D, 1, fullcov);
     init.SetMixWeight(0, 1.0f);
     if (UpdateModelEM(&init, gs) == 0)
@@ -516,14 +616,14 @@ void CModelAdapt::TrainFromScratch(unsigned int nummix, bool fullcov) {
     model_.reset(new GMModel(D, (int)nummix, fullcov));
     model_->SetMixParam(0, init.GetMixtureMean(0), init.GetMixtureVar(0), init.GetMixtureWeight(0));
     nummix_ = (int)nummix;
-    stats_.alloc(D, nummix, true, true, fullcov);
+    stats_.alloc(D, nummix, true, true, fullcov, robust);
     const unsigned int maxIter = nummix < 2 ? 0 : nummix;
     unsigned int nonzero = 1;
     GMModel *model = model_.get();
     for (unsigned int it = 0; it < maxIter; it++) {
         for (int e = 0; e < opt_._iterCountEM; e++) {
             stats_.reset();
-            AccumulateStats(model, stats_, false, true, true);
+            AccumulateStats(model, stats_, false, true, true, robust);
             if (opt_._verbosity)
                 std::cout << "     [LL = " << stats_.getTotLogLike() / (double)stats_.getTotAccSamples()
                           << "]" << std::endl;

@@ -1,8 +1,8 @@
 // Trainer / adaptation driver over the HIP estimator: the subset of
 // CModelAdapt (trainer/OL_ModelAdapt.h:140-180 of GMMTeam/GMM) on the
 // GMM hot path -- EM from a UBM or from scratch (Gaussian splitting; diagonal
-// or full covariance), MAP adaptation, global MLLR (+ MLLR then MAP) of
-// diagonal models -- with the reference's update rules.  All statistics come
+// or full covariance; optionally robust covariances), MAP adaptation, global
+// MLLR (+ MLLR then MAP) -- with the reference's update rules.  All statistics come
 // from GMMStatsEstimator_HIP.
 #pragma once
 
@@ -47,6 +47,7 @@ public:
         int _cudaGPUid = -1;
         unsigned int _numThreads = 1;
         int _precision = GMMHIP_PREC_FP32;
+        bool _robustCov = false;  // robust covariances from the auxiliary statistics
         std::string _transformFileExt = ".mat";
     };
 
@@ -75,7 +76,15 @@ public:
                                bool upmean = true, bool upvar = true);
     void UpdateModelMAP(GMModel *model, GMModel *UBM);
     void AccumulateStats(GMModel *model, GMMStats<double> &stats, bool mixProbsOnly = false,
-                         bool meanStats = true, bool varStats = false);
+                         bool meanStats = true, bool varStats = false, bool auxStats = false);
+
+    // Robust covariance pieces (OL_ModelAdapt.cpp:468-588): independent
+    // frames per component from the auxiliary statistics, the covariance
+    // multipliers, and the in-place rewrite of the second-order statistics.
+    static std::vector<double> GetNIndFrames(const GMMStats<double> &stats);
+    static float getDiagCovMultiplier(double nind);
+    static float getOffDiagCovMultiplier(int D, double nind);
+    static std::vector<double> MakeRobustStats(GMMStats<double> &stats);
     GMMStats<double> &Stats() { return stats_; }
     void SetNumMix(int M) { nummix_ = M; }
 

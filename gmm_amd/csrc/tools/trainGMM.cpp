@@ -2,7 +2,8 @@
 // Option names follow the reference tool (projects/Model_ADAPT/Model_ADAPT.cpp
 // :278-401, trainer/OL_ModelAdapt.cpp:2032-2135); the hot-path subset is
 // supported: --type 0 (from scratch; --full-cov for full covariances),
-// 1 (MLLR), 2 (MAP/EM), 4 (MLLR+MAP).
+// 1 (MLLR), 2 (MAP/EM), 4 (MLLR+MAP); --robust (-R) for the robust
+// covariance estimate (from scratch and MAP/EM).
 // --SSE/--CUDA are accepted and ignored: statistics always run on the GPU.
 #include <cstdio>
 #include <iostream>
@@ -52,12 +53,13 @@ int main(int argc, char **argv) {
     a.option("numThrd", 0, false, "1");
     a.option("precision", 0, false, "fp32");
     a.option("full-cov", 0, true, "0");
+    a.option("robust", 'R', true, "0");
     a.option("verbosity", 'v', true, "0");
     try {
         a.parse(argc, argv);
         if (a.b("help") || a.all("in-file").empty() || !a.has("outFile")) {
             std::cout << "usage: trainGMM -i FEATS [-i ...] --data-T {0,1,2} -o OUT.gmm [-T] "
-                         "--type {0,1,2,4} [--in-UBM UBM.gmm [-L]] [--nummix K [--full-cov]] [--it-EM n] "
+                         "--type {0,1,2,4} [--in-UBM UBM.gmm [-L]] [--nummix K [--full-cov]] [--robust] [--it-EM n] "
                          "[--it-MAP n] [--tau r] [--mean] [--var] [--weight] [--mllr-init] "
                          "[--it-mllr n] [--apply-trn] [--save-trn] [--precision fp32|fp16x3|bf16x3]\n";
             return a.b("help") ? 0 : 1;
@@ -95,6 +97,7 @@ int main(int argc, char **argv) {
         o._cudaGPUid = a.i("GPU-id");
         o._verbosity = a.i("verbosity");
         o._precision = precision_of(a.str("precision"));
+        o._robustCov = a.b("robust");
         ad.SetOptions(o);
         ad.InsertData(feats.rows.data(), feats.N, feats.D);
         if (o._adaptType == FROM_SCRATCH) {

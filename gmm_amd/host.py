@@ -46,6 +46,11 @@ def _L():
                                          ctypes.c_int]
         L.gmmhost_mllr_transform.argtypes = [vp, vp, _dp, _dp, ctypes.c_float, _dp]
         L.gmmhost_mllr_apply.argtypes = [vp, _dp]
+        L.gmmhost_update_em_robust.argtypes = [vp, _dp, _dp, _dp, _dp, ctypes.c_uint64,
+                                               ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.gmmhost_make_robust_stats.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                                _dp, _dp, _dp]
         _configured = True
     return L
 
@@ -141,6 +146,15 @@ class Model:
                                     int(upweight), int(upmean), int(upvar), ctypes.byref(left)))
         return left.value
 
+    def update_em_robust(self, n, F, S, aux, tot, min_var=1e-6, upweight=True, upmean=True,
+                         upvar=True):
+        n, F, S, aux = _d(n), _d(F), _d(S), _d(aux)
+        left = ctypes.c_int()
+        _chk(_L().gmmhost_update_em_robust(self._h, _pd(n), _pd(F), _pd(S), _pd(aux), int(tot),
+                                           min_var, int(upweight), int(upmean), int(upvar),
+                                           ctypes.byref(left)))
+        return left.value
+
     def update_map(self, ubm, n, F, S, tot, tau, min_var=1e-6, upweight=False, upmean=True,
                    upvar=False):
         n, F, S = _d(n), _d(F), _d(S)
@@ -158,3 +172,14 @@ class Model:
     def mllr_apply(self, W):
         W = _d(W)
         _chk(_L().gmmhost_mllr_apply(self._h, _pd(W)))
+
+
+def make_robust_stats(n, F, S, aux, full=False):
+    """(S', nind): MakeRobustStats on a copy of S."""
+    n, F, aux = _d(n), _d(F), _d(aux)
+    S = _d(S).copy()
+    M, D = F.shape
+    nind = np.zeros(M)
+    _chk(_L().gmmhost_make_robust_stats(M, D, int(full), _pd(n), _pd(F), _pd(S), _pd(aux),
+                                        _pd(nind)))
+    return S, nind

@@ -150,6 +150,24 @@ int gmmhip_set_frames_device(gmmhip_t h, const float *dx, uint64_t N, int D,
 int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S,
                       double *tot_ll, uint64_t *nframes);
 
+/* Robust auxiliary statistics of the resident frames under the inserted
+ * model (diagonal or full), added into aux[M*(D+2)]: per component
+ * [aux_d (D) | aux2 | aux3] with, over every consecutive frame pair
+ * (t-1, t) of the frame set and posteriors g before the cut,
+ *   p_m = g_t,m g_t-1,m,  s = sum_m [p_m > min_gamma] p_m,  h_m = p_m / s,
+ *   aux2_m += sqrt(g_t,m) (g_t,m > min_gamma),
+ *   aux_m[d] += h_m |x_td - x_t-1,d|, aux3_m += h_m  (h_m > min_gamma).
+ * Replaces accumulateAuxStats (trainer/OL_GMMStatsEstimator.cpp:1226-1280,
+ * as accumulateStats drives it, :1174-1189) and the GPU's
+ * normAuxGammasKernel + accAuxStatsKernel + the aux part of getAccStats
+ * (trainer/CU_GMMStatsEstimator_CUDA.cu:922-940, :861-862; same
+ * [M][D+2] layout).  Pairs run over the whole frame set: they do not break
+ * at thread or upload-chunk boundaries as the reference's do, and the first
+ * frame of the set starts the first pair.  Computed in FP32 with FP64
+ * sums whatever opts.precision says.  The statistics n/F/S come from
+ * gmmhip_accumulate, as before. */
+int gmmhip_accumulate_aux(gmmhip_t h, double *aux);
+
 /* Per-frame log-likelihoods of the resident frames into out[N] (host).
  * Replaces compVecsLogLikeMT (trainer/OL_GMMStatsEstimator_CUDA.cpp:333-391)
  * = compLogLikes + getLogLikes (trainer/CU_GMMStatsEstimator_CUDA.cu:606-643),

@@ -48,6 +48,20 @@ void gmmhost_model_free(gmmhost_model_t m);
 int gmmhost_update_em(gmmhost_model_t m, const double *n, const double *F, const double *S,
                       uint64_t tot_samples, float min_var, int upweight, int upmean, int upvar,
                       int *left);
+/* UpdateModelEM with _robustCov (OL_ModelAdapt.cpp:848-857, :880): the
+ * statistics are made robust (MakeRobustStats) from the auxiliary statistics
+ * aux[M*(D+2)] (per component [aux (D) | aux2 | aux3], as gmmhip_accumulate_aux
+ * returns them) and the independent-frame counts replace the occupancies in
+ * the minimum-data test. */
+int gmmhost_update_em_robust(gmmhost_model_t m, const double *n, const double *F, const double *S,
+                             const double *aux, uint64_t tot_samples, float min_var, int upweight,
+                             int upmean, int upvar, int *left);
+/* MakeRobustStats (OL_ModelAdapt.cpp:468-588) on statistics of M components:
+ * S (M*D diagonal, or M*D(D+1)/2 packed upper when full) is rewritten in
+ * place; nind[M] (may be NULL) receives GetNIndFrames' counts.  Fails when a
+ * global variance is zero. */
+int gmmhost_make_robust_stats(int M, int D, int full, const double *n, const double *F, double *S,
+                              const double *aux, double *nind);
 /* UpdateModelMAP (OL_ModelAdapt.cpp:993-1106): m starts as (a copy of) the
  * UBM ubm and is adapted with relevance factor tau (full-covariance models:
  * packed S as for gmmhost_update_em). */

@@ -47,6 +47,8 @@ def lib():
         _lib.orc_min_loglike.restype = ctypes.c_double
         _lib.orc_min_gamma.restype = ctypes.c_double
         _lib.orc_mllr_kg.restype = ctypes.c_double
+        _lib.orc_diag_cov_multiplier.restype = ctypes.c_float
+        _lib.orc_offdiag_cov_multiplier.restype = ctypes.c_float
     return _lib
 
 
@@ -288,3 +290,58 @@ def packed_upper_index(D):
     second-order statistics."""
     r, c = np.triu_indices(D)
     return r, c
+
+
+def accumulate_aux(x, gamma, min_gamma=MIN_GAMMA):
+    """Robust auxiliary statistics [M, D+2] = [aux (D) | aux2 | aux3] over
+    every consecutive frame pair, from the posteriors before the cut."""
+    x = _f32(x)
+    gamma = _f64(gamma)
+    N, D = x.shape
+    M = gamma.shape[1]
+    aux = np.zeros((M, D + 2))
+    lib().orc_accumulate_aux(N, D, M, _p(x), _p(gamma), ctypes.c_double(min_gamma), _p(aux))
+    return aux
+
+
+def nind_frames(n, F, S, aux, full=False):
+    n, F, S, aux = _f64(n), _f64(F), _f64(S), _f64(aux)
+    M, D = F.shape
+    nind = np.zeros(M)
+    rc = lib().orc_nind_frames(M, D, int(full), _p(n), _p(F), _p(S), _p(aux), _p(nind))
+    if rc:
+        raise ValueError("global variance equal zero")
+    return nind
+
+
+def make_robust_stats(n, F, S, aux, full=False):
+    """(S', nind): MakeRobustStats applied to a copy of S."""
+    n, F, aux = _f64(n), _f64(F), _f64(aux)
+    S = _f64(S).copy()
+    M, D = F.shape
+    nind = np.zeros(M)
+    rc = lib().orc_make_robust_stats(M, D, int(full), _p(n), _p(F), _p(S), _p(aux), _p(nind))
+    if rc:
+        raise ValueError("global variance equal zero")
+    return S, nind
+
+
+def diag_cov_multiplier(nind):
+    return lib().orc_diag_cov_multiplier(ctypes.c_double(nind))
+
+
+def offdiag_cov_multiplier(D, nind):
+    return lib().orc_offdiag_cov_multiplier(int(D), ctypes.c_double(nind))
+
+
+def update_em_robust(w, mu, var, n, F, S, aux, tot_samples, min_var=1e-6, full=False):
+    """Robust EM update (weights, means, variances / covariances)."""
+    w, mu, var = _f32(w).copy(), _f32(mu).copy(), _f32(var).copy()
+    M, D = mu.shape
+    n, F, S, aux = _f64(n), _f64(F), _f64(S), _f64(aux)
+    left = lib().orc_update_em_robust(M, D, int(full), _p(w), _p(mu), _p(var), _p(n), _p(F),
+                                      _p(S), _p(aux), ctypes.c_double(tot_samples),
+                                      ctypes.c_double(min_var))
+    if left < 0:
+        raise ValueError("global variance equal zero")
+    return w, mu, var, left

@@ -221,10 +221,26 @@ static void mixture_reset(float *mu_m, float *var_m, int D)
  * w/mu/var are updated in place for the first M components (model order),
  * n/F/S are the statistics, tot_samples = GMMStats::_totalAccSamples.
  * Returns the number of components left with non-zero weight. */
+static int update_em_core(int M, int D, float *w, float *mu, float *var,
+                          const double *n, const double *F, const double *S,
+                          double tot_samples, double min_var, int upweight,
+                          int upmean, int upvar, const double *nind);
+
 int orc_update_em(int M, int D, float *w, float *mu, float *var,
                   const double *n, const double *F, const double *S,
                   double tot_samples, double min_var, int upweight,
                   int upmean, int upvar)
+{
+    return update_em_core(M, D, w, mu, var, n, F, S, tot_samples, min_var, upweight, upmean,
+                          upvar, NULL);
+}
+
+/* The EM update with the minimum-data test on nind[j] (the robust
+ * independent-frame counts) when nind is given, else on n[j]. */
+static int update_em_core(int M, int D, float *w, float *mu, float *var,
+                          const double *n, const double *F, const double *S,
+                          double tot_samples, double min_var, int upweight,
+                          int upmean, int upvar, const double *nind)
 {
     const double MIN_IND_SAMPLES = 2.0; /* OL_ModelAdapt.h:54 */
     int nzero = 0, wchange = 0;
@@ -236,7 +252,7 @@ int orc_update_em(int M, int D, float *w, float *mu, float *var,
             continue;
         }
         double np = n[j];
-        if (np < MIN_IND_SAMPLES) {
+        if ((nind ? nind[j] : np) < MIN_IND_SAMPLES) {
             mixture_reset(mu + (size_t)j * D, var + (size_t)j * D, D);
             w[j] = 0.0f;
             wchange = 1;
@@ -679,9 +695,20 @@ void orc_accumulate_full(int N, int D, int M, const float *x, const float *mu,
  * in float (:917-923); no variance floor.  Components with n < 2 are reset
  * (GMMMixture::Reset, OL_GMModel.cpp:73-83: zero mean, identity covariance)
  * and get weight 0; weights are renormalised over the survivors. */
+static int update_em_full_core(int M, int D, float *w, float *mu, float *cov,
+                               const double *n, const double *F, const double *S,
+                               double tot_samples, const double *nind);
+
 int orc_update_em_full(int M, int D, float *w, float *mu, float *cov,
                        const double *n, const double *F, const double *S,
                        double tot_samples)
+{
+    return update_em_full_core(M, D, w, mu, cov, n, F, S, tot_samples, NULL);
+}
+
+static int update_em_full_core(int M, int D, float *w, float *mu, float *cov,
+                               const double *n, const double *F, const double *S,
+                               double tot_samples, const double *nind)
 {
     const double MIN_IND_SAMPLES = 2.0; /* OL_ModelAdapt.h:54 */
     const size_t DP = (size_t)D * (D + 1) / 2;
@@ -695,7 +722,7 @@ int orc_update_em_full(int M, int D, float *w, float *mu, float *cov,
             continue;
         }
         const double np = n[j];
-        if (np < MIN_IND_SAMPLES) {
+        if ((nind ? nind[j] : np) < MIN_IND_SAMPLES) {
             memset(mu + (size_t)j * D, 0, sizeof(float) * D);
             memset(c, 0, sizeof(float) * (size_t)D * D);
             for (int d = 0; d < D; d++)
@@ -842,4 +869,200 @@ void orc_eig_leading(int D, const float *cov, double *lam, double *v)
         v[i] = sgn * V[(size_t)i * D + best];
     free(A);
     free(V);
+}
+
+/* ------------------------------------------------------------------------
+ * Robust ("aux") statistics and the robust covariance estimate.  Paths
+ * relative to GMM/lib_bmssr/.
+ *
+ * How each function is pinned (tests/test_oracle_robust.py):
+ *   orc_accumulate_aux -- numpy brute force over every consecutive frame
+ *       pair on tiny inputs (with a planted pair-normalisation and a planted
+ *       cut, both caught); M = 1 closed form (aux = sum |x_t - x_{t-1}|,
+ *       aux2 = aux3 = N - 1); time reversal leaves aux and aux3 unchanged.
+ *   orc_nind_frames -- i.i.d. Gaussian frames give about N independent
+ *       frames (the 1.12 ~ 2/sqrt(pi) normalisation of the mean absolute
+ *       difference), a slow random walk far fewer; brute force of the
+ *       formula's pieces (global deviation, aux2 share).
+ *   orc_make_robust_stats -- a large independent-frame count leaves the
+ *       statistics almost unchanged (multipliers -> 1); diagonal models
+ *       written as full give the same diagonal; multipliers against their
+ *       closed forms at chosen counts.
+ * ------------------------------------------------------------------------ */
+
+/* accumulateAuxStats without Gaussian pruning
+ * (trainer/OL_GMMStatsEstimator.cpp:1226-1280), driven as accumulateStats
+ * drives it (:1174-1189) over ONE contiguous run of frames (one thread):
+ * for every consecutive pair (t-1, t), t >= 1, with posteriors before the
+ * cut g (gamma[N][M]):
+ *   p_m = g_t,m g_t-1,m;  s = sum_m [p_m > cut] p_m;  h_m = p_m / s (0 if s = 0)
+ *   aux2_m += sqrt(g_t,m)            if g_t,m > cut
+ *   aux_m[k] += h_m |x_t[k] - x_t-1[k]|, aux3_m += h_m   if h_m > cut
+ * aux[M][D+2] = [aux_m (D) | aux2_m | aux3_m] (the layout of the
+ * reference's GPU aux buffer, trainer/CU_GMMStatsEstimator_CUDA.cu:862),
+ * overwritten.  The frame difference is taken in float, as the reference's
+ * TV = float. */
+void orc_accumulate_aux(int N, int D, int M, const float *x, const double *gamma,
+                        double min_gamma, double *aux)
+{
+    const int W = D + 2;
+    memset(aux, 0, sizeof(double) * (size_t)M * W);
+    for (int t = 1; t < N; t++) {
+        const double *g2 = gamma + (size_t)t * M, *g1 = gamma + (size_t)(t - 1) * M;
+        const float *x2 = x + (size_t)t * D, *x1 = x + (size_t)(t - 1) * D;
+        double gs = 0.0;
+        for (int m = 0; m < M; m++) {
+            const double p = g2[m] * g1[m];
+            if (p > min_gamma)
+                gs += p;
+        }
+        if (gs > 0.0)
+            gs = 1.0 / gs;
+        for (int m = 0; m < M; m++) {
+            double *a = aux + (size_t)m * W;
+            const double h = gs * g2[m] * g1[m];
+            if (g2[m] > min_gamma)
+                a[D] += sqrt(g2[m]);
+            if (h <= min_gamma)
+                continue;
+            for (int k = 0; k < D; k++) {
+                const float ad = fabsf(x2[k] - x1[k]);
+                a[k] += h * (double)ad;
+            }
+            a[D + 1] += h;
+        }
+    }
+}
+
+/* Index of S_dd in a component's second-order statistics: d for diagonal
+ * statistics, the packed upper position for full ones. */
+static size_t var_diag_index(int D, int full, int d)
+{
+    return full ? packed_upper(D, d, d) : (size_t)d;
+}
+
+/* GetNIndFrames (trainer/OL_ModelAdapt.cpp:468-535): the effective number of
+ * independent frames per component,
+ *   MII_m  = (1/D) sum_d aux_m[d] / aux3_m / std_d   (std: the global
+ *            deviation from the summed statistics)
+ *   nind_m = 1 + (aux2_m N_glob / sum_m aux2_m - 1) (MII_m / 1.12)^3,
+ * 0 for a component with n = 0 or aux3 = 0.  Returns 0, or -1 when a global
+ * deviation is 0 (the reference throws). */
+int orc_nind_frames(int M, int D, int full, const double *n, const double *F, const double *S,
+                    const double *aux, double *nind)
+{
+    const int W = D + 2;
+    const size_t VS = full ? (size_t)D * (D + 1) / 2 : (size_t)D;
+    double *gstd = (double *)calloc((size_t)D, sizeof(double));
+    double *gmean = (double *)calloc((size_t)D, sizeof(double));
+    double gprob = 0.0, sumNg = 0.0;
+    for (int m = 0; m < M; m++) {
+        gprob += n[m];
+        for (int d = 0; d < D; d++) {
+            gmean[d] += F[(size_t)m * D + d];
+            gstd[d] += S[(size_t)m * VS + var_diag_index(D, full, d)];
+        }
+        nind[m] = 0.0;
+        sumNg += aux[(size_t)m * W + D];
+    }
+    for (int d = 0; d < D; d++) {
+        gmean[d] /= gprob;
+        gstd[d] = sqrt(gstd[d] / gprob - gmean[d] * gmean[d]);
+        if (gstd[d] == 0.0) {
+            free(gstd);
+            free(gmean);
+            return -1;
+        }
+    }
+    for (int m = 0; m < M; m++) {
+        const double *a = aux + (size_t)m * W;
+        if (n[m] == 0.0 || a[D + 1] == 0.0) {
+            nind[m] = 0.0;
+            continue;
+        }
+        double v = 0.0;
+        for (int d = 0; d < D; d++)
+            v += a[d] / a[D + 1] / gstd[d];
+        v /= D;
+        v = pow(v / 1.12, 3);
+        nind[m] = 1 + (a[D] * gprob / sumNg - 1) * v;
+    }
+    free(gstd);
+    free(gmean);
+    return 0;
+}
+
+/* getDiagCovMultiplier / getOffDiagCovMultiplier (OL_ModelAdapt.cpp:537-548),
+ * rounded to float as the reference returns them. */
+float orc_diag_cov_multiplier(double nind)
+{
+    if (nind < 2)
+        return (float)pow(pow(1.0 + 1.0 / (2.0 - 1.25), 3.5), 3 - nind);
+    return (float)pow(1.0 + 1.0 / (nind - 1.25), 3.5);
+}
+
+float orc_offdiag_cov_multiplier(int D, double nind)
+{
+    if (nind < 2)
+        return 0.0f;
+    return (float)(1 - pow(((double)D / ((double)D - 1 + nind)), 1.4));
+}
+
+/* MakeRobustStats (OL_ModelAdapt.cpp:550-588): S is replaced, per component,
+ * by  dc S + (1 - dc) F_k F_kk / n  with dc = diag multiplier x nind/(nind-1)
+ * (float) on the diagonal and the off-diagonal multiplier odc on the
+ * off-diagonal entries of full statistics; (1 - dc) is a float difference,
+ * as in the reference.  nind receives the independent-frame counts.
+ * Returns orc_nind_frames' status. */
+int orc_make_robust_stats(int M, int D, int full, const double *n, const double *F, double *S,
+                          const double *aux, double *nind)
+{
+    const int rc = orc_nind_frames(M, D, full, n, F, S, aux, nind);
+    if (rc)
+        return rc;
+    const size_t VS = full ? (size_t)D * (D + 1) / 2 : (size_t)D;
+    for (int j = 0; j < M; j++) {
+        float dc = orc_diag_cov_multiplier(nind[j]);
+        dc = (float)((double)dc * (nind[j] / (nind[j] - 1)));
+        const double *Fj = F + (size_t)j * D;
+        double *Sj = S + (size_t)j * VS;
+        if (full) {
+            const float odc = orc_offdiag_cov_multiplier(D, nind[j]);
+            for (int k = 0; k < D; k++)
+                for (int kk = k; kk < D; kk++) {
+                    const size_t p = packed_upper(D, k, kk);
+                    if (kk == k)
+                        Sj[p] = dc * Sj[p] + (double)(1.0f - dc) * pow(Fj[k], 2.0) / n[j];
+                    else
+                        Sj[p] = odc * Sj[p] + (double)(1.0f - odc) * Fj[k] * Fj[kk] / n[j];
+                }
+        } else {
+            for (int k = 0; k < D; k++)
+                Sj[k] = dc * Sj[k] + (double)(1.0f - dc) * pow(Fj[k], 2.0) / n[j];
+        }
+    }
+    return 0;
+}
+
+/* UpdateModelEM with _robustCov (OL_ModelAdapt.cpp:848-857, :880): S (a
+ * copy) made robust by orc_make_robust_stats, then the EM update with the
+ * minimum-data test on the independent-frame counts.  full selects the
+ * full-covariance form (cov [M][D][D], S packed upper).  Returns the number
+ * of components left, or -1 when a global variance is 0. */
+int orc_update_em_robust(int M, int D, int full, float *w, float *mu, float *var,
+                         const double *n, const double *F, const double *S,
+                         const double *aux, double tot_samples, double min_var)
+{
+    const size_t VS = full ? (size_t)D * (D + 1) / 2 : (size_t)D;
+    double *Sr = (double *)malloc(sizeof(double) * (size_t)M * VS);
+    double *nind = (double *)malloc(sizeof(double) * (size_t)M);
+    memcpy(Sr, S, sizeof(double) * (size_t)M * VS);
+    int left = -1;
+    if (orc_make_robust_stats(M, D, full, n, F, Sr, aux, nind) == 0)
+        left = full ? update_em_full_core(M, D, w, mu, var, n, F, Sr, tot_samples, nind)
+                    : update_em_core(M, D, w, mu, var, n, F, Sr, tot_samples, min_var, 1, 1, 1,
+                                     nind);
+    free(Sr);
+    free(nind);
+    return left;
 }

@@ -7,7 +7,10 @@ Follows OL_ModelAdapt.cpp:673-821 (TrainFromScratch), :590-660
 (splitMultipleGaussians / splitGaussian, diagonal branch), :965-991
 (GetGaussStats: highest weight, ties to the higher index; zero slots listed
 from the highest index), OL_GMModel.cpp:857-885 (RearrangeMixtures inside
-every insertModel) and :76-86 (GMMMixture::Reset).
+every insertModel) and :76-86 (GMMMixture::Reset).  With robust=True the
+statistics carry the auxiliary frame-pair sums (accumulateAuxStats,
+OL_GMMStatsEstimator.cpp:1226-1280) and every EM update is the robust one
+(OL_ModelAdapt.cpp:848-857: MakeRobustStats, independent-frame test).
 """
 import numpy as np
 
@@ -23,15 +26,31 @@ def _reset_var(D):
     return np.frombuffer(b"\x01" * (4 * D), dtype=np.float32).copy()
 
 
-def train_from_scratch(x, nummix, it_em):
+def _aux(x, w, mu, var, act, nummix, full):
+    D = x.shape[1]
+    if full:
+        _, g = orc.frame_loglik_full(x, w[act], mu[act], var[act], want_gamma=True)
+    else:
+        _, g = orc.frame_loglik(x, w[act], mu[act], var[act], want_gamma=True)
+    aux = np.zeros((nummix, D + 2))
+    aux[act] = orc.accumulate_aux(x, g)
+    return aux
+
+
+def train_from_scratch(x, nummix, it_em, robust=False):
     x = np.ascontiguousarray(x, np.float32)
     N, D = x.shape
     # global statistics with a single all-ones posterior
     n = np.array([float(N)])
     F = x.astype(np.float64).sum(0)[None]
     S = (x.astype(np.float64) * x.astype(np.float64)).sum(0)[None]
-    w0, mu0, var0, left = orc.update_em(np.ones(1), np.zeros((1, D)),
-                                        _reset_var(D)[None], n, F, S, N)
+    if robust:
+        w0, mu0, var0, left = orc.update_em_robust(
+            np.ones(1), np.zeros((1, D)), _reset_var(D)[None], n, F, S,
+            orc.accumulate_aux(x, np.ones((N, 1))), N)
+    else:
+        w0, mu0, var0, left = orc.update_em(np.ones(1), np.zeros((1, D)),
+                                            _reset_var(D)[None], n, F, S, N)
     w = np.zeros(nummix, np.float32)
     mu = np.zeros((nummix, D), np.float32)
     var = np.tile(_reset_var(D), (nummix, 1))
@@ -44,7 +63,10 @@ def train_from_scratch(x, nummix, it_em):
             na, Fa, Sa, _tot = orc.accumulate(x, w[act], mu[act], var[act])
             n = np.zeros(nummix); F = np.zeros((nummix, D)); S = np.zeros((nummix, D))
             n[act], F[act], S[act] = na, Fa, Sa
-            if nonzero > 1:
+            if nonzero > 1 and robust:
+                aux = _aux(x, w, mu, var, act, nummix, False)
+                w, mu, var, nonzero = orc.update_em_robust(w, mu, var, n, F, S, aux, N)
+            elif nonzero > 1:
                 w, mu, var, nonzero = orc.update_em(w, mu, var, n, F, S, N)
             if nonzero < 2:
                 break
@@ -75,7 +97,7 @@ def _rearrange_full(w, mu, cov):
     return w[perm].copy(), mu[perm].copy(), cov[perm].copy(), nz
 
 
-def train_from_scratch_full(x, nummix, it_em):
+def train_from_scratch_full(x, nummix, it_em, robust=False):
     """trainGMM --type 0 --full-cov: the same driver with the full-covariance
     statistics / update, GMMMixture::Reset's identity covariance for empty
     slots (OL_GMModel.cpp:73-83) and the full-covariance splitGaussian
@@ -89,8 +111,13 @@ def train_from_scratch_full(x, nummix, it_em):
     F = xd.sum(0)[None]
     S = (xd.T @ xd)[r, c][None]
     eye = np.eye(D, dtype=np.float32)
-    w0, mu0, cov0, left = orc.update_em_full(np.ones(1), np.zeros((1, D)), eye[None],
-                                             n, F, S, N)
+    if robust:
+        w0, mu0, cov0, left = orc.update_em_robust(
+            np.ones(1), np.zeros((1, D)), eye[None], n, F, S,
+            orc.accumulate_aux(x, np.ones((N, 1))), N, full=True)
+    else:
+        w0, mu0, cov0, left = orc.update_em_full(np.ones(1), np.zeros((1, D)), eye[None],
+                                                 n, F, S, N)
     w = np.zeros(nummix, np.float32)
     mu = np.zeros((nummix, D), np.float32)
     cov = np.tile(eye, (nummix, 1, 1))
@@ -105,7 +132,11 @@ def train_from_scratch_full(x, nummix, it_em):
             F = np.zeros((nummix, D))
             S = np.zeros((nummix, D * (D + 1) // 2))
             n[act], F[act], S[act] = na, Fa, Sa
-            if nonzero > 1:
+            if nonzero > 1 and robust:
+                aux = _aux(x, w, mu, cov, act, nummix, True)
+                w, mu, cov, nonzero = orc.update_em_robust(w, mu, cov, n, F, S, aux, N,
+                                                           full=True)
+            elif nonzero > 1:
                 w, mu, cov, nonzero = orc.update_em_full(w, mu, cov, n, F, S, N)
             if nonzero < 2:
                 break

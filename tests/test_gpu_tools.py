@@ -208,3 +208,48 @@ def test_train_mllr_full_cov_ubm(tmp_path):
     gw, gmu, gcov, _ = fm.read_gmm_txt_full(out)
     assert np.allclose(gmu, orc.mllr_apply(W, umu), rtol=1e-4, atol=1e-4)
     assert np.array_equal(gcov, ucov)
+
+
+@pytest.mark.parametrize("full", [False, True])
+def test_train_from_scratch_robust(tmp_path, golden, full):
+    """--robust (-R): from-scratch training on data.bin with the robust
+    covariance estimate, against the oracle's split driver run the same way
+    (auxiliary statistics from the oracle's posteriors, MakeRobustStats, the
+    independent-frame test)."""
+    import _scratch_driver as sd
+    out = tmp_path / "robust.gmm"
+    args = ["-i", os.path.join(GOLD, "data.bin"), "--data-T", 2, "--type", 0, "--nummix", 4,
+            "--it-EM", 8, "-o", out, "-T", "--robust"]
+    run("trainGMM", *(args + (["--full-cov"] if full else [])))
+    if full:
+        w, mu, cov, _ = fm.read_gmm_txt_full(out)
+        ow, omu, ocov = sd.train_from_scratch_full(golden["x"], 4, 8, robust=True)
+        assert np.abs(w - ow).max() < 2e-5
+        assert np.abs(mu - omu).max() < 2e-4
+        assert np.abs(cov - ocov).max() < 2e-4
+    else:
+        w, mu, var, _ = fm.read_gmm_txt(out)
+        ow, omu, ovar = sd.train_from_scratch(golden["x"], 4, 8, robust=True)
+        assert np.abs(w - ow).max() < 2e-5
+        assert np.abs(mu - omu).max() < 5e-5
+        assert (np.abs(var - ovar) / ovar).max() < 5e-5
+        # the robust estimate differs from the plain one
+        pw, pmu, pvar = sd.train_from_scratch(golden["x"], 4, 8)
+        assert np.abs(var - pvar).max() > 1e-3
+
+
+def test_train_map_robust(tmp_path):
+    """MAP of means and variances with --robust: the auxiliary statistics
+    of the adaptation frames make the second-order statistics robust before
+    the MAP rule (OL_ModelAdapt.cpp:1037-1040)."""
+    x, w, mu, var, ubm_p, data_p = _ubm_case(tmp_path, seed=49, M=16, D=6, N=8000)
+    out = tmp_path / "maprobust.gmm"
+    run("trainGMM", "-i", data_p, "--data-T", 2, "--in-UBM", ubm_p, "-L", "--type", 2,
+        "--it-MAP", 1, "--tau", 16, "--mean", "--var", "--robust", "-o", out, "-T")
+    n, F, S, _ = orc.accumulate(x, w, mu, var)
+    _, g = orc.frame_loglik(x, w, mu, var, want_gamma=True)
+    Sr, _ = orc.make_robust_stats(n, F, S, orc.accumulate_aux(x, g))
+    rw, rmu, rvar = orc.update_map(w, mu, var, n, F, Sr, x.shape[0], 16, upmean=True, upvar=True)
+    gw, gmu, gvar, _ = fm.read_gmm_txt(out)
+    assert np.allclose(gmu, rmu, rtol=1e-5, atol=1e-5)
+    assert np.allclose(gvar, rvar, rtol=1e-4, atol=1e-6)

@@ -231,3 +231,41 @@ def test_mllr_transform_full_model_uses_diagonal():
     assert np.array_equal(Wf, Wd)
     R = orc.mllr_transform(mu, var, n, F, ubm_mu=mu, ubm_var=var)
     assert np.allclose(Wf, R, rtol=1e-9, atol=1e-9)
+
+
+def _robust_case(seed, M=3, D=4, N=3000, full=False):
+    rng = np.random.default_rng(seed)
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, 1.0, (M, D)).astype(np.float32)
+    var = rng.uniform(0.5, 2.0, (M, D)).astype(np.float32)
+    x = (np.cumsum(rng.normal(0, 0.4, (N, D)), axis=0) * 0.2 + rng.normal(0, 1, (N, D))).astype(np.float32)
+    if full:
+        A = rng.normal(size=(M, D, D))
+        cov = (A @ A.transpose(0, 2, 1) / D + np.eye(D)).astype(np.float32)
+        cov = 0.5 * (cov + cov.transpose(0, 2, 1))
+        n, F, S, _ = orc.accumulate_full(x, w, mu, cov)
+        _, g = orc.frame_loglik_full(x, w, mu, cov, want_gamma=True)
+        return x, w, mu, cov, n, F, S, orc.accumulate_aux(x, g)
+    n, F, S, _ = orc.accumulate(x, w, mu, var)
+    _, g = orc.frame_loglik(x, w, mu, var, want_gamma=True)
+    return x, w, mu, var, n, F, S, orc.accumulate_aux(x, g)
+
+
+@pytest.mark.parametrize("full", [False, True])
+def test_make_robust_stats_matches_oracle(full):
+    x, w, mu, v, n, F, S, aux = _robust_case(23, full=full)
+    hS, hn = host.make_robust_stats(n, F, S, aux, full=full)
+    rS, rn = orc.make_robust_stats(n, F, S, aux, full=full)
+    assert np.array_equal(hn, rn)
+    assert np.array_equal(hS, rS)
+
+
+@pytest.mark.parametrize("full", [False, True])
+def test_update_em_robust_matches_oracle(full):
+    x, w, mu, v, n, F, S, aux = _robust_case(24, full=full)
+    m = host.Model.create_full(w, mu, v) if full else host.Model.create(w, mu, v)
+    left = m.update_em_robust(n, F, S, aux, x.shape[0])
+    ow, omu, ov, oleft = orc.update_em_robust(w, mu, v, n, F, S, aux, x.shape[0], full=full)
+    gw, gmu, gv = m.get()
+    assert left == oleft
+    assert np.array_equal(gw, ow) and np.array_equal(gmu, omu) and np.array_equal(gv, ov)
