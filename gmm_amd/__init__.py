@@ -37,7 +37,8 @@ EXTRA_EXPORTS = [
     "gmmhost_model_create_full", "gmmhost_model_full", "gmmhost_model_dims", "gmmhost_model_get", "gmmhost_model_rearrange",
     "gmmhost_model_free", "gmmhost_update_em", "gmmhost_update_em_robust",
     "gmmhost_make_robust_stats", "gmmhost_update_map",
-    "gmmhost_mllr_transform", "gmmhost_mllr_apply", "gmmhost_last_error",
+    "gmmhost_mllr_transform", "gmmhost_fmllr_transform", "gmmhost_mllr_apply",
+    "gmmhost_last_error",
 ]
 
 (OK, ERR_HIP, ERR_OOM, ERR_NAN, ERR_ARG, ERR_VAR, ERR_STATE, ERR_MLLR, ERR_NODEV, ERR_TIMEOUT,
@@ -45,7 +46,7 @@ EXTRA_EXPORTS = [
 REQ_SCORE, REQ_MAP, REQ_MLLR = 0, 1, 2
 REQ_PENDING = -1
 PREC_FP32, PREC_BF16X3, PREC_FP16X3 = 0, 1, 2
-ACC_MEAN, ACC_VAR = 1, 2
+ACC_MEAN, ACC_VAR, ACC_FULLVAR = 1, 2, 4
 
 
 class GmmHipError(RuntimeError):
@@ -262,13 +263,13 @@ class Accumulator:
 
     def accumulate(self, flags=ACC_MEAN | ACC_VAR, into=None):
         """Returns (n[M], F[M,D], S, tot_ll, nframes), S[M,D] for a diagonal
-        model, S[M,D(D+1)/2] (packed upper) for a full one; adds into `into`
-        if given."""
+        model, S[M,D(D+1)/2] (packed upper) for a full one or with
+        ACC_FULLVAR; adds into `into` if given."""
         if into is None:
             n = np.zeros(self.M)
             F = np.zeros((self.M, self.D))
-            S = np.zeros((self.M, self.D * (self.D + 1) // 2 if getattr(self, "full", False)
-                          else self.D))
+            packed = getattr(self, "full", False) or (flags & ACC_FULLVAR)
+            S = np.zeros((self.M, self.D * (self.D + 1) // 2 if packed else self.D))
             tot = ctypes.c_double(0.0)
             nf = ctypes.c_uint64(0)
         else:

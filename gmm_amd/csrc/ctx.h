@@ -172,6 +172,9 @@ void aux_state_free(gmmhip::AuxState *a);
 int full_loglikes_device(gmmhip_ctx *h, float *d_out);
 int full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S, double *tot_ll,
                     uint64_t *nframes);
+// Full second-order statistics of the handle's diagonal model (GMMHIP_ACC_FULLVAR).
+int diag_full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S,
+                         double *tot_ll, uint64_t *nframes);
 
 // Device copies of the active model for the per-utterance kernels: FP32
 // means / variances and, with mllr, the UBM-only MLLR operands

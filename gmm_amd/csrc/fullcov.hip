@@ -293,6 +293,53 @@ int full_loglikes_device(gmmhip_ctx *h, float *d_out) {
     return GMMHIP_OK;
 }
 
+int diag_full_operand(gmmhip_ctx *h) {
+    gmmhip::FullState *fs = nullptr;
+    int rc = gemm_state(h, &fs);
+    if (rc) return rc;
+    if (fs->operand_version == h->model_version) return GMMHIP_OK;
+    const int M = h->M, D = h->D;
+    const int K = 1 + D + D * (D + 1) / 2, KP = (K + 7) / 8 * 8;
+    std::vector<float> theta((size_t)M * KP, 0.0f);
+    for (int m = 0; m < M; m++) {
+        float *th = theta.data() + (size_t)m * KP;
+        double cq = 0.0;
+        int k = 1 + D;
+        for (int i = 0; i < D; i++) {
+            const double iv = 1.0 / (double)h->m_var[(size_t)m * D + i];
+            const double mc = (double)h->m_mu[(size_t)m * D + i] - (double)h->h_off[i];
+            th[1 + i] = (float)(mc * iv);
+            th[k] = (float)(-0.5 * iv);  // (i, i); (i, j > i) stay 0
+            k += D - i;
+            cq += mc * mc * iv;
+        }
+        th[0] = (float)(h->m_gconst[m] - 0.5 * cq);
+    }
+    const std::vector<int> codes = feature_codes(D, true);
+    HIPCHK(h, ensure(&fs->d_theta, &fs->theta_cap, theta.size()));
+    HIPCHK(h, ensure(&fs->d_pairs, &fs->pairs_cap, codes.size()));
+    HIPCHK(h, ensure(&fs->d_cen, &fs->cen_cap, (size_t)D));
+    HIPCHK(h, hipMemcpyAsync(fs->d_theta, theta.data(), theta.size() * sizeof(float),
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(fs->d_pairs, codes.data(), codes.size() * sizeof(int),
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(fs->d_cen, h->h_off.data(), D * sizeof(float), hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    fs->K = K;
+    fs->KP = KP;
+    fs->h_cen = h->h_off;
+    fs->operand_version = h->model_version;
+    return GMMHIP_OK;
+}
+
+int diag_full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S,
+                         double *tot_ll, uint64_t *nframes) {
+    int rc = diag_full_operand(h);
+    if (rc) return rc;
+    return full_accumulate(h, flags, n, F, S, tot_ll, nframes);
+}
+
 int full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S, double *tot_ll,
                     uint64_t *nframes) {
     gmmhip::FullState *fs = h->full;
@@ -438,5 +485,6 @@ int gmmhip_set_model_full(gmmhip_t h, int M, int D, const float *w, const float 
     h->m_mu.assign(mu, mu + (size_t)M * D);
     h->m_var.clear();
     h->model_version++;
+    fs->operand_version = h->model_version;
     return GMMHIP_OK;
 }

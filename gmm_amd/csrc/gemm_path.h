@@ -31,6 +31,7 @@ struct FullState {
     double *d_acc = nullptr;     // [M][KP] FP64 statistics over chunks
     size_t acc_cap = 0;
     std::vector<float> h_cen;
+    uint64_t operand_version = ~0ull;  // model_version theta was built for
 };
 
 }  // namespace gmmhip
@@ -44,6 +45,11 @@ std::vector<int> feature_codes(int D, bool full);
 // The handle's GEMM-path state (created on first use: rocBLAS handle on the
 // handle's stream, atomics off).
 int gemm_state(gmmhip_ctx *h, gmmhip::FullState **out);
+
+// The full-feature operand theta of the handle's diagonal model (zero
+// off-diagonal precisions), so the GEMM path yields full second-order
+// statistics of a diagonal model (fullcov.hip).
+int diag_full_operand(gmmhip_ctx *h);
 
 hipError_t launch_fc_features(const float *X, uint64_t n0, int nc, int ncp, int D, int K, int KP,
                               const float *cen, const int *codes, float *phi, uint8_t *bad,

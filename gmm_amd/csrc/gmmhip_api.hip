@@ -610,6 +610,8 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
         return fail(h, GMMHIP_ERR_ARG, "accumulate: null output for a requested statistic");
     DevGuard g(h->dev);
     if (h->full_model) return full_accumulate(h, flags, n, F, S, tot_ll, nframes);
+    if ((flags & GMMHIP_ACC_FULLVAR) && (flags & GMMHIP_ACC_VAR))
+        return diag_full_accumulate(h, flags, n, F, S, tot_ll, nframes);
     // padded: the mixed statistics kernel stages 32-frame steps of LL2
     HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, pad128(h->N)));
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N, 1)));

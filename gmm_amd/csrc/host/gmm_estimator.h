@@ -114,13 +114,16 @@ public:
         if ((fMean_ && !gmms.hasMeans()) || (fVar_ && !gmms.hasVars()) || (fAux_ && !gmms.hasAux()))
             throw std::logic_error("accumulateStatsMT(): statistics not allocated for the requested flags");
         // a full-covariance model produces full second-order statistics
-        // (the diagonal ones would be their diagonal; not offered)
-        if (fVar_ && gmms.hasFullCov() != p.full)
-            throw std::logic_error("accumulateStatsMT(): full/diagonal statistics do not match the model");
+        // (the diagonal ones would be their diagonal; not offered); a
+        // diagonal model gives full ones on request (setAccFlags' varFull,
+        // fMLLR)
+        if (fVar_ && p.full && !gmms.hasFullCov())
+            throw std::logic_error("accumulateStatsMT(): a full-covariance model needs full statistics");
         if (NSamples == 0) return;
         upload(vectors, NSamples, dim);
         syncOpts();
-        const int flags = (fMean_ ? GMMHIP_ACC_MEAN : 0) | (fVar_ ? GMMHIP_ACC_VAR : 0);
+        const int flags = (fMean_ ? GMMHIP_ACC_MEAN : 0) | (fVar_ ? GMMHIP_ACC_VAR : 0) |
+                          (fVar_ && !p.full && gmms.hasFullCov() ? GMMHIP_ACC_FULLVAR : 0);
         const size_t vs = gmms.hasVars() ? gmms.getVarSize() : (size_t)p.D;
         std::vector<double> n(p.M, 0.0), F(fMean_ ? (size_t)p.M * p.D : 0), S(fVar_ ? (size_t)p.M * vs : 0);
         double tot = 0.0;

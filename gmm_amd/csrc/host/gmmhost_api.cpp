@@ -216,6 +216,20 @@ int gmmhost_mllr_transform(gmmhost_model_t m, gmmhost_model_t ubm, const double 
     return 0;
 }
 
+int gmmhost_fmllr_transform(gmmhost_model_t m, const double *n, const double *F, const double *S,
+                            uint64_t tot_samples, int iters, double *W) {
+    if (!m || !n || !F || !S || !W) return fail("null argument");
+    try {
+        const int M = m->m.GetNumberOfMixtures(), D = m->m.GetDimension();
+        gmm::GMMStats<double> st = make_stats(M, D, n, F, S, tot_samples, true);
+        const gmm::MLLRTransform T = gmm::CModelAdapt::EstimateW_fMLLR(m->m, st, iters);
+        std::memcpy(W, T.W.data(), sizeof(double) * T.W.size());
+    } catch (std::exception &e) {
+        return fail(e.what());
+    }
+    return 0;
+}
+
 int gmmhost_mllr_apply(gmmhost_model_t m, const double *W) {
     if (!m || !W) return fail("null argument");
     gmm::MLLRTransform T;

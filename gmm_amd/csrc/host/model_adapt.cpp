@@ -1,5 +1,6 @@
 #include "model_adapt.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -466,6 +467,163 @@ std::unique_ptr<GMModel> CModelAdapt::AdaptModelMLLR(const GMModel &model, const
     return up;
 }
 
+// LU factorisation with partial pivoting in place (row-major n x n); returns
+// the determinant and fills perm; a zero pivot gives 0.
+static double lu_factor(int n, std::vector<double> &A, std::vector<int> &perm) {
+    perm.resize(n);
+    for (int i = 0; i < n; i++) perm[i] = i;
+    double det = 1.0;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (std::fabs(A[(size_t)i * n + k]) > std::fabs(A[(size_t)p * n + k])) p = i;
+        if (A[(size_t)p * n + k] == 0.0) return 0.0;
+        if (p != k) {
+            for (int j = 0; j < n; j++) std::swap(A[(size_t)k * n + j], A[(size_t)p * n + j]);
+            std::swap(perm[k], perm[p]);
+            det = -det;
+        }
+        det *= A[(size_t)k * n + k];
+        for (int i = k + 1; i < n; i++) {
+            const double l = A[(size_t)i * n + k] / A[(size_t)k * n + k];
+            A[(size_t)i * n + k] = l;
+            for (int j = k + 1; j < n; j++) A[(size_t)i * n + j] -= l * A[(size_t)k * n + j];
+        }
+    }
+    return det;
+}
+
+// Inverse from an LU factorisation (column by column).
+static std::vector<double> lu_inverse(int n, const std::vector<double> &LU,
+                                      const std::vector<int> &perm) {
+    std::vector<double> inv((size_t)n * n), y(n);
+    for (int c = 0; c < n; c++) {
+        for (int i = 0; i < n; i++) {
+            double s = perm[i] == c ? 1.0 : 0.0;
+            for (int j = 0; j < i; j++) s -= LU[(size_t)i * n + j] * y[j];
+            y[i] = s;
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double s = y[i];
+            for (int j = i + 1; j < n; j++) s -= LU[(size_t)i * n + j] * inv[(size_t)j * n + c];
+            inv[(size_t)i * n + c] = s / LU[(size_t)i * n + i];
+        }
+    }
+    return inv;
+}
+
+// ComputeKiGi_fMLLR (OL_ModelAdapt.cpp:1793-1840) + ComputeW_fMLLR /
+// EstimateAb_fmllr (:1614-1791).  Row i of W = [A | b] maximises
+// beta log|det A| + w_i k_i - w_i G_i w_i^T / 2 with the cofactors p_i of
+// the A of the previous sweep: w = (p / f + k) G^-1 for the roots f of
+// beta f^2 - (p G^-1 k) f - p G^-1 p^T = 0, the root with the larger
+// sum w (k - G w / 2) (the reference's criterion) kept.  The reference
+// inverts with unpivoted Gauss-Jordan; here G by LU with partial pivoting
+// (G is SPD) and A likewise.
+MLLRTransform CModelAdapt::EstimateW_fMLLR(const GMModel &model, const GMMStats<double> &st,
+                                           int iters) {
+    const int D = model.GetDimension(), E = D + 1, M = (int)st.getMixNum();
+    if (!st.hasMeans() || !st.hasVars() || !st.hasFullCov())
+        throw std::logic_error("ComputeW_fMLLR(): needs first and full second-order statistics");
+    std::vector<double> G((size_t)D * E * E, 0.0), Gi((size_t)D * E * E), K((size_t)D * E, 0.0);
+    std::vector<float> vd(D);
+    for (int m = 0; m < M; m++) {
+        const double np = st.getProb(m);
+        if (!(np > 0)) continue;
+        std::memcpy(vd.data(), model.GetMixtureVarDiag(m), sizeof(float) * D);
+        const float *mu = model.GetMixtureMean(m);
+        const double *F = st.getMean(m), *S = st.getVar(m);
+        for (int i = 0; i < D; i++) {
+            const double cm = (double)(mu[i] / vd[i]), cv = (double)(1.0f / vd[i]);
+            double *g = &G[(size_t)i * E * E], *k = &K[(size_t)i * E];
+            size_t p = 0;
+            for (int a = 0; a < D; a++) {
+                k[a] += cm * F[a];
+                for (int b = a; b < D; b++, p++) g[(size_t)a * E + b] += cv * S[p];
+                g[(size_t)a * E + D] += cv * F[a];
+            }
+            k[D] += cm * np;
+            g[(size_t)D * E + D] += cv * np;
+        }
+    }
+    std::vector<int> perm;
+    for (int i = 0; i < D; i++) {
+        double *g = &G[(size_t)i * E * E];
+        for (int a = 0; a < E; a++)
+            for (int b = a + 1; b < E; b++) g[(size_t)b * E + a] = g[(size_t)a * E + b];
+        std::vector<double> lu(g, g + (size_t)E * E);
+        if (lu_factor(E, lu, perm) == 0.0)
+            throw std::runtime_error("ComputeW_fMLLR(): matrix G_i is singular");
+        const std::vector<double> inv = lu_inverse(E, lu, perm);
+        std::copy(inv.begin(), inv.end(), Gi.begin() + (size_t)i * E * E);
+    }
+    std::vector<double> A((size_t)D * D, 0.0), b(D, 0.0), p(E), u(E), t(E), w[2], gw(E);
+    w[0].resize(E);
+    w[1].resize(E);
+    for (int d = 0; d < D; d++) A[(size_t)d * D + d] = 1.0;
+    const double beta = (double)(float)st.getTotAccSamples();
+    for (int it = 0; it < iters; it++) {
+        std::vector<double> lu(A);
+        const double det = lu_factor(D, lu, perm);
+        if (det == 0.0) throw std::runtime_error("ComputeW_fMLLR(): Matrix invertion failed!");
+        std::vector<double> cof = lu_inverse(D, lu, perm);  // inv(A) det(A): columns = cofactor rows
+        for (double &c : cof) c *= det;
+        for (int i = 0; i < D; i++) {
+            const double *g = &G[(size_t)i * E * E], *gi = &Gi[(size_t)i * E * E];
+            const double *k = &K[(size_t)i * E];
+            for (int a = 0; a < D; a++) p[a] = cof[(size_t)a * D + i];
+            p[D] = 0.0;
+            double bb = 0.0, cc = 0.0;
+            for (int c = 0; c < E; c++) {
+                double s = 0.0;
+                for (int a = 0; a < E; a++) s += p[a] * gi[(size_t)a * E + c];
+                u[c] = s;
+                bb += s * k[c];
+                cc += s * p[c];
+            }
+            bb = -bb;
+            cc = -cc;
+            const double sq = std::sqrt(bb * bb - 4 * beta * cc);
+            if (!(sq >= 0)) throw std::runtime_error("EstimateAb_fmllr(): Discriminant < 0!");
+            const double f[2] = {(-bb + sq) / (2 * beta), (-bb - sq) / (2 * beta)};
+            double Q[2];
+            for (int j = 0; j < 2; j++) {
+                if (f[j] == 0) throw std::runtime_error("EstimateAb_fmllr(): Dividing by zero!");
+                for (int c = 0; c < E; c++) t[c] = p[c] / f[j] + k[c];
+                for (int c = 0; c < E; c++) {
+                    double s = 0.0;
+                    for (int a = 0; a < E; a++) s += t[a] * gi[(size_t)a * E + c];
+                    w[j][c] = s;
+                }
+                Q[j] = 0.0;
+                for (int a = 0; a < E; a++) {
+                    double s = 0.0;
+                    for (int c = 0; c < E; c++) s += g[(size_t)a * E + c] * w[j][c];
+                    Q[j] += w[j][a] * (k[a] - 0.5 * s);
+                }
+            }
+            const int idx = (Q[0] > Q[1]) ? 0 : 1;
+            for (int c = 0; c < D; c++) A[(size_t)i * D + c] = w[idx][c];
+            b[i] = w[idx][D];
+        }
+    }
+    MLLRTransform W;
+    W.D = D;
+    W.W.resize((size_t)D * E);
+    for (int i = 0; i < D; i++) {
+        for (int c = 0; c < D; c++) W.W[(size_t)i * E + c] = A[(size_t)i * D + c];
+        W.W[(size_t)i * E + D] = b[i];
+    }
+    return W;
+}
+
+MLLRTransform CModelAdapt::ComputeW_fMLLR(GMModel *model) {
+    stats_.reset();
+    AccumulateStats(model, stats_, false, true, true);
+    CheckInvertibility();
+    return EstimateW_fMLLR(*model, stats_, opt_._iter_fMLLR);
+}
+
 // UpdateMLLRMatrix (OL_ModelAdapt.cpp:1319-1340): Wsum <- Wnew o Wsum.
 void CModelAdapt::ComposeMLLR(MLLRTransform &Wsum, const MLLRTransform &Wnew) {
     const int D = Wsum.D, E = D + 1;
@@ -521,6 +679,11 @@ void CModelAdapt::AdaptModel(GMModel *UBM) {
         if (opt_._applyTransforms) model_ = AdaptModelMLLR(*UBM, *W_);
         break;
     }
+    case fMLLR:
+        // full second-order statistics under the (diagonal or full) UBM
+        stats_.alloc(D, nummix_, true, true, true);
+        W_.reset(new MLLRTransform(ComputeW_fMLLR(UBM)));
+        break;
     case MLLR_MAP: {
         stats_.alloc(D, nummix_, true, opt_._AdaptVarBool, fullV, opt_._robustCov);
         MLLRTransform W = IterW_MLLR(UBM, opt_._iterMLLR, false);
@@ -683,7 +846,8 @@ int CModelAdapt::Save(const char *filename, bool save_txt) {
         std::ofstream f(out);
         if (!f) throw std::runtime_error("Save(): Unable to open output file " + out);
         f.precision(10);
-        f << " MLLR\n 1\t" << W_->D << " " << W_->D + 1 << "\n";
+        f << (opt_._adaptType == fMLLR ? " fMLLR" : " MLLR") << "\n 1\t" << W_->D << " "
+          << W_->D + 1 << "\n";
         for (int a = 0; a < W_->D; a++) {
             for (int b = 0; b <= W_->D; b++) f << " " << W_->W[(size_t)a * (W_->D + 1) + b];
             f << "\n";

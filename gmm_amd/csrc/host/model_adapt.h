@@ -2,7 +2,7 @@
 // CModelAdapt (trainer/OL_ModelAdapt.h:140-180 of GMMTeam/GMM) on the
 // GMM hot path -- EM from a UBM or from scratch (Gaussian splitting; diagonal
 // or full covariance; optionally robust covariances), MAP adaptation, global
-// MLLR (+ MLLR then MAP) -- with the reference's update rules.  All statistics come
+// MLLR (+ MLLR then MAP), fMLLR -- with the reference's update rules.  All statistics come
 // from GMMStatsEstimator_HIP.
 #pragma once
 
@@ -48,6 +48,7 @@ public:
         unsigned int _numThreads = 1;
         int _precision = GMMHIP_PREC_FP32;
         bool _robustCov = false;  // robust covariances from the auxiliary statistics
+        int _iter_fMLLR = 20;     // row-update sweeps of fMLLR
         std::string _transformFileExt = ".mat";
     };
 
@@ -97,6 +98,12 @@ public:
     MLLRTransform ComputeW_MLLR(GMModel *model, const std::vector<double> &Ginit,
                                 const std::vector<double> &kinit, bool wantQ);
     static std::unique_ptr<GMModel> AdaptModelMLLR(const GMModel &model, const MLLRTransform &W);
+    // fMLLR (OL_ModelAdapt.cpp:1614-1840): statistics with full second-order
+    // sums under model, then the row-by-row constrained MLLR estimate.
+    MLLRTransform ComputeW_fMLLR(GMModel *model);
+    // fMLLR rows from given statistics (full second-order, packed upper).
+    static MLLRTransform EstimateW_fMLLR(const GMModel &model, const GMMStats<double> &stats,
+                                         int iters);
     static void ComposeMLLR(MLLRTransform &Wsum, const MLLRTransform &Wnew);
 
 private:

@@ -2,8 +2,8 @@
 // Option names follow the reference tool (projects/Model_ADAPT/Model_ADAPT.cpp
 // :278-401, trainer/OL_ModelAdapt.cpp:2032-2135); the hot-path subset is
 // supported: --type 0 (from scratch; --full-cov for full covariances),
-// 1 (MLLR), 2 (MAP/EM), 4 (MLLR+MAP); --robust (-R) for the robust
-// covariance estimate (from scratch and MAP/EM).
+// 1 (MLLR), 2 (MAP/EM), 3 (fMLLR), 4 (MLLR+MAP); --robust (-R) for the
+// robust covariance estimate (from scratch and MAP/EM).
 // --SSE/--CUDA are accepted and ignored: statistics always run on the GPU.
 #include <cstdio>
 #include <iostream>
@@ -42,6 +42,7 @@ int main(int argc, char **argv) {
     a.option("min-var", 0, false, "1e-6");
     a.option("mllr-init", 0, true, "0");
     a.option("it-mllr", 0, false, "1");
+    a.option("it-fMLLR", 0, false, "20");
     a.option("apply-trn", 'a', true, "0");
     a.option("save-trn", 's', true, "0");
     a.option("mllr-Qval", 0, true, "0");
@@ -59,7 +60,7 @@ int main(int argc, char **argv) {
         a.parse(argc, argv);
         if (a.b("help") || a.all("in-file").empty() || !a.has("outFile")) {
             std::cout << "usage: trainGMM -i FEATS [-i ...] --data-T {0,1,2} -o OUT.gmm [-T] "
-                         "--type {0,1,2,4} [--in-UBM UBM.gmm [-L]] [--nummix K [--full-cov]] [--robust] [--it-EM n] "
+                         "--type {0,1,2,3,4} [--in-UBM UBM.gmm [-L]] [--nummix K [--full-cov]] [--robust] [--it-EM n] "
                          "[--it-MAP n] [--tau r] [--mean] [--var] [--weight] [--mllr-init] "
                          "[--it-mllr n] [--apply-trn] [--save-trn] [--precision fp32|fp16x3|bf16x3]\n";
             return a.b("help") ? 0 : 1;
@@ -90,6 +91,7 @@ int main(int argc, char **argv) {
         o._min_var = (float)a.f("min-var");
         o._mllrUBMInit = a.b("mllr-init");
         o._iterMLLR = a.i("it-mllr");
+        o._iter_fMLLR = a.i("it-fMLLR");
         o._applyTransforms = a.b("apply-trn");
         o._saveTransforms = a.b("save-trn");
         o._getQvalMLLR = a.b("mllr-Qval");

@@ -46,6 +46,8 @@ def _L():
                                          ctypes.c_int]
         L.gmmhost_mllr_transform.argtypes = [vp, vp, _dp, _dp, ctypes.c_float, _dp]
         L.gmmhost_mllr_apply.argtypes = [vp, _dp]
+        L.gmmhost_fmllr_transform.argtypes = [vp, _dp, _dp, _dp, ctypes.c_uint64, ctypes.c_int,
+                                              _dp]
         L.gmmhost_update_em_robust.argtypes = [vp, _dp, _dp, _dp, _dp, ctypes.c_uint64,
                                                ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
@@ -167,6 +169,14 @@ class Model:
         W = np.zeros((D, D + 1))
         _chk(_L().gmmhost_mllr_transform(self._h, ubm._h if ubm is not None else None, _pd(n),
                                          _pd(F), min_var, _pd(W)))
+        return W
+
+    def fmllr_transform(self, n, F, S, tot, iters=20):
+        M, D = self.dims()
+        n, F, S = _d(n), _d(F), _d(S)
+        W = np.zeros((D, D + 1))
+        _chk(_L().gmmhost_fmllr_transform(self._h, _pd(n), _pd(F), _pd(S), int(tot), int(iters),
+                                          _pd(W)))
         return W
 
     def mllr_apply(self, W):

@@ -50,7 +50,10 @@ enum {
  * (trainer/OL_GMMStatsEstimator.h:118-123).  n is always produced. */
 enum {
     GMMHIP_ACC_MEAN = 1,     /* export F = sum g*x   */
-    GMMHIP_ACC_VAR = 2       /* export S = sum g*x*x */
+    GMMHIP_ACC_VAR = 2,      /* export S = sum g*x*x */
+    GMMHIP_ACC_FULLVAR = 4   /* with ACC_VAR on a diagonal model: S = packed
+                                upper triangle of sum g x x^T (setAccFlags'
+                                varFull, used by fMLLR); FP32 */
 };
 
 /* Options, the reference's GMMStatsEstimator_GPU::OPTIONS

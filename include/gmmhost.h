@@ -76,6 +76,13 @@ int gmmhost_update_map(gmmhost_model_t m, gmmhost_model_t ubm, const double *n, 
  * (:1851-1878) rejects the statistics. */
 int gmmhost_mllr_transform(gmmhost_model_t m, gmmhost_model_t ubm, const double *n,
                            const double *F, float min_var, double *W);
+/* fMLLR transform W[D*(D+1)] = [A | b] (ComputeW_fMLLR + EstimateAb_fmllr +
+ * ComputeKiGi_fMLLR, OL_ModelAdapt.cpp:1614-1840) of model m (its diagonal
+ * variances) from statistics n[M], F[M*D] and full second-order S[M*D(D+1)/2]
+ * (packed upper, as gmmhip_accumulate with GMMHIP_ACC_FULLVAR gives them)
+ * over tot_samples frames, iters row-update sweeps (the reference: 20). */
+int gmmhost_fmllr_transform(gmmhost_model_t m, const double *n, const double *F, const double *S,
+                            uint64_t tot_samples, int iters, double *W);
 /* AdaptModelMLLR (:1583-1612): means of m replaced by A mu + b. */
 int gmmhost_mllr_apply(gmmhost_model_t m, const double *W);
 

@@ -47,6 +47,7 @@ def lib():
         _lib.orc_min_loglike.restype = ctypes.c_double
         _lib.orc_min_gamma.restype = ctypes.c_double
         _lib.orc_mllr_kg.restype = ctypes.c_double
+        _lib.orc_det.restype = ctypes.c_double
         _lib.orc_diag_cov_multiplier.restype = ctypes.c_float
         _lib.orc_offdiag_cov_multiplier.restype = ctypes.c_float
     return _lib
@@ -345,3 +346,33 @@ def update_em_robust(w, mu, var, n, F, S, aux, tot_samples, min_var=1e-6, full=F
     if left < 0:
         raise ValueError("global variance equal zero")
     return w, mu, var, left
+
+
+def det(A):
+    A = _f64(A)
+    return lib().orc_det(A.shape[0], _p(A))
+
+
+def fmllr_kg(mu, var, n, F, S, i):
+    """(G_i [(D+1)^2], k_i [D+1]) of fMLLR row i from full (packed upper)
+    statistics and the diagonal variances."""
+    mu, var = _f32(mu), _f32(var)
+    M, D = mu.shape
+    n, F, S = _f64(n), _f64(F), _f64(S)
+    G = np.zeros((D + 1, D + 1))
+    k = np.zeros(D + 1)
+    lib().orc_fmllr_kg(M, D, _p(mu), _p(var), _p(n), _p(F), _p(S), int(i), _p(G), _p(k))
+    return G, k
+
+
+def fmllr(mu, var, n, F, S, tot_samples, iters=20):
+    """fMLLR transform W = [A | b] (D x (D+1))."""
+    mu, var = _f32(mu), _f32(var)
+    M, D = mu.shape
+    n, F, S = _f64(n), _f64(F), _f64(S)
+    W = np.zeros((D, D + 1))
+    rc = lib().orc_fmllr(M, D, _p(mu), _p(var), _p(n), _p(F), _p(S),
+                         ctypes.c_double(tot_samples), int(iters), _p(W))
+    if rc:
+        raise ValueError("fMLLR: negative discriminant or zero root")
+    return W

@@ -1066,3 +1066,191 @@ int orc_update_em_robust(int M, int D, int full, float *w, float *mu, float *var
     free(nind);
     return left;
 }
+
+/* ------------------------------------------------------------------------
+ * fMLLR (constrained MLLR: one affine feature transform).  Paths relative
+ * to GMM/lib_bmssr/.
+ *
+ * How it is pinned (tests/test_oracle_fmllr.py): G_i / k_i against numpy
+ * sums over the statistics; frames drawn from the model give a transform
+ * close to the identity; frames pushed through a planted affine map give
+ * that map's inverse; each row update does not decrease the row's
+ * auxiliary function; the determinant against numpy.
+ * ------------------------------------------------------------------------ */
+
+/* MATRIX::_Det_Gen (matrices/Matrix4_LMa.cpp): Gaussian elimination with
+ * partial pivoting (row of the largest magnitude in the column), FP64. */
+double orc_det(int n, const double *A_in)
+{
+    double *A = (double *)malloc(sizeof(double) * (size_t)n * n);
+    memcpy(A, A_in, sizeof(double) * (size_t)n * n);
+    double det = 1.0;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (fabs(A[(size_t)i * n + k]) > fabs(A[(size_t)p * n + k]))
+                p = i;
+        if (A[(size_t)p * n + k] == 0.0) {
+            free(A);
+            return 0.0;
+        }
+        if (p != k) {
+            for (int j = 0; j < n; j++) {
+                const double t = A[(size_t)k * n + j];
+                A[(size_t)k * n + j] = A[(size_t)p * n + j];
+                A[(size_t)p * n + j] = t;
+            }
+            det = -det;
+        }
+        det *= A[(size_t)k * n + k];
+        for (int i = k + 1; i < n; i++) {
+            const double piv = A[(size_t)i * n + k] / A[(size_t)k * n + k];
+            for (int j = k + 1; j < n; j++)
+                A[(size_t)i * n + j] -= piv * A[(size_t)k * n + j];
+        }
+    }
+    free(A);
+    return det;
+}
+
+/* ComputeKiGi_fMLLR (trainer/OL_ModelAdapt.cpp:1793-1840) for row i, from
+ * full second-order statistics S (packed upper) of a model with diagonal
+ * variances var (GetMixtureVarDiag): over components with n > 0,
+ *   G_i += (1/var_mi) [[S_m, F_m], [F_m^T, n_m]],  k_i += (mu_mi/var_mi) [F_m, n_m],
+ * with 1/var_mi and mu_mi/var_mi rounded to float as the reference computes
+ * them from its float model.  G (E x E, E = D+1) and k (E) are overwritten. */
+void orc_fmllr_kg(int M, int D, const float *mu, const float *var, const double *n,
+                  const double *F, const double *S, int i, double *G, double *k)
+{
+    const int E = D + 1;
+    const size_t DP = (size_t)D * (D + 1) / 2;
+    memset(G, 0, sizeof(double) * (size_t)E * E);
+    memset(k, 0, sizeof(double) * E);
+    for (int m = 0; m < M; m++) {
+        const double np = n[m];
+        if (!(np > 0))
+            continue;
+        const float vm = var[(size_t)m * D + i], mm = mu[(size_t)m * D + i];
+        const double cm = (double)(mm / vm), cv = (double)(1.0f / vm);
+        const double *Fm = F + (size_t)m * D, *Sm = S + (size_t)m * DP;
+        for (int a = 0; a < D; a++) {
+            k[a] += cm * Fm[a];
+            for (int b = a; b < D; b++) {
+                G[(size_t)a * E + b] += cv * Sm[packed_upper(D, a, b)];
+                G[(size_t)b * E + a] = G[(size_t)a * E + b];
+            }
+            G[(size_t)a * E + D] += cv * Fm[a];
+            G[(size_t)D * E + a] = G[(size_t)a * E + D];
+        }
+        k[D] += cm * np;
+        G[(size_t)D * E + D] += cv * np;
+    }
+}
+
+/* ComputeW_fMLLR + EstimateAb_fmllr (trainer/OL_ModelAdapt.cpp:1614-1791):
+ * A = I, b = 0; per iteration the cofactors of the current A (columns of
+ * inv(A) det(A), fixed for the whole sweep), then for every row i the
+ * roots f of  beta f^2 - (p Ginv k) f - p Ginv p^T = 0  (beta = the frame
+ * count, rounded to float; p = [cofactor column i, 0]), the candidates
+ * w = (p / f + k) Ginv, and the one with the larger
+ * Q = sum w (k - G w / 2) becomes row i of [A | b].  Inverses by
+ * orc_gj_invert (the reference's Gauss-Jordan), the determinant by
+ * orc_det.  W (D x (D+1), row-major) is written; returns 0, or -1 on a
+ * negative discriminant / zero root (the reference throws). */
+int orc_fmllr(int M, int D, const float *mu, const float *var, const double *n, const double *F,
+              const double *S, double tot_samples, int iters, double *W)
+{
+    const int E = D + 1;
+    double *G = (double *)malloc(sizeof(double) * (size_t)D * E * E);
+    double *Gi = (double *)malloc(sizeof(double) * (size_t)D * E * E);
+    double *K = (double *)malloc(sizeof(double) * (size_t)D * E);
+    double *A = (double *)calloc((size_t)D * D, sizeof(double));
+    double *b = (double *)calloc((size_t)D, sizeof(double));
+    double *cof = (double *)malloc(sizeof(double) * (size_t)D * D);
+    double *p = (double *)malloc(sizeof(double) * E), *u = (double *)malloc(sizeof(double) * E);
+    double *w[2], *gw = (double *)malloc(sizeof(double) * E);
+    double *tv = (double *)malloc(sizeof(double) * E);
+    w[0] = (double *)malloc(sizeof(double) * E);
+    w[1] = (double *)malloc(sizeof(double) * E);
+    int rc = 0;
+    for (int i = 0; i < D; i++) {
+        orc_fmllr_kg(M, D, mu, var, n, F, S, i, G + (size_t)i * E * E, K + (size_t)i * E);
+        memcpy(Gi + (size_t)i * E * E, G + (size_t)i * E * E, sizeof(double) * (size_t)E * E);
+        orc_gj_invert(E, Gi + (size_t)i * E * E);
+    }
+    for (int d = 0; d < D; d++)
+        A[(size_t)d * D + d] = 1.0;
+    const double beta = (double)(float)tot_samples;
+    for (int it = 0; it < iters && rc == 0; it++) {
+        const double det = orc_det(D, A);
+        memcpy(cof, A, sizeof(double) * (size_t)D * D);
+        orc_gj_invert(D, cof);
+        for (int e = 0; e < D * D; e++)
+            cof[e] *= det;
+        for (int i = 0; i < D && rc == 0; i++) {
+            const double *Gr = G + (size_t)i * E * E, *Gir = Gi + (size_t)i * E * E;
+            const double *kr = K + (size_t)i * E;
+            for (int a = 0; a < D; a++)
+                p[a] = cof[(size_t)a * D + i];
+            p[D] = 0.0;
+            /* u = p^T Ginv */
+            for (int c = 0; c < E; c++) {
+                double s = 0.0;
+                for (int a = 0; a < E; a++)
+                    s += p[a] * Gir[(size_t)a * E + c];
+                u[c] = s;
+            }
+            double bb = 0.0, cc = 0.0;
+            for (int c = 0; c < E; c++) {
+                bb += u[c] * kr[c];
+                cc += u[c] * p[c];
+            }
+            bb = -bb;
+            cc = -cc;
+            const double sq = sqrt(pow(bb, 2) - 4 * beta * cc);
+            if (sq < 0 || isnan(sq)) {
+                rc = -1;
+                break;
+            }
+            const double f[2] = {(-bb + sq) / (2 * beta), (-bb - sq) / (2 * beta)};
+            double Q[2];
+            for (int j = 0; j < 2; j++) {
+                if (f[j] == 0) {
+                    rc = -1;
+                    break;
+                }
+                for (int c = 0; c < E; c++)
+                    tv[c] = p[c] / f[j] + kr[c];
+                for (int c = 0; c < E; c++) {
+                    double s = 0.0;
+                    for (int a = 0; a < E; a++)
+                        s += tv[a] * Gir[(size_t)a * E + c];
+                    w[j][c] = s;
+                }
+                for (int a = 0; a < E; a++) {
+                    double s = 0.0;
+                    for (int c = 0; c < E; c++)
+                        s += Gr[(size_t)a * E + c] * w[j][c];
+                    gw[a] = s;
+                }
+                Q[j] = 0.0;
+                for (int c = 0; c < E; c++)
+                    Q[j] += w[j][c] * (kr[c] - 0.5 * gw[c]);
+            }
+            if (rc)
+                break;
+            const int idx = (Q[0] > Q[1]) ? 0 : 1;
+            for (int c = 0; c < D; c++)
+                A[(size_t)i * D + c] = w[idx][c];
+            b[i] = w[idx][D];
+        }
+    }
+    for (int i = 0; i < D; i++) {
+        for (int c = 0; c < D; c++)
+            W[(size_t)i * E + c] = A[(size_t)i * D + c];
+        W[(size_t)i * E + D] = b[i];
+    }
+    free(G); free(Gi); free(K); free(A); free(b); free(cof); free(p); free(u); free(gw); free(tv);
+    free(w[0]); free(w[1]);
+    return rc;
+}

@@ -152,3 +152,36 @@ def test_full_errors(acc):
     acc.set_model(w, mu, np.ones((3, 4), np.float32))
     n, F, S, tot, nf = acc.accumulate()
     assert S.shape == (3, 4)
+
+
+@pytest.mark.parametrize("N,D,M", [(3000, 6, 16), (2000, 39, 64)])
+def test_full_second_order_statistics_of_a_diagonal_model(acc, N, D, M):
+    """GMMHIP_ACC_FULLVAR (setAccFlags' varFull, what fMLLR accumulates):
+    the posteriors of a diagonal model, the packed upper sum g x x^T."""
+    rng = np.random.default_rng(N + D)
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    var = rng.uniform(0.3, 1.5, (M, D)).astype(np.float32)
+    x = rng.normal(0, 2.5, (N, D)).astype(np.float32)
+    acc.set_opts(precision=gmm_amd.PREC_FP32)
+    acc.set_model(w, mu, var)
+    acc.set_frames(x)
+    n, F, S, tot, nf = acc.accumulate(gmm_amd.ACC_MEAN | gmm_amd.ACC_VAR | gmm_amd.ACC_FULLVAR)
+    cov = np.zeros((M, D, D), np.float32)
+    cov[:, np.arange(D), np.arange(D)] = var
+    rn, rF, rS, rtot = orc.accumulate_full(x, w, mu, cov)
+    assert nf == N and tot == pytest.approx(rtot, rel=2e-6)
+    r, c = np.triu_indices(D)
+    rSd = np.maximum(rS[:, r == c], 0)
+    xm = float(np.abs(x).max())
+    assert np.all(np.abs(n - rn) <= 1e-4 * rn + 3e-4)
+    assert np.all(np.abs(F - rF) <= 1e-4 * np.sqrt(rn[:, None] * rSd) + 3e-4 * xm)
+    assert np.all(np.abs(S - rS) <= 1e-4 * np.sqrt(rSd[:, r] * rSd[:, c]) + 3e-4 * xm * xm)
+    # the diagonal of the full statistics matches the diagonal path
+    n2, F2, S2, _, _ = acc.accumulate()
+    assert np.allclose(S[:, r == c], S2, rtol=2e-4, atol=3e-4 * xm * xm)
+    # mixed precision: the GEMM path still runs in FP32
+    acc.set_opts(precision=gmm_amd.PREC_FP16X3)
+    n3, _, S3, _, _ = acc.accumulate(gmm_amd.ACC_MEAN | gmm_amd.ACC_VAR | gmm_amd.ACC_FULLVAR)
+    acc.set_opts(precision=gmm_amd.PREC_FP32)
+    assert np.array_equal(S3, S) and np.array_equal(n3, n)

@@ -253,3 +253,22 @@ def test_train_map_robust(tmp_path):
     gw, gmu, gvar, _ = fm.read_gmm_txt(out)
     assert np.allclose(gmu, rmu, rtol=1e-5, atol=1e-5)
     assert np.allclose(gvar, rvar, rtol=1e-4, atol=1e-6)
+
+
+def test_train_fmllr_from_ubm(tmp_path):
+    """--type 3 (fMLLR): the saved feature transform against the oracle's
+    fMLLR from the oracle's full second-order statistics under the UBM."""
+    x, w, mu, var, ubm_p, data_p = _ubm_case(tmp_path, seed=51, M=32, D=6, N=20000)
+    out = tmp_path / "fmllr.gmm"
+    run("trainGMM", "-i", data_p, "--data-T", 2, "--in-UBM", ubm_p, "-L", "--type", 3,
+        "-o", out, "-T")
+    D = mu.shape[1]
+    cov = np.zeros((len(w), D, D), np.float32)
+    cov[:, np.arange(D), np.arange(D)] = var
+    n, F, S, _ = orc.accumulate_full(x, w, mu, cov)
+    W = orc.fmllr(mu, var, n, F, S, x.shape[0])
+    lines = (tmp_path / "fmllr.mat").read_text().split("\n")
+    assert lines[0].strip() == "fMLLR"
+    Wg = np.array([[float(v) for v in ln.split()] for ln in lines[2:2 + D]])
+    assert np.allclose(Wg, W, rtol=1e-4, atol=1e-4)
+    assert not out.exists()  # fMLLR saves the transform only

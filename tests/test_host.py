@@ -269,3 +269,27 @@ def test_update_em_robust_matches_oracle(full):
     gw, gmu, gv = m.get()
     assert left == oleft
     assert np.array_equal(gw, ow) and np.array_equal(gmu, omu) and np.array_equal(gv, ov)
+
+
+@pytest.mark.parametrize("iters", [1, 20])
+def test_fmllr_transform_matches_oracle(iters):
+    rng = np.random.default_rng(29)
+    M, D, N = 32, 5, 6000
+    w = rng.dirichlet(np.ones(M)).astype(np.float32)
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    var = rng.uniform(0.3, 1.5, (M, D)).astype(np.float32)
+    p = w.astype(np.float64)
+    k = rng.choice(M, size=N, p=p / p.sum())
+    A0 = np.eye(D) + rng.normal(0, 0.1, (D, D))
+    y = mu[k] + np.sqrt(var[k]) * rng.normal(size=(N, D))
+    x = ((y - 0.2) @ np.linalg.inv(A0).T).astype(np.float32)
+    cov = np.zeros((M, D, D), np.float32)
+    cov[:, np.arange(D), np.arange(D)] = var
+    n, F, S, _ = orc.accumulate_full(x, w, mu, cov)
+    m = host.Model.create(w, mu, var)
+    W = m.fmllr_transform(n, F, S, N, iters=iters)
+    R = orc.fmllr(mu, var, n, F, S, N, iters=iters)
+    assert np.allclose(W, R, rtol=1e-9, atol=1e-9)
+    # the same transform from a full-covariance model with those variances
+    mf = host.Model.create_full(w, mu, cov)
+    assert np.allclose(mf.fmllr_transform(n, F, S, N, iters=iters), W, rtol=1e-12, atol=1e-12)
