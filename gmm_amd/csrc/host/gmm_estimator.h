@@ -10,11 +10,14 @@
 // exception classes the reference throws.
 #pragma once
 
+#include <future>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../../include/gmmhip.h"
+#include "feature_io.h"
 #include "gmm_model.h"
 #include "gmm_stats.h"
 
@@ -148,6 +151,58 @@ public:
         gmms._totalAccSamples += nf;
         gmms._totLogLike += (TS)tot;
         sameData_ = false;
+    }
+
+    // accumulateStatsMT over a file list (OL_GMMStatsEstimator.cpp:641-707):
+    // files (prm_type: 0 SVES, 1 HTK, 2 RAW; every dwnsmp-th frame) are
+    // grouped until a group exceeds _memoryBuffDataGB (0: one file per
+    // group) and every group is accumulated; a missing or empty file is
+    // skipped with a warning.  The next group is read by a second thread
+    // while the device accumulates the current one.
+    void accumulateStatsMT(const std::vector<std::string> &files, GMMStats<TS> &gmms, int prm_type,
+                           unsigned int dwnsmp = 1) {
+        const double giga = 1024.0 * 1024.0 * 1024.0;
+        size_t next = 0;
+        auto load_group = [&files, prm_type, dwnsmp, giga, this](size_t &pos) {
+            Features g;
+            while (pos < files.size()) {
+                Features f;
+                std::string err;
+                const std::string &fn = files[pos++];
+                if (LoadFeatures(fn.c_str(), prm_type, f, &err) != 0) {
+                    std::cerr << "\n WARNING: Param file [" << fn << "] not found -> skipped!"
+                              << std::endl;
+                    continue;
+                }
+                if (f.N < 1) {
+                    std::cerr << "\n WARNING: Param file [" << fn << "] empty" << std::endl;
+                    continue;
+                }
+                if (g.D && f.D != g.D)
+                    throw std::logic_error("accumulateStatsMT(): feature dimensions differ across files");
+                g.D = f.D;
+                for (unsigned i = 0; i < f.N; i += (dwnsmp > 1 ? dwnsmp : 1)) {
+                    g.data.insert(g.data.end(), f.data.begin() + (size_t)i * f.D,
+                                  f.data.begin() + (size_t)(i + 1) * f.D);
+                    g.N++;
+                }
+                if (!(_memoryBuffDataGB > (double)g.N * g.D * sizeof(float) / giga)) break;
+            }
+            return g;
+        };
+        std::future<Features> pending = std::async(std::launch::async, load_group, std::ref(next));
+        for (;;) {
+            Features cur = pending.get();
+            if (cur.N == 0) break;
+            if (next < files.size())
+                pending = std::async(std::launch::async, load_group, std::ref(next));
+            else
+                pending = std::async(std::launch::deferred, [] { return Features(); });
+            cur.rows.resize(cur.N);
+            for (unsigned i = 0; i < cur.N; i++) cur.rows[i] = &cur.data[(size_t)i * cur.D];
+            sameData_ = false;
+            accumulateStatsMT(cur.rows.data(), cur.N, cur.D, gmms);
+        }
     }
 
     // Per-frame log-likelihoods of the N frames into outLogLikes[N].

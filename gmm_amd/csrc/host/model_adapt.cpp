@@ -23,6 +23,28 @@ void CModelAdapt::InsertData(float **vectors, unsigned int N, unsigned int D) {
     data_ = vectors;
     N_ = N;
     D_ = D;
+    files_.clear();
+    dataChanged_ = true;
+}
+
+void CModelAdapt::InsertFiles(const std::vector<std::string> &files, int prm_type,
+                              unsigned int dwnsmp, float buffGB) {
+    unsigned int N = 0, D = 0;
+    for (const std::string &fn : files) {
+        Features f;
+        if (LoadFeatures(fn.c_str(), prm_type, f, nullptr) != 0 || f.N == 0) continue;
+        if (D && f.D != D) throw std::logic_error("InsertData(): feature dimensions differ across files");
+        D = f.D;
+        N += dwnsmp > 1 ? (f.N + dwnsmp - 1) / dwnsmp : f.N;
+    }
+    if (N == 0) throw std::runtime_error("InsertData(): no feature vectors in the given files");
+    files_ = files;
+    fileType_ = prm_type;
+    dwnsmp_ = dwnsmp;
+    buffGB_ = buffGB;
+    data_ = nullptr;
+    N_ = N;
+    D_ = D;
     dataChanged_ = true;
 }
 
@@ -40,10 +62,17 @@ void CModelAdapt::initEstimator() {
 // rearranges it), set the flags and add the statistics of the inserted data.
 void CModelAdapt::AccumulateStats(GMModel *model, GMMStats<double> &stats, bool mixProbsOnly,
                                   bool meanStats, bool varStats, bool auxStats) {
-    if (!data_ || N_ == 0) throw std::logic_error("AccumulateStats(): Insert data first!");
+    if ((!data_ && files_.empty()) || N_ == 0)
+        throw std::logic_error("AccumulateStats(): Insert data first!");
     initEstimator();
     if (model) est_->insertModel(*model);
     est_->setAccFlags(mixProbsOnly, meanStats, varStats, stats.hasFullCov(), auxStats);
+    if (!files_.empty()) {
+        est_->_memoryBuffDataGB = buffGB_;
+        est_->accumulateStatsMT(files_, stats, fileType_, dwnsmp_);
+        dataChanged_ = true;
+        return;
+    }
     if (!dataChanged_) est_->setSameDataFlag();
     est_->accumulateStatsMT(data_, N_, D_, stats);
     dataChanged_ = false;
@@ -657,7 +686,7 @@ MLLRTransform CModelAdapt::IterW_MLLR(GMModel *UBM, int iters, bool wantQ) {
 
 void CModelAdapt::AdaptModel(GMModel *UBM) {
     if (!UBM) throw std::logic_error("AdaptModel(): UBM not given!");
-    if (!data_) throw std::logic_error("AdaptModel(): Insert adaptation data first!");
+    if (!data_ && files_.empty()) throw std::logic_error("AdaptModel(): Insert adaptation data first!");
     // statistics as the reference allocates them (OL_ModelAdapt.cpp:281-298):
     // full second-order statistics for a full UBM, none for MLLR
     const bool fullV = UBM->GetFullCovStatus();
@@ -754,7 +783,8 @@ static void leading_eigenpair(int D, const float *cov, double &lam, std::vector<
 // higher index, into the highest-index zero-weight slot, GetGaussStats
 // :965-991).
 void CModelAdapt::TrainFromScratch(unsigned int nummix, bool fullcov) {
-    if (!data_ || N_ <= 3) throw std::runtime_error("TrainFromScratch(): Not enough data to model training.");
+    if ((!data_ && files_.empty()) || N_ <= 3)
+        throw std::runtime_error("TrainFromScratch(): Not enough data to model training.");
     const int D = (int)D_;
     // Global statistics: a single component with a broad variance has
     // posterior 1 for every finite frame.  The first pass finds the mean; the

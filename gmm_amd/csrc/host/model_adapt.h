@@ -58,6 +58,12 @@ public:
 
     // Frames are borrowed (row pointers), as in InsertData.
     void InsertData(float **vectors, unsigned int N, unsigned int D);
+    // Feature files read file by file at every accumulation (the reference's
+    // InsertData(CFileList) + --file-by-file): prm_type 0 SVES, 1 HTK, 2 RAW,
+    // every dwnsmp-th frame, groups of up to buffGB gigabytes.  The files
+    // are scanned once here for their frame count and dimension.
+    void InsertFiles(const std::vector<std::string> &files, int prm_type, unsigned int dwnsmp,
+                     float buffGB);
 
     // MAP / MLLR / MLLR_MAP adaptation of UBM (AdaptModel, :1163-1247).
     void AdaptModel(GMModel *UBM);
@@ -110,6 +116,10 @@ private:
     OPTIONS opt_;
     float **data_ = nullptr;
     unsigned int N_ = 0, D_ = 0;
+    std::vector<std::string> files_;  // file-by-file mode when non-empty
+    int fileType_ = 2;
+    unsigned int dwnsmp_ = 1;
+    float buffGB_ = 0.0f;
     bool dataChanged_ = true;
     std::unique_ptr<GMMStatsEstimator_HIP<double>> est_;
     GMMStats<double> stats_;

@@ -4,7 +4,12 @@
 // supported: --type 0 (from scratch; --full-cov for full covariances),
 // 1 (MLLR), 2 (MAP/EM), 3 (fMLLR), 4 (MLLR+MAP); --robust (-R) for the
 // robust covariance estimate (from scratch and MAP/EM).
-// --SSE/--CUDA are accepted and ignored: statistics always run on the GPU.
+// --file-by-file (-f) reads the feature files at every accumulation in groups
+// of --prm-buffer GB (the next group is read while the device works on the
+// current one); by default all frames stay resident in HBM across the EM
+// iterations.  --SSE/--CUDA are accepted and ignored: statistics always run
+// on the GPU.
+#include <algorithm>
 #include <cstdio>
 #include <iostream>
 #include <string>
@@ -55,6 +60,9 @@ int main(int argc, char **argv) {
     a.option("precision", 0, false, "fp32");
     a.option("full-cov", 0, true, "0");
     a.option("robust", 'R', true, "0");
+    a.option("file-by-file", 'f', true, "0");
+    a.option("prm-buffer", 0, false, "0.005");
+    a.option("dwnsmp", 0, false, "1");
     a.option("verbosity", 'v', true, "0");
     try {
         a.parse(argc, argv);
@@ -67,17 +75,27 @@ int main(int argc, char **argv) {
         }
         Features feats;
         const int dtype = a.i("data-T");
-        for (const std::string &fn : a.all("in-file")) {
-            Features f;
-            std::string err;
-            if (LoadFeatures(fn.c_str(), dtype, f, &err) != 0) throw std::runtime_error(err);
-            if (feats.D && f.D != feats.D) throw std::runtime_error("feature dimensions differ across files");
-            feats.D = f.D;
-            feats.data.insert(feats.data.end(), f.data.begin(), f.data.end());
-            feats.N += f.N;
+        const unsigned dwnsmp = (unsigned)std::max(1, a.i("dwnsmp"));
+        const bool by_file = a.b("file-by-file");
+        if (!by_file) {
+            // all frames resident (in host memory, then in HBM across the EM
+            // iterations)
+            for (const std::string &fn : a.all("in-file")) {
+                Features f;
+                std::string err;
+                if (LoadFeatures(fn.c_str(), dtype, f, &err) != 0) throw std::runtime_error(err);
+                if (feats.D && f.D != feats.D)
+                    throw std::runtime_error("feature dimensions differ across files");
+                feats.D = f.D;
+                for (unsigned i = 0; i < f.N; i += dwnsmp) {
+                    feats.data.insert(feats.data.end(), f.data.begin() + (size_t)i * f.D,
+                                      f.data.begin() + (size_t)(i + 1) * f.D);
+                    feats.N++;
+                }
+            }
+            feats.rows.resize(feats.N);
+            for (unsigned i = 0; i < feats.N; i++) feats.rows[i] = &feats.data[(size_t)i * feats.D];
         }
-        feats.rows.resize(feats.N);
-        for (unsigned i = 0; i < feats.N; i++) feats.rows[i] = &feats.data[(size_t)i * feats.D];
 
         CModelAdapt ad;
         CModelAdapt::OPTIONS o;
@@ -101,7 +119,10 @@ int main(int argc, char **argv) {
         o._precision = precision_of(a.str("precision"));
         o._robustCov = a.b("robust");
         ad.SetOptions(o);
-        ad.InsertData(feats.rows.data(), feats.N, feats.D);
+        if (by_file)
+            ad.InsertFiles(a.all("in-file"), dtype, dwnsmp, (float)a.f("prm-buffer"));
+        else
+            ad.InsertData(feats.rows.data(), feats.N, feats.D);
         if (o._adaptType == FROM_SCRATCH) {
             if (a.i("nummix") < 1) throw std::runtime_error("--type 0 needs --nummix");
             ad.TrainFromScratch((unsigned)a.i("nummix"), a.b("full-cov"));

@@ -272,3 +272,42 @@ def test_train_fmllr_from_ubm(tmp_path):
     Wg = np.array([[float(v) for v in ln.split()] for ln in lines[2:2 + D]])
     assert np.allclose(Wg, W, rtol=1e-4, atol=1e-4)
     assert not out.exists()  # fMLLR saves the transform only
+
+
+@pytest.mark.parametrize("buff", ["0", "0.0002"])
+def test_file_by_file_matches_resident(tmp_path, golden, buff):
+    """--file-by-file (-f): data.bin split into 7 RAW files, read file by
+    file (one file per group, or groups of --prm-buffer GB) at every EM
+    iteration, gives the model of the resident run; --dwnsmp 2 keeps every
+    second frame of every file."""
+    x = golden["x"]
+    parts = np.array_split(x, 7)
+    files = []
+    for i, p in enumerate(parts):
+        fn = tmp_path / ("part%d.raw" % i)
+        fm.write_raw(fn, p)
+        files.append(fn)
+    a_out, b_out = tmp_path / "a.gmm", tmp_path / "b.gmm"
+    common = ["--data-T", 2, "--type", 0, "--nummix", 4, "--it-EM", 8, "-T"]
+    run("trainGMM", *sum([["-i", f] for f in files], []), *common, "-o", a_out, "-f",
+        "--prm-buffer", buff)
+    run("trainGMM", "-i", os.path.join(GOLD, "data.bin"), *common, "-o", b_out)
+    # FP32 partial sums group the frames differently (per file vs per
+    # resident chunk): the models agree to a few units in the 6th digit
+    wa, ma, va, _ = fm.read_gmm_txt(a_out)
+    wb, mb, vb, _ = fm.read_gmm_txt(b_out)
+    print("max rel diff w %.2e mu %.2e var %.2e" % (np.abs(wa / wb - 1).max(),
+          (np.abs(ma - mb) / (np.abs(mb) + 1e-3)).max(), np.abs(va / vb - 1).max()))
+    assert np.allclose(wa, wb, rtol=2e-5) and np.allclose(ma, mb, rtol=2e-5, atol=1e-5)
+    assert np.allclose(va, vb, rtol=2e-5)
+    # down-sampling per file, against the resident run on the kept frames
+    c_out, d_out = tmp_path / "c.gmm", tmp_path / "d.gmm"
+    run("trainGMM", *sum([["-i", f] for f in files], []), *common, "-o", c_out, "-f",
+        "--prm-buffer", buff, "--dwnsmp", 2)
+    kept = np.concatenate([p[::2] for p in parts])
+    fm.write_raw(tmp_path / "kept.raw", kept)
+    run("trainGMM", "-i", tmp_path / "kept.raw", *common, "-o", d_out)
+    wc, mc, vc, _ = fm.read_gmm_txt(c_out)
+    wd, md, vd, _ = fm.read_gmm_txt(d_out)
+    assert np.allclose(wc, wd, rtol=2e-5) and np.allclose(mc, md, rtol=2e-5, atol=1e-5)
+    assert np.allclose(vc, vd, rtol=2e-5)
