@@ -21,7 +21,8 @@ EXPORTS = [
     "gmmhip_default_opts", "gmmhip_device_count", "gmmhip_create",
     "gmmhip_destroy", "gmmhip_set_opts", "gmmhip_last_error",
     "gmmhip_set_model", "gmmhip_set_model_full", "gmmhip_set_frames", "gmmhip_set_frames_contig",
-    "gmmhip_set_frames_device", "gmmhip_accumulate", "gmmhip_accumulate_aux", "gmmhip_loglikes",
+    "gmmhip_set_frames_device", "gmmhip_accumulate", "gmmhip_accumulate_aux", "gmmhip_set_mask",
+    "gmmhip_posteriors", "gmmhip_loglikes",
     "gmmhip_loglikes_device", "gmmhip_stream", "gmmhip_sync",
     "gmmhip_kernel_times", "gmmhip_reset_kernel_times", "gmmhip_batch_adapt",
     "gmmhip_server_create", "gmmhip_server_submit", "gmmhip_server_wait",
@@ -121,6 +122,9 @@ def lib():
         L.gmmhip_accumulate.argtypes = [c_vp, ctypes.c_int, c_dp, c_dp, c_dp, c_dp,
                                         ctypes.POINTER(c_u64)]
         L.gmmhip_accumulate_aux.argtypes = [c_vp, c_dp]
+        L.gmmhip_set_mask.argtypes = [c_vp, c_u64, ctypes.POINTER(c_u64),
+                                      ctypes.POINTER(ctypes.c_int)]
+        L.gmmhip_posteriors.argtypes = [c_vp, c_fp, c_fp]
         L.gmmhip_loglikes.argtypes = [c_vp, c_fp]
         L.gmmhip_loglikes_device.argtypes = [c_vp, c_vp]
         L.gmmhip_stream.restype = c_vp
@@ -286,6 +290,28 @@ class Accumulator:
         aux = np.zeros((self.M, self.D + 2)) if into is None else into
         self._check(lib().gmmhip_accumulate_aux(self._h, _dp(aux)))
         return aux
+
+    def set_mask(self, lists):
+        """Gaussian selection: lists[t] = component indices of frame t (None
+        removes the mask)."""
+        if lists is None:
+            self._check(lib().gmmhip_set_mask(self._h, 0, None, None))
+            return
+        off = np.zeros(len(lists) + 1, np.uint64)
+        off[1:] = np.cumsum([len(l) for l in lists])
+        idx = np.ascontiguousarray(np.concatenate([np.asarray(l, np.int32) for l in lists])
+                                   if off[-1] else np.zeros(1, np.int32), np.int32)
+        self._check(lib().gmmhip_set_mask(
+            self._h, len(lists), off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def posteriors(self):
+        """(ll[N], post[N, M]): per-frame log-likelihoods and posteriors
+        before the cut."""
+        ll = np.zeros(self.N, np.float32)
+        post = np.zeros((self.N, self.M), np.float32)
+        self._check(lib().gmmhip_posteriors(self._h, _fp(ll), _fp(post)))
+        return ll, post
 
     def loglikes(self):
         out = np.zeros(self.N, np.float32)

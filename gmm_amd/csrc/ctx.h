@@ -92,6 +92,14 @@ struct gmmhip_ctx {
     bool full_model = false;
     gmmhip::AuxState *aux = nullptr;
 
+    // Gaussian selection (gmmhip_set_mask): per-frame component bitmap
+    // [mask_N][mask_W] in HBM; while set, every E-step call runs the GEMM
+    // form with it
+    uint32_t *d_mask = nullptr;
+    size_t mask_cap = 0;
+    uint64_t mask_N = 0;
+    int mask_W = 0, mask_M = 0;
+
     // batched per-utterance state (batch_api.hip)
     gmmhip_batch_state *batch = nullptr;
     float *d_mu_f32 = nullptr, *d_var_f32 = nullptr;  // UBM means / variances
@@ -169,12 +177,20 @@ void batch_state_free(gmmhip_batch_state *b);
 // packed upper-triangle second-order statistics.
 void full_state_free(gmmhip::FullState *fs);
 void aux_state_free(gmmhip::AuxState *a);
-int full_loglikes_device(gmmhip_ctx *h, float *d_out);
+// GMMHIP_ERR_ARG when a Gaussian-selection mask does not fit the handle's
+// frames and model.
+int mask_check(gmmhip_ctx *h);
 int full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S, double *tot_ll,
                     uint64_t *nframes);
 // Full second-order statistics of the handle's diagonal model (GMMHIP_ACC_FULLVAR).
 int diag_full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S,
                          double *tot_ll, uint64_t *nframes);
+// The GEMM form for the handle's model (diagonal or full): per-frame
+// log-likelihoods into device memory, statistics (full_accumulate after
+// gemm_operand_for_model), posteriors before the cut into host memory.
+int gemm_operand_for_model(gmmhip_ctx *h);
+int gemm_loglikes_device(gmmhip_ctx *h, float *d_out);
+int gemm_posteriors(gmmhip_ctx *h, float *ll, float *post);
 
 // Device copies of the active model for the per-utterance kernels: FP32
 // means / variances and, with mllr, the UBM-only MLLR operands

@@ -126,6 +126,21 @@ __global__ __launch_bounds__(256) void fc_lse_kernel(float *__restrict__ L, int 
     if (threadIdx.x == 0) blockLL[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// Gaussian selection: log-likelihoods of the components outside frame
+// n0 + t's selection (bit m of its row of the bitmap, W 32-bit words per
+// frame) become -3e38, so they vanish from the log-sum-exp and get posterior
+// 0; a frame with an empty selection is clamped.  One block per frame.
+__global__ __launch_bounds__(256) void fc_mask_kernel(float *__restrict__ L, int nc, int M,
+                                                      const uint32_t *__restrict__ bits, int W,
+                                                      uint64_t n0) {
+    const int t = blockIdx.x;
+    if (t >= nc) return;
+    const uint32_t *row = bits + (n0 + t) * (uint64_t)W;
+    float *l = L + (size_t)t * M;
+    for (int m = threadIdx.x; m < M; m += 256)
+        if (!((row[m >> 5] >> (m & 31)) & 1u)) l[m] = -3e38f;
+}
+
 // acc[e] += sum_s part[s][e] (FP64, s in order), one element per thread:
 // the fixed-order reduction of the FP32 sub-chunk statistics.
 __global__ __launch_bounds__(256) void fc_acc_kernel(const float *__restrict__ part, size_t E,
@@ -210,11 +225,13 @@ uint64_t full_chunk(const gmmhip_ctx *h, int KP) {
     return std::min<uint64_t>(nc, h->N);
 }
 
-// One pass over the resident frames: per chunk features, L = Phi Theta^T,
-// fused log-sum-exp / posteriors, and (stats) Gamma^T Phi into the FP64
-// accumulators.  LLnat (device, optional) receives per-frame
-// log-likelihoods; *tot the FP64 total.
-int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot) {
+// One pass over the resident frames: per chunk features, L = Phi Theta^T
+// (Gaussian selection applied when a mask is set), fused log-sum-exp /
+// posteriors, and (stats) Gamma^T Phi into the FP64 accumulators.  LLnat
+// (device, optional) receives per-frame log-likelihoods; *tot the FP64
+// total; post_out (host, optional, [N][M]) the posteriors before the cut.
+int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot, float *post_out = nullptr) {
+    const int post_cut = post_out ? 0 : 1;
     gmmhip::FullState *fs = h->full;
     const int M = h->M, D = h->D, K = fs->K, KP = fs->KP;
     const uint64_t nc_max = full_chunk(h, KP);
@@ -245,9 +262,18 @@ int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot) {
         // L (column-major M x nc, frame-major rows) = Theta^T (M x KP) * Phi (KP x nc)
         BLASCHK(h, rocblas_sgemm(fs->blas, rocblas_operation_transpose, rocblas_operation_none, M,
                                  nc, KP, &one, fs->d_theta, KP, fs->d_phi, KP, &zero, fs->d_L, M));
+        if (h->d_mask) {
+            hipLaunchKernelGGL(gmmhip::fc_mask_kernel, dim3(nc), dim3(256), 0, h->stream, fs->d_L,
+                               nc, M, h->d_mask, h->mask_W, n0);
+            HIPCHK(h, hipGetLastError());
+        }
         const int blocks = (ncp + 3) / 4;
         HIPCHK(h, launch_fc_lse(fs->d_L, nc, ncp, M, fs->d_bad, h->opt.min_loglike,
-                                h->opt.min_gamma, 1, LLnat, n0, h->d_blockLL + nblk, h->stream));
+                                h->opt.min_gamma, post_cut, LLnat, n0, h->d_blockLL + nblk,
+                                h->stream));
+        if (post_out)
+            HIPCHK(h, hipMemcpyAsync(post_out + n0 * (uint64_t)M, fs->d_L, (size_t)nc * M * sizeof(float),
+                                     hipMemcpyDeviceToHost, h->stream));
         nblk += blocks;
         if (stats) {
             // part[s] (column-major KP x M) = Phi_s (KP x sub) * Gamma_s^T (sub x M), frames
@@ -285,21 +311,24 @@ void full_state_free(gmmhip::FullState *fs) {
     delete fs;
 }
 
-int full_loglikes_device(gmmhip_ctx *h, float *d_out) {
-    double tot = 0.0;
-    int rc = full_pass(h, false, d_out, &tot);
-    if (rc) return rc;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+int mask_check(gmmhip_ctx *h) {
+    if (!h->d_mask) return GMMHIP_OK;
+    if (h->mask_N != h->N || h->mask_M != h->M)
+        return fail(h, GMMHIP_ERR_ARG,
+                    "Gaussian-selection mask set for %llu frames x %d components, but the handle "
+                    "has %llu frames x %d components", (unsigned long long)h->mask_N, h->mask_M,
+                    (unsigned long long)h->N, h->M);
     return GMMHIP_OK;
 }
 
-int diag_full_operand(gmmhip_ctx *h) {
+int diag_operand(gmmhip_ctx *h, bool full_features) {
     gmmhip::FullState *fs = nullptr;
     int rc = gemm_state(h, &fs);
     if (rc) return rc;
-    if (fs->operand_version == h->model_version) return GMMHIP_OK;
+    const int kind = full_features ? gmmhip::OP_DIAG_FULLFEAT : gmmhip::OP_DIAG;
+    if (fs->operand_version == h->model_version && fs->operand_kind == kind) return GMMHIP_OK;
     const int M = h->M, D = h->D;
-    const int K = 1 + D + D * (D + 1) / 2, KP = (K + 7) / 8 * 8;
+    const int K = full_features ? 1 + D + D * (D + 1) / 2 : 1 + 2 * D, KP = (K + 7) / 8 * 8;
     std::vector<float> theta((size_t)M * KP, 0.0f);
     for (int m = 0; m < M; m++) {
         float *th = theta.data() + (size_t)m * KP;
@@ -309,13 +338,13 @@ int diag_full_operand(gmmhip_ctx *h) {
             const double iv = 1.0 / (double)h->m_var[(size_t)m * D + i];
             const double mc = (double)h->m_mu[(size_t)m * D + i] - (double)h->h_off[i];
             th[1 + i] = (float)(mc * iv);
-            th[k] = (float)(-0.5 * iv);  // (i, i); (i, j > i) stay 0
-            k += D - i;
+            th[k] = (float)(-0.5 * iv);  // x'_i^2; with full features (i, j > i) stay 0
+            k += full_features ? D - i : 1;
             cq += mc * mc * iv;
         }
         th[0] = (float)(h->m_gconst[m] - 0.5 * cq);
     }
-    const std::vector<int> codes = feature_codes(D, true);
+    const std::vector<int> codes = feature_codes(D, full_features);
     HIPCHK(h, ensure(&fs->d_theta, &fs->theta_cap, theta.size()));
     HIPCHK(h, ensure(&fs->d_pairs, &fs->pairs_cap, codes.size()));
     HIPCHK(h, ensure(&fs->d_cen, &fs->cen_cap, (size_t)D));
@@ -330,14 +359,44 @@ int diag_full_operand(gmmhip_ctx *h) {
     fs->KP = KP;
     fs->h_cen = h->h_off;
     fs->operand_version = h->model_version;
+    fs->operand_kind = kind;
     return GMMHIP_OK;
 }
 
 int diag_full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S,
                          double *tot_ll, uint64_t *nframes) {
-    int rc = diag_full_operand(h);
+    int rc = diag_operand(h, true);
     if (rc) return rc;
     return full_accumulate(h, flags, n, F, S, tot_ll, nframes);
+}
+
+int gemm_operand_for_model(gmmhip_ctx *h) {
+    // a full model's operand is built by gmmhip_set_model_full; a diagonal
+    // model gets its [1, x', x'^2] operand
+    return h->full_model ? GMMHIP_OK : diag_operand(h, false);
+}
+
+int gemm_loglikes_device(gmmhip_ctx *h, float *d_out) {
+    int rc = gemm_operand_for_model(h);
+    if (rc) return rc;
+    double tot = 0.0;
+    rc = full_pass(h, false, d_out, &tot);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return GMMHIP_OK;
+}
+
+int gemm_posteriors(gmmhip_ctx *h, float *ll, float *post) {
+    int rc = gemm_operand_for_model(h);
+    if (rc) return rc;
+    HIPCHK(h, ensure(&h->d_LLnat, &h->LLnat_cap, (size_t)h->N));
+    double tot = 0.0;
+    rc = full_pass(h, false, h->d_LLnat, &tot, post);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(ll, h->d_LLnat, h->N * sizeof(float), hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return GMMHIP_OK;
 }
 
 int full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S, double *tot_ll,
@@ -356,7 +415,18 @@ int full_accumulate(gmmhip_ctx *h, int flags, double *n, double *F, double *S, d
     // Undo the centring in FP64: x = x' + c ->
     //   F_i = F'_i + n c_i,  S_ij = S'_ij + c_i F'_j + c_j F'_i + n c_i c_j.
     const size_t DP = (size_t)D * (D + 1) / 2;
-    for (int m = 0; m < M; m++) {
+    const bool diag = fs->operand_kind == gmmhip::OP_DIAG;
+    for (int m = 0; m < M && diag; m++) {
+        const double *r = acc.data() + (size_t)m * KP;
+        const double nm = r[0];
+        n[m] += nm;
+        for (int d = 0; d < D; d++) {
+            const double c = fs->h_cen[d], f1 = r[1 + d];
+            if (flags & GMMHIP_ACC_MEAN) F[(size_t)m * D + d] += f1 + nm * c;
+            if (flags & GMMHIP_ACC_VAR) S[(size_t)m * D + d] += r[1 + D + d] + 2.0 * c * f1 + nm * c * c;
+        }
+    }
+    for (int m = 0; m < M && !diag; m++) {
         const double *r = acc.data() + (size_t)m * KP;
         const double nm = r[0];
         const double *f1 = r + 1, *s1 = r + 1 + D;
@@ -486,5 +556,71 @@ int gmmhip_set_model_full(gmmhip_t h, int M, int D, const float *w, const float 
     h->m_var.clear();
     h->model_version++;
     fs->operand_version = h->model_version;
+    fs->operand_kind = gmmhip::OP_FULL;
+    return GMMHIP_OK;
+}
+
+// Gaussian selection (the reference's logLikeMask in "use" mode,
+// trainer/OL_GMMStatsEstimator.h:73-114, :1157-1163: getLikeAndGammas and
+// accumulateStats restricted to each frame's component list).
+int gmmhip_set_mask(gmmhip_t h, uint64_t N, const uint64_t *offsets, const int *idx) {
+    if (!h) return GMMHIP_ERR_ARG;
+    DevGuard g(h->dev);
+    if (!offsets) {
+        h->mask_N = 0;
+        h->mask_M = 0;
+        if (h->d_mask) hipFree(h->d_mask);
+        h->d_mask = nullptr;
+        h->mask_cap = 0;
+        return GMMHIP_OK;
+    }
+    if (h->M <= 0) return fail(h, GMMHIP_ERR_STATE, "set_mask: no model inserted");
+    if (N == 0 || (offsets[N] > 0 && !idx))
+        return fail(h, GMMHIP_ERR_ARG, "set_mask: bad arguments");
+    const int W = (h->M + 31) / 32;
+    std::vector<uint32_t> bits((size_t)N * W, 0u);
+    for (uint64_t t = 0; t < N; t++) {
+        if (offsets[t + 1] < offsets[t])
+            return fail(h, GMMHIP_ERR_ARG, "set_mask: offsets decrease at frame %llu",
+                        (unsigned long long)t);
+        for (uint64_t e = offsets[t]; e < offsets[t + 1]; e++) {
+            const int m = idx[e];
+            if (m < 0 || m >= h->M)
+                return fail(h, GMMHIP_ERR_ARG, "set_mask: component %d of frame %llu out of range",
+                            m, (unsigned long long)t);
+            bits[(size_t)t * W + (m >> 5)] |= 1u << (m & 31);
+        }
+    }
+    HIPCHK(h, ensure(&h->d_mask, &h->mask_cap, bits.size()));
+    HIPCHK(h, hipMemcpyAsync(h->d_mask, bits.data(), bits.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->mask_N = N;
+    h->mask_W = W;
+    h->mask_M = h->M;
+    return GMMHIP_OK;
+}
+
+// The reference's logLikeMask in "fill" mode (mask2fill,
+// trainer/OL_GMMStatsEstimator.cpp:1168-1171): per-frame log-likelihoods and
+// posteriors before the cut; a clamped frame gets min_gamma in every
+// component (compGammas, :540-546; 1 for a single-component model).
+int gmmhip_posteriors(gmmhip_t h, float *ll, float *post) {
+    if (!h) return GMMHIP_ERR_ARG;
+    if (h->M <= 0) return fail(h, GMMHIP_ERR_STATE, "no model inserted");
+    if (!h->d_x || h->N == 0) return fail(h, GMMHIP_ERR_STATE, "no frames inserted");
+    if (h->xD != h->D)
+        return fail(h, GMMHIP_ERR_ARG, "frame dimension %d != model dimension %d", h->xD, h->D);
+    if (!ll || !post) return fail(h, GMMHIP_ERR_ARG, "posteriors: null output");
+    int rc = mask_check(h);
+    if (rc) return rc;
+    DevGuard g(h->dev);
+    rc = gemm_posteriors(h, ll, post);
+    if (rc) return rc;
+    const int M = h->M;
+    for (uint64_t t = 0; t < h->N; t++)
+        if (!(ll[t] > h->opt.min_loglike))
+            std::fill(post + t * (uint64_t)M, post + (t + 1) * (uint64_t)M,
+                      M == 1 ? 1.0f : h->opt.min_gamma);
     return GMMHIP_OK;
 }

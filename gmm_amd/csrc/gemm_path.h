@@ -11,6 +11,11 @@
 
 namespace gmmhip {
 
+// What theta currently holds: a full model, a diagonal model over the full
+// feature map (full second-order statistics of a diagonal model), or a
+// diagonal model over [1, x', x'^2].
+enum GemmOperand { OP_FULL = 0, OP_DIAG_FULLFEAT = 1, OP_DIAG = 2 };
+
 struct FullState {
     rocblas_handle blas = nullptr;
     int K = 0, KP = 0;           // features, padded to a multiple of 8
@@ -32,6 +37,7 @@ struct FullState {
     size_t acc_cap = 0;
     std::vector<float> h_cen;
     uint64_t operand_version = ~0ull;  // model_version theta was built for
+    int operand_kind = OP_FULL;
 };
 
 }  // namespace gmmhip
@@ -46,10 +52,10 @@ std::vector<int> feature_codes(int D, bool full);
 // handle's stream, atomics off).
 int gemm_state(gmmhip_ctx *h, gmmhip::FullState **out);
 
-// The full-feature operand theta of the handle's diagonal model (zero
-// off-diagonal precisions), so the GEMM path yields full second-order
-// statistics of a diagonal model (fullcov.hip).
-int diag_full_operand(gmmhip_ctx *h);
+// The GEMM operand theta of the handle's diagonal model, over the full
+// feature map (zero off-diagonal precisions: full second-order statistics of
+// a diagonal model) or over [1, x', x'^2] (fullcov.hip).
+int diag_operand(gmmhip_ctx *h, bool full_features);
 
 hipError_t launch_fc_features(const float *X, uint64_t n0, int nc, int ncp, int D, int K, int KP,
                               const float *cen, const int *codes, float *phi, uint8_t *bad,

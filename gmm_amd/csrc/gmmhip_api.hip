@@ -345,6 +345,7 @@ int gmmhip_destroy(gmmhip_t h) {
     batch_state_free(h->batch);
     aux_state_free(h->aux);
     full_state_free(h->full);
+    hipFree(h->d_mask);
     hipFree(h->d_mu_f32);
     hipFree(h->d_var_f32);
     if (h->h_res) hipHostFree(h->h_res);
@@ -552,7 +553,8 @@ int gmmhip_loglikes_device(gmmhip_t h, float *d_out) {
     if (rc) return rc;
     if (!d_out) return fail(h, GMMHIP_ERR_ARG, "loglikes_device: null output");
     DevGuard g(h->dev);
-    if (h->full_model) return full_loglikes_device(h, d_out);
+    if ((rc = mask_check(h))) return rc;
+    if (h->full_model || h->d_mask) return gemm_loglikes_device(h, d_out);
     // padded: the mixed statistics kernel stages 32-frame steps of LL2
     HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, pad128(h->N)));
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N, 1)));
@@ -609,9 +611,15 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
         ((flags & GMMHIP_ACC_VAR) && !S))
         return fail(h, GMMHIP_ERR_ARG, "accumulate: null output for a requested statistic");
     DevGuard g(h->dev);
+    if ((rc = mask_check(h))) return rc;
     if (h->full_model) return full_accumulate(h, flags, n, F, S, tot_ll, nframes);
     if ((flags & GMMHIP_ACC_FULLVAR) && (flags & GMMHIP_ACC_VAR))
         return diag_full_accumulate(h, flags, n, F, S, tot_ll, nframes);
+    if (h->d_mask) {
+        // Gaussian selection: the GEMM form with the diagonal operand
+        if ((rc = gemm_operand_for_model(h))) return rc;
+        return full_accumulate(h, flags, n, F, S, tot_ll, nframes);
+    }
     // padded: the mixed statistics kernel stages 32-frame steps of LL2
     HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, pad128(h->N)));
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(h->N, 1)));

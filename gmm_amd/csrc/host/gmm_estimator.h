@@ -10,6 +10,8 @@
 // exception classes the reference throws.
 #pragma once
 
+#include <algorithm>
+#include <cstdint>
 #include <future>
 #include <iostream>
 #include <stdexcept>
@@ -98,6 +100,44 @@ public:
     void resetAccFlags() { flagsSet_ = false; }
     void setSameDataFlag() { sameData_ = true; }
 
+    // Gaussian-selection mask (logLikeMask, OL_GMMStatsEstimator.h:73-114):
+    // "fill" stores every frame's log-likelihood and posteriors before the
+    // cut; "use" restricts frame i to maskIdxs[i].  Both apply to the frames
+    // of each accumulateStatsMT / compVecsLogLikeMT call, frame i of the
+    // call in row i.  The reference's CUDA estimator refuses fill and
+    // ignores use; here both run on the device.
+    struct logLikeMask {
+        unsigned int N = 0;    // >= NSamples
+        unsigned int dim = 0;  // >= number of components
+        float *logLikes = nullptr;  // N
+        float **gammas = nullptr;   // N x dim
+        std::vector<std::vector<unsigned int>> maskIdxs;  // N lists
+    };
+    void mask2fill(logLikeMask &mask) {
+        fillMask_ = true;
+        useMask_ = false;
+        mask_ = &mask;
+    }
+    void mask2use(logLikeMask &mask) {
+        useMask_ = true;
+        fillMask_ = false;
+        mask_ = &mask;
+    }
+    logLikeMask *withdrawMask() {
+        fillMask_ = useMask_ = false;
+        logLikeMask *m = mask_;
+        mask_ = nullptr;
+        return m;
+    }
+    // estimateMaskIdxs (OL_GMMStatsEstimator.cpp:383-394): the components
+    // whose filled posterior exceeds minGamma.
+    static void estimateMaskIdxs(logLikeMask &mask, float minGamma) {
+        mask.maskIdxs.assign(mask.N, {});
+        for (unsigned i = 0; i < mask.N; i++)
+            for (unsigned j = 0; j < mask.dim; j++)
+                if (mask.gammas[i][j] > minGamma) mask.maskIdxs[i].push_back(j);
+    }
+
     // Adds the statistics of the N frames into gmms (allocating it for the
     // model's components if it is not allocated yet).
     void accumulateStatsMT(float **vectors, unsigned int NSamples, unsigned int dim, GMMStats<TS> &gmms) {
@@ -125,6 +165,7 @@ public:
         if (NSamples == 0) return;
         upload(vectors, NSamples, dim);
         syncOpts();
+        MaskScope ms(*this, NSamples);
         const int flags = (fMean_ ? GMMHIP_ACC_MEAN : 0) | (fVar_ ? GMMHIP_ACC_VAR : 0) |
                           (fVar_ && !p.full && gmms.hasFullCov() ? GMMHIP_ACC_FULLVAR : 0);
         const size_t vs = gmms.hasVars() ? gmms.getVarSize() : (size_t)p.D;
@@ -148,6 +189,7 @@ public:
             TS *gA = gmms.auxData();
             for (size_t i = 0; i < aux.size(); i++) gA[i] += (TS)aux[i];
         }
+        ms.fill();
         gmms._totalAccSamples += nf;
         gmms._totLogLike += (TS)tot;
         sameData_ = false;
@@ -212,7 +254,9 @@ public:
         if (NSamples == 0) return;
         upload(vecs, NSamples, dim);
         syncOpts();
+        MaskScope ms(*this, NSamples);
         check(gmmhip_loglikes(h_, outLogLikes));
+        ms.fill();
         for (unsigned n = 0; n < NSamples; n++)
             if (outLogLikes[n] <= _minLogLike) _fNumStabilityTroubles = true;
         sameData_ = false;
@@ -228,6 +272,47 @@ private:
         bool full = false;
         std::vector<float> w, mu, var;  // var: [M][D] or [M][D][D]
     };
+    // Installs the "use" mask for the frames of one call (removed again on
+    // exit) and copies the "fill" outputs after it.
+    struct MaskScope {
+        GMMStatsEstimator_HIP &e;
+        unsigned int N;
+        bool installed = false;
+        MaskScope(GMMStatsEstimator_HIP &est, unsigned int n) : e(est), N(n) {
+            if (!e.mask_) return;
+            if (e.mask_->N < N || (e.useMask_ && e.mask_->maskIdxs.size() < N))
+                throw std::logic_error("accumulateStatsMT(): logLikeMask improperly allocated!");
+            if (!e.useMask_) return;
+            const Packed &p = e.current();
+            std::vector<uint64_t> off(N + 1, 0);
+            std::vector<int> idx;
+            for (unsigned i = 0; i < N; i++) {
+                for (unsigned m : e.mask_->maskIdxs[i])
+                    if ((int)m < p.M) idx.push_back((int)m);  // inactive components never count
+                off[i + 1] = idx.size();
+            }
+            if (idx.empty()) idx.push_back(0);
+            e.check(gmmhip_set_mask(e.h_, N, off.data(), idx.data()));
+            installed = true;
+        }
+        void fill() {
+            if (!e.mask_ || !e.fillMask_) return;
+            const Packed &p = e.current();
+            if (e.mask_->dim < (unsigned)p.Malloc)
+                throw std::logic_error("accumulateStatsMT(): logLikeMask improperly allocated!");
+            std::vector<float> post((size_t)N * p.M);
+            e.check(gmmhip_posteriors(e.h_, e.mask_->logLikes, post.data()));
+            for (unsigned i = 0; i < N; i++) {
+                std::copy(post.begin() + (size_t)i * p.M, post.begin() + (size_t)(i + 1) * p.M,
+                          e.mask_->gammas[i]);
+                std::fill(e.mask_->gammas[i] + p.M, e.mask_->gammas[i] + e.mask_->dim, 0.0f);
+            }
+        }
+        ~MaskScope() {
+            if (installed) gmmhip_set_mask(e.h_, 0, nullptr, nullptr);
+        }
+    };
+
     gmmhip_t h_ = nullptr;
     int dev_ = 0;
     std::string type_ = "GPU HIP";
@@ -237,6 +322,8 @@ private:
          fAux_ = false;
     bool sameData_ = false;
     bool haveFrames_ = false;
+    bool fillMask_ = false, useMask_ = false;
+    logLikeMask *mask_ = nullptr;
 
     const Packed &current() const {
         if (group_.empty()) throw std::logic_error("accumulateStatsMT(): None model inserted!");

@@ -171,6 +171,28 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S,
  * gmmhip_accumulate, as before. */
 int gmmhip_accumulate_aux(gmmhip_t h, double *aux);
 
+/* Gaussian selection: restrict every frame's log-likelihood, posteriors and
+ * statistics to a list of components (the reference's logLikeMask in "use"
+ * mode, mask2use + maskIdxs, trainer/OL_GMMStatsEstimator.h:73-114,
+ * trainer/OL_GMMStatsEstimator.cpp:444-512 with mixIdxs; its CUDA estimator
+ * ignores the mask, OL_GMMStatsEstimator_CUDA.cpp:352, :421).  Frame t's
+ * components are idx[offsets[t] .. offsets[t+1]) (model order, < M); a
+ * frame with none is clamped (LL = min_loglike, no statistics).  The mask
+ * belongs to the inserted model and N resident frames: accumulate,
+ * loglikes and posteriors fail with GMMHIP_ERR_ARG while it does not fit
+ * them; offsets = NULL removes it.  While set, those calls run the GEMM
+ * form in FP32 (whatever opts.precision says); gmmhip_batch_adapt and the
+ * server do not use it. */
+int gmmhip_set_mask(gmmhip_t h, uint64_t N, const uint64_t *offsets, const int *idx);
+
+/* Per-frame log-likelihoods ll[N] and posteriors before the cut
+ * post[N*M] (host, row-major) of the resident frames: the reference's
+ * logLikeMask in "fill" mode (mask2fill, trainer/OL_GMMStatsEstimator.cpp:
+ * 1168-1171; unsupported by its CUDA estimator, OL_GMMStatsEstimator_CUDA.cpp:
+ * 352-353).  A clamped frame gets min_gamma in every component (1 when
+ * M = 1), as compGammas leaves it.  FP32 GEMM form; a set mask applies. */
+int gmmhip_posteriors(gmmhip_t h, float *ll, float *post);
+
 /* Per-frame log-likelihoods of the resident frames into out[N] (host).
  * Replaces compVecsLogLikeMT (trainer/OL_GMMStatsEstimator_CUDA.cpp:333-391)
  * = compLogLikes + getLogLikes (trainer/CU_GMMStatsEstimator_CUDA.cu:606-643),

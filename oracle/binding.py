@@ -376,3 +376,26 @@ def fmllr(mu, var, n, F, S, tot_samples, iters=20):
     if rc:
         raise ValueError("fMLLR: negative discriminant or zero root")
     return W
+
+
+def accumulate_masked(x, w, mu, var, lists, min_gamma=MIN_GAMMA):
+    """Gaussian selection: (LL[N], n, F, S, tot) with frame t restricted to
+    the components lists[t]."""
+    x, mu = _f32(x), _f32(mu)
+    N, D = x.shape
+    M = mu.shape[0]
+    gconst, ivar = prepare_model(w, mu, var)
+    off = np.zeros(N + 1, np.uint64)
+    off[1:] = np.cumsum([len(l) for l in lists])
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(l, np.int32) for l in lists])
+                               if off[-1] else np.zeros(1, np.int32), np.int32)
+    LL = np.zeros(N)
+    n = np.zeros(M)
+    F = np.zeros((M, D))
+    S = np.zeros((M, D))
+    tot = ctypes.c_double(0.0)
+    lib().orc_accumulate_masked(N, D, M, _p(x), _p(mu), _p(ivar), _p(gconst),
+                                off.ctypes.data_as(ctypes.c_void_p),
+                                idx.ctypes.data_as(ctypes.c_void_p), ctypes.c_double(min_gamma),
+                                _p(LL), _p(n), _p(F), _p(S), ctypes.byref(tot))
+    return LL, n, F, S, tot.value

@@ -1254,3 +1254,74 @@ int orc_fmllr(int M, int D, const float *mu, const float *var, const double *n, 
     free(w[0]); free(w[1]);
     return rc;
 }
+
+/* ------------------------------------------------------------------------
+ * Gaussian selection (logLikeMask in "use" mode).
+ *
+ * Pinned (tests/test_oracle_mask.py): a mask holding every component gives
+ * orc_frame_loglik / orc_accumulate exactly; scipy logsumexp and numpy sums
+ * over random subsets; empty selections clamp; a selection of the
+ * components above the cut (estimateMaskIdxs) changes the statistics by no
+ * more than the dropped posterior mass.
+ * ------------------------------------------------------------------------ */
+
+/* getLikeAndGammas with mixIdxs (trainer/OL_GMMStatsEstimator.cpp:444-512)
+ * + compGammas (:515-550) + accumulateStats with mixIdxs (:1334-1378), diagonal
+ * model, exact log-sum-exp: frame t's log-likelihood over components
+ * idx[off[t] .. off[t+1]) only, posteriors exp(l - LL) for those, the
+ * statistics of those with posterior > min_gamma.  An empty selection (or
+ * all components underflowing) clamps the frame to _minLogLike with no
+ * statistics.  LL[N] and (optional) n/F/S (overwritten) and *tot. */
+void orc_accumulate_masked(int N, int D, int M, const float *x, const float *mu,
+                           const double *ivar, const double *gconst, const uint64_t *off,
+                           const int *idx, double min_gamma, double *LL, double *n_out,
+                           double *F, double *S, double *tot_ll)
+{
+    const double minLL = orc_min_loglike();
+    double *l = (double *)malloc(sizeof(double) * (M > 0 ? M : 1));
+    if (n_out) {
+        memset(n_out, 0, sizeof(double) * M);
+        memset(F, 0, sizeof(double) * (size_t)M * D);
+        memset(S, 0, sizeof(double) * (size_t)M * D);
+    }
+    double tot = 0.0;
+    for (int t = 0; t < N; t++) {
+        const float *xt = x + (size_t)t * D;
+        const uint64_t b = off[t], e = off[t + 1];
+        double mx = minLL;
+        for (uint64_t j = b; j < e; j++) {
+            const int m = idx[j];
+            l[m] = comp_ll(xt, D, mu + (size_t)m * D, ivar + (size_t)m * D, gconst[m]);
+            if (l[m] > mx)
+                mx = l[m];
+        }
+        double ll = minLL;
+        if (mx > minLL) {
+            double s = 0.0;
+            for (uint64_t j = b; j < e; j++)
+                s += exp(l[idx[j]] - mx);
+            ll = mx + log(s);
+            if (!isfinite(ll) || ll < minLL)
+                ll = minLL;
+        }
+        LL[t] = ll;
+        tot += ll;
+        if (!n_out || ll <= minLL)
+            continue;
+        for (uint64_t j = b; j < e; j++) {
+            const int m = idx[j];
+            const double g = (M == 1) ? 1.0 : exp(l[m] - ll);
+            if (!(g > min_gamma))
+                continue;
+            n_out[m] += g;
+            for (int d = 0; d < D; d++) {
+                const double xv = xt[d];
+                F[(size_t)m * D + d] += g * xv;
+                S[(size_t)m * D + d] += g * xv * xv;
+            }
+        }
+    }
+    if (tot_ll)
+        *tot_ll = tot;
+    free(l);
+}
