@@ -124,8 +124,9 @@ def traffic_of(kernel, workload):
 
 def bench_batch(args):
     """Batched MAP (tau=16) + global MLLR statistics and transforms for 4,096
-    utterances of uneven length against a 2048-component D=60 UBM: one
-    persistent work-queue launch captured in a hipGraph vs one launch set per
+    utterances of uneven length against a 2048-component D=60 UBM: the whole
+    batch in one pipeline captured in a hipGraph (E-step in --precision; the
+    FP32 form's work-queue launch reported beside it) vs one launch set per
     utterance.  Frames and outputs stay in HBM."""
     import torch
     import gmm_amd
@@ -158,18 +159,30 @@ def bench_batch(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
+    precs = {"fp32": gmm_amd.PREC_FP32, "fp16x3": gmm_amd.PREC_FP16X3,
+             "bf16x3": gmm_amd.PREC_BF16X3}
+    acc.set_opts(precision=precs[args.precision])
     t_q = timed(gmm_amd.BATCH_QUEUE, True, args.steps)
-    t_u = timed(gmm_amd.BATCH_PER_UTTERANCE, False, max(1, args.steps // 4))
     st = status.cpu().numpy()
+    means_q = means.cpu().numpy()
+    # the FP32 queue form on the same batch, and the per-utterance launches
+    acc.set_opts(precision=gmm_amd.PREC_FP32)
+    t_q32 = timed(gmm_amd.BATCH_QUEUE, True, args.steps) if args.precision != "fp32" else t_q
+    means_32 = means.cpu().numpy()
+    t_u = timed(gmm_amd.BATCH_PER_UTTERANCE, False, max(1, args.steps // 4))
     line = {
         "metric": "batched MAP+MLLR utterances/s (persistent queue + hipGraph)",
         "value": U / t_q, "unit": "utterances/s", "n_gpus": 1, "steps": args.steps,
         "warmup": 1, "ms_per_step": t_q * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32 E-step, fp64 MLLR statistics and solve",
+        "vs_baseline": None,
+        "dtype": "%s E-step, fp64 MLLR statistics and solve" % args.precision,
         "data": "synthetic (seeded UBM, per-utterance shifted mixtures, frames in HBM)",
         "config": {"workload": "batch_map", "D": D, "M": M, "U": U, "frames": N, "tau": 16,
                    "mllr_prior": True},
-        "per_utterance_launches": {"ms_per_batch": t_u * 1e3, "utterances_per_s": U / t_u},
+        "fp32_queue": {"ms_per_batch": t_q32 * 1e3, "utterances_per_s": U / t_q32,
+                       "map_means_max_abs_diff_vs_this": float(np.abs(means_q - means_32).max())},
+        "per_utterance_launches": {"ms_per_batch": t_u * 1e3, "utterances_per_s": U / t_u,
+                                   "precision": "fp32"},
         "speedup_vs_per_utterance": t_u / t_q,
         "frame_comp_per_s": N * M / t_q,
         "mllr_underdetermined": int((st != 0).sum()),

@@ -49,6 +49,9 @@ struct MllrGkArgs {
 };
 
 hipError_t launch_fill_u32(unsigned int *p, unsigned int v, int n, hipStream_t s);
+// Utterance rows X[src[u] .. src[u+1]) copied to Xp[dst[u] ..), zero up to dst[u+1].
+hipError_t launch_pad_rows(const float *X, const uint64_t *src, const uint64_t *dst, int U, int D,
+                           float *Xp, hipStream_t s);
 hipError_t launch_mllr_gk(const MllrGkArgs &a, hipStream_t s);
 hipError_t launch_seg_reduce(const SegReduceArgs &a, hipStream_t s);
 hipError_t launch_map_means(const double *n, const double *F, const float *mu, int U, int M, int D,

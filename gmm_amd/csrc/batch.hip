@@ -29,6 +29,27 @@ __global__ void fill_u32_kernel(unsigned int *p, unsigned int v, int n) {
     if (i < n) p[i] = v;
 }
 
+// Utterance u's rows X[src[u] .. src[u+1]) to Xp[dst[u] ..), the rows up to
+// dst[u+1] zero (each utterance of a mixed-precision batch starts on a
+// 32-row block).  One block per utterance.
+__global__ __launch_bounds__(256) void pad_rows_kernel(const float *__restrict__ X,
+                                                       const uint64_t *__restrict__ src,
+                                                       const uint64_t *__restrict__ dst, int D,
+                                                       float *__restrict__ Xp) {
+    const int u = blockIdx.x;
+    const uint64_t n = (src[u + 1] - src[u]) * (uint64_t)D;
+    const uint64_t np = (dst[u + 1] - dst[u]) * (uint64_t)D;
+    const float *x = X + src[u] * (uint64_t)D;
+    float *y = Xp + dst[u] * (uint64_t)D;
+    for (uint64_t e = threadIdx.x; e < np; e += 256) y[e] = e < n ? x[e] : 0.0f;
+}
+
+hipError_t launch_pad_rows(const float *X, const uint64_t *src, const uint64_t *dst, int U, int D,
+                           float *Xp, hipStream_t s) {
+    hipLaunchKernelGGL(pad_rows_kernel, dim3(U), dim3(256), 0, s, X, src, dst, D, Xp);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(unsigned int *p, unsigned int v, int n, hipStream_t s) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
     return hipGetLastError();

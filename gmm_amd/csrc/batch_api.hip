@@ -10,10 +10,18 @@
 //   map_means_kernel, and for MLLR mllr_check / two FP64 GEMMs (G, k)
 //     against UBM-only matrices with the A operand formed from n, F and
 //     1/var while staged / mllr_solve.
+// With opts.precision fp16x3 / bf16x3 the E-step part runs the mixed form:
+// every utterance is copied to a 32-row-aligned place (pad_rows_kernel),
+// prep_frames splits the frames centred and scaled by the UBM's global mean
+// and deviation (so each utterance's result does not depend on the batch),
+// then loglik_mixed and stats_mixed over the utterances' chunks.  A batch
+// with a frame value outside the 16-bit range, or a model whose
+// coefficients overflow it, runs in FP32.
 // With cfg->graph the whole pipeline is captured once into a hipGraph and
 // replayed while the batch shape and buffers stay the same.
 #include "batch.h"
 #include "ctx.h"
+#include "estep_mixed.h"
 
 using namespace gmmhip_detail;
 
@@ -37,6 +45,24 @@ struct gmmhip_batch_state {
     double *d_ivar = nullptr;
     size_t P_cap = 0, Xi_cap = 0, Ginit_cap = 0, kinit_cap = 0, ivar_cap = 0;
     uint64_t mllr_model = ~0ull;
+    // mixed form: padded frames, split features, range flags, UBM-based
+    // centring / scaling and the split model for them
+    float *d_xpad = nullptr;
+    size_t xpad_cap = 0;
+    uint16_t *d_zah = nullptr, *d_zal = nullptr, *d_zbh = nullptr, *d_zbl = nullptr;
+    size_t zah_cap = 0, zal_cap = 0, zbh_cap = 0, zbl_cap = 0;
+    uint8_t *d_bad = nullptr;
+    size_t bad_cap = 0;
+    unsigned int *d_nrange = nullptr;
+    uint64_t *d_src = nullptr, *d_dst = nullptr;
+    size_t src_cap = 0, dst_cap = 0;
+    float *d_zoff = nullptr, *d_zscale = nullptr;
+    size_t zoff_cap = 0, zscale_cap = 0;
+    uint16_t *d_P16 = nullptr;
+    size_t P16_cap = 0;
+    uint64_t p16_model = ~0ull;
+    int p16_prec = -1;
+    bool p16_ok = false;
     // graph cache
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -51,7 +77,10 @@ void batch_state_free(gmmhip_batch_state *b) {
     for (void *p : {(void *)b->d_lo, (void *)b->d_hi, (void *)b->d_first, (void *)b->d_queue,
                     (void *)b->d_n, (void *)b->d_F, (void *)b->d_means, (void *)b->d_G,
                     (void *)b->d_K, (void *)b->d_W, (void *)b->d_status, (void *)b->d_P,
-                    (void *)b->d_Xi, (void *)b->d_Ginit, (void *)b->d_kinit, (void *)b->d_ivar})
+                    (void *)b->d_Xi, (void *)b->d_Ginit, (void *)b->d_kinit, (void *)b->d_ivar,
+                    (void *)b->d_xpad, (void *)b->d_zah, (void *)b->d_zal, (void *)b->d_zbh,
+                    (void *)b->d_zbl, (void *)b->d_bad, (void *)b->d_nrange, (void *)b->d_src,
+                    (void *)b->d_dst, (void *)b->d_zoff, (void *)b->d_zscale, (void *)b->d_P16})
         if (p) hipFree(p);
     delete b;
 }
@@ -149,12 +178,60 @@ namespace {
 struct BatchPlan {
     int U = 0, M = 0, D = 0, KP = 0, Mt = 0, ngroups = 0, nchunks = 0;
     uint64_t N = 0, cf = 0;
+    // mixed form: split precision, padded rows (Np, Npad), KP / tiles of the split model
+    bool mixed = false;
+    int prec = 0, KPm = 0, Mt32 = 0;
+    uint64_t Np = 0, Npad = 0;
     bool want_means = false, want_mllr = false, prior = false, per_utt = false;
     int tau = 0;
     int ll_sub = 2;
     std::vector<uint64_t> lo, hi;
     std::vector<int> first;
 };
+
+// Mixed form operands of the handle's model: centring o = sum_m w_m mu_m and
+// scaling s = 1 / sqrt(sum_m w_m (var_m + (mu_m - o)^2)) (the UBM's global
+// mean and deviation, the same for every batch) and the split model for
+// them; *ok is false when a coefficient overflows the 16-bit type.
+int batch_mixed_operands(gmmhip_ctx *h, gmmhip_batch_state *b, int prec, int KPm, bool *ok) {
+    if (b->p16_model == h->model_version && b->p16_prec == prec) {
+        *ok = b->p16_ok;
+        return GMMHIP_OK;
+    }
+    const int M = h->M, D = h->D;
+    std::vector<float> zoff(D, 0.0f), zscale(D, 1.0f);
+    double wsum = 0.0;
+    for (int m = 0; m < M; m++) wsum += h->m_w[m];
+    if (!(wsum > 0.0)) wsum = 1.0;
+    for (int d = 0; d < D; d++) {
+        double o = 0.0;
+        for (int m = 0; m < M; m++) o += h->m_w[m] * (double)h->m_mu[(size_t)m * D + d];
+        o /= wsum;
+        double v = 0.0;
+        for (int m = 0; m < M; m++) {
+            const double dm = (double)h->m_mu[(size_t)m * D + d] - o;
+            v += h->m_w[m] * ((double)h->m_var[(size_t)m * D + d] + dm * dm);
+        }
+        v /= wsum;
+        zoff[d] = (float)o;
+        if (std::isfinite(v) && v > 0.0) zscale[d] = (float)(1.0 / std::sqrt(v));
+    }
+    bool overflow = false;
+    const std::vector<uint16_t> P = pack_split_model(h, prec, KPm, zoff.data(), zscale.data(), &overflow);
+    b->p16_model = h->model_version;
+    b->p16_prec = prec;
+    b->p16_ok = !overflow;
+    *ok = b->p16_ok;
+    if (overflow) return GMMHIP_OK;
+    HIPCHK(h, ensure(&b->d_zoff, &b->zoff_cap, (size_t)D));
+    HIPCHK(h, ensure(&b->d_zscale, &b->zscale_cap, (size_t)D));
+    HIPCHK(h, ensure(&b->d_P16, &b->P16_cap, P.size()));
+    HIPCHK(h, hipMemcpyAsync(b->d_zoff, zoff.data(), D * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(b->d_zscale, zscale.data(), D * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(b->d_P16, P.data(), P.size() * 2, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return GMMHIP_OK;
+}
 
 // Enqueue the device pipeline (no host synchronisation; graph-capturable:
 // kernels only, the queue counter is reset by a kernel).
@@ -180,7 +257,34 @@ int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
     a.chunk_frames = p.cf;
     a.ll_sub = p.ll_sub;
     const int blocks_cap = std::max(1, h->num_cus) * 2;
-    if (!p.per_utt) {
+    if (p.mixed) {
+        MixedArgs m{};
+        m.N = p.Np;
+        m.Npad = p.Npad;
+        m.D = p.D;
+        m.KP = p.KPm;
+        m.za_hi = b->d_zah;
+        m.za_lo = b->d_zal;
+        m.zb_hi = b->d_zbh;
+        m.zb_lo = b->d_zbl;
+        m.bad = b->d_bad;
+        m.P16 = b->d_P16;
+        m.M = p.M;
+        m.Mt32 = p.Mt32;
+        m.min_ll = h->opt.min_loglike;
+        m.min_gamma = h->opt.min_gamma;
+        m.LL2 = h->d_LL2;
+        m.LLnat = nullptr;
+        m.blockLL = h->d_blockLL;
+        HIPCHK(h, launch_loglik_mixed(m, p.prec, s));
+        m.chunk_frames = p.cf;
+        m.nchunks = p.nchunks;
+        m.ngroups = (p.Mt32 + 3) / 4;
+        m.partials = h->d_part;
+        m.chunk_lo = b->d_lo;
+        m.chunk_hi = b->d_hi;
+        HIPCHK(h, launch_stats_mixed(m, p.prec, s));
+    } else if (!p.per_utt) {
         HIPCHK(h, launch_loglik_fp32(a, s));
         a.nchunks = p.nchunks;
         a.chunk_lo = b->d_lo;
@@ -216,12 +320,13 @@ int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
     SegReduceArgs r{};
     r.partials = h->d_part;
     r.chunk_first = b->d_first;
-    r.off = h->d_off;
+    r.off = p.mixed ? b->d_zoff : h->d_off;
+    r.scale = p.mixed ? b->d_zscale : nullptr;
     r.U = p.U;
     r.M = p.M;
-    r.Mp = p.Mt * 16;
+    r.Mp = p.mixed ? p.Mt32 * 32 : p.Mt * 16;
     r.D = p.D;
-    r.KP = p.KP;
+    r.KP = p.mixed ? p.KPm : p.KP;
     r.n = b->d_n;
     r.F = b->d_F;
     HIPCHK(h, launch_seg_reduce(r, s));
@@ -326,23 +431,82 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
     p.prior = cfg->mllr_prior != 0;
     p.per_utt = cfg->mode == GMMHIP_BATCH_PER_UTTERANCE;
     p.tau = cfg->tau;
+    // mixed form (queue mode): utterances on 32-row blocks, split features
+    // of the padded rows; a frame out of the 16-bit range sends the batch to
+    // FP32
+    std::vector<uint64_t> dst(U + 1, 0);
+    const int prec = h->opt.precision;
+    if (prec != GMMHIP_PREC_FP32 && !p.per_utt) {
+        p.KPm = kp_mixed_for_dim(D);
+        p.Mt32 = (h->M + 31) / 32;
+        bool ok = false;
+        if ((rc = batch_mixed_operands(h, b, prec, p.KPm, &ok))) return rc;
+        if (ok) {
+            for (int u = 0; u < U; u++)
+                dst[u + 1] = dst[u] + (offsets[u + 1] - offsets[u] + 31) / 32 * 32;
+            p.Np = dst[U];
+            p.Npad = pad128(p.Np);
+            HIPCHK(h, ensure(&b->d_src, &b->src_cap, (size_t)U + 1));
+            HIPCHK(h, ensure(&b->d_dst, &b->dst_cap, (size_t)U + 1));
+            HIPCHK(h, ensure(&b->d_xpad, &b->xpad_cap, (size_t)p.Np * D));
+            const size_t zn = (size_t)p.Npad * p.KPm;
+            HIPCHK(h, ensure(&b->d_zah, &b->zah_cap, zn));
+            HIPCHK(h, ensure(&b->d_zal, &b->zal_cap, zn));
+            HIPCHK(h, ensure(&b->d_zbh, &b->zbh_cap, zn));
+            HIPCHK(h, ensure(&b->d_zbl, &b->zbl_cap, zn));
+            HIPCHK(h, ensure(&b->d_bad, &b->bad_cap, (size_t)p.Npad));
+            if (!b->d_nrange) HIPCHK(h, hipMalloc((void **)&b->d_nrange, sizeof(unsigned int)));
+            HIPCHK(h, hipMemcpyAsync(b->d_src, offsets, (U + 1) * 8, hipMemcpyHostToDevice, h->stream));
+            HIPCHK(h, hipMemcpyAsync(b->d_dst, dst.data(), (U + 1) * 8, hipMemcpyHostToDevice,
+                                     h->stream));
+            HIPCHK(h, launch_pad_rows(h->d_x, b->d_src, b->d_dst, U, D, b->d_xpad, h->stream));
+            HIPCHK(h, launch_fill_u32(b->d_nrange, 0u, 1, h->stream));
+            PrepArgs pa{};
+            pa.X = b->d_xpad;
+            pa.N = p.Np;
+            pa.Npad = p.Npad;
+            pa.D = D;
+            pa.KP = p.KPm;
+            pa.off = b->d_zoff;
+            pa.scale = b->d_zscale;
+            pa.za_hi_raw = b->d_zah;
+            pa.za_lo_raw = b->d_zal;
+            pa.zb_hi_raw = b->d_zbh;
+            pa.zb_lo_raw = b->d_zbl;
+            pa.bad = b->d_bad;
+            pa.nrange = b->d_nrange;
+            pa.zlimit = prec == GMMHIP_PREC_FP16X3 ? 240.0f : INFINITY;
+            HIPCHK(h, launch_prep_frames(pa, prec, h->stream));
+            unsigned int nrange = 0;
+            HIPCHK(h, hipMemcpyAsync(&nrange, b->d_nrange, sizeof(nrange), hipMemcpyDeviceToHost,
+                                     h->stream));
+            HIPCHK(h, hipStreamSynchronize(h->stream));
+            p.mixed = nrange == 0;
+            p.prec = prec;
+        }
+    }
     p.first.assign(U + 1, 0);
     for (int u = 0; u < U; u++) {
         p.first[u] = (int)p.lo.size();
-        for (uint64_t s0 = offsets[u]; s0 < offsets[u + 1]; s0 += p.cf) {
+        // chunks in the rows the kernels see: padded rows in the mixed form
+        const uint64_t base = p.mixed ? dst[u] : offsets[u];
+        const uint64_t end = base + (offsets[u + 1] - offsets[u]);
+        for (uint64_t s0 = base; s0 < end; s0 += p.cf) {
             p.lo.push_back(s0);
-            p.hi.push_back(std::min<uint64_t>(offsets[u + 1], s0 + p.cf));
+            p.hi.push_back(std::min<uint64_t>(end, s0 + p.cf));
         }
     }
     p.first[U] = (int)p.lo.size();
     p.nchunks = (int)p.lo.size();
     const size_t M = p.M, E = D + 1, NP = E * (E + 1) / 2;
-    const size_t Mp = (size_t)p.Mt * 16;
+    const size_t Mp = p.mixed ? (size_t)p.Mt32 * 32 : (size_t)p.Mt * 16;
+    const size_t KPx = p.mixed ? (size_t)p.KPm : (size_t)p.KP;
+    const uint64_t Nrows = p.mixed ? p.Npad : N;
 
     // buffers
-    HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, N));
-    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(N, 1) + (size_t)U));
-    HIPCHK(h, ensure(&h->d_part, &h->part_cap, (size_t)p.nchunks * Mp * p.KP));
+    HIPCHK(h, ensure(&h->d_LL2, &h->LL2_cap, Nrows));
+    HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, ll_blocks(Nrows, 1) + (size_t)U));
+    HIPCHK(h, ensure(&h->d_part, &h->part_cap, (size_t)p.nchunks * Mp * KPx));
     HIPCHK(h, ensure(&b->d_lo, &b->lo_cap, (size_t)p.nchunks));
     HIPCHK(h, ensure(&b->d_hi, &b->hi_cap, (size_t)p.nchunks));
     HIPCHK(h, ensure(&b->d_first, &b->first_cap, (size_t)U + 1));
@@ -378,6 +542,10 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
                                      (uint64_t)p.prior,
                                      (uint64_t)p.per_utt,
                                      (uint64_t)p.tau,
+                                     (uint64_t)p.mixed,
+                                     (uint64_t)p.prec,
+                                     (uint64_t)(uintptr_t)b->d_zah,
+                                     (uint64_t)(uintptr_t)b->d_P16,
                                      (uint64_t)(uintptr_t)h->d_part,
                                      (uint64_t)(uintptr_t)b->d_n,
                                      (uint64_t)(uintptr_t)b->d_G};

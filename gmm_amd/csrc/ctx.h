@@ -152,6 +152,10 @@ inline hipError_t ensure(T **p, size_t *cap, size_t n) {
     return e;
 }
 
+// Rows of the mixed form's frame buffers: a multiple of the 128-frame
+// log-likelihood block.
+inline uint64_t pad128(uint64_t n) { return (n + 127) / 128 * 128; }
+
 inline void apply_defaults(gmmhip_opts *o) {
     if (o->min_loglike == 0.0f) o->min_loglike = -1e20f;
     if (o->min_gamma == 0.0f) o->min_gamma = 1e-4f;

@@ -79,7 +79,6 @@ void add_time(gmmhip_ctx *h, int cls, int e0, int e1) {
 
 // Frames per mixed-precision feature row block (the log-likelihood kernel
 // works on 128 frames per block).
-inline uint64_t pad128(uint64_t n) { return (n + 127) / 128 * 128; }
 
 template <typename T>
 void split_store(double v, uint16_t *hi, uint16_t *lo, bool *overflow) {

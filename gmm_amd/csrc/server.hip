@@ -43,7 +43,6 @@ namespace {
 constexpr size_t kAlign = 256;
 inline size_t al(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 inline uint64_t pad32(uint64_t n) { return (n + 31) / 32 * 32; }
-inline uint64_t pad128(uint64_t n) { return (n + 127) / 128 * 128; }
 
 struct Pending {
     gmmhip_request *r;

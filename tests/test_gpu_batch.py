@@ -151,3 +151,55 @@ def test_batch_mllr_wide_and_many_components(acc, D, M):
         assert np.allclose(res["W"][u], Wg, rtol=1e-8, atol=1e-8), u
         Wo = orc.mllr_transform(mu, var, ref[u][0], ref[u][1], ubm_mu=mu, ubm_var=var)
         assert np.allclose(res["W"][u], Wo, rtol=1e-3, atol=1e-3), u
+
+
+@pytest.mark.parametrize("prec,tol", [(gmm_amd.PREC_FP16X3, 2e-4), (gmm_amd.PREC_BF16X3, 2e-3)])
+def test_batch_mixed_precision(acc, prec, tol):
+    """The mixed form in the batched path: per-utterance statistics, MAP
+    means and MLLR transforms against the oracle (mixed-precision
+    tolerances, as the E-step's), graph replays bit-identical to eager runs,
+    and each utterance's result independent of its batch-mates (UBM-based
+    centring and scaling, chunks relative to the utterance start)."""
+    x, off, w, mu, var = make_batch(11, 30, 64, 12, lo=20, hi=600, extra=(0, 5))
+    acc.set_model(w, mu, var)
+    ref = per_utt_oracle(x, off, w, mu, var)
+    acc.set_opts(precision=prec)
+    try:
+        res = acc.batch_adapt(x, off, tau=16, prior=True)
+        graphs = [acc.batch_adapt(x, off, tau=16, prior=True, graph=True) for _ in range(3)]
+        # a sub-batch: utterances 3..9 alone
+        sub = off[3:11] - off[3]
+        res_sub = acc.batch_adapt(x[int(off[3]):int(off[10])], sub, tau=16, prior=True)
+    finally:
+        acc.set_opts(precision=gmm_amd.PREC_FP32)
+    xm = float(np.abs(x).max())
+    for u, (rn, rF) in enumerate(ref):
+        n, F = res["occ"][u], res["first"][u]
+        assert np.all(np.abs(n - rn) <= tol * rn + 3e-4), u
+        assert np.all(np.abs(F - rF) <= tol * np.abs(rF) + 3e-4 * xm + tol * rn[:, None] * xm), u
+        _, rmu, _ = orc.update_map(w, mu, var, n, F, F ** 2, 1, 16)
+        assert np.allclose(res["means"][u], rmu, rtol=1e-5, atol=1e-5 * xm), u
+        if res["status"][u] == 0:
+            Wg = orc.mllr_transform(mu, var, n, F, ubm_mu=mu, ubm_var=var)
+            assert np.allclose(res["W"][u], Wg, rtol=1e-8, atol=1e-8), u
+    for g in graphs:
+        for k in res:
+            assert np.array_equal(res[k], g[k]), k
+    for k in res:
+        assert np.array_equal(res_sub[k], res[k][3:10]), k
+
+
+def test_batch_mixed_out_of_range_runs_fp32(acc):
+    """A frame value outside the FP16 range after centring and scaling sends
+    the whole batch to the FP32 kernels: the results equal the FP32 run."""
+    x, off, w, mu, var = make_batch(12, 8, 32, 6, lo=50, hi=300)
+    x[int(off[2]) + 3, 1] = 5e4  # |z| >> 240
+    acc.set_model(w, mu, var)
+    fp32 = acc.batch_adapt(x, off, tau=16)
+    acc.set_opts(precision=gmm_amd.PREC_FP16X3)
+    try:
+        mixed = acc.batch_adapt(x, off, tau=16)
+    finally:
+        acc.set_opts(precision=gmm_amd.PREC_FP32)
+    for k in fp32:
+        assert np.array_equal(fp32[k], mixed[k]), k
