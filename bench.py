@@ -420,7 +420,7 @@ def main():
     st_avg = st_ms / max(st_n, 1) / 1e3
     achieved = flop_fc * N * M / st_avg
     peak = PEAK["fp32"] if args.precision == "fp32" else PEAK["fp16"]
-    kname = "stats_kernel" if args.precision == "fp32" else "stats_mixed_dma_kernel"
+    kname = "stats_kernel" if args.precision == "fp32" else "stats_mixed_tr_kernel"
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / peak,
                 "traffic": traffic_of(kname, args.workload),

@@ -452,8 +452,11 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
             const size_t zn = (size_t)p.Npad * p.KPm;
             HIPCHK(h, ensure(&b->d_zah, &b->zah_cap, zn));
             HIPCHK(h, ensure(&b->d_zal, &b->zal_cap, zn));
-            HIPCHK(h, ensure(&b->d_zbh, &b->zbh_cap, zn));
-            HIPCHK(h, ensure(&b->d_zbl, &b->zbl_cap, zn));
+            const bool need_zb = mixed_needs_zb(p.KPm);
+            if (need_zb) {
+                HIPCHK(h, ensure(&b->d_zbh, &b->zbh_cap, zn));
+                HIPCHK(h, ensure(&b->d_zbl, &b->zbl_cap, zn));
+            }
             HIPCHK(h, ensure(&b->d_bad, &b->bad_cap, (size_t)p.Npad));
             if (!b->d_nrange) HIPCHK(h, hipMalloc((void **)&b->d_nrange, sizeof(unsigned int)));
             HIPCHK(h, hipMemcpyAsync(b->d_src, offsets, (U + 1) * 8, hipMemcpyHostToDevice, h->stream));
@@ -471,8 +474,8 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
             pa.scale = b->d_zscale;
             pa.za_hi_raw = b->d_zah;
             pa.za_lo_raw = b->d_zal;
-            pa.zb_hi_raw = b->d_zbh;
-            pa.zb_lo_raw = b->d_zbl;
+            pa.zb_hi_raw = need_zb ? b->d_zbh : nullptr;
+            pa.zb_lo_raw = need_zb ? b->d_zbl : nullptr;
             pa.bad = b->d_bad;
             pa.nrange = b->d_nrange;
             pa.zlimit = prec == GMMHIP_PREC_FP16X3 ? 240.0f : INFINITY;

@@ -9,6 +9,12 @@ namespace gmmhip {
 // Contraction depth of the mixed form: [1, z(D), z^2(D)] padded to 32.
 inline int kp_mixed_for_dim(int D) { return (2 * D + 1 + 31) / 32 * 32; }
 
+// The 32-frame-block copy ZB is read only by the statistics kernel of depths
+// that are not a power-of-two number of 16-byte pieces; at KP = 32, 64, 128,
+// 256 the statistics kernel forms it from ZA with transposing LDS reads and
+// ZB is neither allocated nor written.
+inline bool mixed_needs_zb(int KP) { return KP != 32 && KP != 64 && KP != 128 && KP != 256; }
+
 // largest feature dimension of the mixed forms (2D+1 <= 256)
 constexpr int PREP_MAX_D = 127;
 
@@ -19,7 +25,7 @@ struct PrepArgs {
     const float *off;      // [D] centring
     const float *scale;    // [D] 1/std
     uint16_t *za_hi_raw, *za_lo_raw;  // [Npad][KP]
-    uint16_t *zb_hi_raw, *zb_lo_raw;  // [Npad/32][32*KP]
+    uint16_t *zb_hi_raw, *zb_lo_raw;  // [Npad/32][32*KP]; null: not written
     uint8_t *bad;          // [N] written for every frame: 1 non-finite, 2 out of the 16-bit range, 0 ok
     unsigned int *nrange;  // count of (frame, dim) values flagged 2
     float zlimit;          // |z| above this is out of range (fp16: 240)

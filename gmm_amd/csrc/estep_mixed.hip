@@ -162,8 +162,8 @@ __global__ __launch_bounds__(256) void prep_frames_kernel(PrepArgs p) {
     const int pieces = 32 * U;  // per layout and part
     u32x4 *zah = reinterpret_cast<u32x4 *>(p.za_hi_raw) + blk * pieces;
     u32x4 *zal = reinterpret_cast<u32x4 *>(p.za_lo_raw) + blk * pieces;
-    u32x4 *zbh = reinterpret_cast<u32x4 *>(p.zb_hi_raw) + blk * pieces;
-    u32x4 *zbl = reinterpret_cast<u32x4 *>(p.zb_lo_raw) + blk * pieces;
+    u32x4 *zbh = p.zb_hi_raw ? reinterpret_cast<u32x4 *>(p.zb_hi_raw) + blk * pieces : nullptr;
+    u32x4 *zbl = p.zb_lo_raw ? reinterpret_cast<u32x4 *>(p.zb_lo_raw) + blk * pieces : nullptr;
     // ZA[n][k]: piece q of frame f holds k = 8q .. 8q+7
     for (int e = tid; e < pieces; e += 256) {
         const int f = e / U, k0 = (e - f * U) * 8;
@@ -181,6 +181,7 @@ __global__ __launch_bounds__(256) void prep_frames_kernel(PrepArgs p) {
     }
     // ZB[blk][u][ct][h][c][j]: slot ((u*CT + ct)*2 + h)*32 + c holds feature
     // 32ct + c of frames 16u + 8(j/4) + 4h + j%4, j = 0..7
+    if (!zbh) return;
     for (int e = tid; e < pieces; e += 256) {
         const int c = e & 31, h = (e >> 5) & 1, rest = e >> 6;
         const int ct = rest % CT, u = rest / CT;
@@ -395,18 +396,23 @@ __device__ __forceinline__ void split_pair(float a, float b, unsigned int &hi, u
 
 // ---------------------------------------------------------------------------
 // Direct-to-LDS staging (gfx950 global_load_lds_dwordx4: lane L of a wave
-// writes its 16 bytes at the wave-uniform LDS base + 16 L) into two LDS
-// buffers, for depths whose feature row is a power-of-two number of 16-byte
-// pieces (KP = 32, 64, 128, 256).  The copy of step s+1 runs while step s is
+// writes its 16 bytes at the wave-uniform LDS base + 16 L), for depths whose
+// feature row is a power-of-two number of 16-byte pieces (KP = 32, 64, 128,
+// 256).  The copies of a later step run while the current one is
 // contracted, with one barrier per step and no staging registers.
 //
 // ZA rows are stored swizzled: piece w of frame row c sits at slot
-// c*U + (w ^ za_swz(c)), so the 16 lanes that an LDS cycle serves (same w,
-// consecutive c) hit 16 distinct bank groups.
+// c*U + (w ^ za_swz(c)).  za_swz is a bijection on any 16 consecutive rows
+// (of the rows that share a 256-byte bank window), so the 16 lanes of a
+// 16-byte read of piece w of consecutive rows hit 16 distinct bank groups;
+// and for the four rows of an aligned block it differs in the bits above the
+// lowest two piece bits (U >= 8), so a transposing read of four pieces
+// 4i .. 4i+3 of each of four rows is conflict-free as well.
 template <int U>
-__device__ __forceinline__ int za_swz(int c) {
-    if constexpr (U >= 16) return c & 15;
-    else return (c / (16 / U)) & (U - 1);
+__host__ __device__ constexpr int za_swz(int c) {
+    if constexpr (U >= 16) return ((c & 3) << 2) | ((c >> 2) & 3);
+    else if constexpr (U == 8) return (((c >> 1) & 1) << 2) | ((c >> 2) & 3);
+    else return (c >> 2) & 3;
 }
 
 __device__ __forceinline__ void lds_dma16(const void *src, void *lds_wave_base) {
@@ -428,29 +434,41 @@ __device__ __forceinline__ void lds_dma16(__amdgpu_buffer_rsrc_t r, unsigned int
         r, (__attribute__((address_space(3))) void *)lds_wave_base, 16, voff, 0, 0, 0);
 }
 
+// Transposing LDS read (gfx950 ds_read_b64_tr_b16).  Within each group of 16
+// lanes, lane j supplies the address of an 8-byte piece P_j (four 16-bit
+// values); lane i of the group receives P_{4k + i/4}[i % 4] as element k,
+// k = 0..3.  With lane j pointing at frame fb + j/4, features b + 4 (j % 4),
+// lane i receives feature b + i of frames fb .. fb+3.
+typedef short v4s16 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4s16 lds_read_tr16(const void *lds_addr) {
+    return __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+        (__attribute__((address_space(3))) v4s16 *)const_cast<void *>(lds_addr));
+}
+
 // Statistics with direct-to-LDS staging.  A block of four waves owns four
 // component tiles (32 components each) and walks its frame chunk in 32-frame
-// steps; per step each wave copies one of the four staged parts — ZA hi, ZA
-// lo, ZB hi, ZB lo (U pieces x 32 rows each) — and the last wave also the
-// step's 32 per-frame log-likelihoods.  Iteration s contracts ZA of step s+1
-// (the first contraction, interleaved with step s's posteriors) and ZB of
-// step s (the frame contraction), so ZA is copied two steps ahead and ZB one
-// step ahead, each into the slot its previous occupant left at the start of
-// the iteration.  Each slot is a __shared__ variable of its own and the loop
-// is unrolled by two so that every access names its slot: the compiler can
-// then see that the copies in flight do not alias the reads of the
-// iteration and does not wait for them.
+// steps; per step the four waves copy the step's ZA hi and lo rows (U pieces
+// x 32 rows each, a quarter each) and the last wave also the step's 32
+// per-frame log-likelihoods.  Iteration s contracts ZA of step s+1 with the
+// split model (the first contraction, interleaved with step s's posteriors)
+// and the posteriors of step s with the features of step s (the frame
+// contraction), whose operand -- 8 frames of one feature per lane -- comes
+// out of the ZA rows of step s through transposing LDS reads.  So ZA is
+// copied two steps ahead into three rotating slots.  Each slot is a
+// __shared__ variable of its own and the loop is unrolled by three so that
+// every access names its slot: the compiler can then see that the copies in
+// flight do not alias the reads of the iteration and does not wait for them.
 template <int KP, typename T>
-__global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kernel(MixedArgs a) {
+__global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_tr_kernel(MixedArgs a) {
     typedef typename Vec8<T>::type V8;
     constexpr int KS = KP / 16;
     constexpr int CT = KP / 32;
     constexpr int U = KP / 8;               // 16-byte pieces per feature row
-    constexpr int PART = 32 * U;            // pieces per part and step
-    constexpr int NDMA = PART / 64;         // copies per wave and step
-    static_assert((U & (U - 1)) == 0 && PART % 64 == 0, "power-of-two feature rows");
-    __shared__ u32x4 ZA0[2][PART], ZA1[2][PART], ZB0[2][PART], ZB1[2][PART];
-    __shared__ float Ls0[32], Ls1[32];
+    constexpr int PART = 32 * U;            // pieces per part (hi or lo) and step
+    constexpr int NDMA = PART / 128;        // copies per wave and step (2 parts over 4 waves)
+    static_assert((U & (U - 1)) == 0 && PART % 128 == 0, "power-of-two feature rows");
+    __shared__ u32x4 ZA0[2][PART], ZA1[2][PART], ZA2[2][PART];
+    __shared__ float Ls0[32], Ls1[32], Ls2[32];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -465,7 +483,6 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
     const int tile = group * 4 + wave;
     const bool active = tile < a.Mt32;
     const bool cvalid = (32 * tile + c) < a.M;
-    // posterior threshold: padded components never pass
     const float thr = cvalid ? a.min_gamma : INFINITY;
 
     V8 bhi[KS], blo[KS];
@@ -496,42 +513,54 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
         if (f1 > a.N) f1 = a.N;
     }
 
-    // this wave's part (0 ZA hi, 1 ZA lo, 2 ZB hi, 3 ZB lo) and the source
-    // piece of each of its copies within a 32-frame block (ZA parts
-    // de-swizzle, ZB parts copy straight); LL2 holds at least ceil32(f1)
-    // entries, those past f1 are replaced by +inf below
-    const int part = wave;
-    const bool za_wave = part < 2;
-    const bool ll_wave = part == 3;
-    const int hl = part & 1;
-    const u32x4 *src = reinterpret_cast<const u32x4 *>(
-        part == 0 ? a.za_hi : part == 1 ? a.za_lo : part == 2 ? a.zb_hi : a.zb_lo);
-    // (byte offsets within the step's 32-frame block of the part)
+    // waves 0, 1 copy halves of ZA hi, waves 2, 3 of ZA lo; wave 3 also the
+    // step's 32 per-frame log-likelihoods.  Copy j of a wave moves piece
+    // p = q PART/2 + 64 j + lane of its part (row p / U, de-swizzled source).
+    const int hl = wave >> 1, q = wave & 1;
+    const bool ll_wave = wave == 3;
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(hl ? a.za_lo : a.za_hi);
     unsigned int soff[NDMA];
 #pragma unroll
     for (int j = 0; j < NDMA; j++) {
-        const int p = j * 64 + lane;
-        if (za_wave) {
-            const int row = p / U, v = p % U;
-            soff[j] = (unsigned int)(row * U + (v ^ za_swz<U>(row))) * 16u;
-        } else {
-            soff[j] = (unsigned int)p * 16u;
-        }
+        const int p = q * (PART / 2) + j * 64 + lane;
+        const int row = p / U, v = p % U;
+        soff[j] = (unsigned int)(row * U + (v ^ za_swz<U>(row))) * 16u;
+    }
+    // transposing reads of the frame contraction: this lane's two 8-byte
+    // pieces per (u, ct) are frames 16u + 8r + 4h + j/4, r = 0, 1, features
+    // 32 ct + 16 (lane / 16 % 2) + 4 (j % 4), j = lane % 16, at byte offset
+    // (fr U + (w ^ za_swz(fr))) 16 + 8 (j % 2) of a slot's hi (or lo) array,
+    // fr the frame row, w = fr's feature / 8.  The swizzle has period 16 in
+    // fr, so u only adds 16 rows.
+    static_assert(za_swz<U>(16) == za_swz<U>(0) && za_swz<U>(31) == za_swz<U>(15),
+                  "row swizzle of period 16");
+    unsigned int troff[2][CT];
+    {
+        const int g = lane >> 4, j = lane & 15;
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                const int fr = 8 * r + 4 * h + (j >> 2);
+                const int feat = 32 * ct + 16 * (g & 1) + 4 * (j & 3);
+                const int w = feat >> 3;
+                troff[r][ct] = (unsigned int)((fr * U + (w ^ za_swz<U>(fr))) * 16 + (feat & 4) * 2);
+            }
     }
     const unsigned int ulane = (unsigned int)lane;
     const int nsteps = (int)((f1 - f0 + 31) / 32);
-#define GMMHIP_DMA_PART(K_, ZD_, LD_)                                              \
+#define GMMHIP_TR_DMA(K_, ZD_, LD_)                                                \
     {                                                                              \
         const __amdgpu_buffer_rsrc_t rs =                                          \
             dma_rsrc(src + (f0 + 32 * (uint64_t)(K_)) * U, PART * 16);             \
         _Pragma("unroll") for (int j = 0; j < NDMA; j++)                           \
-            lds_dma16(rs, soff[j], &ZD_[hl][j * 64]);                              \
+            lds_dma16(rs, soff[j], &ZD_[hl][q * (PART / 2) + j * 64]);             \
         if (ll_wave && lane < 32)                                                  \
             __builtin_amdgcn_global_load_lds(                                      \
                 a.LL2 + f0 + 32 * (uint64_t)(K_) + ulane,                          \
                 (__attribute__((address_space(3))) void *)&LD_[0], 4, 0, 0);       \
     }
-#define GMMHIP_CONTRACT1(ACC_, ZR_)                                                \
+#define GMMHIP_TR_CONTRACT1(ACC_, ZR_)                                             \
     _Pragma("unroll") for (int ks = 0; ks < KS; ks++) {                            \
         const int slot = c * U + ((2 * ks + h) ^ za_swz<U>(c));                    \
         const V8 zh = __builtin_bit_cast(V8, ZR_[0][slot]);                        \
@@ -540,44 +569,48 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
         ACC_ = mfma32(zh, blo[ks], ACC_);                                          \
         ACC_ = mfma32(zl, bhi[ks], ACC_);                                          \
     }
-    // prologue: ZA of steps 0 and 1, ZB and LL2 of step 0
-    if (za_wave) {
-        GMMHIP_DMA_PART(0, ZA0, Ls0)
-        if (nsteps > 1) GMMHIP_DMA_PART(1, ZA1, Ls1)
-    } else {
-        GMMHIP_DMA_PART(0, ZB0, Ls0)
+    // prologue: ZA of steps 0 and 1, LL2 of step 0
+    GMMHIP_TR_DMA(0, ZA0, Ls0)
+    if (nsteps > 1) {
+        const __amdgpu_buffer_rsrc_t rs = dma_rsrc(src + (f0 + 32) * U, PART * 16);
+#pragma unroll
+        for (int j = 0; j < NDMA; j++) lds_dma16(rs, soff[j], &ZA1[hl][q * (PART / 2) + j * 64]);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    // one FP32 accumulator takes hi*hi, hi*lo and lo*hi
     f32x16 acc = {};
     if (active) {
-        GMMHIP_CONTRACT1(acc, ZA0)
+        GMMHIP_TR_CONTRACT1(acc, ZA0)
     }
 
-    // iteration s with parity P: reads ZA of step s+1 (ZAn), ZB and LL2 of
-    // step s (ZBc, Lc); copies ZA of step s+2 into ZAc and ZB / LL2 of step
-    // s+1 into ZBn / Ln
-#define GMMHIP_STATS_STEP(S_, ZAc, ZAn, ZBc, ZBn, Lc, Ln, ACC_, NACC_)           \
+    // iteration s: slot C holds ZA / LL2 of step s (frame contraction,
+    // posteriors), slot N ZA of step s+1 (first contraction) and receives
+    // LL2 of step s+1, slot F receives ZA of step s+2
+#define GMMHIP_TR_STEP(S_, ZC, ZN, ZF, LC, LN, ACC_, NACC_)                        \
     {                                                                              \
         const int s_ = (S_);                                                       \
         __builtin_amdgcn_s_waitcnt(0);                                             \
         if (ll_wave && s_ == nsteps - 1 && lane < 32 && f0 + 32 * (uint64_t)s_ + lane >= f1) \
-            Lc[lane] = INFINITY; /* frames past the chunk end: posterior 0 */      \
+            LC[lane] = INFINITY; /* frames past the chunk end: posterior 0 */      \
         __syncthreads();                                                           \
-        if (za_wave) {                                                             \
-            if (s_ + 2 < nsteps) GMMHIP_DMA_PART(s_ + 2, ZAc, Lc)                   \
-        } else if (s_ + 1 < nsteps) {                                              \
-            GMMHIP_DMA_PART(s_ + 1, ZBn, Ln)                                       \
+        if (s_ + 2 < nsteps) {                                                     \
+            const __amdgpu_buffer_rsrc_t rs =                                      \
+                dma_rsrc(src + (f0 + 32 * (uint64_t)(s_ + 2)) * U, PART * 16);     \
+            _Pragma("unroll") for (int j = 0; j < NDMA; j++)                       \
+                lds_dma16(rs, soff[j], &ZF[hl][q * (PART / 2) + j * 64]);          \
         }                                                                          \
+        if (ll_wave && lane < 32 && s_ + 1 < nsteps)                               \
+            __builtin_amdgcn_global_load_lds(                                      \
+                a.LL2 + f0 + 32 * (uint64_t)(s_ + 1) + ulane,                      \
+                (__attribute__((address_space(3))) void *)&LN[0], 4, 0, 0);        \
         if (active) {                                                              \
             NACC_ = (f32x16){};                                                    \
-            GMMHIP_CONTRACT1(NACC_, ZAn)                                           \
+            GMMHIP_TR_CONTRACT1(NACC_, ZN)                                         \
             u32x4 gh4[2], gl4[2];                                                  \
             _Pragma("unroll") for (int r = 0; r < 16; r += 2) {                    \
                 const int row = 8 * (r >> 2) + 4 * h + (r & 3);                    \
-                float g0 = exp2f_fast(ACC_[r] - Lc[row]);                          \
-                float g1 = exp2f_fast(ACC_[r + 1] - Lc[row + 1]);                  \
+                float g0 = exp2f_fast(ACC_[r] - LC[row]);                          \
+                float g1 = exp2f_fast(ACC_[r + 1] - LC[row + 1]);                  \
                 g0 = (g0 > thr) ? g0 : 0.0f;                                       \
                 g1 = (g1 > thr) ? g1 : 0.0f;                                       \
                 unsigned int wh, wl;                                               \
@@ -596,9 +629,18 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
                 const V8 ghi = __builtin_bit_cast(V8, gh4[u]);                     \
                 const V8 glo = __builtin_bit_cast(V8, gl4[u]);                     \
                 _Pragma("unroll") for (int ct = 0; ct < CT; ct++) {                \
-                    const int slot = ((u * CT + ct) * 2 + h) * 32 + c;             \
-                    const V8 yh = __builtin_bit_cast(V8, ZBc[0][slot]);            \
-                    const V8 yl = __builtin_bit_cast(V8, ZBc[1][slot]);            \
+                    const char *bh = reinterpret_cast<const char *>(&ZC[0][0]);    \
+                    const char *bl = reinterpret_cast<const char *>(&ZC[1][0]);    \
+                    const char *bhu = bh + u * (16 * U * 16);                      \
+                    const char *blu = bl + u * (16 * U * 16);                      \
+                    const v4s16 h0 = lds_read_tr16(bhu + troff[0][ct]);            \
+                    const v4s16 h1 = lds_read_tr16(bhu + troff[1][ct]);            \
+                    const v4s16 l0 = lds_read_tr16(blu + troff[0][ct]);            \
+                    const v4s16 l1 = lds_read_tr16(blu + troff[1][ct]);            \
+                    const V8 yh = __builtin_bit_cast(                              \
+                        V8, __builtin_shufflevector(h0, h1, 0, 1, 2, 3, 4, 5, 6, 7)); \
+                    const V8 yl = __builtin_bit_cast(                              \
+                        V8, __builtin_shufflevector(l0, l1, 0, 1, 2, 3, 4, 5, 6, 7)); \
                     acc2[ct] = mfma32(ghi, yh, acc2[ct]);                          \
                     acc2[ct] = mfma32(ghi, yl, acc2[ct]);                          \
                     acc2[ct] = mfma32(glo, yh, acc2[ct]);                          \
@@ -606,23 +648,22 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_dma_kern
             }                                                                      \
         }                                                                          \
     }
-    // Step s's posteriors (register r = 8u + j holds frame 16u + 8(j/4) +
-    // 4h + j%4; rows r and r+1, r even, are consecutive frames and one
-    // packed word) interleave with the first contraction of step s+1 (on
-    // the last step it runs on stale pieces and is discarded); a 16-frame
-    // half whose posteriors are all under the cut (exact zeros: hi is zero
-    // only when g is) adds nothing to the frame contraction and is skipped.
-    // (the two unrolled halves swap the roles of acc and acc_n)
     f32x16 acc_n;
     int s = 0;
-    for (; s + 1 < nsteps; s += 2) {
-        GMMHIP_STATS_STEP(s, ZA0, ZA1, ZB0, ZB1, Ls0, Ls1, acc, acc_n)
-        GMMHIP_STATS_STEP(s + 1, ZA1, ZA0, ZB1, ZB0, Ls1, Ls0, acc_n, acc)
+    for (; s + 2 < nsteps; s += 3) {
+        GMMHIP_TR_STEP(s, ZA0, ZA1, ZA2, Ls0, Ls1, acc, acc_n)
+        GMMHIP_TR_STEP(s + 1, ZA1, ZA2, ZA0, Ls1, Ls2, acc_n, acc)
+        GMMHIP_TR_STEP(s + 2, ZA2, ZA0, ZA1, Ls2, Ls0, acc, acc_n)
+        acc = acc_n;
     }
-    if (s < nsteps) GMMHIP_STATS_STEP(s, ZA0, ZA1, ZB0, ZB1, Ls0, Ls1, acc, acc_n)
-#undef GMMHIP_STATS_STEP
-#undef GMMHIP_DMA_PART
-#undef GMMHIP_CONTRACT1
+    if (s < nsteps) {
+        GMMHIP_TR_STEP(s, ZA0, ZA1, ZA2, Ls0, Ls1, acc, acc_n)
+        acc = acc_n;
+    }
+    if (s + 1 < nsteps) GMMHIP_TR_STEP(s + 1, ZA1, ZA2, ZA0, Ls1, Ls2, acc, acc_n)
+#undef GMMHIP_TR_STEP
+#undef GMMHIP_TR_DMA
+#undef GMMHIP_TR_CONTRACT1
 
     if (!active) return;
     const size_t Mp = (size_t)a.Mt32 * 32;
@@ -899,9 +940,9 @@ hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s) {
 #define XD(K)                                                                                     \
     case K:                                                                                       \
         if (prec == 2)                                                                            \
-            hipLaunchKernelGGL((stats_mixed_dma_kernel<K, _Float16>), grid, dim3(256), 0, s, a);  \
+            hipLaunchKernelGGL((stats_mixed_tr_kernel<K, _Float16>), grid, dim3(256), 0, s, a);  \
         else                                                                                      \
-            hipLaunchKernelGGL((stats_mixed_dma_kernel<K, __bf16>), grid, dim3(256), 0, s, a);    \
+            hipLaunchKernelGGL((stats_mixed_tr_kernel<K, __bf16>), grid, dim3(256), 0, s, a);    \
         break;
         XD(32) XD(64) X(96) XD(128) X(160) X(192) X(224) XD(256)
 #undef X

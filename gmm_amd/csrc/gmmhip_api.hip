@@ -195,15 +195,21 @@ int prepare_mixed(gmmhip_ctx *h, bool *use_mixed) {
         if (h->z_cap < zn) {
             hipFree(h->d_za_hi);
             hipFree(h->d_za_lo);
-            hipFree(h->d_zb_hi);
-            hipFree(h->d_zb_lo);
-            h->d_za_hi = h->d_za_lo = h->d_zb_hi = h->d_zb_lo = nullptr;
+            h->d_za_hi = h->d_za_lo = nullptr;
             h->z_cap = 0;
             HIPCHK(h, hipMalloc((void **)&h->d_za_hi, zn * 2));
             HIPCHK(h, hipMalloc((void **)&h->d_za_lo, zn * 2));
+            h->z_cap = zn;
+        }
+        const bool need_zb = mixed_needs_zb(KPm);
+        if (need_zb && h->zb_cap < zn) {
+            hipFree(h->d_zb_hi);
+            hipFree(h->d_zb_lo);
+            h->d_zb_hi = h->d_zb_lo = nullptr;
+            h->zb_cap = 0;
             HIPCHK(h, hipMalloc((void **)&h->d_zb_hi, zn * 2));
             HIPCHK(h, hipMalloc((void **)&h->d_zb_lo, zn * 2));
-            h->z_cap = zn;
+            h->zb_cap = zn;
         }
         HIPCHK(h, ensure(&h->d_bad, &h->bad_cap, (size_t)N));
         if (!h->d_nrange) HIPCHK(h, hipMalloc((void **)&h->d_nrange, sizeof(unsigned int)));
@@ -218,8 +224,8 @@ int prepare_mixed(gmmhip_ctx *h, bool *use_mixed) {
         p.scale = h->d_zscale;
         p.za_hi_raw = h->d_za_hi;
         p.za_lo_raw = h->d_za_lo;
-        p.zb_hi_raw = h->d_zb_hi;
-        p.zb_lo_raw = h->d_zb_lo;
+        p.zb_hi_raw = need_zb ? h->d_zb_hi : nullptr;
+        p.zb_lo_raw = need_zb ? h->d_zb_lo : nullptr;
         p.bad = h->d_bad;
         p.nrange = h->d_nrange;
         p.zlimit = (prec == GMMHIP_PREC_FP16X3) ? 240.0f : INFINITY;

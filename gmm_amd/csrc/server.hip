@@ -110,8 +110,9 @@ Layout layout_of(const Shape &s, const Dims &d) {
     const size_t zb = s.mixed ? Nz * KP * 2 : 0;
     L.zah = take(zb);
     L.zal = take(zb);
-    L.zbh = take(zb);
-    L.zbl = take(zb);
+    const size_t zbb = mixed_needs_zb(KP) ? zb : 0;
+    L.zbh = take(zbb);
+    L.zbl = take(zbb);
     L.bad = take(s.mixed ? s.N : 0);
     L.nrange = take(s.mixed ? 4 : 0);
     L.total = p;
@@ -328,8 +329,8 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
         p.scale = s->d_zscale;
         p.za_hi_raw = (uint16_t *)(A + L.zah);
         p.za_lo_raw = (uint16_t *)(A + L.zal);
-        p.zb_hi_raw = (uint16_t *)(A + L.zbh);
-        p.zb_lo_raw = (uint16_t *)(A + L.zbl);
+        p.zb_hi_raw = mixed_needs_zb(d.KPm) ? (uint16_t *)(A + L.zbh) : nullptr;
+        p.zb_lo_raw = mixed_needs_zb(d.KPm) ? (uint16_t *)(A + L.zbl) : nullptr;
         p.bad = bad;
         p.nrange = nrange;
         p.zlimit = s->zlimit;
