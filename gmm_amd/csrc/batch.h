@@ -19,7 +19,7 @@ struct SegReduceArgs {
 struct MllrSolveArgs {
     const double *Gp;     // [(u,i)][ldg] packed upper triangle of G
     const double *Kp;     // [(u,i)][ldk] k
-    const double *Ginit;  // [D][E][E] UBM prior or null
+    const double *Ginit;  // [D][E(E+1)/2] UBM prior, packed as Gp, or null
     const double *kinit;  // [D][E] or null
     int U, D;
     int ldg, ldk;

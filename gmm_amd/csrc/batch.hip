@@ -291,13 +291,14 @@ __global__ __launch_bounds__(64) void mllr_solve_kernel(MllrSolveArgs a) {
         return;
     }
     const double *gp = a.Gp + ui * a.ldg;
+    const int NP = E * (E + 1) / 2;
     for (int p = lane; p < E * E; p += 64) {
         const int r = p / E, cc = p - r * E;
         const int lo = r < cc ? r : cc, hi = r < cc ? cc : r;
         // packed index of (lo, hi), lo <= hi: lo*E - lo(lo-1)/2 + (hi - lo)
         const int pk = lo * E - (lo * (lo - 1)) / 2 + (hi - lo);
         double g = gp[pk];
-        if (a.Ginit) g += a.Ginit[(size_t)i * E * E + p];
+        if (a.Ginit) g += a.Ginit[(size_t)i * NP + pk];
         G[r * LDS + cc] = g;
     }
     for (int p = lane; p < E; p += 64) {
@@ -394,14 +395,14 @@ hipError_t launch_mllr_check(const double *n, int U, int M, int D, int *status, 
 // ---------------------------------------------------------------------------
 // Register-resident solve for E = D+1 <= 64.  One wave per (u, i): lane r
 // holds row r of [G | k] (G symmetric, unpacked from the packed upper
-// triangle staged in LDS, plus the prior) in registers.  Gauss-Jordan
+// triangle of G plus the prior, staged in LDS) in registers.  Gauss-Jordan
 // elimination without pivoting, as the reference inverts G_i
 // (MATRIX::_Invert_Gen, Matrix4_LMa.cpp:2232; stable for SPD G, whose pivots
 // are the ratios of its leading principal minors and so all positive): at
-// step j the pivot row is broadcast element by element with v_readlane,
-// every other row subtracts its multiple of it, and every row shifts left
-// one column, so the current column is always register 0 and every register
-// index is a compile-time constant.  After E steps lane r holds k'_r and its
+// step j the pivot row is broadcast element by element with v_readlane and
+// every other row subtracts its multiple of it.  Steps run in groups of 8
+// with the columns updated in place, and every row shifts left by 8 columns
+// between groups, so every register index is a compile-time constant.  After E steps lane r holds k'_r and its
 // pivot g'_rr, and w_r = k'_r / g'_rr.  A pivot <= 0 (G not positive
 // definite) marks the utterance's status.
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
@@ -424,46 +425,62 @@ __global__ __launch_bounds__(64) void mllr_solve_gj_kernel(MllrSolveArgs a) {
         return;
     }
     const double *gp = a.Gp + ui * a.ldg;
-    for (int p = r; p < NP; p += 64) sG[p] = gp[p];
+    if (a.Ginit) {
+        const double *gip = a.Ginit + (size_t)i * NP;
+        for (int p = r; p < NP; p += 64) sG[p] = gp[p] + gip[p];
+    } else {
+        for (int p = r; p < NP; p += 64) sG[p] = gp[p];
+    }
     __syncthreads();
-    const double *gi = a.Ginit ? a.Ginit + (size_t)i * E * E + (size_t)r * E : nullptr;
+    // every element is read unconditionally at clamped indices (zeroed
+    // afterwards outside the E x E system), so the reads issue back to back
+    // instead of one per branch
+    const int rr = r < E ? r : E - 1;
     double g[EMAX];
 #pragma unroll
     for (int c = 0; c < EMAX; c++) {
-        double v = 0.0;
-        if (r < E && c < E) {
-            const int lo = r < c ? r : c, hi = r < c ? c : r;
-            // packed index of (lo, hi), lo <= hi: lo*E - lo(lo-1)/2 + (hi - lo)
-            v = sG[lo * E - (lo * (lo - 1)) / 2 + (hi - lo)];
-            if (gi) v += gi[c];
-        }
-        g[c] = v;
+        const int cc = c < E ? c : E - 1;
+        const int lo = rr < cc ? rr : cc, hi = rr < cc ? cc : rr;
+        // packed index of (lo, hi), lo <= hi: lo*E - lo(lo-1)/2 + (hi - lo)
+        g[c] = sG[lo * E - (lo * (lo - 1)) / 2 + (hi - lo)];
     }
-    double kr = 0.0;
-    if (r < E) {
-        kr = a.Kp[ui * a.ldk + r];
-        if (a.kinit) kr += a.kinit[(size_t)i * E + r];
-    }
+#pragma unroll
+    for (int c = 0; c < EMAX; c++)
+        if (!(r < E && c < E)) g[c] = 0.0;
+    double kr = a.Kp[ui * a.ldk + rr];
+    if (a.kinit) kr += a.kinit[(size_t)i * E + rr];
+    if (r >= E) kr = 0.0;
     double diag = 1.0;
     bool bad = false;
-    for (int j = 0; j < E; j++) {
-        const double piv = readlane_f64(g[0], j);
-        const double kj = readlane_f64(kr, j);
-        bad = bad || !(piv > 0.0);
-        const double f = (r == j) ? 0.0 : g[0] / piv;
-        if (r == j) diag = piv;
-        kr = fma(-f, kj, kr);
-        // columns to the right of the pivot, in blocks of 8 behind uniform
-        // branches; registers past the active columns are dead after the step
-        const int rem = E - 1 - j;
+    // steps in groups of 8: within a group, register c holds column
+    // 8 jb + c and columns are updated in place; after the group every
+    // register moves down by 8 columns
+    static_assert(EMAX % 8 == 0, "groups of 8 columns");
+    for (int jb = 0; 8 * jb < E; jb++) {
 #pragma unroll
-        for (int c0 = 0; c0 < EMAX - 1; c0 += 8) {
-            if (c0 < rem) {
+        for (int jj = 0; jj < 8; jj++) {
+            const int j = 8 * jb + jj;
+            if (j < E) {
+                const double piv = readlane_f64(g[jj], j);
+                const double kj = readlane_f64(kr, j);
+                bad = bad || !(piv > 0.0);
+                const double f = (r == j) ? 0.0 : g[jj] / piv;
+                if (r == j) diag = piv;
+                kr = fma(-f, kj, kr);
+                // columns right of the pivot, in blocks of 8 behind uniform
+                // branches (block cb holds columns 8 (jb + cb) ..)
 #pragma unroll
-                for (int c = c0; c < c0 + 8 && c < EMAX - 1; c++)
-                    g[c] = fma(-f, readlane_f64(g[c + 1], j), g[c + 1]);
+                for (int cb = jj / 8; cb < EMAX / 8; cb++) {
+                    if (8 * (jb + cb) < E) {
+#pragma unroll
+                        for (int c = (8 * cb > jj + 1 ? 8 * cb : jj + 1); c < 8 * cb + 8; c++)
+                            g[c] = fma(-f, readlane_f64(g[c], j), g[c]);
+                    }
+                }
             }
         }
+#pragma unroll
+        for (int c = 0; c < EMAX - 8; c++) g[c] = g[c + 8];
     }
     if (r < E) a.W[ui * E + r] = kr / diag;
     if (bad && r == 0) atomicOr(&a.status[u], 8);

@@ -122,15 +122,20 @@ int build_mllr_operands(gmmhip_ctx *h, gmmhip_batch_state *b) {
             k[D] += ck;
         }
     }
+    // the prior of row i, packed like the per-utterance G (upper triangle)
+    std::vector<double> Gip((size_t)D * NP);
+    for (int i = 0; i < D; i++)
+        for (int a = 0, pk = 0; a < E; a++)
+            for (int c = a; c < E; c++, pk++) Gip[(size_t)i * NP + pk] = Gi[((size_t)i * E + a) * E + c];
     HIPCHK(h, ensure(&b->d_P, &b->P_cap, P.size()));
     HIPCHK(h, ensure(&b->d_Xi, &b->Xi_cap, Xi.size()));
-    HIPCHK(h, ensure(&b->d_Ginit, &b->Ginit_cap, Gi.size()));
+    HIPCHK(h, ensure(&b->d_Ginit, &b->Ginit_cap, Gip.size()));
     HIPCHK(h, ensure(&b->d_kinit, &b->kinit_cap, ki.size()));
     HIPCHK(h, ensure(&b->d_ivar, &b->ivar_cap, iv.size()));
     HIPCHK(h, hipMemcpyAsync(b->d_ivar, iv.data(), iv.size() * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(b->d_P, P.data(), P.size() * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(b->d_Xi, Xi.data(), Xi.size() * 8, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(b->d_Ginit, Gi.data(), Gi.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(b->d_Ginit, Gip.data(), Gip.size() * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(b->d_kinit, ki.data(), ki.size() * 8, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     b->mllr_model = h->model_version;

@@ -199,8 +199,8 @@ int gemm_posteriors(gmmhip_ctx *h, float *ll, float *post);
 
 // Device copies of the active model for the per-utterance kernels: FP32
 // means / variances and, with mllr, the UBM-only MLLR operands
-// P[m][packed a<=b] = xi_a xi_b, Xi[m][a] (xi = [mu, 1]), the prior G/k
-// and 1/var in FP64 (batch_api.hip).  Valid until the next gmmhip_set_model.
+// P[m][packed a<=b] = xi_a xi_b, Xi[m][a] (xi = [mu, 1]), the prior G (one
+// per row i, packed like P), the prior k and 1/var in FP64 (batch_api.hip).  Valid until the next gmmhip_set_model.
 struct ModelOperands {
     const float *mu, *var;
     const double *P, *Xi, *Ginit, *kinit, *ivar;
