@@ -9,16 +9,20 @@
 // z^2 inside the 16-bit exponent range and removes the x^2 cancellation of
 // offset data; the model coefficients absorb o and s.
 //
-// Frame features [1, z, z^2, 0...] are split once per frame set into two
-// layouts (prep_frames_kernel):
+// Frame features [1, z, z^2, 0...] are split once per frame set
+// (prep_frames_kernel) into
 //   ZA[n][k]                   row-major: the log-likelihood contraction
 //                              reads 8 consecutive k of one frame per lane;
-//   ZB[blk][u][ct][h][c][j]    per 32-frame block: the statistics
-//                              contraction over frames reads, for column
-//                              32ct+c, the 8 frames 16u + 8(j/4) + 4h + j%4
-//                              that lane (h, c) holds after the first
-//                              contraction (the 32 lanes of a half-wave read
-//                              consecutive 16-byte pieces).
+//                              the statistics kernel of power-of-two depths
+//                              also takes its frame-contraction operand from
+//                              these rows, by transposing LDS reads;
+//   ZB[blk][u][ct][h][c][j]    (other depths only) per 32-frame block: the
+//                              register-staged statistics kernel's frame
+//                              contraction reads, for column 32ct+c, the 8
+//                              frames 16u + 8(j/4) + 4h + j%4 that lane
+//                              (h, c) holds after the first contraction (the
+//                              32 lanes of a half-wave read consecutive
+//                              16-byte pieces).
 // MFMA 32x32x16 lane layout (lane = 32h + c): A[row c][k 8h+j],
 // B[k 8h+j][col c], C reg r -> row 8(r/4) + 4h + r%4, col c.
 #include "estep.h"
