@@ -61,8 +61,12 @@ def check(acc, prec, x, w, mu, var, ref_ll=None, ref_st=None):
 
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("N,D,M", [(1, 3, 1), (77, 5, 3), (1000, 20, 33), (5000, 60, 100),
-                                   (3000, 39, 64), (400, 127, 40), (2049, 60, 257)])
+                                   (3000, 39, 64), (400, 127, 40), (2049, 60, 257),
+                                   (600, 70, 20), (500, 90, 9), (300, 100, 12)])
 def test_mixed_shapes(acc, prec, N, D, M):
+    """Depths KP = 32, 64, 128, 256 (statistics from the ZA rows by
+    transposing LDS reads) and 96, 160, 192, 224 (register-staged statistics
+    kernel with the ZB copy)."""
     x, w, mu, var = case(500 + N + D + M, N, D, M)
     check(acc, prec, x, w, mu, var)
 
