@@ -154,13 +154,16 @@ def test_batch_mllr_wide_and_many_components(acc, D, M):
 
 
 @pytest.mark.parametrize("prec,tol", [(gmm_amd.PREC_FP16X3, 2e-4), (gmm_amd.PREC_BF16X3, 2e-3)])
-def test_batch_mixed_precision(acc, prec, tol):
+@pytest.mark.parametrize("D", [12, 30, 45])
+def test_batch_mixed_precision(acc, prec, tol, D):
     """The mixed form in the batched path: per-utterance statistics, MAP
     means and MLLR transforms against the oracle (mixed-precision
     tolerances, as the E-step's), graph replays bit-identical to eager runs,
     and each utterance's result independent of its batch-mates (UBM-based
-    centring and scaling, chunks relative to the utterance start)."""
-    x, off, w, mu, var = make_batch(11, 30, 64, 12, lo=20, hi=600, extra=(0, 5))
+    centring and scaling, chunks relative to the utterance start).  Depths
+    KP = 32, 64 (statistics by transposing LDS reads) and 96 (the
+    register-staged statistics kernel with the ZB copy)."""
+    x, off, w, mu, var = make_batch(11, 30, 64, D, lo=20, hi=600, extra=(0, 5))
     acc.set_model(w, mu, var)
     ref = per_utt_oracle(x, off, w, mu, var)
     acc.set_opts(precision=prec)
