@@ -734,7 +734,10 @@ extern "C" int gmmhip_server_create(int device, const gmmhip_server_cfg *cfg, in
             free_server(s);
             return GMMHIP_ERR_OOM;
         }
-        const size_t stage = std::min<size_t>(c.slot_bytes, 64u << 20);
+        // a batch's IN and OUT parts each fit the slot budget, so staging of
+        // that size never regrows on the dispatch path (a regrowth frees
+        // pinned memory, which waits for the device, and pins anew)
+        const size_t stage = c.slot_bytes;
         if (grow_pinned(s, &sl.h_in, &sl.h_in_cap, stage) ||
             grow_pinned(s, &sl.h_out, &sl.h_out_cap, stage)) {
             free_server(s);

@@ -314,7 +314,9 @@ typedef struct gmmhip_server_cfg {
                                 UBM's global mean and deviation) */
     int slots;               /* batches in flight, 0 = 2 */
     uint64_t slot_bytes;     /* device budget of one batch: frames, per-frame
-                                values, partial sums and outputs; 0 = 1 GiB */
+                                values, partial sums and outputs; 0 = 1 GiB.
+                                Each slot also pins 2 x slot_bytes of host
+                                memory for its staging */
     int max_batch;           /* requests per batch, 0 = 1024 */
     int chunk_frames;        /* frames per statistics work item, 0 = 512 */
     int tau;                 /* MAP relevance factor */
