@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 namespace gmmhip {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -58,31 +60,34 @@ hipError_t launch_fill_u32(unsigned int *p, unsigned int v, int n, hipStream_t s
 // ---------------------------------------------------------------------------
 // Per-utterance reduction of the chunk partials [chunk][Mp][KP] (columns
 // [1, x', x'^2] of centred frames) into n[u][m] and un-centred F[u][m][d],
-// FP64, chunks of an utterance summed in order.  One thread per (u, m, j),
-// j = 0..D.
+// FP64, chunks of an utterance summed in order.  A block of 4 waves takes 4
+// components of one utterance (blockIdx.y, strided by gridDim.y), wave w
+// component 4 blockIdx.x + w and lane j the columns j, j + 64, ... of
+// 0..D, so no thread divides an index.
 __global__ __launch_bounds__(256) void seg_reduce_kernel(SegReduceArgs a) {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const int m = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (m >= a.M) return;
     const int J = a.D + 1;
-    const uint64_t total = (uint64_t)a.U * a.M * J;
-    if (i >= total) return;
-    const int j = (int)(i % J);
-    const uint64_t um = i / J;
-    const int m = (int)(um % a.M);
-    const int u = (int)(um / a.M);
     const size_t E = (size_t)a.Mp * a.KP;
-    double s = 0.0, n = 0.0;
-    for (int c = a.chunk_first[u]; c < a.chunk_first[u + 1]; c++) {
-        const float *p = a.partials + (size_t)c * E + (size_t)m * a.KP;
-        s += (double)p[j];
-        if (j > 0) n += (double)p[0];
-    }
-    if (j == 0) {
-        a.n[um] = s;
-    } else {
-        // x = x' + o (FP32 form) or x = z / scale + o (mixed form):
-        // F = F' + n o or F = F' / scale + n o
-        const double f = a.scale ? s / (double)a.scale[j - 1] : s;
-        a.F[um * a.D + (j - 1)] = f + n * (double)a.off[j - 1];
+    for (int u = blockIdx.y; u < a.U; u += gridDim.y) {
+        const int c0 = a.chunk_first[u], c1 = a.chunk_first[u + 1];
+        const uint64_t um = (uint64_t)u * a.M + m;
+        for (int j = (int)(threadIdx.x & 63); j < J; j += 64) {
+            double s = 0.0, n = 0.0;
+            for (int c = c0; c < c1; c++) {
+                const float *p = a.partials + (size_t)c * E + (size_t)m * a.KP;
+                s += (double)p[j];
+                if (j > 0) n += (double)p[0];
+            }
+            if (j == 0) {
+                a.n[um] = s;
+            } else {
+                // x = x' + o (FP32 form) or x = z / scale + o (mixed form):
+                // F = F' + n o or F = F' / scale + n o
+                const double f = a.scale ? s / (double)a.scale[j - 1] : s;
+                a.F[um * a.D + (j - 1)] = f + n * (double)a.off[j - 1];
+            }
+        }
     }
 }
 
@@ -373,9 +378,9 @@ hipError_t launch_seg_sum(const float *v, const uint64_t *begin, const uint64_t 
 }
 
 hipError_t launch_seg_reduce(const SegReduceArgs &a, hipStream_t s) {
-    const uint64_t total = (uint64_t)a.U * a.M * (a.D + 1);
-    hipLaunchKernelGGL(seg_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       a);
+    if (a.U <= 0 || a.M <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((a.M + 3) / 4), (unsigned)std::min(a.U, 65535));
+    hipLaunchKernelGGL(seg_reduce_kernel, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
