@@ -17,7 +17,13 @@ batch_map (batched MAP + MLLR of 4,096 utterances), serve (serving loop),
 fullcov_estep (full-covariance E-step, D=60, M=512, N=1e6).
 
 Prints ONE JSON line (rank 0).  Frames are generated directly in HBM, so the
-timed region starts with inputs resident.
+timed region starts with inputs resident.  Beside the headline the line
+carries: the first E-step on a new frame set (the mixed form's
+once-per-frame-set preparation included), errors against FP64 (total LL,
+per-frame LL over all frames, posteriors of a frame sample, occupancies,
+M-step means and variances), the serving loop at 5,000 req/s on the
+2048-component UBM (p50/p99 latency, FP32 and fp16x3), and the restated
+CPU path on the host cores.
 """
 import argparse
 import json
@@ -35,11 +41,23 @@ PEAK = {"fp32": 157.3e12, "bf16": 2.5e15, "fp16": 2.5e15}
 HBM_PEAK = 8.0e12
 
 
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(x_sample, w, mu, var, budget_s=12.0, max_frames=10_000_000):
     """Restated SSE CPU path (oracle/sse_port.c) on the host cores over a
-    bounded prefix of the same frames."""
+    bounded prefix of the same frames: one thread per core this process may
+    run on, at most 16 (the host-core share of one GPU on a GPU box)."""
     from oracle import binding as orc
-    threads = max(1, min(16, os.cpu_count() or 1))
+    visible = len(os.sched_getaffinity(0))
+    threads = max(1, min(16, visible))
     M = mu.shape[0]
     n0 = min(4000, x_sample.shape[0])
     t0 = time.perf_counter()
@@ -50,15 +68,17 @@ def cpu_baseline(x_sample, w, mu, var, budget_s=12.0, max_frames=10_000_000):
     _, _, _, _, used = orc.sse_accumulate(x_sample[:n], w, mu, var, threads)
     dt = time.perf_counter() - t0
     return {"value": n * M / dt, "unit": "frame*comp/s", "cores": int(used),
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "cores_visible": visible,
             "sample": "first %d frames of the workload, %.1f s, reference SSE CPU path restated "
-                      "(oracle/sse_port.c), %d threads" % (n, dt, used)}
+                      "(oracle/sse_port.c), %d threads on %s (%d cores visible to the "
+                      "process)" % (n, dt, used, cpu_model(), visible)}
 
 
-def fp64_reference(x, w, mu, var, min_gamma=1e-4, chunk=262144):
+def fp64_reference(x, w, mu, var, min_gamma=1e-4, chunk=262144, frame_ll=None):
     """FP64 E-step over all frames on the GPU (torch float64; a checker,
     not the product): total log-likelihood and n, F, S with the 1e-4
-    posterior cut, for the error report."""
+    posterior cut, for the error report; per-frame log-likelihoods into
+    frame_ll (float64 tensor [N]) when given."""
     import torch
     dev = x.device
     M, D = mu.shape
@@ -79,6 +99,8 @@ def fp64_reference(x, w, mu, var, min_gamma=1e-4, chunk=262144):
         x2 = xc * xc
         L = torch.mm(xc, B1) + torch.mm(x2, B2) + gconst
         lse = torch.logsumexp(L, dim=1)
+        if frame_ll is not None:
+            frame_ll[a:a + chunk] = lse
         tot += lse.sum()
         g = torch.exp(L - lse[:, None])
         g = torch.where(g > min_gamma, g, torch.zeros_like(g))
@@ -106,6 +128,74 @@ def accuracy(res, ref):
             "mean_max_abs_err": float(np.max(np.abs(mu - rmu))),
             "var_max_abs_err": float(np.max(np.abs(v - rv))),
             "var_max_rel_err": float(np.max(np.abs(v - rv) / rv))}
+
+
+def frame_ll_error(acc, x, ref_ll):
+    """Max relative error of the per-frame log-likelihoods (the handle's
+    current form, written to device memory) against FP64, over all frames."""
+    import torch
+    got = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+    acc.loglikes_device(got.data_ptr())
+    torch.cuda.synchronize()
+    e = 0.0
+    for a in range(0, x.shape[0], 1 << 22):
+        r = ref_ll[a:a + (1 << 22)]
+        e = max(e, float(((got[a:a + (1 << 22)].double() - r).abs() / r.abs()).max()))
+    return e
+
+
+def posterior_error(acc, x, w, mu, var, nsample=2048):
+    """Posteriors of a sample of frames (every N/nsample-th) through the
+    handle's current form, against FP64: each sampled frame is one
+    utterance of the batched path with the cut disabled, so its occupancies
+    are its posteriors.  The batched mixed form centres and scales by the
+    model's global mean and deviation instead of the frame set's."""
+    import torch
+    N, D = x.shape
+    M = mu.shape[0]
+    idx = torch.arange(0, N, max(1, N // nsample), device=x.device)[:nsample]
+    xs = x[idx].contiguous()
+    U = xs.shape[0]
+    off = np.arange(U + 1, dtype=np.uint64)
+    occ = torch.empty((U, M), dtype=torch.float64, device=x.device)
+    acc.set_opts(min_gamma=1e-30)
+    try:
+        acc.batch_adapt_device(xs.data_ptr(), off, D, {"occ": occ.data_ptr()}, tau=16)
+    finally:
+        acc.set_opts(min_gamma=0.0)
+    torch.cuda.synchronize()
+    x64 = xs.double()
+    mu64 = torch.as_tensor(mu, dtype=torch.float64, device=x.device)
+    var64 = torch.as_tensor(var, dtype=torch.float64, device=x.device)
+    L = (torch.log(torch.as_tensor(w, dtype=torch.float64, device=x.device))
+         - 0.5 * (D * np.log(2 * np.pi) + torch.log(var64).sum(1))
+         - 0.5 * (((x64[:, None, :] - mu64[None]) ** 2) / var64[None]).sum(-1))
+    g = torch.softmax(L, dim=1)
+    d = (occ - g).abs()
+    big = g > 1e-3
+    return {"frames": int(U), "max_abs_err": float(d.max()),
+            "max_rel_err_above_1e-3": float((d[big] / g[big]).max()) if bool(big.any()) else 0.0}
+
+
+def serve_leg(rate=5000, seconds=2.0, precisions=("fp32", "fp16x3")):
+    """The serving loop at one arrival rate (serveLoad, a child process that
+    links the same library): p50/p99 latency from the scheduled arrival,
+    throughput and the alone-vs-mix check, per precision."""
+    import subprocess
+    exe = os.path.join(ROOT, "gmm_amd", "_build", "serveLoad")
+    out = {}
+    for prec in precisions:
+        cmd = [exe, "--rates", str(rate), "--seconds", str(seconds), "--precision", prec]
+        print("bench: " + " ".join(cmd), file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            out[prec] = {"error": "serveLoad rc %d" % r.returncode}
+            continue
+        row = [json.loads(ln) for ln in r.stdout.splitlines() if ln.startswith("{")][0]
+        out[prec] = {k: row[k] for k in ("rate", "requests", "failed", "throughput_rps",
+                                         "p50_ms", "p90_ms", "p99_ms", "max_ms", "map_p99_ms",
+                                         "mllr_p99_ms", "verify_checked", "verify_identical")}
+    return out
 
 
 def traffic_of(kernel, workload):
@@ -351,6 +441,10 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 reference")
     ap.add_argument("--rates", default="2000,5000,10000", help="serve: arrival rates (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="serve: seconds per rate")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving leg of the E-step workloads")
+    ap.add_argument("--serve-seconds-headline", type=float, default=2.0,
+                    help="seconds of the serving leg (5000 req/s) in the E-step line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -409,9 +503,32 @@ def main():
                    "tot_loglike": r32[3]}
     res, el, kt = run(args.precision)
     ref = None
+    ll_err = {}
     if not args.no_accuracy:
-        ref = fp64_reference(x, w, mu, var)
+        ref_ll = torch.empty(N, dtype=torch.float64, device=x.device)
+        ref = fp64_reference(x, w, mu, var, frame_ll=ref_ll)
+        for prec in ([args.precision] + (["fp32"] if compare is not None else [])):
+            acc.set_opts(profile=1, precision=precs[prec])
+            acc.set_frames_device(x.data_ptr(), N, D)  # the posterior sample replaces them
+            ll_err[prec] = {"frame_loglike_max_rel_err": frame_ll_error(acc, x, ref_ll),
+                            "posteriors": posterior_error(acc, x, w, mu, var)}
+        del ref_ll
+        acc.set_frames_device(x.data_ptr(), N, D)
     value = N * M * args.steps / el
+
+    # the first E-step on a new frame set: the mixed form's once-per-frame-set
+    # work (per-dimension frame statistics for the centring and the split
+    # features) is included, the timed steps above reuse it
+    first = {}
+    for prec in ([args.precision] + (["fp32"] if compare is not None else [])):
+        acc.set_opts(profile=1, precision=precs[prec])
+        acc.set_frames_device(x.data_ptr(), N, D)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        acc.accumulate()
+        torch.cuda.synchronize()
+        first[prec] = (time.perf_counter() - t0) * 1e3
+    acc.set_opts(profile=1, precision=precs[args.precision])
 
     flop_fc = 8 * D + 4
     st_ms, st_n = kt["stats"]
@@ -457,10 +574,19 @@ def main():
         compare["tot_loglike_rel_diff"] = abs(res[3] - compare["tot_loglike"]) / abs(
             compare["tot_loglike"])
         line["fp32_compare"] = compare
+    line["first_call"] = {"ms": {k: v for k, v in first.items()},
+                          "note": "one E-step on a frame set just handed over (frames "
+                                  "resident), once-per-frame-set preparation included"}
+    if compare is not None:
+        line["first_call"]["speedup_vs_fp32"] = first["fp32"] / first[args.precision]
     if ref is not None:
         line["accuracy_vs_fp64"] = {args.precision: accuracy(res, ref)}
         if compare is not None:
             line["accuracy_vs_fp64"]["fp32"] = accuracy(r32, ref)
+        for prec, e in ll_err.items():
+            line["accuracy_vs_fp64"][prec].update(e)
+    if not args.no_serve:
+        line["serving"] = serve_leg(rate=5000, seconds=args.serve_seconds_headline)
     if not args.no_cpu_baseline:
         ns = min(N, 10_000_000)
         xs = x[:ns].cpu().numpy()

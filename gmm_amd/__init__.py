@@ -25,6 +25,7 @@ EXPORTS = [
     "gmmhip_posteriors", "gmmhip_loglikes",
     "gmmhip_loglikes_device", "gmmhip_stream", "gmmhip_sync",
     "gmmhip_kernel_times", "gmmhip_reset_kernel_times", "gmmhip_batch_adapt",
+    "gmmhip_batch_graph_captures",
     "gmmhip_server_create", "gmmhip_server_submit", "gmmhip_server_wait",
     "gmmhip_server_drain", "gmmhip_server_stats_get", "gmmhip_server_last_error",
     "gmmhip_server_destroy", "gmmhip_tune_load", "gmmhip_tune_lookup", "gmmhip_tiles_used",
@@ -138,6 +139,7 @@ def lib():
                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(BatchCfg),
                                          ctypes.POINTER(BatchOut)]
         L.gmmhip_reset_kernel_times.argtypes = [c_vp]
+        L.gmmhip_batch_graph_captures.argtypes = [c_vp, ctypes.POINTER(c_u64)]
         L.gmmhip_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerCfg), ctypes.c_int,
                                            ctypes.c_int, c_fp, c_fp, c_fp, ctypes.POINTER(c_vp)]
         L.gmmhip_server_submit.argtypes = [c_vp, ctypes.POINTER(Request)]
@@ -377,6 +379,12 @@ class Accumulator:
             self._h, ctypes.c_void_p(x_ptr), 1, off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
             U, D, ctypes.byref(cfg), ctypes.byref(o)))
         self.N = int(off[-1])
+
+    def graph_captures(self):
+        """How many times this handle captured the batch pipeline into a hipGraph."""
+        c = ctypes.c_uint64(0)
+        self._check(lib().gmmhip_batch_graph_captures(self._h, ctypes.byref(c)))
+        return c.value
 
     def tiles_used(self):
         """(ll_frames, chunk_frames) of the last accumulate / loglikes / batch call."""

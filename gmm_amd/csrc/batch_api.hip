@@ -67,6 +67,7 @@ struct gmmhip_batch_state {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<uint64_t> key;
+    uint64_t captures = 0;
 };
 
 namespace gmmhip_detail {
@@ -556,7 +557,12 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
                                      (uint64_t)(uintptr_t)b->d_P16,
                                      (uint64_t)(uintptr_t)h->d_part,
                                      (uint64_t)(uintptr_t)b->d_n,
-                                     (uint64_t)(uintptr_t)b->d_G};
+                                     (uint64_t)(uintptr_t)b->d_G,
+                                     // every buffer the pipeline captures
+                                     // (handle-wide scratch included) stays
+                                     // where it was only while no device
+                                     // buffer has been reallocated since
+                                     g_alloc_generation.load(std::memory_order_relaxed)};
         key.insert(key.end(), offsets, offsets + U + 1);
         if (!b->exec || key != b->key) {
             if (b->exec) HIPCHK(h, hipGraphExecDestroy(b->exec));
@@ -576,6 +582,7 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
             b->graph = graph;
             HIPCHK(h, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
             b->key = key;
+            b->captures++;
         }
         HIPCHK(h, hipGraphLaunch(b->exec, h->stream));
     } else {
@@ -601,5 +608,11 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
     if (out->mllr_W && !cfg->outputs_on_device)
         for (int u = 0; u < U; u++)
             if (out->mllr_status[u]) out->mllr_status[u] = GMMHIP_ERR_MLLR;
+    return GMMHIP_OK;
+}
+
+extern "C" int gmmhip_batch_graph_captures(gmmhip_t h, uint64_t *captures) {
+    if (!h || !captures) return GMMHIP_ERR_ARG;
+    *captures = h->batch ? h->batch->captures : 0;
     return GMMHIP_OK;
 }

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -138,10 +139,19 @@ inline int fail(gmmhip_ctx *h, int code, const char *fmt, ...) {
                         "%s failed: %s", #expr, hipGetErrorString(e_));             \
     } while (0)
 
+// Bumped whenever a device buffer of any handle is (re)allocated or freed
+// for regrowth.  A captured hipGraph holds raw device addresses, so its
+// cache key includes this generation: a replay after any reallocation would
+// otherwise touch freed memory.
+inline std::atomic<uint64_t> g_alloc_generation{0};
+
+inline void bump_alloc_generation() { g_alloc_generation.fetch_add(1, std::memory_order_relaxed); }
+
 // Grow-only device buffer.
 template <typename T>
 inline hipError_t ensure(T **p, size_t *cap, size_t n) {
     if (*cap >= n && *p) return hipSuccess;
+    bump_alloc_generation();
     if (*p) {
         hipError_t e = hipFree(*p);
         if (e != hipSuccess) return e;

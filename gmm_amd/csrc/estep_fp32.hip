@@ -1,8 +1,8 @@
 // FP32 E-step kernels for gfx950 (plain form).
 //
 // The per-frame-by-component log-likelihood is the dense contraction
-//   l2[t,m] = sum_k Z[t,k] * B[k,m],  Z[t,:] = [1, x', x'^2, 0...],
-//   B[:,m]  = log2(e) * [c_m, mu'_m * ivar_m, -ivar_m / 2, 0...]
+//   l2[t,m] = sum_k Z[t,k] * B[k,m],  Z[t,:] = [1, x', x'^2, 1, 0...],
+//   B[:,m]  = log2(e) * [c_m hi, mu'_m * ivar_m, -ivar_m / 2, c_m lo, 0...]
 // with x' = x - off (mean-centred frames, mu' = mu - off), evaluated on
 // v_mfma_f32_16x16x4_f32.  It restates gammasKernel
 // (trainer/CU_GMMStatsEstimator_CUDAkernels.cuh:681-750), which forms
@@ -45,7 +45,8 @@ __device__ __forceinline__ float fast_log2(float x) {
     return __builtin_amdgcn_logf(x);
 }
 
-// Row k of the frame feature vector [1, x', x'^2, 0...] for one frame.
+// Row k of the frame feature vector [1, x', x'^2, 1, 0...] for one frame:
+// the second 1 (column 2D+1) carries the low part of the model constant.
 __device__ __forceinline__ float zval(const float *__restrict__ xr,
                                       const float *__restrict__ off, int D,
                                       int k) {
@@ -55,7 +56,7 @@ __device__ __forceinline__ float zval(const float *__restrict__ xr,
         float v = xr[k - 1 - D] - off[k - 1 - D];
         return v * v;
     }
-    return 0.0f;
+    return k == 2 * D + 1 ? 1.0f : 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -341,7 +342,9 @@ __device__ __forceinline__ void stats_item(const EstepArgs &a, StatsShared<KP> &
             }
         }
         if (tid < FB) {
-            Zr[tid * S] = (base + tid < f1) ? 1.0f : 0.0f;
+            const float one = (base + tid < f1) ? 1.0f : 0.0f;
+            Zr[tid * S] = one;
+            Zr[tid * S + 2 * D + 1] = one;  // low part of the model constant
             Ls[tid] = pl;
         }
         __syncthreads();

@@ -193,6 +193,7 @@ int prepare_mixed(gmmhip_ctx *h, bool *use_mixed) {
                                  hipMemcpyHostToDevice, h->stream));
         const size_t zn = (size_t)Npad * KPm;
         if (h->z_cap < zn) {
+            bump_alloc_generation();
             hipFree(h->d_za_hi);
             hipFree(h->d_za_lo);
             h->d_za_hi = h->d_za_lo = nullptr;
@@ -203,6 +204,7 @@ int prepare_mixed(gmmhip_ctx *h, bool *use_mixed) {
         }
         const bool need_zb = mixed_needs_zb(KPm);
         if (need_zb && h->zb_cap < zn) {
+            bump_alloc_generation();
             hipFree(h->d_zb_hi);
             hipFree(h->d_zb_lo);
             h->d_zb_hi = h->d_zb_lo = nullptr;
@@ -448,19 +450,30 @@ int gmmhip_set_model(gmmhip_t h, int M, int D, const float *w, const float *mu,
     // Padding components (M..16*Mt-1): constant -1e30 (log2 units), so their
     // log-likelihood vanishes in every exp2 without a lane mask.
     for (int m = M; m < Mt * 16; m++) P[pidx(m, 0)] = -1e30f;
-    std::vector<double> coef(KP);
+    // Coefficients (log2 units) b_d = mu'_d ivar_d, a_d = -ivar_d / 2 are
+    // rounded to FP32; the constant is then chosen in FP64 so that the
+    // rounded coefficients give the exact log-likelihood at the component's
+    // own mean, c = l(mu') - sum_d (b_d mu'_d + a_d mu'_d^2), and it is
+    // carried as hi + lo: hi in column 0, lo in column 2D+1 (a padding
+    // column, KP >= 2D+2 always, whose frame feature is 1).  Without this the
+    // rounding of the coefficients and of the constant (|c| up to hundreds)
+    // shifts every posterior of a component by the same ~1e-5 relative.
     for (int m = 0; m < M; m++) {
-        std::fill(coef.begin(), coef.end(), 0.0);
-        double cq = 0.0;
+        double cq = 0.0, cadj = 0.0;
         for (int d = 0; d < D; d++) {
             const double iv = 1.0 / (double)var[(size_t)m * D + d];
             const double mc = (double)mu[(size_t)m * D + d] - (double)off[d];
-            coef[1 + d] = mc * iv;
-            coef[1 + D + d] = -0.5 * iv;
+            const double b = mc * iv * log2e, a = -0.5 * iv * log2e;
+            const float bf = (float)b, af = (float)a;
+            P[pidx(m, 1 + d)] = bf;
+            P[pidx(m, 1 + D + d)] = af;
             cq += mc * mc * iv;
+            cadj += (b - (double)bf) * mc + (a - (double)af) * mc * mc;
         }
-        coef[0] = gconst[m] - 0.5 * cq;
-        for (int k = 0; k < KP; k++) P[pidx(m, k)] = (float)(coef[k] * log2e);
+        const double c = (gconst[m] - 0.5 * cq) * log2e + cadj;
+        const float chi = (float)c;
+        P[pidx(m, 0)] = chi;
+        P[pidx(m, 2 * D + 1)] = (float)(c - (double)chi);
     }
     DevGuard g(h->dev);
     HIPCHK(h, ensure(&h->d_P, &h->P_cap, P.size()));

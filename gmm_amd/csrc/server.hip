@@ -151,8 +151,13 @@ struct gmmhip_server {
         hipStream_t stream = nullptr;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
         char *arena = nullptr;
+        // pinned staging of exactly slot_bytes, allocated at creation: a
+        // batch's IN region [0, in_end) and OUT region [ll, out_end) are
+        // disjoint parts of one layout of at most slot_bytes, so IN is staged
+        // at h_stage and OUT lands at h_stage + ll (h_in / h_out point there)
+        char *h_stage = nullptr;
+        size_t h_stage_cap = 0;
         char *h_in = nullptr, *h_out = nullptr;
-        size_t h_in_cap = 0, h_out_cap = 0;
         SlotState state = SLOT_FREE;
         std::vector<Pending> batch;
         std::vector<uint64_t> beg;  // first frame row of each request
@@ -206,19 +211,6 @@ void finish_locked(gmmhip_server *s, gmmhip_request *r, int status, Clock::time_
         }                                                                                 \
     } while (0)
 
-int grow_pinned(gmmhip_server *s, char **p, size_t *cap, size_t need) {
-    if (*cap >= need) return GMMHIP_OK;
-    // geometric growth: hipHostFree waits for the device, so regrowth must
-    // stay rare on the dispatch path
-    const size_t n = std::max(need + need / 2, *cap * 2);
-    if (*p) SRV_HIPCHK(hipHostFree(*p));
-    *p = nullptr;
-    *cap = 0;
-    SRV_HIPCHK(hipHostMalloc((void **)p, n, hipHostMallocDefault));
-    *cap = n;
-    return GMMHIP_OK;
-}
-
 // Stage and enqueue one batch on its slot (dispatcher thread).
 int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
     const Dims &d = s->d;
@@ -254,9 +246,13 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
     const Layout L = layout_of(sh, d);
     sl.shape = sh;
     sl.lay = L;
-    int rc;
-    if ((rc = grow_pinned(s, &sl.h_in, &sl.h_in_cap, L.in_end))) return rc;
-    if ((rc = grow_pinned(s, &sl.h_out, &sl.h_out_cap, L.out_copy))) return rc;
+    if (L.out_end > sl.h_stage_cap) {
+        // admission keeps every layout within slot_bytes
+        set_err(s, "batch layout exceeds the slot's staging");
+        return GMMHIP_ERR_STATE;
+    }
+    sl.h_in = sl.h_stage;
+    sl.h_out = sl.h_stage + L.ll;
 
     // IN staging; the padding rows of a mixed batch are zero
     std::memcpy(sl.h_in + L.beg, sl.beg.data(), (size_t)sh.U * 8);
@@ -587,8 +583,7 @@ void dispatcher_main(gmmhip_server *s) {
 void free_server(gmmhip_server *s) {
     for (auto &sl : s->slots) {
         if (sl.arena) hipFree(sl.arena);
-        if (sl.h_in) hipHostFree(sl.h_in);
-        if (sl.h_out) hipHostFree(sl.h_out);
+        if (sl.h_stage) hipHostFree(sl.h_stage);
         if (sl.ev0) hipEventDestroy(sl.ev0);
         if (sl.ev1) hipEventDestroy(sl.ev1);
         if (sl.stream) hipStreamDestroy(sl.stream);
@@ -734,15 +729,16 @@ extern "C" int gmmhip_server_create(int device, const gmmhip_server_cfg *cfg, in
             free_server(s);
             return GMMHIP_ERR_OOM;
         }
-        // a batch's IN and OUT parts each fit the slot budget, so staging of
-        // that size never regrows on the dispatch path (a regrowth frees
-        // pinned memory, which waits for the device, and pins anew)
-        const size_t stage = c.slot_bytes;
-        if (grow_pinned(s, &sl.h_in, &sl.h_in_cap, stage) ||
-            grow_pinned(s, &sl.h_out, &sl.h_out_cap, stage)) {
+        // one pinned staging buffer of the slot budget: every admitted
+        // batch's IN and OUT regions fit in it, so it never regrows on the
+        // dispatch path (a regrowth frees pinned memory, which waits for the
+        // device, and pins anew)
+        if (hipHostMalloc((void **)&sl.h_stage, c.slot_bytes, hipHostMallocDefault) != hipSuccess) {
+            sl.h_stage = nullptr;
             free_server(s);
             return GMMHIP_ERR_OOM;
         }
+        sl.h_stage_cap = c.slot_bytes;
     }
     s->t_reset = Clock::now();
     for (int i = 0; i < c.slots; i++) s->slots[i].completer = std::thread(completer_main, s, i);

@@ -257,6 +257,13 @@ typedef struct gmmhip_batch_out {
 int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on_device,
                        const uint64_t *offsets, int U, int D, const gmmhip_batch_cfg *cfg,
                        gmmhip_batch_out *out);
+/* Number of times the handle has captured the batch pipeline into a
+ * hipGraph (cfg.graph = 1).  A graph is replayed only while the batch shape,
+ * the model and every device buffer it captured are unchanged; any device
+ * reallocation in between (a larger loglikes / auxiliary / full-covariance
+ * call on the same handle, for example) makes the next call capture anew.
+ * Diagnostic; no counterpart in the reference. */
+int gmmhip_batch_graph_captures(gmmhip_t h, uint64_t *captures);
 
 /* ---- Tuning table -----------------------------------------------------
  * Tile choices per (path, precision, D, M) picked by a timed sweep
@@ -315,8 +322,9 @@ typedef struct gmmhip_server_cfg {
     int slots;               /* batches in flight, 0 = 2 */
     uint64_t slot_bytes;     /* device budget of one batch: frames, per-frame
                                 values, partial sums and outputs; 0 = 1 GiB.
-                                Each slot also pins 2 x slot_bytes of host
-                                memory for its staging */
+                                Each slot also pins slot_bytes of host
+                                memory for its staging (a batch's input and
+                                output regions are disjoint parts of it) */
     int max_batch;           /* requests per batch, 0 = 1024 */
     int chunk_frames;        /* frames per statistics work item, 0 = 512 */
     int tau;                 /* MAP relevance factor */

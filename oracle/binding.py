@@ -69,8 +69,11 @@ def prepare_model(w, mu, var, min_var=1e-6):
     M, D = mu.shape
     gconst = np.zeros(M)
     ivar = np.zeros((M, D))
+    # the reference compares float variances with a float _min_var
+    # (OL_GMMStatsEstimator.h:62, OL_GMMStatsEstimator.cpp:279)
     rc = lib().orc_prepare_model(M, D, _p(w), _p(mu), _p(var),
-                                 ctypes.c_double(min_var), _p(gconst), _p(ivar))
+                                 ctypes.c_double(float(np.float32(min_var))), _p(gconst),
+                                 _p(ivar))
     if rc != 0:
         raise ValueError("variance below min_var in component %d" % (-rc - 1))
     return gconst, ivar
@@ -101,6 +104,40 @@ def accumulate(x, w, mu, var, lpdif=False, min_gamma=MIN_GAMMA):
                          int(lpdif), ctypes.c_double(min_gamma), _p(n), _p(F),
                          _p(S), ctypes.byref(tot))
     return n, F, S, tot.value
+
+
+def near_cut(x, w, mu, var, band, min_gamma=MIN_GAMMA):
+    """Per component: the number of frames whose FP64 posterior lies within a
+    relative band of the cut, [min_gamma(1-band), min_gamma(1+band)], and the
+    sums of |x| and x^2 over them (what such frames move the statistics by
+    when they land on the other side of the cut, divided by the posterior)."""
+    x, mu = _f32(x), _f32(mu)
+    N, D = x.shape
+    M = mu.shape[0]
+    gconst, ivar = prepare_model(w, mu, var)
+    cnt = np.zeros(M)
+    gx = np.zeros((M, D))
+    gx2 = np.zeros((M, D))
+    lib().orc_near_cut(N, D, M, _p(x), _p(mu), _p(ivar), _p(gconst),
+                       ctypes.c_double(min_gamma * (1.0 - band)),
+                       ctypes.c_double(min_gamma * (1.0 + band)), _p(cnt), _p(gx), _p(gx2))
+    return cnt, gx, gx2
+
+
+def tail_mass(x, w, mu, var, min_gamma=MIN_GAMMA):
+    """Per component: the sums of g(1-g), g(1-g)|x| and g(1-g) x^2 over the
+    posteriors g that pass the cut (what an absolute error E in the
+    component log-likelihoods can move the statistics by, divided by 2E)."""
+    x, mu = _f32(x), _f32(mu)
+    N, D = x.shape
+    M = mu.shape[0]
+    gconst, ivar = prepare_model(w, mu, var)
+    tn = np.zeros(M)
+    tx = np.zeros((M, D))
+    tx2 = np.zeros((M, D))
+    lib().orc_tail_mass(N, D, M, _p(x), _p(mu), _p(ivar), _p(gconst), ctypes.c_double(min_gamma),
+                        _p(tn), _p(tx), _p(tx2))
+    return tn, tx, tx2
 
 
 def update_em(w, mu, var, n, F, S, tot_samples, min_var=1e-6,

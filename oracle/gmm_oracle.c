@@ -25,7 +25,7 @@
  *                          planted both sides of 1e-4); sum of n == N when
  *                          no posterior is cut; permutation invariance.
  *   orc_update_em       -- example/myModel.gmm reproduced by the from-scratch
- *                          split driver (tests/golden/train_from_scratch.py).
+ *                          split driver (tests/_scratch_driver.py).
  *   orc_update_map      -- tau=0 equals ML means; n=0 keeps the UBM; alpha
  *                          closed form on a one-component case.
  *   orc_mllr_init/kg    -- G symmetric positive definite; brute-force outer
@@ -205,6 +205,79 @@ void orc_accumulate(int N, int D, int M, const float *x, const float *mu,
         }
     }
     *tot_ll = tot;
+    free(g);
+}
+
+/* Posteriors near the _minGamma cut (OL_GMMStatsEstimator.cpp:1350): for
+ * every frame and component whose FP64 posterior lies in [lo, hi] (a band
+ * around the cut), cnt[m] += 1 and the bounds of what one such frame can
+ * move the statistics when an implementation lands on the other side of the
+ * cut: gx[m*D+d] += |x_d|, gx2[m*D+d] += x_d^2.  Outputs are overwritten.
+ * Used by the parity tests to give each component an allowance for cut
+ * flips of exactly the frames that can flip; pinned by a numpy brute force
+ * over the posteriors of orc_frame_loglik. */
+void orc_near_cut(int N, int D, int M, const float *x, const float *mu,
+                  const double *ivar, const double *gconst, double lo,
+                  double hi, double *cnt, double *gx, double *gx2)
+{
+    double *g = (double *)malloc(sizeof(double) * (M > 0 ? M : 1));
+    double LL;
+    memset(cnt, 0, sizeof(double) * M);
+    memset(gx, 0, sizeof(double) * (size_t)M * D);
+    memset(gx2, 0, sizeof(double) * (size_t)M * D);
+    for (int n = 0; n < N; n++) {
+        const float *xn = x + (size_t)n * D;
+        orc_frame_loglik(1, D, M, xn, mu, ivar, gconst, 0, &LL, g);
+        for (int m = 0; m < M; m++) {
+            if (!(g[m] >= lo && g[m] <= hi))
+                continue;
+            cnt[m] += 1.0;
+            double *a = gx + (size_t)m * D, *b = gx2 + (size_t)m * D;
+            for (int d = 0; d < D; d++) {
+                double xv = xn[d];
+                a[d] += fabs(xv);
+                b[d] += xv * xv;
+            }
+        }
+    }
+    free(g);
+}
+
+/* Posterior-weighted sums of the posteriors that pass the cut, each taken
+ * with the weight gamma (1 - gamma): for every frame and component with
+ * gamma > min_gamma (FP64 posteriors as in orc_frame_loglik),
+ * tn[m] += gamma (1 - gamma), tx[m*D+d] += gamma (1 - gamma) |x_d|,
+ * tx2[m*D+d] += gamma (1 - gamma) x_d^2.  Outputs are overwritten.
+ * If every component log-likelihood of a frame carries an absolute error of
+ * at most E, a posterior gamma_m = exp(l_m - LL) moves by at most
+ * 2 E gamma_m (1 - gamma_m); the parity tests turn these sums into the
+ * statistics' allowance for that.  Pinned by a numpy brute force over the
+ * posteriors. */
+void orc_tail_mass(int N, int D, int M, const float *x, const float *mu,
+                   const double *ivar, const double *gconst, double min_gamma,
+                   double *tn, double *tx, double *tx2)
+{
+    double *g = (double *)malloc(sizeof(double) * (M > 0 ? M : 1));
+    double LL;
+    memset(tn, 0, sizeof(double) * M);
+    memset(tx, 0, sizeof(double) * (size_t)M * D);
+    memset(tx2, 0, sizeof(double) * (size_t)M * D);
+    for (int n = 0; n < N; n++) {
+        const float *xn = x + (size_t)n * D;
+        orc_frame_loglik(1, D, M, xn, mu, ivar, gconst, 0, &LL, g);
+        for (int m = 0; m < M; m++) {
+            if (!(g[m] > min_gamma))
+                continue;
+            const double v = g[m] * (1.0 - g[m]);
+            tn[m] += v;
+            double *a = tx + (size_t)m * D, *b = tx2 + (size_t)m * D;
+            for (int d = 0; d < D; d++) {
+                double xv = xn[d];
+                a[d] += v * fabs(xv);
+                b[d] += v * xv * xv;
+            }
+        }
+    }
     free(g);
 }
 

@@ -1,7 +1,9 @@
 /*
  * sse_port.c -- restatement of the reference's multi-threaded SSE CPU
  * accumulator, used ONLY as the timed CPU baseline beside the GPU numbers
- * (bench.py's cpu_baseline leg) and checked against gmm_oracle.c in tests/.
+ * (bench.py's cpu_baseline leg) and checked against gmm_oracle.c (with the
+ * same LPDIF rule) at 1, 4 and 16 threads in
+ * tests/test_oracle.py::test_sse_port_matches_oracle_at_every_thread_count.
  * It is test infrastructure, never part of the product.
  *
  * Follows (paths relative to GMM/lib_bmssr/):

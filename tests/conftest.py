@@ -22,3 +22,18 @@ def golden():
     w, mu, var, _ = fm.read_gmm_txt(os.path.join(g, "myModel.gmm"))
     ll = fm.read_savemat_txt(os.path.join(g, "data.logL"))[0]
     return dict(x=x, w=w, mu=mu, var=var, ll=ll)
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _torch_device_first(request):
+    """Tests that use torch device tensors next to the library: torch's own
+    HIP runtime must initialise the device before the library's runtime
+    does, as bench.py does (torch first, then the library's handles)."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.empty(1, device="cuda")
+        except ImportError:
+            pass
+    yield

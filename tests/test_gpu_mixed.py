@@ -5,11 +5,14 @@ Stated tolerances (per-frame log-likelihood relative / total
 log-likelihood relative / statistics rtol on n, sqrt(nS)-scaled F, S):
   fp16x3   1e-5 / 1e-6 / 2e-4
   bf16x3   2e-4 / 2e-5 / 2e-3
-The statistics tolerance also absorbs posteriors that cross the 1e-4 cut.
+Posteriors that cross the 1e-4 cut get a per-component allowance for the
+frames whose FP64 posterior lies within 1 % (fp16x3) or 10 % (bf16x3) of the
+cut (tests/_check.py).
 """
 import numpy as np
 import pytest
 
+import _check
 import _oracle as orc
 import gmm_amd
 from gmm_amd import synth
@@ -18,6 +21,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = {gmm_amd.PREC_FP16X3: (1e-5, 1e-6, 2e-4), gmm_amd.PREC_BF16X3: (2e-4, 2e-5, 2e-3)}
 PRECS = [gmm_amd.PREC_FP16X3, gmm_amd.PREC_BF16X3]
+NAME = {gmm_amd.PREC_FP16X3: "fp16x3", gmm_amd.PREC_BF16X3: "bf16x3"}
+BAND = {p: _check.BAND[NAME[p]] for p in PRECS}
 
 
 @pytest.fixture(scope="module")
@@ -49,13 +54,11 @@ def check(acc, prec, x, w, mu, var, ref_ll=None, ref_st=None):
     rll = orc.frame_loglik(x, w, mu, var) if ref_ll is None else ref_ll
     rn, rF, rS, rtot = orc.accumulate(x, w, mu, var) if ref_st is None else ref_st
     assert np.all(np.abs(ll - rll) <= lt * np.abs(rll) + 1e-6), np.max(np.abs(ll - rll) / np.abs(rll))
-    assert got[3] == pytest.approx(rtot, rel=tt)
-    n, F, S = got[:3]
-    xm = float(np.abs(x).max())
-    assert np.all(np.abs(n - rn) <= st * rn + 3e-4), np.max(np.abs(n - rn) / np.maximum(rn, 1e-30))
-    scaleF = np.sqrt(np.maximum(rn[:, None] * rS, 0))
-    assert np.all(np.abs(F - rF) <= st * scaleF + 3e-4 * xm)
-    assert np.all(np.abs(S - rS) <= st * rS + 3e-4 * xm * xm)
+    used = _check.check_stats(got, (rn, rF, rS, rtot), x, w, mu, var, rtol=st, tot_rtol=tt,
+                              band=BAND[prec])
+    print("%s N=%d D=%d M=%d: max rel LL %.3g, allowance used %.3f" % (
+        NAME[prec], x.shape[0], x.shape[1], mu.shape[0],
+        np.max(np.abs(ll - rll) / np.abs(rll)), used))
     return ll, got
 
 
@@ -91,8 +94,10 @@ def test_mixed_nan_frames(acc, prec):
     n = acc.accumulate()[0]
     acc.set_opts(precision=gmm_amd.PREC_FP32)
     assert np.all(ll[[5, 300]] == np.float32(-1e20))
-    rn = orc.accumulate(np.delete(x, [5, 300], 0), w, mu, var)[0]
-    assert np.all(np.abs(n - rn) <= TOL[prec][2] * rn + 3e-4)
+    xs = np.delete(x, [5, 300], 0)
+    rn, rF, _, _ = orc.accumulate(xs, w, mu, var)
+    _check.check_stats((n, rF, None, None), (rn, rF, None, None), xs, w, mu, var,
+                       rtol=TOL[prec][2], band=BAND[prec])
 
 
 def test_fp16_out_of_range_runs_fp32(acc):
@@ -129,13 +134,23 @@ def test_mixed_determinism_and_chunking(acc, prec):
 
 def test_mixed_error_report_ubm_like(acc):
     """Errors of both mixed forms and the FP32 form against FP64 on a
-    UBM-like case (D=60, M=512, N=20,000; the numbers are printed)."""
+    UBM-like case (D=60, M=512, N=20,000, the mixed workload's mixture with
+    frame offsets): per-frame LL, total LL, occupancies and M-step means,
+    each within the form's stated tolerance."""
     cfg = synth.CONFIGS["mixed_estep"]
     w0, mu0, var0 = synth.true_mixture(**dict(cfg, M=512))
     x = synth.sample_frames_np(w0, mu0, var0, 20000, 99)
     w, mu, var = synth.init_model_from_frames(x, 512, 99)
     rll = orc.frame_loglik(x, w, mu, var)
     rst = orc.accumulate(x, w, mu, var)
+    rn = rst[0]
+    # (per-frame LL rel, total LL rel, occupancy rel and mean abs for
+    # components with n >= 100, where a posterior crossing the cut moves
+    # the mean by <= 1e-4 |x - mu| / 100); the per-component check below is
+    # the exact one
+    lim = {"fp32": (1e-5, 1e-6, 2e-5, 5e-5), "fp16x3": (1e-5, 1e-6, 2e-4, 2e-4),
+           "bf16x3": (2e-4, 2e-5, 2e-3, 2e-3)}
+    big = rn >= 100
     for prec, name in [(gmm_amd.PREC_FP32, "fp32"), (gmm_amd.PREC_FP16X3, "fp16x3"),
                        (gmm_amd.PREC_BF16X3, "bf16x3")]:
         acc.set_opts(precision=prec)
@@ -143,9 +158,15 @@ def test_mixed_error_report_ubm_like(acc):
         acc.set_frames(x)
         ll = acc.loglikes().astype(np.float64)
         n, F, S, tot, _ = acc.accumulate()
+        e_ll = np.max(np.abs(ll - rll) / np.abs(rll))
+        e_tot = abs(tot - rst[3]) / abs(rst[3])
+        e_n = np.max(np.abs(n - rn)[big] / rn[big])
+        e_mu = np.max(np.abs(F / np.maximum(n, 1e-30)[:, None] -
+                             rst[1] / np.maximum(rn, 1e-30)[:, None])[big])
         print("%s: max rel LL %.3g, rel totLL %.3g, max rel n %.3g, max |dmu| %.3g" % (
-            name, np.max(np.abs(ll - rll) / np.abs(rll)), abs(tot - rst[3]) / abs(rst[3]),
-            np.max(np.abs(n - rst[0]) / np.maximum(rst[0], 1)),
-            np.max(np.abs(F / np.maximum(n, 1e-30)[:, None] -
-                          rst[1] / np.maximum(rst[0], 1e-30)[:, None])[rst[0] > 10])))
+            name, e_ll, e_tot, e_n, e_mu))
+        L = lim[name]
+        assert e_ll <= L[0] and e_tot <= L[1] and e_n <= L[2] and e_mu <= L[3], name
+        _check.check_stats((n, F, S, tot), rst, x, w, mu, var, rtol=max(L[2], 1e-5),
+                           tot_rtol=L[1], band=_check.BAND[name], what=name + " ")
     acc.set_opts(precision=gmm_amd.PREC_FP32)

@@ -322,3 +322,94 @@ def test_mllr_check_codes():
     assert orc.mllr_check(10, 4, n) == 2
     n = np.full(10, 0.1)
     assert orc.mllr_check(10, 4, n) == 3
+
+
+# ---------------------------------------------------------------- near-cut counts
+def test_near_cut_matches_brute_force():
+    """orc_near_cut: per-component count of frames whose posterior lies in a
+    band around the 1e-4 cut, and the sums of |x| and x^2 over them, against
+    numpy posteriors (logsumexp) on a case with many posteriors near 1e-4."""
+    rng = np.random.default_rng(71)
+    w, mu, var = rand_model(rng, 9, 4, mean_sd=1.5)
+    x = rng.normal(0, 2.5, (4000, 4)).astype(np.float32)
+    l = np_loglik_matrix(x, w, mu, var)
+    g = np.exp(l - logsumexp(l, axis=1)[:, None])
+    for band in (0.05, 0.3):
+        lo, hi = orc.MIN_GAMMA * (1 - band), orc.MIN_GAMMA * (1 + band)
+        sel = (g >= lo) & (g <= hi)
+        cnt, gx, gx2 = orc.near_cut(x, w, mu, var, band)
+        assert sel.sum() > 20  # the case does exercise the band
+        assert np.array_equal(cnt, sel.sum(0).astype(np.float64))
+        xa = np.abs(x.astype(np.float64))
+        assert np.allclose(gx, sel.T.astype(np.float64) @ xa, rtol=1e-12)
+        assert np.allclose(gx2, sel.T.astype(np.float64) @ (xa * xa), rtol=1e-12)
+    # a planted mistake (band taken on the log-likelihood instead of the
+    # posterior) selects different frames
+    sel_wrong = (np.abs(l - np.log(orc.MIN_GAMMA)) <= 0.05)
+    assert not np.array_equal(sel_wrong.sum(0), orc.near_cut(x, w, mu, var, 0.05)[0])
+
+
+def test_tail_mass_matches_brute_force():
+    """orc_tail_mass: per-component sums of g(1-g), g(1-g)|x|, g(1-g)x^2
+    over the posteriors above the cut, against numpy posteriors; and the
+    bound it stands for: with every component log-likelihood of a frame
+    moved by at most E, each posterior moves by at most 2 E g (1 - g)."""
+    rng = np.random.default_rng(72)
+    w, mu, var = rand_model(rng, 11, 5, mean_sd=1.0)
+    x = rng.normal(0, 2.0, (3000, 5)).astype(np.float32)
+    l = np_loglik_matrix(x, w, mu, var)
+    g = np.exp(l - logsumexp(l, axis=1)[:, None])
+    xa = np.abs(x.astype(np.float64))
+    gs = np.where(g > orc.MIN_GAMMA, g * (1 - g), 0.0)
+    tn, tx, tx2 = orc.tail_mass(x, w, mu, var)
+    assert np.allclose(tn, gs.sum(0), rtol=1e-10)
+    assert np.allclose(tx, gs.T @ xa, rtol=1e-10)
+    assert np.allclose(tx2, gs.T @ (xa * xa), rtol=1e-10)
+    E = 1e-3
+    for _ in range(3):
+        lp = l + rng.uniform(-E, E, l.shape)
+        gp = np.exp(lp - logsumexp(lp, axis=1)[:, None])
+        assert np.all(np.abs(gp - g) <= 2 * E * g * (1 - g) * (1 + 1e-3) + 1e-15)
+    # a planted mistake (weight g instead of g(1-g)) gives other sums
+    assert not np.allclose(tn, np.where(g > orc.MIN_GAMMA, g, 0.0).sum(0))
+
+
+# ---------------------------------------------------------------- CPU baseline port
+def test_sse_port_matches_oracle_at_every_thread_count():
+    """oracle/sse_port.c (the timed CPU baseline: the reference's SSE path
+    with float per-frame arithmetic, addLog with LPDIF = 15, g > 1e-4 into
+    double, contiguous thread split) against the FP64 oracle with the same
+    LPDIF rule, at 1, 4 and 16 threads; the thread split changes only the
+    order of the final double sums."""
+    import _check
+    rng = np.random.default_rng(73)
+    M, D, N = 40, 13, 6000
+    w, mu, var = rand_model(rng, M, D)
+    k = rng.choice(M, size=N, p=w.astype(np.float64) / w.sum())
+    x = (mu[k] + np.sqrt(var[k]) * rng.standard_normal((N, D))).astype(np.float32)
+    ref = orc.accumulate(x, w, mu, var, lpdif=True)
+    near = orc.near_cut(x, w, mu, var, 1e-2)
+    outs = []
+    for t in (1, 4, 16):
+        n, F, S, tot, used = orc.sse_accumulate(x, w, mu, var, t)
+        assert used == t
+        _check.check_stats((n, F, S, tot), ref, x, w, mu, var, rtol=1e-5, tot_rtol=1e-6,
+                           near=near, band=1e-2, what="threads=%d " % t)
+        outs.append((n, F, S, tot))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.allclose(a, b, rtol=1e-12, atol=1e-9)
+    # the LPDIF early-out is what the port implements: a frame whose second
+    # component is 14 log-units below the first gets its addLog term, one 16
+    # below does not (the oracle agrees on both)
+    w2 = np.array([0.5, 0.5], np.float32)
+    mu2 = np.zeros((2, 1), np.float32)
+    for gap, keep in ((14.0, True), (16.0, False)):
+        var2 = np.array([[1.0], [1.0]], np.float32)
+        mu2[1, 0] = np.sqrt(2 * gap)
+        xx = np.zeros((1, 1), np.float32)
+        t_port = orc.sse_accumulate(xx, w2, mu2, var2, 1)[3]
+        t_orc = orc.accumulate(xx, w2, mu2, var2, lpdif=True)[3]
+        t_exact = orc.accumulate(xx, w2, mu2, var2, lpdif=False)[3]
+        assert t_port == pytest.approx(t_orc, abs=2e-6)
+        assert (abs(t_orc - t_exact) < 1e-12) == keep
