@@ -25,7 +25,7 @@ EXPORTS = [
     "gmmhip_posteriors", "gmmhip_loglikes",
     "gmmhip_loglikes_device", "gmmhip_stream", "gmmhip_sync",
     "gmmhip_kernel_times", "gmmhip_reset_kernel_times", "gmmhip_batch_adapt",
-    "gmmhip_batch_graph_captures",
+    "gmmhip_batch_graph_captures", "gmmhip_accumulate_noalloc",
     "gmmhip_server_create", "gmmhip_server_submit", "gmmhip_server_wait",
     "gmmhip_server_drain", "gmmhip_server_stats_get", "gmmhip_server_last_error",
     "gmmhip_server_destroy", "gmmhip_tune_load", "gmmhip_tune_lookup", "gmmhip_tiles_used",
@@ -140,6 +140,8 @@ def lib():
                                          ctypes.POINTER(BatchOut)]
         L.gmmhip_reset_kernel_times.argtypes = [c_vp]
         L.gmmhip_batch_graph_captures.argtypes = [c_vp, ctypes.POINTER(c_u64)]
+        L.gmmhip_accumulate_noalloc.argtypes = [c_vp, c_vp, c_u64, ctypes.c_int, ctypes.c_int,
+                                                c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_u64)]
         L.gmmhip_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerCfg), ctypes.c_int,
                                            ctypes.c_int, c_fp, c_fp, c_fp, ctypes.POINTER(c_vp)]
         L.gmmhip_server_submit.argtypes = [c_vp, ctypes.POINTER(Request)]

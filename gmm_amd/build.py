@@ -27,7 +27,7 @@ COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
           "-Wno-unused-result", "-Wno-unused-value"]
 
 HIP_SOURCES = ["estep_fp32.hip", "estep_mixed.hip", "batch.hip", "gmmhip_api.hip",
-               "batch_api.hip", "server.hip", "fullcov.hip", "auxstats.hip"]
+               "batch_api.hip", "server.hip", "fullcov.hip", "auxstats.hip", "noalloc.hip"]
 HOST_SOURCES = ["host/gmm_model.cpp", "host/model_adapt.cpp", "host/feature_io.cpp",
                 "host/gmmhost_api.cpp", "tune.cpp"]
 TOOLS = ["trainGMM", "getLogLike", "serveLoad"]

@@ -109,6 +109,10 @@ struct gmmhip_ctx {
     uint64_t mu_f32_version = ~0ull;
     int num_cus = 0;
 
+    // device address of the no-allocation entry point's file-scope scratch
+    // array on this handle's device (noalloc.hip), fetched at creation
+    unsigned char *noalloc = nullptr;
+
     // tiles of the last call (gmmhip_tiles_used)
     int used_ll_frames = 0, used_chunk = 0;
 
@@ -184,6 +188,11 @@ struct DevGuard {
     }
 };
 
+// Device address of gmmhip_accumulate_noalloc's scratch array on device dev
+// (which must be current), fetched with hipGetSymbolAddress once per device;
+// nullptr when the runtime cannot give it (noalloc.hip).
+unsigned char *noalloc_scratch(int dev);
+
 // Frees the batched state of a handle (batch_api.hip).
 void batch_state_free(gmmhip_batch_state *b);
 
@@ -223,6 +232,12 @@ int model_operands(gmmhip_ctx *h, bool mllr, ModelOperands *o);
 // set when a coefficient does not fit the 16-bit type.
 std::vector<uint16_t> pack_split_model(const gmmhip_ctx *h, int prec, int KPm, const float *zoff,
                                        const float *zscale, bool *overflow);
+
+// Frames per statistics work item of an E-step over N frames: the
+// opts.chunk_frames override, else enough items for ~4096 blocks, capped by
+// the tuning table (8192 without an entry), a multiple of the statistics
+// kernel's 64-frame step (gmmhip_api.hip).
+uint64_t estep_chunk_frames(const gmmhip_ctx *h, uint64_t N, int ngroups, bool mixed);
 
 // Log-likelihood frame tile of the FP32 form (in units of 64 frames):
 // opts.ll_frames, else the tuning table's entry for the path (then the

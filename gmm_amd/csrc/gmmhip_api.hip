@@ -9,15 +9,14 @@
 #include "ctx.h"
 #include "tune.h"
 
-namespace {
-using namespace gmmhip_detail;
+namespace gmmhip_detail {
 
 // Frames per FP32 partial accumulator: enough chunks to give >= ~4096
 // blocks, at most 8192 frames per FP32 accumulator, a multiple of the
 // statistics kernel's frame step.
 // The upper bound comes from the tuning table when it has an entry for the
 // precision in use.
-uint64_t pick_chunk_frames(const gmmhip_ctx *h, int ngroups, bool mixed) {
+uint64_t estep_chunk_frames(const gmmhip_ctx *h, uint64_t N, int ngroups, bool mixed) {
     if (h->opt.chunk_frames > 0) {
         uint64_t cf = (uint64_t)h->opt.chunk_frames;
         return (cf + ST_FRAMES_PER_ITER - 1) / ST_FRAMES_PER_ITER * ST_FRAMES_PER_ITER;
@@ -25,11 +24,20 @@ uint64_t pick_chunk_frames(const gmmhip_ctx *h, int ngroups, bool mixed) {
     uint64_t cap = tuned_chunk(h, GMMHIP_TUNE_ESTEP, mixed ? h->opt.precision : GMMHIP_PREC_FP32);
     if (cap == 0) cap = 8192;
     const uint64_t target_chunks = std::max<uint64_t>(1, 4096 / std::max(1, ngroups));
-    uint64_t cf = (h->N + target_chunks - 1) / target_chunks;
+    uint64_t cf = (N + target_chunks - 1) / target_chunks;
     cf = (cf + ST_FRAMES_PER_ITER - 1) / ST_FRAMES_PER_ITER * ST_FRAMES_PER_ITER;
     cf = std::max<uint64_t>(cf, 128);
     cf = std::min<uint64_t>(cf, cap);
     return cf;
+}
+
+}  // namespace gmmhip_detail
+
+namespace {
+using namespace gmmhip_detail;
+
+uint64_t pick_chunk_frames(const gmmhip_ctx *h, int ngroups, bool mixed) {
+    return estep_chunk_frames(h, h->N, ngroups, mixed);
 }
 
 int check_ready(gmmhip_ctx *h) {
@@ -322,6 +330,9 @@ int gmmhip_create(int device, const gmmhip_opts *opts, gmmhip_t *out) {
         delete h;
         return GMMHIP_ERR_OOM;
     }
+    // the no-allocation entry point's scratch on this device (its address is
+    // fetched once per device, with the device current)
+    h->noalloc = noalloc_scratch(device);
     *out = h;
     return GMMHIP_OK;
 }

@@ -153,6 +153,28 @@ int gmmhip_set_frames_device(gmmhip_t h, const float *dx, uint64_t N, int D,
 int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S,
                       double *tot_ll, uint64_t *nframes);
 
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The plain FP32 E-step of gmmhip_accumulate on N frames
+ * [N][D] in DEVICE memory (row-major, borrowed for the call), adding into
+ * DEVICE outputs n[M], F[M*D], S[M*D] and tot_ll[1] (FP64) and into the host
+ * counter *nframes, with no device or host allocation, no host
+ * synchronisation and no memset: the work is enqueued on the handle's
+ * stream (gmmhip_stream / gmmhip_sync), its scratch is a file-scope device
+ * array of GMMHIP_NOALLOC_SCRATCH_BYTES per device, held until the process
+ * ends.  Results: n, F, S bit-identical to gmmhip_accumulate in FP32 on the
+ * same frames; tot_ll equal to FP64 rounding.  Runs the FP32 form whatever
+ * opts.precision says.  Limits: Mp x KP <= 524288 (Mp = M rounded up to 16,
+ * KP = 2D+1 rounded up to 16; e.g. M <= 4096 at D <= 63, M <= 2048 at
+ * D <= 127), GMMHIP_ERR_ARG beyond; any N.  Needs a diagonal model and no
+ * Gaussian-selection mask.  Every call on one device shares the scratch
+ * array: calls that overlap on one device (two streams, two threads) race.
+ * The host cannot see a NaN total here: check *tot_ll after syncing. */
+#define GMMHIP_NOALLOC_SCRATCH_BYTES 25297152u /* 24.1 MiB */
+int gmmhip_accumulate_noalloc(gmmhip_t h, const float *frames, uint64_t N, int D, int flags,
+                              double *n, double *F, double *S, double *tot_ll,
+                              uint64_t *nframes);
+
 /* Robust auxiliary statistics of the resident frames under the inserted
  * model (diagonal or full), added into aux[M*(D+2)]: per component
  * [aux_d (D) | aux2 | aux3] with, over every consecutive frame pair
