@@ -11,8 +11,11 @@ Entries for other shapes in the table are kept.
     python -m gmm_amd.autotune --D 60 --M 512,2048 --prec fp32,fp16x3 \\
         --frames 4000000 [--batch] [--out PATH]
 
-The frame tile only exists in the FP32 form (the mixed kernels use a fixed
-32 x 32 MFMA tile per wave); the work-item size is swept for every form.
+Frame tiles: the FP32 log-likelihood kernel's 64 / 128 / 256 frames per
+block; the mixed form's 128 / 256 frames per block (four or eight waves of 32
+frames sharing each model tile staged in LDS; 256 only where the depth
+2D+1 rounded up to 32 is a multiple of 64).  The work-item size is swept for
+every form, in the E-step and (--batch) in the batched path, per precision.
 Frames come from a seeded true mixture (means ~ N(0, 2^2), variances
 ~ U[0.3, 3], weights ~ Dirichlet(0.5)) generated in HBM.
 """
@@ -28,6 +31,15 @@ PREC_NAME = {v: k for k, v in PREC.items()}
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "gfx950.tune")
 
 LL_FRAMES = [64, 128, 256]
+LL_FRAMES_MIXED = [128, 256]
+
+
+def frame_tiles(prec, D):
+    """Candidate log-likelihood frame tiles of a form at dimension D."""
+    if prec == PREC["fp32"]:
+        return [ll for ll in LL_FRAMES if not (ll == 256 and 2 * D + 1 > 128)]
+    kpm = (2 * D + 1 + 31) // 32 * 32
+    return [ll for ll in LL_FRAMES_MIXED if not (ll == 256 and kpm % 64)]
 CHUNKS = [1024, 2048, 4096, 8192, 16384]
 BATCH_CHUNKS = [256, 512, 1024, 2048]
 
@@ -68,11 +80,8 @@ def _time(fn, reps):
 def sweep_estep(acc, prec, D, M, reps, log):
     """Returns (ll_frames, chunk_frames, ms) of the fastest E-step."""
     import gmm_amd
-    lls = LL_FRAMES if prec == gmm_amd.PREC_FP32 else [128]
     best = None
-    for ll in lls:
-        if ll == 256 and 2 * D + 1 > 128:
-            continue
+    for ll in frame_tiles(prec, D):
         for cf in CHUNKS:
             acc.set_opts(precision=prec, ll_frames=ll, chunk_frames=cf)
             ms = _time(acc.accumulate, reps)
@@ -83,10 +92,10 @@ def sweep_estep(acc, prec, D, M, reps, log):
     return best
 
 
-def sweep_batch(acc, D, M, U, reps, seed, log):
+def sweep_batch(acc, prec, D, M, U, reps, seed, log):
     """Returns (ll_frames, chunk_frames, ms) of the fastest batched MAP+MLLR
-    pass over U utterances (lengths lognormal(median 300, sigma 0.8) in
-    [50, 3000])."""
+    pass in one precision over U utterances (lengths lognormal(median 300,
+    sigma 0.8) in [50, 3000])."""
     import torch
     import gmm_amd
     from gmm_amd import synth
@@ -100,20 +109,19 @@ def sweep_batch(acc, D, M, U, reps, seed, log):
             "W": torch.empty((U, D, D + 1), dtype=torch.float64, device=dev).data_ptr(),
             "status": torch.empty(U, dtype=torch.int32, device=dev).data_ptr()}
     best = None
-    for ll in LL_FRAMES:
-        if ll == 256 and 2 * D + 1 > 128:
-            continue
-        acc.set_opts(ll_frames=ll)
+    for ll in frame_tiles(prec, D):
+        acc.set_opts(precision=prec, ll_frames=ll)
         for cf in BATCH_CHUNKS:
             def run():
                 acc.batch_adapt_device(x.data_ptr(), off, D, outs, tau=16, prior=True,
                                        chunk_frames=cf, graph=True)
                 torch.cuda.synchronize()
             ms = _time(run, reps)
-            log("  batch D=%d M=%d U=%d ll=%d chunk=%d: %.3f ms" % (D, M, U, ll, cf, ms))
+            log("  batch %s D=%d M=%d U=%d ll=%d chunk=%d: %.3f ms" % (
+                PREC_NAME[prec], D, M, U, ll, cf, ms))
             if best is None or ms < best[2]:
                 best = (ll, cf, ms)
-    acc.set_opts(ll_frames=0)
+    acc.set_opts(precision=PREC["fp32"], ll_frames=0)
     return best
 
 
@@ -123,7 +131,8 @@ def main(argv=None):
     ap.add_argument("--M", default="512,2048", help="comma-separated component counts")
     ap.add_argument("--prec", default="fp32,fp16x3,bf16x3")
     ap.add_argument("--frames", type=int, default=4_000_000)
-    ap.add_argument("--batch", action="store_true", help="also tune the batched path (FP32)")
+    ap.add_argument("--batch", action="store_true",
+                    help="also tune the batched path (every precision in --prec)")
     ap.add_argument("--utterances", type=int, default=2048)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--seed", type=int, default=20261209)
@@ -155,9 +164,12 @@ def main(argv=None):
             del x
             torch.cuda.empty_cache()
             if args.batch:
-                ll, cf, ms = sweep_batch(acc, D, M, args.utterances, args.reps, args.seed, log)
-                rows[("batch", "fp32", D, M)] = (ll, cf, ms)
-                log("batch fp32 D=%d M=%d -> ll_frames=%d chunk=%d (%.3f ms)" % (D, M, ll, cf, ms))
+                for pname in args.prec.split(","):
+                    ll, cf, ms = sweep_batch(acc, PREC[pname], D, M, args.utterances, args.reps,
+                                             args.seed, log)
+                    rows[("batch", pname, D, M)] = (ll, cf, ms)
+                    log("batch %s D=%d M=%d -> ll_frames=%d chunk=%d (%.3f ms)" % (
+                        pname, D, M, ll, cf, ms))
             write_table(args.out, rows)
     acc.close()
     print(args.out)

@@ -191,6 +191,7 @@ struct BatchPlan {
     bool want_means = false, want_mllr = false, prior = false, per_utt = false;
     int tau = 0;
     int ll_sub = 2;
+    int ll_nw = 4;  // mixed log-likelihood waves per block
     std::vector<uint64_t> lo, hi;
     std::vector<int> first;
 };
@@ -282,6 +283,7 @@ int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
         m.LL2 = h->d_LL2;
         m.LLnat = nullptr;
         m.blockLL = h->d_blockLL;
+        m.ll_nw = p.ll_nw;
         HIPCHK(h, launch_loglik_mixed(m, p.prec, s));
         m.chunk_frames = p.cf;
         m.nchunks = p.nchunks;
@@ -426,10 +428,13 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
     p.Mt = h->Mt;
     p.ngroups = (h->Mt + 3) / 4;
     p.N = N;
-    p.cf = cfg->chunk_frames > 0 ? (uint64_t)(cfg->chunk_frames + 63) / 64 * 64
-                                 : tuned_chunk(h, GMMHIP_TUNE_BATCH, GMMHIP_PREC_FP32);
+    // the table's entry for the handle's precision, else the FP32 entry
+    uint64_t tcf = tuned_chunk(h, GMMHIP_TUNE_BATCH, h->opt.precision);
+    if (tcf == 0) tcf = tuned_chunk(h, GMMHIP_TUNE_BATCH, GMMHIP_PREC_FP32);
+    p.cf = cfg->chunk_frames > 0 ? (uint64_t)(cfg->chunk_frames + 63) / 64 * 64 : tcf;
     if (p.cf == 0) p.cf = 1024;
     p.ll_sub = choose_ll_sub(h, GMMHIP_TUNE_BATCH);
+    p.ll_nw = choose_mixed_ll_nw(h, GMMHIP_TUNE_BATCH);
     h->used_ll_frames = 64 * p.ll_sub;
     h->used_chunk = (int)p.cf;
     p.want_means = out->means != nullptr;
@@ -546,6 +551,7 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
                                      h->model_version,
                                      p.cf,
                                      (uint64_t)p.ll_sub,
+                                     (uint64_t)p.ll_nw,
                                      (uint64_t)p.want_means,
                                      (uint64_t)p.want_mllr,
                                      (uint64_t)p.prior,

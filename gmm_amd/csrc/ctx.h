@@ -244,6 +244,11 @@ uint64_t estep_chunk_frames(const gmmhip_ctx *h, uint64_t N, int ngroups, bool m
 // E-step entry), else 2; 4 only for contraction depths up to 128
 // (gmmhip_api.hip).
 int choose_ll_sub(const gmmhip_ctx *h, int path);
+// Log-likelihood waves per block of the mixed form (4: 128-frame blocks,
+// 8: 256-frame blocks, only where the depth is a multiple of 64):
+// opts.ll_frames, else the tuning table's entry for (path, precision),
+// else 4 (gmmhip_api.hip).
+int choose_mixed_ll_nw(const gmmhip_ctx *h, int path);
 // Most frames per statistics work item from the tuning table for (path,
 // precision) and the handle's model, rounded to the 64-frame step; 0 when
 // the table has no entry.

@@ -41,6 +41,7 @@ struct MixedArgs {
     float min_ll, min_gamma;
     float *LL2, *LLnat;
     double *blockLL;
+    int ll_nw;             // log-likelihood waves per block: 4 (128 frames) or 8 (256, KP % 64 == 0)
     uint64_t chunk_frames;
     int nchunks, ngroups;
     float *partials;       // [nchunks][Mt32*32][KP]
@@ -55,6 +56,10 @@ hipError_t launch_colstats(const float *X, uint64_t N, int D, int blocks, uint64
 // prec: GMMHIP_PREC_BF16X3 (1) or GMMHIP_PREC_FP16X3 (2)
 hipError_t launch_prep_frames(const PrepArgs &p, int prec, hipStream_t s);
 hipError_t launch_loglik_mixed(const MixedArgs &a, int prec, hipStream_t s);
+// Frames per block of the mixed log-likelihood kernel at depth KP with
+// ll_nw waves (4, or 8 where KP is a multiple of 64): the number of
+// per-block log-likelihood sums it writes is ceil(Npad / that).
+int mixed_ll_frames_per_block(int KP, int nw);
 hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s);
 
 }  // namespace gmmhip

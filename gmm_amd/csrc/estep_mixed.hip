@@ -622,11 +622,7 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_tr_kerne
                 gh4[r >> 3][(r & 7) >> 1] = wh;                                    \
                 gl4[r >> 3][(r & 7) >> 1] = wl;                                    \
             }                                                                      \
-            _Pragma("unroll") for (int i = 0; i < KS; i++) {                       \
-                __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);                 \
-                __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);                 \
-                __builtin_amdgcn_sched_group_barrier(0x002, 10, 0);                \
-            }                                                                      \
+            GMMHIP_TR_SCHED                                                        \
             _Pragma("unroll") for (int u = 0; u < 2; u++) {                        \
                 const unsigned int any = gh4[u][0] | gh4[u][1] | gh4[u][2] | gh4[u][3]; \
                 if (__builtin_amdgcn_ballot_w64(any != 0u) == 0) continue;         \
@@ -654,6 +650,15 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_tr_kerne
     }
     f32x16 acc_n;
     int s = 0;
+    // the first contraction's LDS reads one step ahead of their MFMAs, the
+    // step's posterior references first
+#define GMMHIP_TR_SCHED                                                            \
+    __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);                             \
+    _Pragma("unroll") for (int i = 0; i < KS; i++) {                               \
+        if (i + 1 < KS) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);         \
+        __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);                         \
+        __builtin_amdgcn_sched_group_barrier(0x002, 10, 0);                        \
+    }
     for (; s + 2 < nsteps; s += 3) {
         GMMHIP_TR_STEP(s, ZA0, ZA1, ZA2, Ls0, Ls1, acc, acc_n)
         GMMHIP_TR_STEP(s + 1, ZA1, ZA2, ZA0, Ls1, Ls2, acc_n, acc)
@@ -666,6 +671,7 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_tr_kerne
     }
     if (s + 1 < nsteps) GMMHIP_TR_STEP(s + 1, ZA1, ZA2, ZA0, Ls1, Ls2, acc, acc_n)
 #undef GMMHIP_TR_STEP
+#undef GMMHIP_TR_SCHED
 #undef GMMHIP_TR_DMA
 #undef GMMHIP_TR_CONTRACT1
 
@@ -699,36 +705,103 @@ __device__ __forceinline__ float row_max32(float v) {
 // far above a row's reference are summed without moving it.
 static constexpr float LSE_HEADROOM = 64.0f;
 
+// Last tile (padded components >= M masked per lane), then the 32 component
+// lanes of each frame row combined (xor within 32 lanes), the per-frame
+// log-likelihoods written (clamped as the reference does) and the block's
+// FP64 sum.  base = first frame of the block (NW waves x 32 frames).
+template <int NW>
+__device__ __forceinline__ void ll_finish(const MixedArgs &a, const f32x16 &acc, float (&mx)[16],
+                                          float (&sm)[16], uint64_t base, double *red) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, c = lane & 31;
+    {
+        const bool valid = (32 * (a.Mt32 - 1) + c) < a.M;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            // (a padded lane that saw no component keeps mx = -inf, sm = 0)
+            const float d = acc[r] - mx[r];
+            const bool up = valid && d > LSE_HEADROOM;
+            const float e = valid ? exp2f_fast(up ? -d : d) : 0.0f;
+            sm[r] = up ? fmaf(sm[r], e, 1.0f) : sm[r] + e;
+            mx[r] = up ? acc[r] : mx[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const float m2 = __shfl_xor(mx[r], o);
+            const float s2 = __shfl_xor(sm[r], o);
+            const float hi = fmaxf(mx[r], m2);
+            // lanes that only saw padded components carry (-inf, 0)
+            const float e1 = (mx[r] == -INFINITY) ? 0.0f : sm[r] * exp2f_fast(mx[r] - hi);
+            const float e2 = (m2 == -INFINITY) ? 0.0f : s2 * exp2f_fast(m2 - hi);
+            sm[r] = e1 + e2;
+            mx[r] = hi;
+        }
+    }
+    double local = 0.0;
+    if (c == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int row = 8 * (r >> 2) + 4 * h + (r & 3);
+            const uint64_t fr = base + 32 * wave + row;
+            float l2 = mx[r] + log2f_fast(sm[r]);
+            float nat = l2 * kLn2m;
+            if (!(nat > a.min_ll) || isinf(nat) || (fr < a.N && a.bad[fr])) {
+                nat = a.min_ll;
+                l2 = INFINITY;
+            }
+            if (fr < a.N) {
+                a.LL2[fr] = l2;
+                if (a.LLnat) a.LLnat[fr] = nat;
+                local += (double)nat;
+            }
+        }
+        red[wave * 2 + h] = local;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int i = 0; i < 2 * NW; i++) s += red[i];
+        a.blockLL[blockIdx.x] = s;
+    }
+}
+
 // Per-frame log-sum-exp: block = 4 waves x 32 frames; a wave keeps the split
 // A fragments of its frames in registers and streams 32-component tiles of
 // the split model through LDS by direct-to-LDS copies (two LDS buffers, one
 // barrier per tile).  hi*hi + hi*lo + lo*hi accumulate into one chain per
 // tile; the log-sum-exp of tile t runs on the VALU between tile t+1's MFMAs.
-template <int KP, typename T>
-__global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
+template <int KP, typename T, int NW>
+__global__ __launch_bounds__(64 * NW) void loglik_mixed_dma_kernel(MixedArgs a) {
     typedef typename Vec8<T>::type V8;
     constexpr int KS = KP / 16;
     constexpr int TILE16 = KS * 2 * 64;      // 16-byte pieces per component tile
-    constexpr int PF = TILE16 / 256;
-    static_assert(TILE16 % 256 == 0, "component tile must split evenly over 256 threads");
+    constexpr int PF = TILE16 / (64 * NW);
+    static_assert(TILE16 % (64 * NW) == 0, "component tile must split evenly over the block");
     // one __shared__ variable per buffer: the compiler then sees that a copy
     // in flight does not alias the reads of the other buffer
     __shared__ u32x4 Pl0[TILE16], Pl1[TILE16];
-    __shared__ double red[8];
+    __shared__ double red[2 * NW];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, c = lane & 31;
-    const uint64_t base = (uint64_t)blockIdx.x * 128;
+    const uint64_t base = (uint64_t)blockIdx.x * (32 * NW);
 
     V8 ahi[KS], alo[KS];
     {
-        const uint64_t fr = base + 32 * wave + c;  // < Npad
-        const u32x4 *zh = reinterpret_cast<const u32x4 *>(a.za_hi) + fr * (KP / 8);
-        const u32x4 *zl = reinterpret_cast<const u32x4 *>(a.za_lo) + fr * (KP / 8);
+        // rows past Npad (a multiple of 128) belong to no frame: zero operands
+        const uint64_t fr = base + 32 * wave + c;
+        const bool ok = fr < a.Npad;
+        const u32x4 *zh = reinterpret_cast<const u32x4 *>(a.za_hi) + (ok ? fr : 0) * (KP / 8);
+        const u32x4 *zl = reinterpret_cast<const u32x4 *>(a.za_lo) + (ok ? fr : 0) * (KP / 8);
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-            const u32x4 x = zh[2 * ks + h], y = zl[2 * ks + h];
+            const u32x4 z = (u32x4){0u, 0u, 0u, 0u};
+            const u32x4 x = ok ? zh[2 * ks + h] : z, y = ok ? zl[2 * ks + h] : z;
             ahi[ks] = __builtin_bit_cast(V8, x);
             alo[ks] = __builtin_bit_cast(V8, y);
         }
@@ -793,7 +866,7 @@ __global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
             GMMHIP_DMA_TILE((T_) + 1, PW_)                                           \
         }                                                                            \
         _Pragma("unroll") for (int j = 0; j < 2 * KS; j++) bt[j] = PR_[j * 64 + lane]; \
-        __builtin_amdgcn_sched_barrier(0);                                           \
+        GMMHIP_LL_READ_ORDER                                                         \
         NACC_ = (f32x16){};                                                          \
         GMMHIP_TILE_MFMA(NACC_)                                                      \
         float d[16], e[16];                                                          \
@@ -803,10 +876,7 @@ __global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
         }                                                                            \
         float dm = fmaxf(fmaxf(d[0], d[1]), d[2]);                                   \
         _Pragma("unroll") for (int r = 3; r < 16; r += 2) dm = fmaxf(fmaxf(dm, d[r]), d[r + 1 < 16 ? r + 1 : r]); \
-        _Pragma("unroll") for (int i = 0; i < 3 * KS; i++) {                         \
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                       \
-            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);                       \
-        }                                                                            \
+        GMMHIP_LL_SCHED                                                              \
         if (__builtin_amdgcn_ballot_w64(dm > LSE_HEADROOM) == 0) {                   \
             _Pragma("unroll") for (int r = 0; r < 16; r++) sm[r] += e[r];            \
         } else {                                                                     \
@@ -821,6 +891,20 @@ __global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
             }                                                                        \
         }                                                                            \
     }
+    // the tile's LDS reads spread through its MFMA chain, two contraction
+    // steps ahead of their use, so that a wave never holds a run of reads in
+    // front of MFMAs it could issue (with all 16 reads first, the LDS command
+    // queue fills and in-order issue stalls the MFMAs)
+#define GMMHIP_LL_READ_ORDER
+#define GMMHIP_LL_SCHED                                                              \
+    __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);                               \
+    _Pragma("unroll") for (int ks = 0; ks < KS; ks++) {                              \
+        if (ks + 2 < KS) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);          \
+        _Pragma("unroll") for (int i = 0; i < 3; i++) {                              \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                       \
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);                       \
+        }                                                                            \
+    }
     f32x16 acc2;
     int t = 1;
     // tile t sits in Pl1 for odd t, Pl0 for even t
@@ -833,62 +917,10 @@ __global__ __launch_bounds__(256) void loglik_mixed_dma_kernel(MixedArgs a) {
         acc = acc2;
     }
 #undef GMMHIP_LL_STEP
-    // last tile: padded components (>= M) are masked per lane
-    {
-        const bool valid = (32 * (a.Mt32 - 1) + c) < a.M;
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            // (a padded lane that saw no component keeps mx = -inf, sm = 0)
-            const float d = acc[r] - mx[r];
-            const bool up = valid && d > LSE_HEADROOM;
-            const float e = valid ? exp2f_fast(up ? -d : d) : 0.0f;
-            sm[r] = up ? fmaf(sm[r], e, 1.0f) : sm[r] + e;
-            mx[r] = up ? acc[r] : mx[r];
-        }
-    }
+#undef GMMHIP_LL_READ_ORDER
+#undef GMMHIP_LL_SCHED
 #undef GMMHIP_TILE_MFMA
-
-    // combine the 32 component lanes of each frame row (xor within 32 lanes)
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const float m2 = __shfl_xor(mx[r], o);
-            const float s2 = __shfl_xor(sm[r], o);
-            const float hi = fmaxf(mx[r], m2);
-            // lanes that only saw padded components carry (-inf, 0)
-            const float e1 = (mx[r] == -INFINITY) ? 0.0f : sm[r] * exp2f_fast(mx[r] - hi);
-            const float e2 = (m2 == -INFINITY) ? 0.0f : s2 * exp2f_fast(m2 - hi);
-            sm[r] = e1 + e2;
-            mx[r] = hi;
-        }
-    }
-    double local = 0.0;
-    if (c == 0) {
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int row = 8 * (r >> 2) + 4 * h + (r & 3);
-            const uint64_t fr = base + 32 * wave + row;
-            float l2 = mx[r] + log2f_fast(sm[r]);
-            float nat = l2 * kLn2m;
-            if (!(nat > a.min_ll) || isinf(nat) || (fr < a.N && a.bad[fr])) {
-                nat = a.min_ll;
-                l2 = INFINITY;
-            }
-            if (fr < a.N) {
-                a.LL2[fr] = l2;
-                if (a.LLnat) a.LLnat[fr] = nat;
-                local += (double)nat;
-            }
-        }
-        red[wave * 2 + h] = local;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double s = 0.0;
-        for (int i = 0; i < 8; i++) s += red[i];
-        a.blockLL[blockIdx.x] = s;
-    }
+    ll_finish<NW>(a, acc, mx, sm, base, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -911,15 +943,23 @@ hipError_t launch_prep_frames(const PrepArgs &p, int prec, hipStream_t s) {
     return hipGetLastError();
 }
 
+int mixed_ll_frames_per_block(int KP, int nw) { return (nw == 8 && KP % 64 == 0) ? 256 : 128; }
+
 hipError_t launch_loglik_mixed(const MixedArgs &a, int prec, hipStream_t s) {
-    const dim3 grid((unsigned)((a.Npad + 127) / 128));
+    const int fpb = mixed_ll_frames_per_block(a.KP, a.ll_nw);
+    const dim3 grid((unsigned)((a.Npad + fpb - 1) / fpb));
     switch (a.KP) {
 #define X(K)                                                                                      \
     case K:                                                                                       \
-        if (prec == 2)                                                                            \
-            hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, _Float16>), grid, dim3(256), 0, s, a); \
+        if (fpb == 256) {                                                                         \
+            if (prec == 2)                                                                        \
+                hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, _Float16, (K % 64 == 0 ? 8 : 4)>), grid, dim3(512), 0, s, a); \
+            else                                                                                  \
+                hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, __bf16, (K % 64 == 0 ? 8 : 4)>), grid, dim3(512), 0, s, a); \
+        } else if (prec == 2)                                                                     \
+            hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, _Float16, 4>), grid, dim3(256), 0, s, a); \
         else                                                                                      \
-            hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, __bf16>), grid, dim3(256), 0, s, a);   \
+            hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, __bf16, 4>), grid, dim3(256), 0, s, a);   \
         break;
         GMMHIP_KP32_CASES(X)
 #undef X

@@ -116,6 +116,15 @@ int choose_ll_sub(const gmmhip_ctx *h, int path) {
     return sub;
 }
 
+int choose_mixed_ll_nw(const gmmhip_ctx *h, int path) {
+    int f = h->opt.ll_frames;
+    if (f <= 0) {
+        gmmhip_tune::Entry e;
+        f = gmmhip_tune::lookup(path, h->opt.precision, h->D, h->M, &e) ? e.ll_frames : 128;
+    }
+    return (f >= 256 && kp_mixed_for_dim(h->D) % 64 == 0) ? 8 : 4;
+}
+
 uint64_t tuned_chunk(const gmmhip_ctx *h, int path, int prec) {
     gmmhip_tune::Entry e;
     if (!gmmhip_tune::lookup(path, prec, h->D, h->M, &e)) return 0;
@@ -287,6 +296,7 @@ MixedArgs make_mixed_args(gmmhip_ctx *h) {
     m.Mt32 = (h->M + 31) / 32;
     m.min_ll = h->opt.min_loglike;
     m.min_gamma = h->opt.min_gamma;
+    m.ll_nw = choose_mixed_ll_nw(h, GMMHIP_TUNE_ESTEP);
     return m;
 }
 
@@ -595,7 +605,7 @@ int gmmhip_loglikes_device(gmmhip_t h, float *d_out) {
         m.LL2 = h->d_LL2;
         m.LLnat = d_out;
         m.blockLL = h->d_blockLL;
-        h->used_ll_frames = 128;
+        h->used_ll_frames = mixed_ll_frames_per_block(m.KP, m.ll_nw);
         HIPCHK(h, launch_loglik_mixed(m, h->opt.precision, h->stream));
         return GMMHIP_OK;
     }
@@ -662,7 +672,9 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
     const int Mtx = (h->M + tile - 1) / tile;
     const int ngroups = (Mtx + 3) / 4;
     const uint64_t chunk_frames = pick_chunk_frames(h, ngroups, mixed);
-    const int ll_sub = mixed ? 2 : choose_ll_sub(h, GMMHIP_TUNE_ESTEP);
+    const int ll_sub =
+        mixed ? mixed_ll_frames_per_block(h->KPm, choose_mixed_ll_nw(h, GMMHIP_TUNE_ESTEP)) / 64
+              : choose_ll_sub(h, GMMHIP_TUNE_ESTEP);
     h->used_ll_frames = 64 * ll_sub;
     h->used_chunk = (int)chunk_frames;
     const int nchunks = (int)((h->N + chunk_frames - 1) / chunk_frames);

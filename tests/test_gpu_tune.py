@@ -74,3 +74,39 @@ def test_table_entry_is_used(acc, tmp_path):
         assert acc.tiles_used() == (64, 256)
     finally:
         gmm_amd.tune_load(None)
+
+
+@pytest.mark.parametrize("prec", [gmm_amd.PREC_FP16X3, gmm_amd.PREC_BF16X3])
+@pytest.mark.parametrize("D,M,N", [(60, 300, 20000 + 77), (20, 70, 3001)])
+def test_mixed_frame_tiles_identical(acc, prec, D, M, N):
+    """The mixed form's log-likelihood frame tile (128 or 256 frames per
+    block; 256 only where the depth is a multiple of 64, else the handle
+    keeps 128): per-frame log-likelihoods and statistics bit-identical, the
+    total to the FP64 grouping of block sums; the batched path the same."""
+    rng = np.random.default_rng(D + M + prec)
+    w = rng.dirichlet(np.ones(M))
+    mu = rng.normal(0, 2, (M, D)).astype(np.float32)
+    var = rng.uniform(0.5, 2, (M, D)).astype(np.float32)
+    x = synth.sample_frames_np(w, mu, var, N, 5)
+    acc.set_model(w.astype(np.float32), mu, var)
+    acc.set_frames(x)
+    kpm = (2 * D + 1 + 31) // 32 * 32
+    res = {}
+    try:
+        for ll in (128, 256):
+            acc.set_opts(precision=prec, ll_frames=ll, chunk_frames=1024)
+            lls = acc.loglikes()
+            st = acc.accumulate()
+            used = acc.tiles_used()
+            assert used == ((ll if kpm % 64 == 0 else 128), 1024), used
+            off = np.array([0, 500, 3000, N], np.uint64)
+            bt = acc.batch_adapt(x, off, mllr=False)
+            res[ll] = (lls, st, bt)
+    finally:
+        acc.set_opts(precision=gmm_amd.PREC_FP32, ll_frames=0, chunk_frames=0)
+    assert np.array_equal(res[128][0], res[256][0])
+    for a, b in zip(res[128][1][:3], res[256][1][:3]):
+        assert np.array_equal(a, b)
+    assert res[128][1][3] == pytest.approx(res[256][1][3], rel=1e-12)
+    for k in res[128][2]:
+        assert np.array_equal(res[128][2][k], res[256][2][k]), k

@@ -74,3 +74,14 @@ def test_committed_table_parses():
     for (kind, prec, D, M), (ll, cf, ms) in rows.items():
         assert kind in ("estep", "batch") and prec in autotune.PREC
         assert ll in (64, 128, 256) and cf >= 64 and ms > 0
+
+
+def test_autotune_frame_tile_candidates():
+    """FP32: 64/128/256 (256 only up to depth 128); mixed: 128/256 (256 only
+    where 2D+1 rounded up to 32 is a multiple of 64)."""
+    fp32, fp16 = autotune.PREC["fp32"], autotune.PREC["fp16x3"]
+    assert autotune.frame_tiles(fp32, 60) == [64, 128, 256]
+    assert autotune.frame_tiles(fp32, 64) == [64, 128]
+    assert autotune.frame_tiles(fp16, 60) == [128, 256]   # depth 128
+    assert autotune.frame_tiles(fp16, 39) == [128]        # depth 96
+    assert autotune.frame_tiles(fp16, 20) == [128, 256]   # depth 64
