@@ -40,7 +40,9 @@ EXTRA_EXPORTS = [
     "gmmhost_model_free", "gmmhost_update_em", "gmmhost_update_em_robust",
     "gmmhost_make_robust_stats", "gmmhost_update_map",
     "gmmhost_mllr_transform", "gmmhost_fmllr_transform", "gmmhost_mllr_apply",
-    "gmmhost_last_error",
+    "gmmhost_last_error", "gmmhost_features_load", "gmmhost_features_load_ranges",
+    "gmmhost_features_dims", "gmmhost_features_get", "gmmhost_features_free",
+    "gmmhost_features_save",
 ]
 
 (OK, ERR_HIP, ERR_OOM, ERR_NAN, ERR_ARG, ERR_VAR, ERR_STATE, ERR_MLLR, ERR_NODEV, ERR_TIMEOUT,

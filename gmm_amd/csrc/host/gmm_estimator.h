@@ -198,53 +198,46 @@ public:
     // accumulateStatsMT over a file list (OL_GMMStatsEstimator.cpp:641-707):
     // files (prm_type: 0 SVES, 1 HTK, 2 RAW; every dwnsmp-th frame) are
     // grouped until a group exceeds _memoryBuffDataGB (0: one file per
-    // group) and every group is accumulated; a missing or empty file is
-    // skipped with a warning.  The next group is read by a second thread
-    // while the device accumulates the current one.
+    // group) and every group is accumulated; a file that cannot be opened or
+    // holds no frames is skipped with a warning, a malformed one throws.
+    // The next group is read by a second thread while the device accumulates
+    // the current one.
     void accumulateStatsMT(const std::vector<std::string> &files, GMMStats<TS> &gmms, int prm_type,
                            unsigned int dwnsmp = 1) {
-        const double giga = 1024.0 * 1024.0 * 1024.0;
-        size_t next = 0;
-        auto load_group = [&files, prm_type, dwnsmp, giga, this](size_t &pos) {
-            Features g;
-            while (pos < files.size()) {
-                Features f;
-                std::string err;
-                const std::string &fn = files[pos++];
-                if (LoadFeatures(fn.c_str(), prm_type, f, &err) != 0) {
-                    std::cerr << "\n WARNING: Param file [" << fn << "] not found -> skipped!"
-                              << std::endl;
-                    continue;
+        const unsigned step = dwnsmp > 1 ? dwnsmp : 1;
+        accumulateGroups(files.size(), gmms, [&files, prm_type, step](size_t i, Features &f, std::string &name,
+                                                                      std::string &err) {
+            name = files[i];
+            const int rc = LoadFeatures(name.c_str(), prm_type, f, &err);
+            if (rc == 0 && step > 1 && f.N > 0) {
+                Features k;
+                k.D = f.D;
+                for (unsigned t = 0; t < f.N; t += step) {
+                    k.data.insert(k.data.end(), f.data.begin() + (size_t)t * f.D,
+                                  f.data.begin() + (size_t)(t + 1) * f.D);
+                    k.N++;
                 }
-                if (f.N < 1) {
-                    std::cerr << "\n WARNING: Param file [" << fn << "] empty" << std::endl;
-                    continue;
-                }
-                if (g.D && f.D != g.D)
-                    throw std::logic_error("accumulateStatsMT(): feature dimensions differ across files");
-                g.D = f.D;
-                for (unsigned i = 0; i < f.N; i += (dwnsmp > 1 ? dwnsmp : 1)) {
-                    g.data.insert(g.data.end(), f.data.begin() + (size_t)i * f.D,
-                                  f.data.begin() + (size_t)(i + 1) * f.D);
-                    g.N++;
-                }
-                if (!(_memoryBuffDataGB > (double)g.N * g.D * sizeof(float) / giga)) break;
+                f = std::move(k);
             }
-            return g;
-        };
-        std::future<Features> pending = std::async(std::launch::async, load_group, std::ref(next));
-        for (;;) {
-            Features cur = pending.get();
-            if (cur.N == 0) break;
-            if (next < files.size())
-                pending = std::async(std::launch::async, load_group, std::ref(next));
-            else
-                pending = std::async(std::launch::deferred, [] { return Features(); });
-            cur.rows.resize(cur.N);
-            for (unsigned i = 0; i < cur.N; i++) cur.rows[i] = &cur.data[(size_t)i * cur.D];
-            sameData_ = false;
-            accumulateStatsMT(cur.rows.data(), cur.N, cur.D, gmms);
-        }
+            return rc;
+        });
+    }
+
+    // accumulateStatsMT over frame intervals of files
+    // (OL_GMMStatsEstimator.cpp:712-780): per file, the frames of its
+    // intervals [x0, x1) in sample-period units (LoadFeatureRanges; SVES and
+    // RAW take samplePeriod, HTK its header's), grouped and accumulated as
+    // above, in file-name order.
+    void accumulateStatsMT(const file_2_ranges &file2ranges, GMMStats<TS> &gmms, int prm_type,
+                           unsigned int samplePeriod) {
+        std::vector<const file_2_ranges::value_type *> items;
+        for (const auto &kv : file2ranges) items.push_back(&kv);
+        accumulateGroups(items.size(), gmms, [&items, prm_type, samplePeriod](size_t i, Features &f,
+                                                                              std::string &name,
+                                                                              std::string &err) {
+            name = items[i]->first;
+            return LoadFeatureRanges(name.c_str(), prm_type, items[i]->second, samplePeriod, f, &err);
+        });
     }
 
     // Per-frame log-likelihoods of the N frames into outLogLikes[N].
@@ -312,6 +305,51 @@ private:
             if (installed) gmmhip_set_mask(e.h_, 0, nullptr, nullptr);
         }
     };
+
+    // The grouping loop of the file overloads: load(i, features, name, err)
+    // reads item i (0, or -1 to skip with a warning, or -2 to throw).
+    template <typename Load>
+    void accumulateGroups(size_t count, GMMStats<TS> &gmms, Load load) {
+        const double giga = 1024.0 * 1024.0 * 1024.0;
+        size_t next = 0;
+        auto load_group = [&load, count, giga, this](size_t &pos) {
+            Features g;
+            while (pos < count) {
+                Features f;
+                std::string err, name;
+                const int rc = load(pos++, f, name, err);
+                if (rc == -1) {
+                    std::cerr << "\n WARNING: Param file [" << name << "] not found -> skipped!" << std::endl;
+                    continue;
+                }
+                if (rc != 0) throw std::runtime_error(err);
+                if (f.N < 1) {
+                    std::cerr << "\n WARNING: Param file [" << name << "] empty" << std::endl;
+                    continue;
+                }
+                if (g.D && f.D != g.D)
+                    throw std::logic_error("accumulateStatsMT(): feature dimensions differ across files");
+                g.D = f.D;
+                g.data.insert(g.data.end(), f.data.begin(), f.data.end());
+                g.N += f.N;
+                if (!(_memoryBuffDataGB > (double)g.N * g.D * sizeof(float) / giga)) break;
+            }
+            return g;
+        };
+        std::future<Features> pending = std::async(std::launch::async, load_group, std::ref(next));
+        for (;;) {
+            Features cur = pending.get();
+            if (cur.N == 0) break;
+            if (next < count)
+                pending = std::async(std::launch::async, load_group, std::ref(next));
+            else
+                pending = std::async(std::launch::deferred, [] { return Features(); });
+            cur.rows.resize(cur.N);
+            for (unsigned i = 0; i < cur.N; i++) cur.rows[i] = &cur.data[(size_t)i * cur.D];
+            sameData_ = false;
+            accumulateStatsMT(cur.rows.data(), cur.N, cur.D, gmms);
+        }
+    }
 
     gmmhip_t h_ = nullptr;
     int dev_ = 0;

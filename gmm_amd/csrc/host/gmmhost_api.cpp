@@ -2,13 +2,19 @@
 #include "../../../include/gmmhost.h"
 
 #include <cstring>
+#include <memory>
 #include <string>
 
+#include "feature_io.h"
 #include "gmm_model.h"
 #include "model_adapt.h"
 
 struct gmmhost_model {
     gmm::GMModel m;
+};
+
+struct gmmhost_features {
+    gmm::Features f;
 };
 
 namespace {
@@ -38,6 +44,83 @@ struct RuleHost : gmm::CModelAdapt {};
 extern "C" {
 
 const char *gmmhost_last_error(void) { return g_err.c_str(); }
+
+static int feat_rc(int rc, const std::string &err) {
+    if (rc == 0) return 0;
+    return fail(err, rc == -1 ? GMMHOST_FEAT_MISSING : 1);
+}
+
+int gmmhost_features_load(const char *path, int type, uint32_t dwnsmp, gmmhost_features_t *out) {
+    if (!path || !out) return fail("gmmhost_features_load: null argument");
+    try {
+        std::unique_ptr<gmmhost_features> h(new gmmhost_features);
+        std::string err;
+        const int rc = gmm::LoadFeatures(path, type, h->f, &err);
+        if (rc) return feat_rc(rc, err);
+        if (dwnsmp > 1 && h->f.N > 0) {
+            gmm::Features k;
+            k.D = h->f.D;
+            k.samplePeriod = h->f.samplePeriod;
+            for (uint32_t t = 0; t < h->f.N; t += dwnsmp) {
+                k.data.insert(k.data.end(), h->f.data.begin() + (size_t)t * k.D,
+                              h->f.data.begin() + (size_t)(t + 1) * k.D);
+                k.N++;
+            }
+            h->f = std::move(k);
+        }
+        *out = h.release();
+        return 0;
+    } catch (std::exception &e) {
+        return fail(e.what());
+    }
+}
+
+int gmmhost_features_load_ranges(const char *path, int type, const uint32_t *ranges, int nranges,
+                                 uint32_t sample_period, gmmhost_features_t *out) {
+    if (!path || !out || nranges < 0 || (nranges > 0 && !ranges))
+        return fail("gmmhost_features_load_ranges: bad argument");
+    try {
+        gmm::element_ranges r;
+        for (int k = 0; k < nranges; k++) r.push_back({ranges[2 * k], ranges[2 * k + 1]});
+        std::unique_ptr<gmmhost_features> h(new gmmhost_features);
+        std::string err;
+        const int rc = gmm::LoadFeatureRanges(path, type, r, sample_period, h->f, &err);
+        if (rc) return feat_rc(rc, err);
+        *out = h.release();
+        return 0;
+    } catch (std::exception &e) {
+        return fail(e.what());
+    }
+}
+
+int gmmhost_features_dims(gmmhost_features_t f, uint32_t *N, uint32_t *D, uint32_t *sample_period) {
+    if (!f) return fail("gmmhost_features_dims: null handle");
+    if (N) *N = f->f.N;
+    if (D) *D = f->f.D;
+    if (sample_period) *sample_period = f->f.samplePeriod;
+    return 0;
+}
+
+int gmmhost_features_get(gmmhost_features_t f, float *x) {
+    if (!f || (!x && f->f.N)) return fail("gmmhost_features_get: null argument");
+    if (f->f.N) std::memcpy(x, f->f.data.data(), sizeof(float) * f->f.data.size());
+    return 0;
+}
+
+void gmmhost_features_free(gmmhost_features_t f) { delete f; }
+
+int gmmhost_features_save(const char *path, int type, const float *x, uint32_t N, uint32_t D,
+                          uint32_t sample_period) {
+    if (!path || (!x && N && D)) return fail("gmmhost_features_save: null argument");
+    int rc;
+    switch (type) {
+    case 0: rc = gmm::SaveSVES(path, x, N, D); break;
+    case 1: rc = gmm::SaveHTK(path, x, N, D, sample_period); break;
+    case 2: rc = gmm::SaveRaw(path, x, N, D); break;
+    default: return fail("gmmhost_features_save: unknown type");
+    }
+    return rc ? fail(std::string("cannot write ") + path) : 0;
+}
 
 int gmmhost_model_load(const char *path, int txt, gmmhost_model_t *out) {
     if (!path || !out) return fail("null argument");

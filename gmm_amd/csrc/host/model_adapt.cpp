@@ -6,6 +6,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <limits>
 #include <stdexcept>
 
 namespace gmm {
@@ -17,6 +18,10 @@ constexpr double MIN_IND_SAMPLES = 2.0;  // OL_ModelAdapt.h:54
 void CModelAdapt::SetOptions(const OPTIONS &opt) {
     opt_ = opt;
     if (opt_._min_var < 1.17549435e-38f) opt_._min_var = 1.17549435e-38f;  // CheckOptions :2141-2143
+    // CheckOptions :2191-2212: thresholds clamped into range
+    if (opt_._llTh < 0.0f) opt_._llTh = 0.0f;
+    opt_._splitXpercAtOnce = std::min(1.0f, std::max(0.0f, opt_._splitXpercAtOnce));
+    opt_._splitTh = std::min(1.0f, std::max(0.0f, opt_._splitTh));
 }
 
 void CModelAdapt::InsertData(float **vectors, unsigned int N, unsigned int D) {
@@ -24,6 +29,7 @@ void CModelAdapt::InsertData(float **vectors, unsigned int N, unsigned int D) {
     N_ = N;
     D_ = D;
     files_.clear();
+    ranges_.clear();
     dataChanged_ = true;
 }
 
@@ -32,15 +38,44 @@ void CModelAdapt::InsertFiles(const std::vector<std::string> &files, int prm_typ
     unsigned int N = 0, D = 0;
     for (const std::string &fn : files) {
         Features f;
-        if (LoadFeatures(fn.c_str(), prm_type, f, nullptr) != 0 || f.N == 0) continue;
+        std::string err;
+        const int rc = LoadFeatures(fn.c_str(), prm_type, f, &err);
+        if (rc == -1 || (rc == 0 && f.N == 0)) continue;
+        if (rc != 0) throw std::runtime_error(err);
         if (D && f.D != D) throw std::logic_error("InsertData(): feature dimensions differ across files");
         D = f.D;
         N += dwnsmp > 1 ? (f.N + dwnsmp - 1) / dwnsmp : f.N;
     }
     if (N == 0) throw std::runtime_error("InsertData(): no feature vectors in the given files");
+    ranges_.clear();
     files_ = files;
     fileType_ = prm_type;
     dwnsmp_ = dwnsmp;
+    buffGB_ = buffGB;
+    data_ = nullptr;
+    N_ = N;
+    D_ = D;
+    dataChanged_ = true;
+}
+
+void CModelAdapt::InsertRanges(const file_2_ranges &ranges, int prm_type, unsigned int samplePeriod,
+                               float buffGB) {
+    unsigned int N = 0, D = 0;
+    for (const auto &kv : ranges) {
+        Features f;
+        std::string err;
+        const int rc = LoadFeatureRanges(kv.first.c_str(), prm_type, kv.second, samplePeriod, f, &err);
+        if (rc == -1 || (rc == 0 && f.N == 0)) continue;
+        if (rc != 0) throw std::runtime_error(err);
+        if (D && f.D != D) throw std::logic_error("InsertData(): feature dimensions differ across files");
+        D = f.D;
+        N += f.N;
+    }
+    if (N == 0) throw std::runtime_error("InsertData(): no feature vectors in the given intervals");
+    files_.clear();
+    ranges_ = ranges;
+    fileType_ = prm_type;
+    samplePeriod_ = samplePeriod;
     buffGB_ = buffGB;
     data_ = nullptr;
     N_ = N;
@@ -62,14 +97,16 @@ void CModelAdapt::initEstimator() {
 // rearranges it), set the flags and add the statistics of the inserted data.
 void CModelAdapt::AccumulateStats(GMModel *model, GMMStats<double> &stats, bool mixProbsOnly,
                                   bool meanStats, bool varStats, bool auxStats) {
-    if ((!data_ && files_.empty()) || N_ == 0)
-        throw std::logic_error("AccumulateStats(): Insert data first!");
+    if (!hasData() || N_ == 0) throw std::logic_error("AccumulateStats(): Insert data first!");
     initEstimator();
     if (model) est_->insertModel(*model);
     est_->setAccFlags(mixProbsOnly, meanStats, varStats, stats.hasFullCov(), auxStats);
-    if (!files_.empty()) {
+    if (!files_.empty() || !ranges_.empty()) {
         est_->_memoryBuffDataGB = buffGB_;
-        est_->accumulateStatsMT(files_, stats, fileType_, dwnsmp_);
+        if (!files_.empty())
+            est_->accumulateStatsMT(files_, stats, fileType_, dwnsmp_);
+        else
+            est_->accumulateStatsMT(ranges_, stats, fileType_, samplePeriod_);
         dataChanged_ = true;
         return;
     }
@@ -686,7 +723,7 @@ MLLRTransform CModelAdapt::IterW_MLLR(GMModel *UBM, int iters, bool wantQ) {
 
 void CModelAdapt::AdaptModel(GMModel *UBM) {
     if (!UBM) throw std::logic_error("AdaptModel(): UBM not given!");
-    if (!data_ && files_.empty()) throw std::logic_error("AdaptModel(): Insert adaptation data first!");
+    if (!hasData()) throw std::logic_error("AdaptModel(): Insert adaptation data first!");
     // statistics as the reference allocates them (OL_ModelAdapt.cpp:281-298):
     // full second-order statistics for a full UBM, none for MLLR
     const bool fullV = UBM->GetFullCovStatus();
@@ -776,14 +813,92 @@ static void leading_eigenpair(int D, const float *cov, double &lam, std::vector<
     for (int i = 0; i < D; i++) v[i] = sg * V[(size_t)i * D + best];
 }
 
-// TrainFromScratch (OL_ModelAdapt.cpp:673-821) with splitGaussian
-// (:614-660; diagonal: shift +-0.5*sqrt(var); full covariance: shift
-// +-sqrt(lambda) v along the leading eigenpair) and splitMultipleGaussians
-// with split-X = 0 (one split per round: heaviest component, ties to the
-// higher index, into the highest-index zero-weight slot, GetGaussStats
-// :965-991).
-void CModelAdapt::TrainFromScratch(unsigned int nummix, bool fullcov) {
-    if ((!data_ && files_.empty()) || N_ <= 3)
+// splitGaussian (OL_ModelAdapt.cpp:614-660): component m2 halves its weight
+// with the empty slot mn, which takes its covariance; the two means move
+// apart by +-0.5 sqrt(var) (diagonal) or +-sqrt(lambda) v along the leading
+// eigenpair (full covariance).
+static void split_component(GMModel *model, int m2, int mn) {
+    const int D = model->GetDimension();
+    std::memcpy(model->GetMixtureVar(mn), model->GetMixtureVar(m2), sizeof(float) * model->GetDimVar());
+    const float half = model->GetMixtureWeight(m2) / 2.0f;
+    model->SetMixWeight(mn, half);
+    model->SetMixWeight(m2, half);
+    float *mi = model->GetMixtureMean(m2), *mv = model->GetMixtureMean(mn);
+    const float *vi = model->GetMixtureVar(m2);
+    std::vector<float> sh(D);
+    if (model->GetFullCovStatus()) {
+        double lam = 0.0;
+        std::vector<double> ev;
+        leading_eigenpair(D, vi, lam, ev);
+        if (!(lam >= 0.0)) throw std::runtime_error("splitGaussian(): covariance matrix not positive definite!");
+        for (int d = 0; d < D; d++) sh[d] = (float)(std::sqrt(lam) * ev[d]);
+    } else {
+        for (int d = 0; d < D; d++) sh[d] = (float)(std::sqrt(vi[d]) * 0.5);
+    }
+    for (int d = 0; d < D; d++) {
+        mv[d] = mi[d] + sh[d];
+        mi[d] -= sh[d];
+    }
+}
+
+// splitMultipleGaussians + GetGaussStats (OL_ModelAdapt.cpp:590-611,
+// :951-990).  The scan runs from the last component down: the heaviest
+// component is the highest-index one among equal maxima, empty slots are
+// taken from the highest index down, and the split order is by weight,
+// descending, lower index first among equal weights.  ceil(split-X x
+// non-zero) < 2 splits the heaviest component only; otherwise components are
+// split in that order while they weigh at least split-th x the heaviest and
+// empty slots remain.  Returns false when there was no empty slot.
+static bool split_multiple(GMModel *model, float splitX, float splitTh) {
+    const int M = model->GetNumberOfMixtures();
+    int imax = 0;
+    float wm = 0.0f;
+    std::vector<int> zero, order;
+    for (int m = M - 1; m >= 0; m--) {
+        const float w = model->GetMixtureWeight(m);
+        if (wm < w) {
+            wm = w;
+            imax = m;
+        }
+        if (w == 0.0f)
+            zero.push_back(m);
+        else
+            order.push_back(m);
+    }
+    if (zero.empty()) return false;
+    const unsigned nsplit = (unsigned)std::ceil(splitX * (float)order.size());
+    if (nsplit < 2) {
+        split_component(model, imax, zero.front());
+        return true;
+    }
+    // order holds indices descending; a stable sort by weight keeps the
+    // insertion rule of GetGaussStats (an equal weight goes before the
+    // entries already placed, i.e. lower index first)
+    std::reverse(order.begin(), order.end());
+    std::vector<float> w0(M);
+    for (int m = 0; m < M; m++) w0[m] = model->GetMixtureWeight(m);
+    std::stable_sort(order.begin(), order.end(), [&w0](int a, int b) { return w0[a] > w0[b]; });
+    const float th = splitTh * model->GetMixtureWeight(imax);
+    size_t z = 0;
+    for (unsigned n = 0; n < order.size(); n++) {
+        if (model->GetMixtureWeight(order[n]) < th || z == zero.size() || n >= nsplit) break;
+        split_component(model, order[n], zero[z++]);
+    }
+    return true;
+}
+
+// TrainFromScratch (OL_ModelAdapt.cpp:663-821): EM rounds of --it-EM
+// iterations, a split after each round, up to nummix components.  --ll-Th
+// (:738-741, :781-797) needs at least two EM iterations and stops from the
+// fourth round on once a round gains less than the threshold over the better
+// of the two rounds before it, keeping the best of those models.  The mean
+// log-likelihood of the last EM iteration is computed at every verbosity
+// here; the reference computes it only when verbose (:766-769), so at
+// verbosity 0 its check compares -FLT_MAX with itself and stops at the
+// fourth round whatever the data.
+void CModelAdapt::TrainFromScratch(unsigned int nummix, bool fullcov, const char *FileName,
+                                   bool save_txt) {
+    if (!hasData() || N_ <= 3)
         throw std::runtime_error("TrainFromScratch(): Not enough data to model training.");
     const int D = (int)D_;
     // Global statistics: a single component with a broad variance has
@@ -811,15 +926,22 @@ void CModelAdapt::TrainFromScratch(unsigned int nummix, bool fullcov) {
     nummix_ = (int)nummix;
     stats_.alloc(D, nummix, true, true, fullcov, robust);
     const unsigned int maxIter = nummix < 2 ? 0 : nummix;
+    int iterEM = opt_._iterCountEM;
+    if (iterEM < 2 && opt_._llTh > 0.0f) {
+        std::cout << " (num of EM iteartions < 2 -> it-EM set to 2)" << std::endl;
+        iterEM = 2;
+    }
+    float ll[3] = {-std::numeric_limits<float>::max(), -std::numeric_limits<float>::max(),
+                   -std::numeric_limits<float>::max()};
+    GMModel m2, m3;
     unsigned int nonzero = 1;
     GMModel *model = model_.get();
     for (unsigned int it = 0; it < maxIter; it++) {
-        for (int e = 0; e < opt_._iterCountEM; e++) {
+        for (int e = 0; e < iterEM; e++) {
             stats_.reset();
             AccumulateStats(model, stats_, false, true, true, robust);
-            if (opt_._verbosity)
-                std::cout << "     [LL = " << stats_.getTotLogLike() / (double)stats_.getTotAccSamples()
-                          << "]" << std::endl;
+            ll[0] = (float)(stats_.getTotLogLike() / (double)stats_.getTotAccSamples());
+            if (opt_._verbosity) std::cout << "     [LL = " << ll[0] << "]" << std::endl;
             if (nonzero > 1) nonzero = UpdateModelEM(model, stats_);
             if (nonzero < 2) {
                 if (nonzero < 1) *model = init;
@@ -827,40 +949,26 @@ void CModelAdapt::TrainFromScratch(unsigned int nummix, bool fullcov) {
             }
         }
         if (nonzero == 0 || nonzero == nummix) break;
+        if (opt_._llTh > 0.0f) {
+            const float mx = std::max(ll[1], ll[2]);
+            if (it > 2 && ll[0] - mx < opt_._llTh) {
+                if (mx > ll[0]) *model = (mx == ll[1]) ? m2 : m3;
+                break;
+            }
+            m3 = m2;
+            m2 = *model;
+            ll[2] = ll[1];
+            ll[1] = ll[0];
+        }
+        if (opt_._saveModelAfterNSplits > 0 && (it + 1) % (unsigned)opt_._saveModelAfterNSplits == 0 &&
+            FileName) {
+            char tag[64];
+            std::snprintf(tag, sizeof(tag), ".it%05u.G%05u.gmm.bckup", it, nonzero);
+            if (model->Save((std::string(FileName) + tag).c_str(), save_txt) != 0)
+                throw std::runtime_error(model->LastError());
+        }
         if (it < maxIter - 1) {
-            int imax = 0;
-            float wm = 0.0f;
-            for (int m = (int)nummix - 1; m >= 0; m--)
-                if (wm < model->GetMixtureWeight(m)) {
-                    wm = model->GetMixtureWeight(m);
-                    imax = m;
-                }
-            int znew = -1;
-            for (int m = (int)nummix - 1; m >= 0 && znew < 0; m--)
-                if (model->GetMixtureWeight(m) == 0) znew = m;
-            if (znew < 0) break;
-            std::memcpy(model->GetMixtureVar(znew), model->GetMixtureVar(imax),
-                        sizeof(float) * model->GetDimVar());
-            const float half = model->GetMixtureWeight(imax) / 2.0f;
-            model->SetMixWeight(znew, half);
-            model->SetMixWeight(imax, half);
-            float *mi = model->GetMixtureMean(imax), *mn = model->GetMixtureMean(znew);
-            const float *vi = model->GetMixtureVar(imax);
-            std::vector<float> sh(D);
-            if (fullcov) {
-                double lam = 0.0;
-                std::vector<double> ev;
-                leading_eigenpair(D, vi, lam, ev);
-                if (!(lam >= 0.0))
-                    throw std::runtime_error("splitGaussian(): covariance matrix not positive definite!");
-                for (int d = 0; d < D; d++) sh[d] = (float)(std::sqrt(lam) * ev[d]);
-            } else {
-                for (int d = 0; d < D; d++) sh[d] = (float)(std::sqrt(vi[d]) * 0.5);
-            }
-            for (int d = 0; d < D; d++) {
-                mn[d] = mi[d] + sh[d];
-                mi[d] -= sh[d];
-            }
+            if (!split_multiple(model, opt_._splitXpercAtOnce, opt_._splitTh)) break;
             nonzero += 1;
         }
     }

@@ -50,6 +50,15 @@ public:
         bool _robustCov = false;  // robust covariances from the auxiliary statistics
         int _iter_fMLLR = 20;     // row-update sweeps of fMLLR
         std::string _transformFileExt = ".mat";
+        // TrainFromScratch (OL_ModelAdapt.h:69-80, :212-229): split-X, the
+        // fraction of the non-zero components split at once per round
+        // (those heavier than split-th times the heaviest); ll-Th, stop when
+        // the mean log-likelihood gains less than this over the better of
+        // the two previous rounds; tmp-N, save a backup model every N rounds.
+        float _splitXpercAtOnce = 0.0f;
+        float _splitTh = 0.0f;
+        float _llTh = 0.0f;
+        int _saveModelAfterNSplits = 0;
     };
 
     CModelAdapt() = default;
@@ -64,13 +73,19 @@ public:
     // are scanned once here for their frame count and dimension.
     void InsertFiles(const std::vector<std::string> &files, int prm_type, unsigned int dwnsmp,
                      float buffGB);
+    // Frame intervals of files, read at every accumulation (the reference's
+    // InsertData(file_2_ranges*) of the MLF mode, OL_ModelAdapt.cpp:321-328,
+    // accumulated by the file-to-ranges accumulateStatsMT, :440-443).
+    void InsertRanges(const file_2_ranges &ranges, int prm_type, unsigned int samplePeriod,
+                      float buffGB);
 
     // MAP / MLLR / MLLR_MAP adaptation of UBM (AdaptModel, :1163-1247).
     void AdaptModel(GMModel *UBM);
     // EM from scratch with binary splitting up to nummix components
-    // (TrainFromScratch, :673-821, no log-lik threshold), diagonal or full
-    // covariance.
-    void TrainFromScratch(unsigned int nummix, bool fullcov = false);
+    // (TrainFromScratch, :673-821), diagonal or full covariance.  FileName
+    // names the backups of --tmp-N (FileName.itNNNNN.GNNNNN.gmm.bckup).
+    void TrainFromScratch(unsigned int nummix, bool fullcov = false, const char *FileName = nullptr,
+                          bool save_txt = false);
 
     GMModel *GetClientModel() { return model_.get(); }
     const MLLRTransform *GetTransform() const { return W_.get(); }
@@ -117,6 +132,8 @@ private:
     float **data_ = nullptr;
     unsigned int N_ = 0, D_ = 0;
     std::vector<std::string> files_;  // file-by-file mode when non-empty
+    file_2_ranges ranges_;            // file-by-file interval mode when non-empty
+    unsigned int samplePeriod_ = 0;
     int fileType_ = 2;
     unsigned int dwnsmp_ = 1;
     float buffGB_ = 0.0f;
@@ -129,6 +146,7 @@ private:
     int nummix_ = 0;
 
     void initEstimator();
+    bool hasData() const { return data_ || !files_.empty() || !ranges_.empty(); }
     std::unique_ptr<GMModel> AdaptModelMAP(GMModel *ubm);
     void IterEM(GMModel *model, int iters);
     void IterMAP(GMModel *model, int iters);

@@ -54,6 +54,8 @@ public:
         }
     }
     bool has(const std::string &n) const { return values_.count(n) || defaults_.count(n); }
+    // given on the command line (defaults aside)
+    bool given(const std::string &n) const { return values_.count(n) > 0; }
     std::string str(const std::string &n) const {
         auto it = values_.find(n);
         if (it != values_.end()) return it->second.back();

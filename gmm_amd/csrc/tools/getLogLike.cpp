@@ -58,10 +58,12 @@ int main(int argc, char **argv) {
         for (const std::string &fn : a.all("in-file")) {
             Features f;
             std::string err;
-            if (LoadFeatures(fn.c_str(), a.i("data-T"), f, &err) != 0) {
+            const int rc = LoadFeatures(fn.c_str(), a.i("data-T"), f, &err);
+            if (rc == -1) {
                 std::cerr << "WARNING: Param file [" << fn << "] not found -> skipped!" << std::endl;
                 continue;
             }
+            if (rc != 0) throw std::runtime_error(err);
             if (f.N < 1) continue;
             ll.resize(f.N);
             est.compVecsLogLikeMT(f.rows.data(), f.N, f.D, ll.data());

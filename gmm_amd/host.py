@@ -1,5 +1,6 @@
 """ctypes binding of the host-layer C ABI (include/gmmhost.h): the `.gmm`
-model file, RearrangeMixtures and the EM / MAP / MLLR update rules."""
+model file, RearrangeMixtures, the EM / MAP / MLLR update rules and the
+SVES / HTK / RAW feature files."""
 import ctypes
 
 import numpy as np
@@ -12,7 +13,13 @@ HOST_EXPORTS = [
     "gmmhost_model_dims", "gmmhost_model_get", "gmmhost_model_rearrange",
     "gmmhost_model_free", "gmmhost_update_em", "gmmhost_update_map",
     "gmmhost_mllr_transform", "gmmhost_mllr_apply", "gmmhost_last_error",
+    "gmmhost_fmllr_transform", "gmmhost_update_em_robust", "gmmhost_make_robust_stats",
+    "gmmhost_features_load", "gmmhost_features_load_ranges", "gmmhost_features_dims",
+    "gmmhost_features_get", "gmmhost_features_free", "gmmhost_features_save",
 ]
+
+SVES, HTK, RAW = 0, 1, 2
+FEAT_MISSING = 2
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -53,17 +60,30 @@ def _L():
                                                ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.gmmhost_make_robust_stats.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                                 _dp, _dp, _dp]
+        u32 = ctypes.c_uint32
+        L.gmmhost_features_load.argtypes = [ctypes.c_char_p, ctypes.c_int, u32, ctypes.POINTER(vp)]
+        L.gmmhost_features_load_ranges.argtypes = [ctypes.c_char_p, ctypes.c_int,
+                                                   ctypes.POINTER(u32), ctypes.c_int, u32,
+                                                   ctypes.POINTER(vp)]
+        L.gmmhost_features_dims.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                            ctypes.POINTER(u32)]
+        L.gmmhost_features_get.argtypes = [vp, _fp]
+        L.gmmhost_features_free.argtypes = [vp]
+        L.gmmhost_features_free.restype = None
+        L.gmmhost_features_save.argtypes = [ctypes.c_char_p, ctypes.c_int, _fp, u32, u32, u32]
         _configured = True
     return L
 
 
 class HostError(RuntimeError):
-    pass
+    def __init__(self, msg, code=1):
+        super().__init__(msg)
+        self.code = code
 
 
 def _chk(rc):
     if rc != 0:
-        raise HostError("gmmhost error %d: %s" % (rc, _L().gmmhost_last_error().decode()))
+        raise HostError("gmmhost error %d: %s" % (rc, _L().gmmhost_last_error().decode()), rc)
 
 
 def _f(a):
@@ -193,3 +213,43 @@ def make_robust_stats(n, F, S, aux, full=False):
     _chk(_L().gmmhost_make_robust_stats(M, D, int(full), _pd(n), _pd(F), _pd(S), _pd(aux),
                                         _pd(nind)))
     return S, nind
+
+
+def _take_features(h):
+    u32 = ctypes.c_uint32
+    N, D, P = u32(), u32(), u32()
+    try:
+        _chk(_L().gmmhost_features_dims(h, ctypes.byref(N), ctypes.byref(D), ctypes.byref(P)))
+        x = np.zeros((N.value, D.value), np.float32)
+        _chk(_L().gmmhost_features_get(h, x.ctypes.data_as(_fp)))
+    finally:
+        _L().gmmhost_features_free(h)
+    return x, P.value
+
+
+def load_features(path, kind, dwnsmp=1):
+    """(frames [N, D] float32, sample period) of a SVES / HTK / RAW file,
+    every dwnsmp-th frame (Param::Load).  HostError with code FEAT_MISSING
+    when the file cannot be opened."""
+    h = ctypes.c_void_p()
+    _chk(_L().gmmhost_features_load(str(path).encode(), int(kind), int(dwnsmp), ctypes.byref(h)))
+    return _take_features(h)
+
+
+def load_feature_ranges(path, kind, ranges, sample_period=0):
+    """Frames of the intervals [x0, x1) (sample-period units) of one file,
+    (Param::Load with frame intervals)."""
+    r = np.ascontiguousarray(np.asarray(ranges, np.uint32).reshape(-1, 2))
+    h = ctypes.c_void_p()
+    _chk(_L().gmmhost_features_load_ranges(str(path).encode(), int(kind),
+                                           r.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                           r.shape[0], int(sample_period), ctypes.byref(h)))
+    return _take_features(h)
+
+
+def save_features(path, kind, x, sample_period=100000):
+    x = _f(x)
+    if x.ndim != 2:
+        raise ValueError("frames must be [N, D]")
+    _chk(_L().gmmhost_features_save(str(path).encode(), int(kind), x.ctypes.data_as(_fp),
+                                    x.shape[0], x.shape[1], int(sample_period)))

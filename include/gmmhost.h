@@ -1,8 +1,8 @@
 /*
  * gmmhost.h -- C ABI of the host layer around the gfx950 accumulator: the
  * `.gmm` model file, the RearrangeMixtures component order and the EM / MAP /
- * MLLR update rules that turn statistics into models.  No device work; these
- * entry points run on any host.  Reference paths are relative to
+ * MLLR update rules that turn statistics into models, and the feature files
+ * the trainer reads.  No device work; these entry points run on any host.  Reference paths are relative to
  * GMM/lib_bmssr/ of GMMTeam/GMM.
  */
 #ifndef GMMHOST_H
@@ -85,6 +85,34 @@ int gmmhost_fmllr_transform(gmmhost_model_t m, const double *n, const double *F,
                             uint64_t tot_samples, int iters, double *W);
 /* AdaptModelMLLR (:1583-1612): means of m replaced by A mu + b. */
 int gmmhost_mllr_apply(gmmhost_model_t m, const double *W);
+
+/* Feature files (param/OL_Param.h:59-127).  type: 0 SVES, 1 HTK, 2 RAW
+ * (--data-T).  Loaders return 0, GMMHOST_FEAT_MISSING when the file cannot
+ * be opened (the reference returns -1 and its callers skip the file) or 1
+ * for a malformed / truncated file or bad interval (the reference throws);
+ * an empty file loads as N = D = 0. */
+#define GMMHOST_FEAT_MISSING 2
+typedef struct gmmhost_features *gmmhost_features_t;
+/* Param::Load(filename, loadType, dwnsmp) (OL_Param.cpp:222-244; SVES
+ * :246-354, RAW :358-453, HTK :477-576): every dwnsmp-th frame from the
+ * first (dwnsmp 0 or 1: all). */
+int gmmhost_features_load(const char *path, int type, uint32_t dwnsmp, gmmhost_features_t *out);
+/* Param::Load with frame intervals (OL_Param.cpp:581-1037): nranges
+ * intervals [ranges[2k], ranges[2k+1]) in sample-period units, ascending;
+ * the period is the HTK header's, or sample_period for SVES and RAW.  The
+ * reference's HTK variant (:897-1000) is followed for every type (its SVES
+ * and RAW variants count each interval twice). */
+int gmmhost_features_load_ranges(const char *path, int type, const uint32_t *ranges, int nranges,
+                                 uint32_t sample_period, gmmhost_features_t *out);
+/* Frames, dimension and (HTK) sample period of loaded features. */
+int gmmhost_features_dims(gmmhost_features_t f, uint32_t *N, uint32_t *D, uint32_t *sample_period);
+/* Copy the N x D frames (row-major) into x. */
+int gmmhost_features_get(gmmhost_features_t f, float *x);
+void gmmhost_features_free(gmmhost_features_t f);
+/* Write N x D frames: SVES (Param::Save, OL_Param.cpp:177-220, confidence
+ * factors 1), HTK (the layout LoadHTK reads, parmKind USER) or RAW. */
+int gmmhost_features_save(const char *path, int type, const float *x, uint32_t N, uint32_t D,
+                          uint32_t sample_period);
 
 const char *gmmhost_last_error(void);
 

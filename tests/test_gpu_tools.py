@@ -311,3 +311,149 @@ def test_file_by_file_matches_resident(tmp_path, golden, buff):
     wd, md, vd, _ = fm.read_gmm_txt(d_out)
     assert np.allclose(wc, wd, rtol=2e-5) and np.allclose(mc, md, rtol=2e-5, atol=1e-5)
     assert np.allclose(vc, vd, rtol=2e-5)
+
+
+def _clustered(seed=5, M=12, D=6, N=12000):
+    rng = np.random.default_rng(seed)
+    w = rng.dirichlet(np.ones(M) * 3)
+    mu = rng.normal(0, 4, (M, D)).astype(np.float32)
+    var = rng.uniform(0.3, 1.0, (M, D)).astype(np.float32)
+    return synth.sample_frames_np(w, mu, var, N, 9)
+
+
+def _cmp_models(path, ref, what):
+    w, mu, var, _ = fm.read_gmm_txt(path)
+    # the model file holds the components of non-zero weight only
+    # (GMModel::WriteTXT, model/OL_GMModel.cpp:497-505)
+    keep = ref[0] > 0
+    rw, rmu, rvar = ref[0][keep], ref[1][keep], ref[2][keep]
+    assert w.shape == rw.shape, (what, w.shape, rw.shape)
+    dw = np.abs(w - rw).max() / rw.max()
+    dmu = np.abs(mu - rmu).max() / np.abs(rmu).max()
+    dvar = (np.abs(var - rvar) / rvar).max()
+    print("%s: max diff w %.2e mu %.2e var %.2e" % (what, dw, dmu, dvar))
+    # FP32 E-step against the FP64 oracle over ~30 EM iterations of
+    # well-separated clusters (the file keeps 7 digits)
+    assert dw < 1e-4 and dmu < 1e-4 and dvar < 1e-4, what
+
+
+@pytest.mark.parametrize("split_x,split_th", [(0.5, 0.2), (1.0, 0.0)])
+def test_train_from_scratch_split_x(tmp_path, split_x, split_th):
+    """--split-X / --split-th (splitMultipleGaussians, OL_ModelAdapt.cpp:
+    590-611): several components split per round, in GetGaussStats' weight
+    order, against the oracle's split driver."""
+    import _scratch_driver as sd
+    x = _clustered()
+    fm.write_raw(tmp_path / "x.raw", x)
+    out = tmp_path / "m.gmm"
+    run("trainGMM", "-i", tmp_path / "x.raw", "--data-T", 2, "--type", 0, "--nummix", 16,
+        "--it-EM", 4, "--split-X", split_x, "--split-th", split_th, "-o", out, "-T")
+    hist = []
+    ref = sd.train_from_scratch(x, 16, 4, split_x=split_x, split_th=split_th, history=hist)
+    # more than one split per round happened
+    assert len(hist) < 16 and hist[-1][2] == 16, hist
+    _cmp_models(out, ref, "split-X %g" % split_x)
+
+
+def test_train_from_scratch_ll_threshold_and_backups(tmp_path):
+    """--ll-Th (OL_ModelAdapt.cpp:781-797) stops the splitting once a round
+    gains less than the threshold over the better of the two rounds before;
+    --tmp-N writes OUT.itNNNNN.GNNNNN.gmm.bckup every N rounds.  The oracle's
+    driver gives per-round gains of >= 0.17 up to round 13 and 4e-4 at round
+    14, so a threshold of 0.05 stops at round 14 (15 components) with a
+    wide margin on both sides."""
+    import _scratch_driver as sd
+    x = _clustered()
+    fm.write_raw(tmp_path / "x.raw", x)
+    out = tmp_path / "m.gmm"
+    run("trainGMM", "-i", tmp_path / "x.raw", "--data-T", 2, "--type", 0, "--nummix", 32,
+        "--it-EM", 3, "--ll-Th", 0.05, "--tmp-N", 5, "-o", out, "-T")
+    hist = []
+    ref = sd.train_from_scratch(x, 32, 3, ll_th=0.05, history=hist)
+    assert hist[-1][0] == 14 and (ref[0] > 0).sum() == 15, hist
+    _cmp_models(out, ref, "ll-Th")
+    for it, _, nz in hist:
+        name = "%s.it%05d.G%05d.gmm.bckup" % (out, it, nz)
+        assert os.path.exists(name) == ((it + 1) % 5 == 0 and it < 14), name
+
+
+def _mlf_case(tmp_path, kind):
+    """Three feature files (two under a sub-directory), an MLF labelling
+    their frames with 'sil', 'a' and 'b', and a classes file."""
+    x = _clustered(N=1500)
+    parts = {"utt1": x[:400], "sub/utt2": x[400:1000], "sub/utt3": x[1000:]}
+    data = tmp_path / "data"
+    (data / "sub").mkdir(parents=True)
+    ext = ".htk" if kind == host.HTK else ".prm"
+    for name, p in parts.items():
+        host.save_features(data / (name + ext), kind, p, sample_period=100000)
+    # label times in 100 ns: frame k starts at k * 100000
+    labels = {
+        "utt1": [(0, 50, "sil"), (50, 230, "a"), (230, 400, "b")],
+        "sub/utt2": [(0, 120, "b"), (120, 130, "sil"), (130, 600, "a")],
+        "sub/utt3": [(0, 90, "a"), (90, 500, "sil")],
+    }
+    lines = ["#!MLF!#"]
+    for name, labs in labels.items():
+        lines.append('"*/%s.lab"' % name)
+        lines += ["%d %d %s" % (b * 100000, e * 100000, lab) for b, e, lab in labs]
+        lines.append(".")
+    (tmp_path / "labels.mlf").write_text("\n".join(lines) + "\n")
+    (tmp_path / "classes.txt").write_text("speech a b\nsilence sil\nnothing zz\n")
+    # the frames of each class, files in path order, intervals in time order
+    want = {}
+    for cls, elems in (("speech", ("a", "b")), ("silence", ("sil",))):
+        rows = []
+        for name in sorted(parts, key=lambda n: str(data / (n + ext))):
+            for b, e, lab in labels[name]:
+                if lab in elems:
+                    rows.append(parts[name][b:e])
+        want[cls] = np.concatenate(rows)
+    return data, ext, want
+
+
+@pytest.mark.parametrize("kind", [host.HTK, host.SVES])
+@pytest.mark.parametrize("by_file", [False, True])
+def test_mlf_mode_trains_one_model_per_class(tmp_path, kind, by_file):
+    """MLF mode (projects/Model_ADAPT/Model_ADAPT.cpp:97-240, :478-513;
+    tools/OL_MLFParser.cpp:57-163): one model per class from the labelled
+    intervals of its elements, resident or read file by file at every
+    accumulation through the file-to-ranges accumulateStatsMT
+    (OL_GMMStatsEstimator.cpp:712-780).  Each model is the one trainGMM
+    trains from a RAW file of exactly that class's frames, byte for byte;
+    a class without labels gets no model."""
+    data, ext, want = _mlf_case(tmp_path, kind)
+    out = tmp_path / "models"
+    out.mkdir()
+    common = ["--type", 0, "--nummix", 4, "--it-EM", 5, "-T"]
+    args = ["-S", tmp_path / "classes.txt", "-i", tmp_path / "labels.mlf", "-e", ext, "-I", data,
+            "--data-T", kind, "-D", out, "-x", ".gmm"] + common
+    if kind != host.HTK:
+        args += ["--sample-period", 100000]
+    if by_file:
+        args += ["-f"]
+    run("trainGMM", *args)
+    assert sorted(p.name for p in out.iterdir()) == ["silence.gmm", "speech.gmm"]
+    for cls, x in want.items():
+        fm.write_raw(tmp_path / (cls + ".raw"), x)
+        ref = tmp_path / (cls + "_ref.gmm")
+        run("trainGMM", "-i", tmp_path / (cls + ".raw"), "--data-T", 2, "-o", ref, *common)
+        assert (out / (cls + ".gmm")).read_bytes() == ref.read_bytes(), cls
+
+
+def test_class_mode_trains_from_whole_files(tmp_path):
+    """CLASS mode: 'CLASS file1 file2 ...' trains each class from whole
+    files dir/fileK.ext."""
+    data, ext, _ = _mlf_case(tmp_path, host.HTK)
+    (tmp_path / "cls.txt").write_text("one utt1\ntwo sub/utt2 sub/utt3\n")
+    out = tmp_path / "models"
+    out.mkdir()
+    common = ["--type", 0, "--nummix", 2, "--it-EM", 4, "-T"]
+    run("trainGMM", "-S", tmp_path / "cls.txt", "-e", ext, "-I", data, "--data-T", 1, "-D", out,
+        *common)
+    x = _clustered(N=1500)
+    for cls, rows in (("one", x[:400]), ("two", x[400:])):
+        fm.write_raw(tmp_path / (cls + ".raw"), rows)
+        ref = tmp_path / (cls + "_ref.gmm")
+        run("trainGMM", "-i", tmp_path / (cls + ".raw"), "--data-T", 2, "-o", ref, *common)
+        assert (out / (cls + ".gmm")).read_bytes() == ref.read_bytes(), cls
