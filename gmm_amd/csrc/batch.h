@@ -14,6 +14,11 @@ struct SegReduceArgs {
     int U, M, Mp, D, KP;
     double *n;               // [U][M]
     double *F;               // [U][M][D]
+    // MAP means of the first Umeans utterances (null: none)
+    const float *mu;         // [M][D] UBM means
+    int tau;                 // relevance factor
+    int Umeans;
+    float *means;            // [Umeans][M][D]
 };
 
 struct MllrSolveArgs {
@@ -54,8 +59,6 @@ hipError_t launch_pad_rows(const float *X, const uint64_t *src, const uint64_t *
                            float *Xp, hipStream_t s);
 hipError_t launch_mllr_gk(const MllrGkArgs &a, hipStream_t s);
 hipError_t launch_seg_reduce(const SegReduceArgs &a, hipStream_t s);
-hipError_t launch_map_means(const double *n, const double *F, const float *mu, int U, int M, int D,
-                            int tau, float *out, hipStream_t s);
 hipError_t launch_mllr_check(const double *n, int U, int M, int D, int *status, hipStream_t s);
 hipError_t launch_mllr_solve(const MllrSolveArgs &a, hipStream_t s);
 // out[u] = sum of v[begin[u] .. end[u]) in FP64; the summation order is

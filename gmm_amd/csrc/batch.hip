@@ -60,10 +60,14 @@ hipError_t launch_fill_u32(unsigned int *p, unsigned int v, int n, hipStream_t s
 // ---------------------------------------------------------------------------
 // Per-utterance reduction of the chunk partials [chunk][Mp][KP] (columns
 // [1, x', x'^2] of centred frames) into n[u][m] and un-centred F[u][m][d],
-// FP64, chunks of an utterance summed in order.  A block of 4 waves takes 4
-// components of one utterance (blockIdx.y, strided by gridDim.y), wave w
-// component 4 blockIdx.x + w and lane j the columns j, j + 64, ... of
-// 0..D, so no thread divides an index.
+// FP64, chunks of an utterance summed in order, and, for the first Umeans
+// utterances, the MAP means of UpdateModelMAP (trainer/OL_ModelAdapt.cpp:
+// 993-1106; means only): alpha = n/(n+tau), mu' = alpha F/n + (1-alpha) mu
+// for n > 0, the UBM mean otherwise.  A block of 4 waves takes 4 components
+// of one utterance (blockIdx.y, strided by gridDim.y), wave w component
+// 4 blockIdx.x + w and lane j the columns j, j + 64, ... of 0..D, so no
+// thread divides an index; every lane sums the occupancy column itself, in
+// the same order, so each has n without a broadcast.
 __global__ __launch_bounds__(256) void seg_reduce_kernel(SegReduceArgs a) {
     const int m = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     if (m >= a.M) return;
@@ -85,33 +89,20 @@ __global__ __launch_bounds__(256) void seg_reduce_kernel(SegReduceArgs a) {
                 // x = x' + o (FP32 form) or x = z / scale + o (mixed form):
                 // F = F' + n o or F = F' / scale + n o
                 const double f = a.scale ? s / (double)a.scale[j - 1] : s;
-                a.F[um * a.D + (j - 1)] = f + n * (double)a.off[j - 1];
+                const double Fv = f + n * (double)a.off[j - 1];
+                a.F[um * a.D + (j - 1)] = Fv;
+                if (u < a.Umeans) {
+                    const float ub = a.mu[(size_t)m * a.D + (j - 1)];
+                    float v = ub;
+                    if (n > 0.0) {
+                        const double alfa = n / (n + (double)a.tau);
+                        v = (float)(alfa * Fv / n + (1.0 - alfa) * (double)ub);
+                    }
+                    a.means[um * a.D + (j - 1)] = v;
+                }
             }
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// MAP mean adaptation, alpha = n/(n+tau), mu' = alpha F/n + (1-alpha) mu for
-// n > 0, the UBM mean otherwise.  One thread per (u, m, d).
-__global__ __launch_bounds__(256) void map_means_kernel(const double *__restrict__ n,
-                                                        const double *__restrict__ F,
-                                                        const float *__restrict__ mu, int U, int M,
-                                                        int D, int tau, float *__restrict__ out) {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint64_t total = (uint64_t)U * M * D;
-    if (i >= total) return;
-    const uint64_t um = i / D;
-    const int d = (int)(i - um * D);
-    const int m = (int)(um % M);
-    const double nm = n[um];
-    const float ub = mu[(size_t)m * D + d];
-    float v = ub;
-    if (nm > 0.0) {
-        const double alfa = nm / (nm + (double)tau);
-        v = (float)(alfa * F[i] / nm + (1.0 - alfa) * (double)ub);
-    }
-    out[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -384,118 +375,204 @@ hipError_t launch_seg_reduce(const SegReduceArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_map_means(const double *n, const double *F, const float *mu, int U, int M, int D,
-                            int tau, float *out, hipStream_t s) {
-    const uint64_t total = (uint64_t)U * M * D;
-    hipLaunchKernelGGL(map_means_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, n,
-                       F, mu, U, M, D, tau, out);
-    return hipGetLastError();
-}
-
 hipError_t launch_mllr_check(const double *n, int U, int M, int D, int *status, hipStream_t s) {
     hipLaunchKernelGGL(mllr_check_kernel, dim3(U), dim3(256), 0, s, n, U, M, D, status);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Register-resident solve for E = D+1 <= 64.  One wave per (u, i): lane r
-// holds row r of [G | k] (G symmetric, unpacked from the packed upper
-// triangle of G plus the prior, staged in LDS) in registers.  Gauss-Jordan
-// elimination without pivoting, as the reference inverts G_i
-// (MATRIX::_Invert_Gen, Matrix4_LMa.cpp:2232; stable for SPD G, whose pivots
-// are the ratios of its leading principal minors and so all positive): at
-// step j the pivot row is broadcast element by element with v_readlane and
-// every other row subtracts its multiple of it.  Steps run in groups of 8
-// with the columns updated in place, and every row shifts left by 8 columns
-// between groups, so every register index is a compile-time constant.  After E steps lane r holds k'_r and its
-// pivot g'_rr, and w_r = k'_r / g'_rr.  A pivot <= 0 (G not positive
-// definite) marks the utterance's status.
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-    const uint64_t b = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
-    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
-template <int EMAX>
-__global__ __launch_bounds__(64) void mllr_solve_gj_kernel(MllrSolveArgs a) {
+// Blocked solve for E = D+1 <= 64 on FP64 MFMA.  One wave per (u, i).  G
+// (symmetric, unpacked from the packed upper triangle of G plus the prior,
+// staged in LDS) is padded to 64 x 64 with an identity block and held in
+// registers in the accumulator layout of v_mfma_f64_16x16x4f64: tile (I, J)
+// of 16 x 16, lane (q, c), register r = element (16 I + q + 4 r, 16 J + c);
+// the right-hand side k sits one row per lane.  Gauss-Jordan elimination
+// without pivoting, as the reference inverts G_i (MATRIX::_Invert_Gen,
+// Matrix4_LMa.cpp:2232; stable for SPD G, whose pivots are the ratios of
+// its leading principal minors and so all positive), four columns at a
+// time: step s takes the rows and columns K = 4s .. 4s+3, the 4 x 4 pivot
+// block B = A[K, K], the row block R = A[K, :] and the column block
+// P = A[:, K] (through LDS), forms R' = B^-1 R (lane j solves for column j
+// with an LU of B that every lane repeats) and
+//   A <- A - P_eff R',   P_eff = P with B - I4 in the rows K,
+// which eliminates the columns K from every other row and leaves R' in the
+// rows K, as one rank-4 MFMA update per 16 x 16 tile (tiles of finished
+// columns skipped: their R' is zero).  k takes the same update with vector
+// FMAs.  After the last step A is the identity and k holds w.  The pivots
+// of B's LU are the pivots of the unblocked elimination, so a non-positive
+// one (G not positive definite) marks the utterance's status.  Blocking turns the E^2 single-element broadcasts of a
+// one-row-per-lane elimination into 8 KB of LDS traffic per step.
+// two waves per SIMD: the compiler then keeps the 64 x 64 block in 251
+// VGPRs without spilling instead of 379 VGPR + AGPR at one wave
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mllr_solve_mfma_kernel(
+    MllrSolveArgs a) {
+    constexpr int EMAX = 64;
     __shared__ double sG[EMAX * (EMAX + 1) / 2];
     const int E = a.D + 1;
     const int NP = E * (E + 1) / 2;
     const uint64_t ui = blockIdx.x;
     const int u = (int)(ui / a.D), i = (int)(ui % a.D);
-    const int r = threadIdx.x;
+    const int lane = threadIdx.x, q = lane >> 4, c = lane & 15;
     if (a.status[u] != 0) {
-        if (r < E) a.W[ui * E + r] = 0.0;
+        if (lane < E) a.W[ui * E + lane] = 0.0;
         return;
     }
+    // all of this lane's loads in flight at once (one HBM round trip, not
+    // one per element)
+    constexpr int NIT = (EMAX * (EMAX + 1) / 2 + 63) / 64;
     const double *gp = a.Gp + ui * a.ldg;
-    if (a.Ginit) {
-        const double *gip = a.Ginit + (size_t)i * NP;
-        for (int p = r; p < NP; p += 64) sG[p] = gp[p] + gip[p];
-    } else {
-        for (int p = r; p < NP; p += 64) sG[p] = gp[p];
+    const double *gip = a.Ginit ? a.Ginit + (size_t)i * NP : nullptr;
+    double t[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+        const int p = lane + 64 * it;
+        t[it] = p < NP ? gp[p] : 0.0;
+    }
+    if (gip) {
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int p = lane + 64 * it;
+            if (p < NP) t[it] += gip[p];
+        }
+    }
+    double rhs = 0.0;
+    if (lane < E) {
+        rhs = a.Kp[ui * a.ldk + lane];
+        if (a.kinit) rhs += a.kinit[(size_t)i * E + lane];
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+        const int p = lane + 64 * it;
+        if (p < NP) sG[p] = t[it];
     }
     __syncthreads();
-    // every element is read unconditionally at clamped indices (zeroed
-    // afterwards outside the E x E system), so the reads issue back to back
-    // instead of one per branch
-    const int rr = r < E ? r : E - 1;
-    double g[EMAX];
+    f64x4 A[4][4];
 #pragma unroll
-    for (int c = 0; c < EMAX; c++) {
-        const int cc = c < E ? c : E - 1;
-        const int lo = rr < cc ? rr : cc, hi = rr < cc ? cc : rr;
-        // packed index of (lo, hi), lo <= hi: lo*E - lo(lo-1)/2 + (hi - lo)
-        g[c] = sG[lo * E - (lo * (lo - 1)) / 2 + (hi - lo)];
-    }
+    for (int I = 0; I < 4; I++)
 #pragma unroll
-    for (int c = 0; c < EMAX; c++)
-        if (!(r < E && c < E)) g[c] = 0.0;
-    double kr = a.Kp[ui * a.ldk + rr];
-    if (a.kinit) kr += a.kinit[(size_t)i * E + rr];
-    if (r >= E) kr = 0.0;
-    double diag = 1.0;
+        for (int J = 0; J < 4; J++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = 16 * I + q + 4 * r, col = 16 * J + c;
+                double v = row == col ? 1.0 : 0.0;
+                if (row < E && col < E) {
+                    const int lo = row < col ? row : col, hi = row < col ? col : row;
+                    v = sG[lo * E - (lo * (lo - 1)) / 2 + (hi - lo)];
+                }
+                A[I][J][r] = v;
+            }
+    // LDS from here on: R and R' (4 x 64 each), P (64 x 4), k of the rows K
+    double *sR = sG;
+    double *sRp = sG + 256;
+    double *sP = sG + 512;
+    double *sK = sG + 768;
     bool bad = false;
-    // steps in groups of 8: within a group, register c holds column
-    // 8 jb + c and columns are updated in place; after the group every
-    // register moves down by 8 columns
-    static_assert(EMAX % 8 == 0, "groups of 8 columns");
-    for (int jb = 0; 8 * jb < E; jb++) {
+    const int S = (E + 3) / 4;
 #pragma unroll
-        for (int jj = 0; jj < 8; jj++) {
-            const int j = 8 * jb + jj;
-            if (j < E) {
-                const double piv = readlane_f64(g[jj], j);
-                const double kj = readlane_f64(kr, j);
+    for (int s = 0; s < 16; s++) {
+        if (s < S) {
+            const int Ib = s >> 2, rr = s & 3, cb = 4 * (s & 3);
+            __syncthreads();  // the previous step's reads are done
+            // R[q][16 J + c] is register rr of tile (Ib, J): row 4s + q
+#pragma unroll
+            for (int J = 0; J < 4; J++) sR[q * 64 + 16 * J + c] = A[Ib][J][rr];
+            // P[row][t], t = c - cb, from the lanes holding columns 4s .. 4s+3
+            if (c >= cb && c < cb + 4) {
+#pragma unroll
+                for (int I = 0; I < 4; I++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) sP[(16 * I + q + 4 * r) * 4 + (c - cb)] = A[I][Ib][r];
+            }
+            if (lane >= 4 * s && lane < 4 * s + 4) sK[lane - 4 * s] = rhs;
+            __syncthreads();
+            // B = L U without pivoting (every lane; B is 4 x 4)
+            double b[4][4];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) b[t][v] = sR[t * 64 + 4 * s + v];
+            double rp[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const double piv = b[j][j];
                 bad = bad || !(piv > 0.0);
-                const double f = (r == j) ? 0.0 : g[jj] / piv;
-                if (r == j) diag = piv;
-                kr = fma(-f, kj, kr);
-                // columns right of the pivot, in blocks of 8 behind uniform
-                // branches (block cb holds columns 8 (jb + cb) ..)
+                // 1/piv: the hardware reciprocal and two Newton steps
+                const double pp = piv > 0.0 ? piv : 1.0;
+                double rc = __builtin_amdgcn_rcp(pp);
+                rc = fma(rc, fma(-pp, rc, 1.0), rc);
+                rp[j] = fma(rc, fma(-pp, rc, 1.0), rc);
 #pragma unroll
-                for (int cb = jj / 8; cb < EMAX / 8; cb++) {
-                    if (8 * (jb + cb) < E) {
+                for (int t = j + 1; t < 4; t++) {
+                    const double l = b[t][j] * rp[j];
+                    b[t][j] = l;
 #pragma unroll
-                        for (int c = (8 * cb > jj + 1 ? 8 * cb : jj + 1); c < 8 * cb + 8; c++)
-                            g[c] = fma(-f, readlane_f64(g[c], j), g[c]);
-                    }
+                    for (int v = j + 1; v < 4; v++) b[t][v] = fma(-l, b[j][v], b[t][v]);
                 }
             }
-        }
+            // x = B^-1 y: forward with the unit lower factor, backward with U
+            auto solve = [&](double y0, double y1, double y2, double y3, double *x) {
+                double z[4] = {y0, y1, y2, y3};
 #pragma unroll
-        for (int c = 0; c < EMAX - 8; c++) g[c] = g[c + 8];
+                for (int t = 1; t < 4; t++)
+#pragma unroll
+                    for (int j = 0; j < t; j++) z[t] = fma(-b[t][j], z[j], z[t]);
+#pragma unroll
+                for (int t = 3; t >= 0; t--) {
+                    double v = z[t];
+#pragma unroll
+                    for (int j = t + 1; j < 4; j++) v = fma(-b[t][j], x[j], v);
+                    x[t] = v * rp[t];
+                }
+            };
+            // R'[:, lane] = B^-1 R[:, lane]
+            double x[4];
+            solve(sR[lane], sR[64 + lane], sR[128 + lane], sR[192 + lane], x);
+#pragma unroll
+            for (int t = 0; t < 4; t++) sRp[t * 64 + lane] = x[t];
+            // k: rhs[row] -= P_eff[row][:] . (B^-1 k_K)
+            double xk[4];
+            solve(sK[0], sK[1], sK[2], sK[3], xk);
+            double rk = 0.0;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const double pv = sP[lane * 4 + t] - (lane == 4 * s + t ? 1.0 : 0.0);
+                rk = fma(pv, xk[t], rk);
+            }
+            rhs -= rk;
+            __syncthreads();
+            // B operand R'[q][16 J + c], A operand -P_eff[16 I + c][q]
+            double bop[4] = {0.0, 0.0, 0.0, 0.0}, aop[4];
+#pragma unroll
+            for (int J = 0; J < 4; J++)
+                if (J >= Ib) bop[J] = sRp[q * 64 + 16 * J + c];
+#pragma unroll
+            for (int I = 0; I < 4; I++) {
+                const int row = 16 * I + c;
+                aop[I] = (row == 4 * s + q ? 1.0 : 0.0) - sP[row * 4 + q];
+            }
+            // the tiles the next step reads first (its pivot tile row and
+            // column), so that step can start while the rest still run
+            const int In = (s + 1) >> 2;
+#pragma unroll
+            for (int pass = 0; pass < 2; pass++)
+#pragma unroll
+                for (int I = 0; I < 4; I++)
+#pragma unroll
+                    for (int J = 0; J < 4; J++) {
+                        if (J < Ib || ((I == In || J == In) != (pass == 0))) continue;
+                        A[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[I], bop[J], A[I][J], 0, 0, 0);
+                    }
+        }
     }
-    if (r < E) a.W[ui * E + r] = kr / diag;
-    if (bad && r == 0) atomicOr(&a.status[u], 8);
+    if (lane < E) a.W[ui * E + lane] = rhs;
+    if (bad && lane == 0) atomicOr(&a.status[u], 8);
 }
 
 hipError_t launch_mllr_solve(const MllrSolveArgs &a, hipStream_t s) {
     const int E = a.D + 1;
     const dim3 grid((unsigned)((uint64_t)a.U * a.D));
     if (E <= 64) {
-        hipLaunchKernelGGL(mllr_solve_gj_kernel<64>, grid, dim3(64), 0, s, a);
+        hipLaunchKernelGGL(mllr_solve_mfma_kernel, grid, dim3(64), 0, s, a);
     } else {
         const size_t shm = ((size_t)E * (E + 1) + E) * sizeof(double);
         hipLaunchKernelGGL(mllr_solve_kernel, grid, dim3(64), shm, s, a);

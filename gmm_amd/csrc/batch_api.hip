@@ -6,8 +6,8 @@
 //     of the utterance),
 //   stats_queue_kernel: a persistent grid pulls (chunk, component group)
 //     items from a device counter; chunks never straddle utterances,
-//   seg_reduce_kernel: per-utterance FP64 sums of the chunk partials,
-//   map_means_kernel, and for MLLR mllr_check / two FP64 GEMMs (G, k)
+//   seg_reduce_kernel: per-utterance FP64 sums of the chunk partials and
+//     the MAP means, and for MLLR mllr_check / two FP64 GEMMs (G, k)
 //     against UBM-only matrices with the A operand formed from n, F and
 //     1/var while staged / mllr_solve.
 // With opts.precision fp16x3 / bf16x3 the E-step part runs the mixed form:
@@ -337,9 +337,11 @@ int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
     r.KP = p.mixed ? p.KPm : p.KP;
     r.n = b->d_n;
     r.F = b->d_F;
+    r.mu = h->d_mu_f32;
+    r.tau = p.tau;
+    r.Umeans = p.want_means ? p.U : 0;
+    r.means = b->d_means;
     HIPCHK(h, launch_seg_reduce(r, s));
-    if (p.want_means)
-        HIPCHK(h, launch_map_means(b->d_n, b->d_F, h->d_mu_f32, p.U, p.M, p.D, p.tau, b->d_means, s));
     if (p.want_mllr) {
         const int E = p.D + 1, NP = E * (E + 1) / 2;
         const int R = p.U * p.D;

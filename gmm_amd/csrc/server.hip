@@ -15,8 +15,8 @@
 //               chunks of the MAP and MLLR requests only;
 //   mixed form: prep_frames (z = (x - o) * s split hi/lo), loglik_mixed,
 //               stats_mixed over the MAP/MLLR chunks;
-//   seg_reduce (n, F per request), seg_sum (LL per request),
-//   map_means (MAP requests), mllr_check / the two FP64 G, k GEMMs /
+//   seg_reduce (n, F per request, MAP means of the MAP requests), seg_sum
+//   (LL per request), mllr_check / the two FP64 G, k GEMMs /
 //   mllr_solve (MLLR requests),
 //   one D2H copy of [LL | MLLR status | n | W | means | per-frame LL].
 // Chunks start at each request's first frame, the mixed form's centring o
@@ -377,13 +377,14 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
         r.KP = rKP;
         r.n = (double *)(A + L.n);
         r.F = (double *)(A + L.F);
+        r.mu = s->mo.mu;
+        r.tau = s->cfg.tau;
+        r.Umeans = sh.Umap;
+        r.means = (float *)(A + L.means);
         SRV_HIPCHK(launch_seg_reduce(r, st));
     }
     SRV_HIPCHK(launch_seg_sum(LLnat, (const uint64_t *)(A + L.beg), (const uint64_t *)(A + L.end),
                               sh.U, (double *)(A + L.ll), st));
-    if (sh.Umap > 0)
-        SRV_HIPCHK(launch_map_means((const double *)(A + L.n), (const double *)(A + L.F), s->mo.mu,
-                                    sh.Umap, d.M, d.D, s->cfg.tau, (float *)(A + L.means), st));
     if (sh.Umllr > 0) {
         const double *n = (const double *)(A + L.n) + (size_t)sh.Umap * d.M;
         const double *F = (const double *)(A + L.F) + (size_t)sh.Umap * d.M * d.D;
