@@ -27,7 +27,8 @@ COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
           "-Wno-unused-result", "-Wno-unused-value"]
 
 HIP_SOURCES = ["estep_fp32.hip", "estep_mixed.hip", "batch.hip", "gmmhip_api.hip",
-               "batch_api.hip", "server.hip", "fullcov.hip", "auxstats.hip", "noalloc.hip"]
+               "batch_api.hip", "server.hip", "fullcov.hip", "auxstats.hip", "noalloc.hip",
+               "gemm_mfma.hip"]
 HOST_SOURCES = ["host/gmm_model.cpp", "host/model_adapt.cpp", "host/feature_io.cpp",
                 "host/gmmhost_api.cpp", "tune.cpp"]
 TOOLS = ["trainGMM", "getLogLike", "serveLoad"]
@@ -76,7 +77,7 @@ def build_lib(jobs=8):
         objs = [f.result() for f in hip] + [f.result() for f in host]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-L/opt/rocm/lib", "-lrocblas", "-pthread", "-ldl", "-Wl,-rpath,/opt/rocm/lib"])
+         ["-L/opt/rocm/lib", "-pthread", "-ldl", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     for t in TOOLS:
         _link_tool(t)

@@ -12,8 +12,9 @@
 // a frame chunk are formed as a whole: the GEMM form of the E-step
 // (gemm_path.h) gives L = Theta^T phi and the uncut posteriors in place; one
 // wave per frame then forms the pair weights H = [h | sqrt-cut g] (2M per
-// frame) and the pair features psi = [1, |x_t - x_t-1|]; the sums are one
-// more GEMM, psi^T H, over frame sub-chunks, into FP64 in a fixed order.  The
+// frame) and the pair features psi = [1, |x_t - x_t-1|]; the sums H^T psi
+// run on the statistics kernel of the GEMM form (features psi_d = psi[d] *
+// psi[0]) over frame sub-chunks, into FP64 in a fixed order.  The
 // posteriors of a chunk's last frame are carried to the next chunk, so
 // pairs never break at chunk boundaries.
 #include "gemm_path.h"
@@ -25,8 +26,10 @@ struct AuxState {
     int K = 0, KP = 0;
     float *d_theta = nullptr;    // [M][KP] diagonal models: [const, mu' ivar, -ivar/2]
     size_t theta_cap = 0;
-    int *d_codes = nullptr;      // [K]
-    size_t codes_cap = 0;
+    uint32_t *d_pairs = nullptr; // [KP] feature pairs of the diagonal operand
+    size_t pairs_cap = 0;
+    uint32_t *d_psipairs = nullptr;  // [DP1] (d, 0): psi_d = psi[d] * psi[0]
+    size_t psipairs_cap = 0;
     float *d_cen = nullptr;      // [D]
     size_t cen_cap = 0;
     float *d_H = nullptr;        // [ncp][2M] pair weights | sqrt-cut posteriors
@@ -107,7 +110,7 @@ namespace {
 int aux_diag_operand(gmmhip_ctx *h, gmmhip::AuxState *a) {
     if (a->version == h->model_version) return GMMHIP_OK;
     const int M = h->M, D = h->D;
-    const int K = 1 + 2 * D, KP = (K + 7) / 8 * 8;
+    const int K = 1 + 2 * D, KP = feature_pad(K);
     std::vector<float> theta((size_t)M * KP, 0.0f);
     for (int m = 0; m < M; m++) {
         float *th = theta.data() + (size_t)m * KP;
@@ -121,13 +124,13 @@ int aux_diag_operand(gmmhip_ctx *h, gmmhip::AuxState *a) {
         }
         th[0] = (float)(h->m_gconst[m] - 0.5 * cq);
     }
-    const std::vector<int> codes = feature_codes(D, false);
+    const std::vector<uint32_t> pairs = feature_pairs(feature_codes(D, false), D, KP);
     HIPCHK(h, ensure(&a->d_theta, &a->theta_cap, theta.size()));
-    HIPCHK(h, ensure(&a->d_codes, &a->codes_cap, codes.size()));
+    HIPCHK(h, ensure(&a->d_pairs, &a->pairs_cap, pairs.size()));
     HIPCHK(h, ensure(&a->d_cen, &a->cen_cap, (size_t)D));
     HIPCHK(h, hipMemcpyAsync(a->d_theta, theta.data(), theta.size() * sizeof(float),
                              hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(a->d_codes, codes.data(), codes.size() * sizeof(int),
+    HIPCHK(h, hipMemcpyAsync(a->d_pairs, pairs.data(), pairs.size() * sizeof(uint32_t),
                              hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(a->d_cen, h->h_off.data(), D * sizeof(float), hipMemcpyHostToDevice,
                              h->stream));
@@ -145,7 +148,8 @@ namespace gmmhip_detail {
 void aux_state_free(gmmhip::AuxState *a) {
     if (!a) return;
     hipFree(a->d_theta);
-    hipFree(a->d_codes);
+    hipFree(a->d_pairs);
+    hipFree(a->d_psipairs);
     hipFree(a->d_cen);
     hipFree(a->d_H);
     hipFree(a->d_psi);
@@ -171,32 +175,36 @@ int gmmhip_accumulate_aux(gmmhip_t h, double *aux) {
     if (!h->aux) h->aux = new gmmhip::AuxState();
     gmmhip::AuxState *a = h->aux;
     const int M = h->M, D = h->D, DP1 = (D + 1 + 7) / 8 * 8;
-    int K, KP;
+    int KP;
     const float *theta, *cen;
-    const int *codes;
+    const uint32_t *pairs;
     if (h->full_model) {
-        K = fs->K, KP = fs->KP, theta = fs->d_theta, cen = fs->d_cen, codes = fs->d_pairs;
+        KP = fs->KP, theta = fs->d_theta, cen = fs->d_cen, pairs = fs->d_pairs;
     } else {
         rc = aux_diag_operand(h, a);
         if (rc) return rc;
-        K = a->K, KP = a->KP, theta = a->d_theta, cen = a->d_cen, codes = a->d_codes;
+        KP = a->KP, theta = a->d_theta, cen = a->d_cen, pairs = a->d_pairs;
     }
-    // chunk: phi, L, H and psi rows of a chunk within ~768 MB
+    if (!a->d_psipairs || a->psipairs_cap < (size_t)DP1) {
+        std::vector<uint32_t> pp(DP1);
+        for (int d = 0; d < DP1; d++) pp[d] = (uint32_t)d;  // (d, 0)
+        HIPCHK(h, ensure(&a->d_psipairs, &a->psipairs_cap, (size_t)DP1));
+        HIPCHK(h, hipMemcpyAsync(a->d_psipairs, pp.data(), pp.size() * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, h->stream));
+    }
+    const int DXP = ext_row_floats(D);
+    // chunk: extended rows, L, H and psi rows of a chunk within ~768 MB
     uint64_t nc_max;
     if (h->opt.chunk_frames > 0) {
         nc_max = std::max<uint64_t>(256, (uint64_t)h->opt.chunk_frames);
     } else {
-        const uint64_t per = 4ull * (uint64_t)(KP + 3 * M + DP1) + 1;
+        const uint64_t per = 4ull * (uint64_t)(DXP + 3 * M + DP1) + 1;
         nc_max = std::max<uint64_t>(1024, (768ull << 20) / per / 256 * 256);
     }
     nc_max = std::min<uint64_t>(nc_max, h->N);
-    // sub-chunks of the sums GEMM: enough (64 x 128)-tiles of its DP1 x 2M
-    // output to cover the CUs, each sub-chunk >= 256 frames
-    const int tiles = ((DP1 + 63) / 64) * ((2 * M + 127) / 128);
-    int S = std::max(1, std::min(32, (2 * (h->num_cus > 0 ? h->num_cus : 256) + tiles - 1) / tiles));
-    S = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)S, nc_max / 256));
-    const uint64_t ncp_max = (nc_max + S - 1) / S * S;
-    HIPCHK(h, ensure(&fs->d_phi, &fs->phi_cap, (size_t)ncp_max * KP));
+    const int S = stats_subchunks(2 * M, DP1, h->num_cus, nc_max);
+    const uint64_t ncp_max = (uint64_t)stats_sub(nc_max, S) * S;
+    HIPCHK(h, ensure(&fs->d_xt, &fs->xt_cap, (size_t)ncp_max * DXP));
     HIPCHK(h, ensure(&fs->d_L, &fs->L_cap, (size_t)ncp_max * M));
     HIPCHK(h, ensure(&fs->d_bad, &fs->bad_cap, (size_t)ncp_max));
     HIPCHK(h, ensure(&a->d_H, &a->H_cap, (size_t)ncp_max * 2 * M));
@@ -205,28 +213,22 @@ int gmmhip_accumulate_aux(gmmhip_t h, double *aux) {
     HIPCHK(h, ensure(&a->d_part, &a->part_cap, (size_t)S * 2 * M * DP1));
     HIPCHK(h, ensure(&a->d_acc, &a->acc_cap, (size_t)2 * M * DP1));
     HIPCHK(h, ensure(&h->d_blockLL, &h->blockLL_cap, (size_t)(ncp_max + 3) / 4));
-    const float one = 1.0f, zero = 0.0f;
     int first = 1;
     for (uint64_t n0 = 0; n0 < h->N; n0 += nc_max) {
         const int nc = (int)std::min<uint64_t>(nc_max, h->N - n0);
-        const int sub = (nc + S - 1) / S;
+        const int sub = stats_sub(nc, S);
         const int ncp = sub * S;
-        HIPCHK(h, launch_fc_features(h->d_x, n0, nc, ncp, D, K, KP, cen, codes, fs->d_phi,
-                                     fs->d_bad, h->stream));
-        BLASCHK(h, rocblas_sgemm(fs->blas, rocblas_operation_transpose, rocblas_operation_none, M,
-                                 nc, KP, &one, theta, KP, fs->d_phi, KP, &zero, fs->d_L, M));
+        HIPCHK(h, launch_fc_rows(h->d_x, n0, nc, ncp, D, cen, fs->d_xt, fs->d_bad, h->stream));
+        HIPCHK(h, launch_fc_ll(fs->d_xt, nc, D, theta, M, KP, pairs, fs->d_L, h->stream));
         HIPCHK(h, launch_fc_lse(fs->d_L, nc, ncp, M, fs->d_bad, h->opt.min_loglike,
                                 h->opt.min_gamma, 0, nullptr, n0, h->d_blockLL, h->stream));
         hipLaunchKernelGGL(gmmhip::aux_pair_kernel, dim3((ncp + 3) / 4), dim3(256), 0, h->stream,
                            fs->d_L, a->d_carry, h->d_x, n0, nc, ncp, M, D, DP1, h->opt.min_gamma,
                            a->d_H, a->d_psi);
         HIPCHK(h, hipGetLastError());
-        // part[s] (column-major DP1 x 2M) = Psi_s (DP1 x sub) * H_s^T (sub x 2M)
-        BLASCHK(h, rocblas_sgemm_strided_batched(
-                       fs->blas, rocblas_operation_none, rocblas_operation_transpose, DP1, 2 * M,
-                       sub, &one, a->d_psi, DP1, (rocblas_stride)sub * DP1, a->d_H, 2 * M,
-                       (rocblas_stride)sub * 2 * M, &zero, a->d_part, DP1,
-                       (rocblas_stride)DP1 * 2 * M, S));
+        // part[s] [2M][DP1] = H_s^T Psi_s over frames [s sub, (s+1) sub)
+        HIPCHK(h, launch_fc_stats(a->d_psi, DP1, a->d_H, 2 * M, DP1, a->d_psipairs, sub, S,
+                                  a->d_part, h->stream));
         HIPCHK(h, launch_fc_acc(a->d_part, (size_t)2 * M * DP1, S, first, a->d_acc, h->stream));
         first = 0;
         // the next chunk's first pair needs this chunk's last posteriors

@@ -11,68 +11,17 @@
 //     l[t][m]   = phi(x'_t) . theta_m
 //       theta_m = [log w + gconst - mu'^T P mu' / 2, (P mu')_d, -P_ii/2 (i == j), -P_ij (i < j)]
 //     stats_m   = sum_t g_tm phi(x'_t) = [n, sum g x', sum g x'_i x'_j]
-// so a frame chunk runs: feature expansion (HBM-bound), L = Phi Theta^T and
-// the statistics Gamma^T Phi as plain FP32 GEMMs (rocBLAS, deterministic:
-// atomics off), and between them one fused pass per frame computing the
-// log-sum-exp, the clamp rules and the cut posteriors in place.  The
-// statistics GEMM has a small output (K x M) and a long contraction (the
-// chunk's frames), so it runs as S independent GEMMs over frame sub-chunks
-// (one strided-batched call) to fill the chip; the S partials and the chunks
+// so a frame chunk runs: extended rows [1, x'] (fc_rows_kernel), L = Phi
+// Theta^T with phi formed from those rows inside the MFMA kernel
+// (fc_ll_kernel), one fused pass per frame computing the log-sum-exp, the
+// clamp rules and the cut posteriors in place, and the statistics
+// Gamma^T Phi the same way (fc_stats_kernel).  The statistics have a small
+// output (K x M) and a long contraction (the chunk's frames), so they run
+// over S frame sub-chunks to fill the chip; the S partials and the chunks
 // are summed into FP64 accumulators in a fixed order.
 #include "gemm_path.h"
 
 namespace gmmhip {
-
-// phi rows of frames [n0, n0 + nc) (rows up to ncp zero): one block of 256
-// threads per 8 frames;
-// the frames are centred in LDS and every thread writes consecutive
-// features.  A frame holding a non-finite (or overflowing, |x'| > 1e18)
-// value gets a zero row and bad = 1: its log-likelihood is clamped and it
-// contributes no statistics, as the reference (non-finite component
-// log-likelihoods -> _minLogLike -> no posterior above the cut).
-__global__ __launch_bounds__(256) void fc_features_kernel(const float *__restrict__ X, uint64_t n0,
-                                                          int nc, int ncp, int D, int K, int KP,
-                                                          const float *__restrict__ cen,
-                                                          const int *__restrict__ pairs,
-                                                          float *__restrict__ phi,
-                                                          uint8_t *__restrict__ bad) {
-    __shared__ float xs[8][128];
-    __shared__ int badf[8];
-    const int f0 = blockIdx.x * 8;
-    const int tid = threadIdx.x;
-    if (tid < 8) badf[tid] = 0;
-    __syncthreads();
-    for (int e = tid; e < 8 * D; e += 256) {
-        const int f = e / D, d = e - f * D;
-        float v = 0.0f;
-        if (f0 + f < nc) {
-            v = X[(n0 + f0 + f) * (uint64_t)D + d] - cen[d];
-            if (!(fabsf(v) <= 1e18f)) {
-                badf[f] = 1;
-                v = 0.0f;
-            }
-        }
-        xs[f][d] = v;
-    }
-    __syncthreads();
-    // rows [nc, ncp) pad the chunk to whole sub-chunks: zero features
-    for (int f = 0; f < 8; f++) {
-        if (f0 + f >= ncp) break;
-        float *row = phi + (size_t)(f0 + f) * KP;
-        const bool fb = badf[f] != 0 || f0 + f >= nc;
-        for (int k = tid; k < KP; k += 256) {
-            float v = 0.0f;
-            if (k < K && !fb) {
-                const int p = pairs[k];
-                if (p == -1) v = 1.0f;
-                else if (p < 0) v = xs[f][-2 - p];
-                else v = xs[f][p >> 16] * xs[f][p & 0xffff];
-            }
-            row[k] = v;
-        }
-        if (tid == 0) bad[f0 + f] = fb ? 1 : 0;
-    }
-}
 
 // One wave per frame: log-sum-exp of the frame's M component
 // log-likelihoods (exact, max-shifted), the clamp rules of logLikeKernel
@@ -173,23 +122,9 @@ std::vector<int> feature_codes(int D, bool full) {
 int gemm_state(gmmhip_ctx *h, gmmhip::FullState **out) {
     if (!h->full) {
         h->full = new gmmhip::FullState();
-        DevGuard g(h->dev);
-        if (rocblas_create_handle(&h->full->blas) != rocblas_status_success)
-            return fail(h, GMMHIP_ERR_HIP, "rocblas_create_handle failed");
-        rocblas_set_stream(h->full->blas, h->stream);
-        // deterministic kernels: no atomics (the default, stated explicitly)
-        rocblas_set_atomics_mode(h->full->blas, rocblas_atomics_not_allowed);
     }
     *out = h->full;
     return GMMHIP_OK;
-}
-
-hipError_t launch_fc_features(const float *X, uint64_t n0, int nc, int ncp, int D, int K, int KP,
-                              const float *cen, const int *codes, float *phi, uint8_t *bad,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(gmmhip::fc_features_kernel, dim3((ncp + 7) / 8), dim3(256), 0, s, X, n0, nc,
-                       ncp, D, K, KP, cen, codes, phi, bad);
-    return hipGetLastError();
 }
 
 hipError_t launch_fc_lse(float *L, int nc, int ncp, int M, const uint8_t *bad, float min_ll,
@@ -211,15 +146,15 @@ hipError_t launch_fc_acc(const float *part, size_t E, int S, int first, double *
 
 namespace {
 
-// Frames per chunk: the feature rows and the log-likelihood rows of a chunk
+// Frames per chunk: the extended rows and the log-likelihood rows of a chunk
 // take at most ~512 MB;
 // opts.chunk_frames (at least 256) overrides.
-uint64_t full_chunk(const gmmhip_ctx *h, int KP) {
+uint64_t full_chunk(const gmmhip_ctx *h) {
     uint64_t nc;
     if (h->opt.chunk_frames > 0) {
         nc = std::max<uint64_t>(256, (uint64_t)h->opt.chunk_frames);
     } else {
-        const uint64_t per = 4ull * (uint64_t)(KP + h->M) + 1;
+        const uint64_t per = 4ull * (uint64_t)(ext_row_floats(h->D) + h->M) + 1;
         nc = std::max<uint64_t>(1024, (512ull << 20) / per / 256 * 256);
     }
     return std::min<uint64_t>(nc, h->N);
@@ -233,15 +168,11 @@ uint64_t full_chunk(const gmmhip_ctx *h, int KP) {
 int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot, float *post_out = nullptr) {
     const int post_cut = post_out ? 0 : 1;
     gmmhip::FullState *fs = h->full;
-    const int M = h->M, D = h->D, K = fs->K, KP = fs->KP;
-    const uint64_t nc_max = full_chunk(h, KP);
-    // sub-chunks of the statistics GEMM: enough (256 x 128)-tiles of the
-    // K x M output to cover the CUs, each sub-chunk >= 256 frames
-    const int tiles = ((KP + 255) / 256) * ((M + 127) / 128);
-    int S = std::max(1, std::min(32, (2 * (h->num_cus > 0 ? h->num_cus : 256) + tiles - 1) / tiles));
-    S = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)S, nc_max / 256));
-    const uint64_t ncp_max = (nc_max + S - 1) / S * S;
-    HIPCHK(h, ensure(&fs->d_phi, &fs->phi_cap, (size_t)ncp_max * KP));
+    const int M = h->M, D = h->D, KP = fs->KP, DXP = ext_row_floats(D);
+    const uint64_t nc_max = full_chunk(h);
+    const int S = stats_subchunks(M, KP, h->num_cus, nc_max);
+    const uint64_t ncp_max = (uint64_t)stats_sub(nc_max, S) * S;
+    HIPCHK(h, ensure(&fs->d_xt, &fs->xt_cap, (size_t)ncp_max * DXP));
     HIPCHK(h, ensure(&fs->d_L, &fs->L_cap, (size_t)ncp_max * M));
     HIPCHK(h, ensure(&fs->d_bad, &fs->bad_cap, (size_t)ncp_max));
     const uint64_t nblk_total = (h->N + nc_max - 1) / nc_max * ((ncp_max + 3) / 4);
@@ -250,18 +181,15 @@ int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot, float *post_
         HIPCHK(h, ensure(&fs->d_part, &fs->part_cap, (size_t)S * M * KP));
         HIPCHK(h, ensure(&fs->d_acc, &fs->acc_cap, (size_t)M * KP));
     }
-    const float one = 1.0f, zero = 0.0f;
     uint64_t nblk = 0;
     int first = 1;
     for (uint64_t n0 = 0; n0 < h->N; n0 += nc_max) {
         const int nc = (int)std::min<uint64_t>(nc_max, h->N - n0);
-        const int sub = (nc + S - 1) / S;  // frames per sub-chunk
+        const int sub = stats_sub(nc, S);  // frames per sub-chunk
         const int ncp = sub * S;
-        HIPCHK(h, launch_fc_features(h->d_x, n0, nc, ncp, D, K, KP, fs->d_cen, fs->d_pairs,
-                                     fs->d_phi, fs->d_bad, h->stream));
-        // L (column-major M x nc, frame-major rows) = Theta^T (M x KP) * Phi (KP x nc)
-        BLASCHK(h, rocblas_sgemm(fs->blas, rocblas_operation_transpose, rocblas_operation_none, M,
-                                 nc, KP, &one, fs->d_theta, KP, fs->d_phi, KP, &zero, fs->d_L, M));
+        HIPCHK(h, launch_fc_rows(h->d_x, n0, nc, ncp, D, fs->d_cen, fs->d_xt, fs->d_bad, h->stream));
+        // L [nc][M] = Phi Theta^T
+        HIPCHK(h, launch_fc_ll(fs->d_xt, nc, D, fs->d_theta, M, KP, fs->d_pairs, fs->d_L, h->stream));
         if (h->d_mask) {
             hipLaunchKernelGGL(gmmhip::fc_mask_kernel, dim3(nc), dim3(256), 0, h->stream, fs->d_L,
                                nc, M, h->d_mask, h->mask_W, n0);
@@ -276,13 +204,9 @@ int full_pass(gmmhip_ctx *h, bool stats, float *LLnat, double *tot, float *post_
                                      hipMemcpyDeviceToHost, h->stream));
         nblk += blocks;
         if (stats) {
-            // part[s] (column-major KP x M) = Phi_s (KP x sub) * Gamma_s^T (sub x M), frames
-            // [s sub, (s+1) sub) of the chunk
-            BLASCHK(h, rocblas_sgemm_strided_batched(
-                           fs->blas, rocblas_operation_none, rocblas_operation_transpose, KP, M, sub,
-                           &one, fs->d_phi, KP, (rocblas_stride)sub * KP, fs->d_L, M,
-                           (rocblas_stride)sub * M, &zero, fs->d_part, KP, (rocblas_stride)KP * M,
-                           S));
+            // part[s] [M][KP] = Gamma_s^T Phi_s over frames [s sub, (s+1) sub)
+            HIPCHK(h, launch_fc_stats(fs->d_xt, DXP, fs->d_L, M, KP, fs->d_pairs, sub, S,
+                                      fs->d_part, h->stream));
             const size_t E = (size_t)M * KP;
             HIPCHK(h, launch_fc_acc(fs->d_part, E, S, first, fs->d_acc, h->stream));
             first = 0;
@@ -299,11 +223,10 @@ namespace gmmhip_detail {
 
 void full_state_free(gmmhip::FullState *fs) {
     if (!fs) return;
-    if (fs->blas) rocblas_destroy_handle(fs->blas);
     hipFree(fs->d_theta);
     hipFree(fs->d_pairs);
     hipFree(fs->d_cen);
-    hipFree(fs->d_phi);
+    hipFree(fs->d_xt);
     hipFree(fs->d_L);
     hipFree(fs->d_bad);
     hipFree(fs->d_part);
@@ -328,7 +251,7 @@ int diag_operand(gmmhip_ctx *h, bool full_features) {
     const int kind = full_features ? gmmhip::OP_DIAG_FULLFEAT : gmmhip::OP_DIAG;
     if (fs->operand_version == h->model_version && fs->operand_kind == kind) return GMMHIP_OK;
     const int M = h->M, D = h->D;
-    const int K = full_features ? 1 + D + D * (D + 1) / 2 : 1 + 2 * D, KP = (K + 7) / 8 * 8;
+    const int K = full_features ? 1 + D + D * (D + 1) / 2 : 1 + 2 * D, KP = feature_pad(K);
     std::vector<float> theta((size_t)M * KP, 0.0f);
     for (int m = 0; m < M; m++) {
         float *th = theta.data() + (size_t)m * KP;
@@ -344,13 +267,13 @@ int diag_operand(gmmhip_ctx *h, bool full_features) {
         }
         th[0] = (float)(h->m_gconst[m] - 0.5 * cq);
     }
-    const std::vector<int> codes = feature_codes(D, full_features);
+    const std::vector<uint32_t> pairs = feature_pairs(feature_codes(D, full_features), D, KP);
     HIPCHK(h, ensure(&fs->d_theta, &fs->theta_cap, theta.size()));
-    HIPCHK(h, ensure(&fs->d_pairs, &fs->pairs_cap, codes.size()));
+    HIPCHK(h, ensure(&fs->d_pairs, &fs->pairs_cap, pairs.size()));
     HIPCHK(h, ensure(&fs->d_cen, &fs->cen_cap, (size_t)D));
     HIPCHK(h, hipMemcpyAsync(fs->d_theta, theta.data(), theta.size() * sizeof(float),
                              hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(fs->d_pairs, codes.data(), codes.size() * sizeof(int),
+    HIPCHK(h, hipMemcpyAsync(fs->d_pairs, pairs.data(), pairs.size() * sizeof(uint32_t),
                              hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(fs->d_cen, h->h_off.data(), D * sizeof(float), hipMemcpyHostToDevice,
                              h->stream));
@@ -468,7 +391,7 @@ int gmmhip_set_model_full(gmmhip_t h, int M, int D, const float *w, const float 
     int rc = gemm_state(h, &fs);
     if (rc) return rc;
     const int K = 1 + D + D * (D + 1) / 2;
-    const int KP = (K + 7) / 8 * 8;
+    const int KP = feature_pad(K);
     // centre: the weighted mean of the component means
     std::vector<double> c(D, 0.0);
     double ws = 0.0;
@@ -533,14 +456,14 @@ int gmmhip_set_model_full(gmmhip_t h, int M, int D, const float *w, const float 
             for (int j = i; j < D; j++, k++)
                 th[k] = (float)((i == j ? -0.5 : -1.0) * P[(size_t)i * D + j]);
     }
-    const std::vector<int> pairs = feature_codes(D, true);
+    const std::vector<uint32_t> pairs = feature_pairs(feature_codes(D, true), D, KP);
     DevGuard g(h->dev);
     HIPCHK(h, ensure(&fs->d_theta, &fs->theta_cap, theta.size()));
     HIPCHK(h, ensure(&fs->d_pairs, &fs->pairs_cap, pairs.size()));
     HIPCHK(h, ensure(&fs->d_cen, &fs->cen_cap, (size_t)D));
     HIPCHK(h, hipMemcpyAsync(fs->d_theta, theta.data(), theta.size() * sizeof(float),
                              hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(fs->d_pairs, pairs.data(), pairs.size() * sizeof(int),
+    HIPCHK(h, hipMemcpyAsync(fs->d_pairs, pairs.data(), pairs.size() * sizeof(uint32_t),
                              hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(fs->d_cen, cf.data(), D * sizeof(float), hipMemcpyHostToDevice,
                              h->stream));
