@@ -309,8 +309,9 @@ def bench_fullcov(args):
     seeded 512-component full-covariance mixture; the model is initialised
     from 512 random frames with the global covariance (the first EM
     iteration).  One step = gmmhip_accumulate over the resident frames:
-    feature expansion, the two FP32 GEMMs (log-likelihoods, gamma^T Phi),
-    the fused log-sum-exp/posterior pass and the FP64 chunk reduction.
+    extended rows, the two FP32 MFMA contractions with the feature map
+    formed in LDS (log-likelihoods, gamma^T Phi), the fused
+    log-sum-exp/posterior pass and the FP64 chunk reduction.
     Algorithmic work 4K FLOP per frame x component with K = 1 + D + D(D+1)/2
     features."""
     import torch
@@ -367,8 +368,8 @@ def bench_fullcov(args):
                    "tot_loglike_per_frame": res[3] / N},
         "roofline": {"bound": "mfma", "achieved": flop / step / 1e12, "peak": PEAK["fp32"] / 1e12,
                      "unit": "TFLOP/s", "frac": flop / step / PEAK["fp32"], "traffic": None,
-                     "kernel": "whole E-step (features + two FP32 GEMMs + log-sum-exp + FP64 "
-                               "reduction)", "avg_launch_ms": step * 1e3},
+                     "kernel": "whole E-step (extended rows + fc_ll + log-sum-exp + fc_stats + "
+                               "FP64 reduction)", "avg_launch_ms": step * 1e3},
     }
     if not args.no_cpu_baseline:
         from oracle import binding as orc

@@ -48,8 +48,7 @@ namespace gmmhip_detail {
 // full: [1, x', x'_i x'_j (i <= j, packed upper)]; diagonal: [1, x', x'_d^2].
 std::vector<int> feature_codes(int D, bool full);
 
-// The handle's GEMM-path state (created on first use: rocBLAS handle on the
-// handle's stream, atomics off).
+// The handle's GEMM-path state (created on first use).
 int gemm_state(gmmhip_ctx *h, gmmhip::FullState **out);
 
 // The GEMM operand theta of the handle's diagonal model, over the full
