@@ -63,7 +63,8 @@ class Opts(ctypes.Structure):
     _fields_ = [("min_loglike", ctypes.c_float), ("min_gamma", ctypes.c_float),
                 ("min_var", ctypes.c_float), ("precision", ctypes.c_int),
                 ("chunk_frames", ctypes.c_int), ("verbosity", ctypes.c_int),
-                ("profile", ctypes.c_int), ("ll_frames", ctypes.c_int)]
+                ("profile", ctypes.c_int), ("ll_frames", ctypes.c_int),
+                ("comp_tile", ctypes.c_int)]
 
 
 class BatchCfg(ctypes.Structure):
@@ -155,9 +156,8 @@ def lib():
         L.gmmhip_server_destroy.argtypes = [c_vp]
         L.gmmhip_server_destroy.restype = None
         L.gmmhip_tune_load.argtypes = [ctypes.c_char_p]
-        L.gmmhip_tune_lookup.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 2
-        L.gmmhip_tiles_used.argtypes = [c_vp, ctypes.POINTER(ctypes.c_int),
-                                        ctypes.POINTER(ctypes.c_int)]
+        L.gmmhip_tune_lookup.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)] * 3
+        L.gmmhip_tiles_used.argtypes = [c_vp] + [ctypes.POINTER(ctypes.c_int)] * 3
         _lib = L
     return _lib
 
@@ -176,10 +176,11 @@ def tune_load(path=None):
 
 
 def tune_lookup(path, precision, D, M):
-    """(ll_frames, chunk_frames) the tuning table picks, or None."""
-    a, b = ctypes.c_int(0), ctypes.c_int(0)
-    rc = lib().gmmhip_tune_lookup(path, precision, D, M, ctypes.byref(a), ctypes.byref(b))
-    return (a.value, b.value) if rc == 0 else None
+    """(ll_frames, chunk_frames, comp_tile) the tuning table picks, or None."""
+    a, b, c = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    rc = lib().gmmhip_tune_lookup(path, precision, D, M, ctypes.byref(a), ctypes.byref(b),
+                                  ctypes.byref(c))
+    return (a.value, b.value, c.value) if rc == 0 else None
 
 
 def device_count():
@@ -391,10 +392,12 @@ class Accumulator:
         return c.value
 
     def tiles_used(self):
-        """(ll_frames, chunk_frames) of the last accumulate / loglikes / batch call."""
-        a, b = ctypes.c_int(0), ctypes.c_int(0)
-        self._check(lib().gmmhip_tiles_used(self._h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+        """(ll_frames, chunk_frames, comp_tile) of the last accumulate / loglikes /
+        batch call (comp_tile 0 when no mixed statistics kernel ran)."""
+        a, b, c = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        self._check(lib().gmmhip_tiles_used(self._h, ctypes.byref(a), ctypes.byref(b),
+                                            ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def stream(self):
         return lib().gmmhip_stream(self._h)

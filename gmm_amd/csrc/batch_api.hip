@@ -192,6 +192,7 @@ struct BatchPlan {
     int tau = 0;
     int ll_sub = 2;
     int ll_nw = 4;  // mixed log-likelihood waves per block
+    int st_nw = 4;  // mixed statistics waves per block (32 components each)
     std::vector<uint64_t> lo, hi;
     std::vector<int> first;
 };
@@ -287,7 +288,8 @@ int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
         HIPCHK(h, launch_loglik_mixed(m, p.prec, s));
         m.chunk_frames = p.cf;
         m.nchunks = p.nchunks;
-        m.ngroups = (p.Mt32 + 3) / 4;
+        m.st_nw = p.st_nw;
+        m.ngroups = (p.Mt32 + p.st_nw - 1) / p.st_nw;
         m.partials = h->d_part;
         m.chunk_lo = b->d_lo;
         m.chunk_hi = b->d_hi;
@@ -437,6 +439,7 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
     if (p.cf == 0) p.cf = 1024;
     p.ll_sub = choose_ll_sub(h, GMMHIP_TUNE_BATCH);
     p.ll_nw = choose_mixed_ll_nw(h, GMMHIP_TUNE_BATCH);
+    p.st_nw = choose_mixed_st_nw(h, GMMHIP_TUNE_BATCH);
     h->used_ll_frames = 64 * p.ll_sub;
     h->used_chunk = (int)p.cf;
     p.want_means = out->means != nullptr;
@@ -501,6 +504,7 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
             p.prec = prec;
         }
     }
+    h->used_comp_tile = p.mixed ? 32 * p.st_nw : 0;
     p.first.assign(U + 1, 0);
     for (int u = 0; u < U; u++) {
         p.first[u] = (int)p.lo.size();
@@ -554,6 +558,7 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
                                      p.cf,
                                      (uint64_t)p.ll_sub,
                                      (uint64_t)p.ll_nw,
+                                     (uint64_t)p.st_nw,
                                      (uint64_t)p.want_means,
                                      (uint64_t)p.want_mllr,
                                      (uint64_t)p.prior,

@@ -114,7 +114,7 @@ struct gmmhip_ctx {
     unsigned char *noalloc = nullptr;
 
     // tiles of the last call (gmmhip_tiles_used)
-    int used_ll_frames = 0, used_chunk = 0;
+    int used_ll_frames = 0, used_chunk = 0, used_comp_tile = 0;
 
     // kernel timers (opts.profile)
     hipEvent_t ev[5] = {};
@@ -249,6 +249,9 @@ int choose_ll_sub(const gmmhip_ctx *h, int path);
 // opts.ll_frames, else the tuning table's entry for (path, precision),
 // else 4 (gmmhip_api.hip).
 int choose_mixed_ll_nw(const gmmhip_ctx *h, int path);
+// Mixed statistics waves per block: opts.comp_tile, else the tuning table's
+// entry for (path, precision), else 128 components (4 waves).
+int choose_mixed_st_nw(const gmmhip_ctx *h, int path);
 // Most frames per statistics work item from the tuning table for (path,
 // precision) and the handle's model, rounded to the 64-frame step; 0 when
 // the table has no entry.

@@ -42,6 +42,8 @@ struct MixedArgs {
     float *LL2, *LLnat;
     double *blockLL;
     int ll_nw;             // log-likelihood waves per block: 4 (128 frames) or 8 (256, KP % 64 == 0)
+    int st_nw;             // statistics waves per block: 4 (128 components) or 8 (256; KP 64
+                           // or 128); ngroups = ceil(Mt32 / st_nw)
     uint64_t chunk_frames;
     int nchunks, ngroups;
     float *partials;       // [nchunks][Mt32*32][KP]
@@ -60,6 +62,9 @@ hipError_t launch_loglik_mixed(const MixedArgs &a, int prec, hipStream_t s);
 // ll_nw waves (4, or 8 where KP is a multiple of 64): the number of
 // per-block log-likelihood sums it writes is ceil(Npad / that).
 int mixed_ll_frames_per_block(int KP, int nw);
+// Statistics waves per block for a component tile of comp_tile components
+// (128 or 256) at depth KP: 8 for 256 at KP 64 or 128, else 4.
+int mixed_stats_waves(int KP, int comp_tile);
 hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s);
 
 }  // namespace gmmhip

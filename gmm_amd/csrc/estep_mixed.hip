@@ -449,11 +449,15 @@ __device__ __forceinline__ v4s16 lds_read_tr16(const void *lds_addr) {
         (__attribute__((address_space(3))) v4s16 *)const_cast<void *>(lds_addr));
 }
 
-// Statistics with direct-to-LDS staging.  A block of four waves owns four
-// component tiles (32 components each) and walks its frame chunk in 32-frame
-// steps; per step the four waves copy the step's ZA hi and lo rows (U pieces
-// x 32 rows each, a quarter each) and the last wave also the step's 32
-// per-frame log-likelihoods.  Iteration s contracts ZA of step s+1 with the
+// Statistics with direct-to-LDS staging.  A block of NWS waves (4 or 8: the
+// component tile, 128 or 256 components per block) owns NWS component tiles
+// (32 components each) and walks its frame chunk in 32-frame steps; per step
+// the waves copy the step's ZA hi and lo rows (U pieces x 32 rows each, the
+// first half of the waves hi, the second lo) and the last wave also the
+// step's 32 per-frame log-likelihoods.  Eight waves share one copy of the
+// rows between twice the components (half the copies per MFMA) at one block
+// per CU (depths 64 and 128; at 256 the two waves per SIMD would spill); the
+// tuning table picks per (D, M).  Iteration s contracts ZA of step s+1 with the
 // split model (the first contraction, interleaved with step s's posteriors)
 // and the posteriors of step s with the features of step s (the frame
 // contraction), whose operand -- 8 frames of one feature per lane -- comes
@@ -462,15 +466,17 @@ __device__ __forceinline__ v4s16 lds_read_tr16(const void *lds_addr) {
 // __shared__ variable of its own and the loop is unrolled by three so that
 // every access names its slot: the compiler can then see that the copies in
 // flight do not alias the reads of the iteration and does not wait for them.
-template <int KP, typename T>
-__global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_tr_kernel(MixedArgs a) {
+template <int KP, typename T, int NWS>
+__global__ __launch_bounds__(64 * NWS, (NWS == 4 && KP <= 128 ? 2 : 1)) void stats_mixed_tr_kernel(MixedArgs a) {
     typedef typename Vec8<T>::type V8;
     constexpr int KS = KP / 16;
     constexpr int CT = KP / 32;
     constexpr int U = KP / 8;               // 16-byte pieces per feature row
     constexpr int PART = 32 * U;            // pieces per part (hi or lo) and step
-    constexpr int NDMA = PART / 128;        // copies per wave and step (2 parts over 4 waves)
-    static_assert((U & (U - 1)) == 0 && PART % 128 == 0, "power-of-two feature rows");
+    constexpr int HW = NWS / 2;             // waves per part
+    constexpr int PW = PART / HW;           // pieces a wave copies per step
+    constexpr int NDMA = PW / 64;           // copies per wave and step
+    static_assert((U & (U - 1)) == 0 && PW % 64 == 0 && NDMA >= 1, "power-of-two feature rows");
     __shared__ u32x4 ZA0[2][PART], ZA1[2][PART], ZA2[2][PART];
     __shared__ float Ls0[32], Ls1[32], Ls2[32];
 
@@ -484,7 +490,7 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_tr_kerne
     const int group = idx % a.ngroups;
     const int chunk = (idx / a.ngroups) * 8 + xcd;
     if (chunk >= a.nchunks) return;
-    const int tile = group * 4 + wave;
+    const int tile = group * NWS + wave;
     const bool active = tile < a.Mt32;
     const bool cvalid = (32 * tile + c) < a.M;
     const float thr = cvalid ? a.min_gamma : INFINITY;
@@ -517,16 +523,17 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_tr_kerne
         if (f1 > a.N) f1 = a.N;
     }
 
-    // waves 0, 1 copy halves of ZA hi, waves 2, 3 of ZA lo; wave 3 also the
-    // step's 32 per-frame log-likelihoods.  Copy j of a wave moves piece
-    // p = q PART/2 + 64 j + lane of its part (row p / U, de-swizzled source).
-    const int hl = wave >> 1, q = wave & 1;
-    const bool ll_wave = wave == 3;
+    // waves 0 .. HW-1 copy ZA hi, waves HW .. NWS-1 ZA lo, PW pieces each;
+    // the last wave also the step's 32 per-frame log-likelihoods.  Copy j of
+    // a wave moves piece p = q PW + 64 j + lane of its part (row p / U,
+    // de-swizzled source).
+    const int hl = wave / HW, q = wave % HW;
+    const bool ll_wave = wave == NWS - 1;
     const u32x4 *src = reinterpret_cast<const u32x4 *>(hl ? a.za_lo : a.za_hi);
     unsigned int soff[NDMA];
 #pragma unroll
     for (int j = 0; j < NDMA; j++) {
-        const int p = q * (PART / 2) + j * 64 + lane;
+        const int p = q * PW + j * 64 + lane;
         const int row = p / U, v = p % U;
         soff[j] = (unsigned int)(row * U + (v ^ za_swz<U>(row))) * 16u;
     }
@@ -558,7 +565,7 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_tr_kerne
         const __amdgpu_buffer_rsrc_t rs =                                          \
             dma_rsrc(src + (f0 + 32 * (uint64_t)(K_)) * U, PART * 16);             \
         _Pragma("unroll") for (int j = 0; j < NDMA; j++)                           \
-            lds_dma16(rs, soff[j], &ZD_[hl][q * (PART / 2) + j * 64]);             \
+            lds_dma16(rs, soff[j], &ZD_[hl][q * PW + j * 64]);             \
         if (ll_wave && lane < 32)                                                  \
             __builtin_amdgcn_global_load_lds(                                      \
                 a.LL2 + f0 + 32 * (uint64_t)(K_) + ulane,                          \
@@ -578,7 +585,7 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_tr_kerne
     if (nsteps > 1) {
         const __amdgpu_buffer_rsrc_t rs = dma_rsrc(src + (f0 + 32) * U, PART * 16);
 #pragma unroll
-        for (int j = 0; j < NDMA; j++) lds_dma16(rs, soff[j], &ZA1[hl][q * (PART / 2) + j * 64]);
+        for (int j = 0; j < NDMA; j++) lds_dma16(rs, soff[j], &ZA1[hl][q * PW + j * 64]);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -601,7 +608,7 @@ __global__ __launch_bounds__(256, (KP <= 128 ? 2 : 1)) void stats_mixed_tr_kerne
             const __amdgpu_buffer_rsrc_t rs =                                      \
                 dma_rsrc(src + (f0 + 32 * (uint64_t)(s_ + 2)) * U, PART * 16);     \
             _Pragma("unroll") for (int j = 0; j < NDMA; j++)                       \
-                lds_dma16(rs, soff[j], &ZF[hl][q * (PART / 2) + j * 64]);          \
+                lds_dma16(rs, soff[j], &ZF[hl][q * PW + j * 64]);          \
         }                                                                          \
         if (ll_wave && lane < 32 && s_ + 1 < nsteps)                               \
             __builtin_amdgcn_global_load_lds(                                      \
@@ -970,7 +977,15 @@ hipError_t launch_loglik_mixed(const MixedArgs &a, int prec, hipStream_t s) {
 
 // Feature rows of a power-of-two number of 16-byte pieces take the
 // direct-to-LDS statistics kernel; the others the register-staged one.
+int mixed_stats_waves(int KP, int comp_tile) {
+    // (at KP = 256 eight waves would share a 256-register budget and spill)
+    return (comp_tile == 256 && (KP == 64 || KP == 128)) ? 8 : 4;
+}
+
 hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s) {
+    if (a.st_nw != 4 && a.st_nw != 8) return hipErrorInvalidValue;
+    if (a.st_nw == 8 && a.KP != 64 && a.KP != 128) return hipErrorInvalidValue;
+    if (a.ngroups != (a.Mt32 + a.st_nw - 1) / a.st_nw) return hipErrorInvalidValue;
     const int nchunks_pad = (a.nchunks + 7) / 8 * 8;
     const dim3 grid((unsigned)(nchunks_pad * a.ngroups));
     switch (a.KP) {
@@ -983,10 +998,15 @@ hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s) {
         break;
 #define XD(K)                                                                                     \
     case K:                                                                                       \
-        if (prec == 2)                                                                            \
-            hipLaunchKernelGGL((stats_mixed_tr_kernel<K, _Float16>), grid, dim3(256), 0, s, a);  \
+        if (a.st_nw == 8 && (K == 64 || K == 128)) {                                              \
+            if (prec == 2)                                                                        \
+                hipLaunchKernelGGL((stats_mixed_tr_kernel<K, _Float16, (K == 64 || K == 128 ? 8 : 4)>), grid, dim3(512), 0, s, a); \
+            else                                                                                  \
+                hipLaunchKernelGGL((stats_mixed_tr_kernel<K, __bf16, (K == 64 || K == 128 ? 8 : 4)>), grid, dim3(512), 0, s, a);   \
+        } else if (prec == 2)                                                                     \
+            hipLaunchKernelGGL((stats_mixed_tr_kernel<K, _Float16, 4>), grid, dim3(256), 0, s, a); \
         else                                                                                      \
-            hipLaunchKernelGGL((stats_mixed_tr_kernel<K, __bf16>), grid, dim3(256), 0, s, a);    \
+            hipLaunchKernelGGL((stats_mixed_tr_kernel<K, __bf16, 4>), grid, dim3(256), 0, s, a);   \
         break;
         XD(32) XD(64) X(96) XD(128) X(160) X(192) X(224) XD(256)
 #undef X

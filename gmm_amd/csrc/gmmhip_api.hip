@@ -116,6 +116,15 @@ int choose_ll_sub(const gmmhip_ctx *h, int path) {
     return sub;
 }
 
+int choose_mixed_st_nw(const gmmhip_ctx *h, int path) {
+    int ct = h->opt.comp_tile;
+    if (ct <= 0) {
+        gmmhip_tune::Entry e;
+        ct = gmmhip_tune::lookup(path, h->opt.precision, h->D, h->M, &e) ? e.comp_tile : 128;
+    }
+    return mixed_stats_waves(kp_mixed_for_dim(h->D), ct);
+}
+
 int choose_mixed_ll_nw(const gmmhip_ctx *h, int path) {
     int f = h->opt.ll_frames;
     if (f <= 0) {
@@ -297,6 +306,7 @@ MixedArgs make_mixed_args(gmmhip_ctx *h) {
     m.min_ll = h->opt.min_loglike;
     m.min_gamma = h->opt.min_gamma;
     m.ll_nw = choose_mixed_ll_nw(h, GMMHIP_TUNE_ESTEP);
+    m.st_nw = choose_mixed_st_nw(h, GMMHIP_TUNE_ESTEP);
     return m;
 }
 
@@ -670,13 +680,15 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
     const int KPx = mixed ? h->KPm : h->KP;
     const int tile = mixed ? 32 : 16;
     const int Mtx = (h->M + tile - 1) / tile;
-    const int ngroups = (Mtx + 3) / 4;
+    const int st_nw = mixed ? choose_mixed_st_nw(h, GMMHIP_TUNE_ESTEP) : 4;
+    const int ngroups = (Mtx + st_nw - 1) / st_nw;
     const uint64_t chunk_frames = pick_chunk_frames(h, ngroups, mixed);
     const int ll_sub =
         mixed ? mixed_ll_frames_per_block(h->KPm, choose_mixed_ll_nw(h, GMMHIP_TUNE_ESTEP)) / 64
               : choose_ll_sub(h, GMMHIP_TUNE_ESTEP);
     h->used_ll_frames = 64 * ll_sub;
     h->used_chunk = (int)chunk_frames;
+    h->used_comp_tile = mixed ? 32 * st_nw : 0;
     const int nchunks = (int)((h->N + chunk_frames - 1) / chunk_frames);
     const size_t Mp = (size_t)Mtx * tile;
     const size_t E = Mp * KPx;
@@ -760,19 +772,22 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
 
 int gmmhip_tune_load(const char *path) { return gmmhip_tune::load(path); }
 
-int gmmhip_tune_lookup(int path, int precision, int D, int M, int *ll_frames, int *chunk_frames) {
+int gmmhip_tune_lookup(int path, int precision, int D, int M, int *ll_frames, int *chunk_frames,
+                       int *comp_tile) {
     if (D <= 0 || M <= 0) return GMMHIP_ERR_ARG;
     gmmhip_tune::Entry e;
     if (!gmmhip_tune::lookup(path, precision, D, M, &e)) return GMMHIP_ERR_STATE;
     if (ll_frames) *ll_frames = e.ll_frames;
     if (chunk_frames) *chunk_frames = e.chunk_frames;
+    if (comp_tile) *comp_tile = e.comp_tile;
     return GMMHIP_OK;
 }
 
-int gmmhip_tiles_used(gmmhip_t h, int *ll_frames, int *chunk_frames) {
+int gmmhip_tiles_used(gmmhip_t h, int *ll_frames, int *chunk_frames, int *comp_tile) {
     if (!h) return GMMHIP_ERR_ARG;
     if (ll_frames) *ll_frames = h->used_ll_frames;
     if (chunk_frames) *chunk_frames = h->used_chunk;
+    if (comp_tile) *comp_tile = h->used_comp_tile;
     return GMMHIP_OK;
 }
 

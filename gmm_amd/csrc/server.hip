@@ -353,6 +353,7 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
         if (Us > 0) {
             m.chunk_frames = s->cf;
             m.nchunks = sh.nchunks;
+            m.st_nw = 4;
             m.ngroups = (m.Mt32 + 3) / 4;
             m.partials = (float *)(A + L.part);
             m.chunk_lo = clo;

@@ -55,6 +55,8 @@ int load_locked(const char *path) {
         Entry e{};
         if (!(ss >> kind >> prec >> e.D >> e.M >> e.ll_frames >> e.chunk_frames)) continue;
         if (!(ss >> e.ms)) e.ms = 0.0;
+        if (!(ss >> e.comp_tile)) e.comp_tile = 128;
+        if (e.comp_tile != 128 && e.comp_tile != 256) continue;
         e.path = kind == "estep" ? GMMHIP_TUNE_ESTEP : kind == "batch" ? GMMHIP_TUNE_BATCH : -1;
         e.prec = parse_prec(prec);
         if (e.path < 0 || e.prec < 0 || e.D <= 0 || e.M <= 0) continue;

@@ -1,8 +1,10 @@
 // Tuning table of the E-step kernels (internal): per (path, precision, D, M)
-// the log-likelihood frame tile and the frames per statistics work item that
-// a timed sweep chose.  Text format, one entry per line:
-//   <estep|batch> <fp32|bf16x3|fp16x3> D M ll_frames chunk_frames ms
-// '#' starts a comment.  Host-only code (tune.cpp).
+// the log-likelihood frame tile, the frames per statistics work item and the
+// mixed statistics component tile that a timed sweep chose.  Text format,
+// one entry per line:
+//   <estep|batch> <fp32|bf16x3|fp16x3> D M ll_frames chunk_frames ms [comp_tile]
+// (comp_tile 128 when absent).  '#' starts a comment.  Host-only code
+// (tune.cpp).
 #pragma once
 
 namespace gmmhip_tune {
@@ -14,6 +16,7 @@ struct Entry {
     int ll_frames;     // 64, 128 or 256
     int chunk_frames;  // frames per statistics work item (upper bound)
     double ms;         // time measured by the sweep
+    int comp_tile;     // mixed statistics components per block: 128 or 256
 };
 
 // (Re)load the table; path == nullptr: $GMMHIP_TUNE_FILE, else

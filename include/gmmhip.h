@@ -70,6 +70,9 @@ typedef struct gmmhip_opts {
     int profile;         /* 1: time every kernel with HIP events */
     int ll_frames;       /* FP32 log-likelihood frame tile: 64, 128, 256;
                             0 = tuning table, else 128 */
+    int comp_tile;       /* mixed-form statistics component tile: 128 or 256
+                            components per block (256 at mixed depths 64
+                            and 128 only); 0 = tuning table, else 128 */
 } gmmhip_opts;
 
 /* Fill *o with the defaults above. */
@@ -290,11 +293,12 @@ int gmmhip_batch_graph_captures(gmmhip_t h, uint64_t *captures);
 /* ---- Tuning table -----------------------------------------------------
  * Tile choices per (path, precision, D, M) picked by a timed sweep
  * (python -m gmm_amd.autotune) and stored as text in
- * gmm_amd/tuning/gfx950.tune (or $GMMHIP_TUNE_FILE): the FP32
- * log-likelihood frame tile and the most frames per statistics work item.
- * Used when opts.ll_frames / opts.chunk_frames (cfg->chunk_frames for
- * batches) are 0.  No counterpart in the reference, whose launch shapes are
- * compile-time constants (trainer/CU_GMMStatsEstimator_CUDA.cu:606-643). */
+ * gmm_amd/tuning/gfx950.tune (or $GMMHIP_TUNE_FILE): the log-likelihood
+ * frame tile, the most frames per statistics work item and (mixed forms)
+ * the statistics component tile.  Used when opts.ll_frames /
+ * opts.chunk_frames (cfg->chunk_frames for batches) / opts.comp_tile are 0.
+ * No counterpart in the reference, whose launch shapes are compile-time
+ * constants (trainer/CU_GMMStatsEstimator_CUDA.cu:606-643). */
 enum { GMMHIP_TUNE_ESTEP = 0, GMMHIP_TUNE_BATCH = 1 };
 /* Reload the table from path (NULL: the default); returns the number of
  * entries, or -1 if the file cannot be read (the table is then empty). */
@@ -303,9 +307,11 @@ int gmmhip_tune_load(const char *path);
  * there is no exact entry; GMMHIP_ERR_STATE when it has none for the path
  * and precision. */
 int gmmhip_tune_lookup(int path, int precision, int D, int M, int *ll_frames,
-                       int *chunk_frames);
-/* Tiles the handle's last accumulate / loglikes / batch call used. */
-int gmmhip_tiles_used(gmmhip_t h, int *ll_frames, int *chunk_frames);
+                       int *chunk_frames, int *comp_tile);
+/* Tiles the handle's last accumulate / loglikes / batch call used
+ * (comp_tile: components per mixed statistics block, 0 when no mixed
+ * statistics kernel ran). */
+int gmmhip_tiles_used(gmmhip_t h, int *ll_frames, int *chunk_frames, int *comp_tile);
 
 /* ---- Serving loop -------------------------------------------------------
  * A server keeps one UBM resident in HBM and answers requests of three kinds
