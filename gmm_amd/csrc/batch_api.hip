@@ -14,7 +14,11 @@
 // every utterance is copied to a 32-row-aligned place (pad_rows_kernel),
 // prep_frames splits the frames centred and scaled by the UBM's global mean
 // and deviation (so each utterance's result does not depend on the batch),
-// then loglik_mixed and stats_mixed over the utterances' chunks.  A batch
+// then loglik_mixed and the statistics over the utterances' chunks -- at
+// depths of a power-of-two number of 16-byte pieces (D <= 127 but for 2D+1
+// in 65..96 or 129..224) the persistent form of stats_mixed_tr, whose blocks
+// pull (chunk, group) items from per-XCD device counters, else the grid
+// form of the register-staged kernel.  A batch
 // with a frame value outside the 16-bit range, or a model whose
 // coefficients overflow it, runs in FP32.
 // With cfg->graph the whole pipeline is captured once into a hipGraph and
@@ -293,7 +297,17 @@ int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
         m.partials = h->d_part;
         m.chunk_lo = b->d_lo;
         m.chunk_hi = b->d_hi;
-        HIPCHK(h, launch_stats_mixed(m, p.prec, s));
+        if (!mixed_needs_zb(p.KPm)) {
+            // persistent grid over the (chunk, group) items; counters reset
+            // by a kernel so the pipeline stays capturable
+            m.queue = b->d_queue;
+            m.queue_blocks =
+                mixed_stats_queue_blocks(p.KPm, p.st_nw, h->num_cus, p.nchunks * m.ngroups);
+            HIPCHK(h, launch_fill_u32(b->d_queue, 0u, 8, s));
+            HIPCHK(h, launch_stats_mixed_queue(m, p.prec, s));
+        } else {
+            HIPCHK(h, launch_stats_mixed(m, p.prec, s));
+        }
     } else if (!p.per_utt) {
         HIPCHK(h, launch_loglik_fp32(a, s));
         a.nchunks = p.nchunks;

@@ -51,6 +51,10 @@ struct MixedArgs {
     // chunk_lo must be a multiple of 32 (null: chunk c covers chunk_frames
     // frames from c * chunk_frames)
     const uint64_t *chunk_lo, *chunk_hi;
+    // persistent form (launch_stats_mixed_queue): eight item counters, zero
+    // at launch, and the number of blocks
+    unsigned int *queue;
+    int queue_blocks;
 };
 
 hipError_t launch_colstats(const float *X, uint64_t N, int D, int blocks, uint64_t per_block,
@@ -66,5 +70,14 @@ int mixed_ll_frames_per_block(int KP, int nw);
 // (128 or 256) at depth KP: 8 for 256 at KP 64 or 128, else 4.
 int mixed_stats_waves(int KP, int comp_tile);
 hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s);
+// Persistent form of the statistics kernel at depths whose feature row is a
+// power-of-two number of 16-byte pieces (mixed_needs_zb false): queue_blocks
+// blocks pull (chunk, group) items, chunk-major, from eight per-XCD lists
+// (counters a.queue[0..7], zero at launch) until all are empty.  Same
+// partials as launch_stats_mixed, bit for bit.
+hipError_t launch_stats_mixed_queue(const MixedArgs &a, int prec, hipStream_t s);
+// Blocks for the persistent form: the resident blocks of the chip (at most
+// the item count rounded up to a multiple of 8).
+int mixed_stats_queue_blocks(int KP, int st_nw, int num_cus, int nitems);
 
 }  // namespace gmmhip

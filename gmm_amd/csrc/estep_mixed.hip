@@ -449,6 +449,29 @@ __device__ __forceinline__ v4s16 lds_read_tr16(const void *lds_addr) {
         (__attribute__((address_space(3))) v4s16 *)const_cast<void *>(lds_addr));
 }
 
+// Next (chunk, group) item of the persistent statistics grid, or -1 when
+// none is left.  XCD x's list (counter a.queue[x]) holds the chunks x, x+8,
+// x+16, ..., each with all its component groups in a row, so the groups
+// that share a chunk's frames run under one L2; a block (on XCD blockIdx % 8)
+// drains its own list first, then takes items from the others in turn.
+// drained: bit x set once list x was found empty.
+__device__ __forceinline__ int mixed_queue_pull(const MixedArgs &a, int xcd,
+                                                unsigned int &drained) {
+    for (int k = 0; k < 8; k++) {
+        const int x = (xcd + k) & 7;
+        if (drained & (1u << x)) continue;
+        const unsigned int nch = a.nchunks > x ? (unsigned int)(a.nchunks - x + 7) / 8u : 0u;
+        const unsigned int n = nch * (unsigned int)a.ngroups;
+        const unsigned int j = n ? atomicAdd(a.queue + x, 1u) : n;
+        if (j < n) {
+            const unsigned int jc = j / (unsigned int)a.ngroups;
+            return (int)((x + 8 * jc) * (unsigned int)a.ngroups + (j - jc * (unsigned int)a.ngroups));
+        }
+        drained |= 1u << x;
+    }
+    return -1;
+}
+
 // Statistics with direct-to-LDS staging.  A block of NWS waves (4 or 8: the
 // component tile, 128 or 256 components per block) owns NWS component tiles
 // (32 components each) and walks its frame chunk in 32-frame steps; per step
@@ -466,7 +489,11 @@ __device__ __forceinline__ v4s16 lds_read_tr16(const void *lds_addr) {
 // __shared__ variable of its own and the loop is unrolled by three so that
 // every access names its slot: the compiler can then see that the copies in
 // flight do not alias the reads of the iteration and does not wait for them.
-template <int KP, typename T, int NWS>
+//
+// QUEUE: persistent form; the blocks pull (chunk, group) items through
+// mixed_queue_pull until every list is empty, with a barrier between items
+// (the next item's copies overwrite slots the last one read).
+template <int KP, typename T, int NWS, bool QUEUE>
 __global__ __launch_bounds__(64 * NWS, (NWS == 4 && KP <= 128 ? 2 : 1)) void stats_mixed_tr_kernel(MixedArgs a) {
     typedef typename Vec8<T>::type V8;
     constexpr int KS = KP / 16;
@@ -481,15 +508,32 @@ __global__ __launch_bounds__(64 * NWS, (NWS == 4 && KP <= 128 ? 2 : 1)) void sta
     __shared__ float Ls0[32], Ls1[32], Ls2[32];
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int h = lane >> 5, c = lane & 31;
 
-    const int b = blockIdx.x;
-    const int xcd = b & 7, idx = b >> 3;
-    const int group = idx % a.ngroups;
-    const int chunk = (idx / a.ngroups) * 8 + xcd;
-    if (chunk >= a.nchunks) return;
+    __shared__ int sh_item;
+    unsigned int drained = 0;  // (thread 0) item lists found empty
+    for (;;) {
+    // lane-derived offsets formed anew per item: hoisted out of the item
+    // loop they would stay live beside the body's 255 registers and spill
+    int lane = tid & 63;
+    if constexpr (QUEUE) asm volatile("" : "+v"(lane));
+    const int h = lane >> 5, c = lane & 31;
+    int chunk, group;
+    if constexpr (QUEUE) {
+        __syncthreads();
+        if (tid == 0) sh_item = mixed_queue_pull(a, (int)(blockIdx.x & 7), drained);
+        __syncthreads();
+        const int item = sh_item;
+        if (item < 0) break;
+        chunk = item / a.ngroups;
+        group = item - chunk * a.ngroups;
+    } else {
+        const int b = blockIdx.x;
+        const int xcd = b & 7, idx = b >> 3;
+        group = idx % a.ngroups;
+        chunk = (idx / a.ngroups) * 8 + xcd;
+        if (chunk >= a.nchunks) return;
+    }
     const int tile = group * NWS + wave;
     const bool active = tile < a.Mt32;
     const bool cvalid = (32 * tile + c) < a.M;
@@ -682,16 +726,19 @@ __global__ __launch_bounds__(64 * NWS, (NWS == 4 && KP <= 128 ? 2 : 1)) void sta
 #undef GMMHIP_TR_DMA
 #undef GMMHIP_TR_CONTRACT1
 
-    if (!active) return;
-    const size_t Mp = (size_t)a.Mt32 * 32;
-    float *out = a.partials + ((size_t)chunk * Mp + (size_t)tile * 32) * KP;
+    if (active) {
+        const size_t Mp = (size_t)a.Mt32 * 32;
+        float *out = a.partials + ((size_t)chunk * Mp + (size_t)tile * 32) * KP;
 #pragma unroll
-    for (int ct = 0; ct < CT; ct++)
+        for (int ct = 0; ct < CT; ct++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int row = 8 * (r >> 2) + 4 * h + (r & 3);
-            out[(size_t)row * KP + 32 * ct + c] = acc2[ct][r];
-        }
+            for (int r = 0; r < 16; r++) {
+                const int row = 8 * (r >> 2) + 4 * h + (r & 3);
+                out[(size_t)row * KP + 32 * ct + c] = acc2[ct][r];
+            }
+    }
+    if constexpr (!QUEUE) break;
+    }
 }
 
 // Maximum over the 32 lanes of a half-wave (lanes 32h .. 32h+31): DPP
@@ -982,15 +1029,15 @@ int mixed_stats_waves(int KP, int comp_tile) {
     return (comp_tile == 256 && (KP == 64 || KP == 128)) ? 8 : 4;
 }
 
-hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s) {
+template <bool Q>
+static hipError_t launch_stats_mixed_form(const MixedArgs &a, int prec, dim3 grid, hipStream_t s) {
     if (a.st_nw != 4 && a.st_nw != 8) return hipErrorInvalidValue;
     if (a.st_nw == 8 && a.KP != 64 && a.KP != 128) return hipErrorInvalidValue;
     if (a.ngroups != (a.Mt32 + a.st_nw - 1) / a.st_nw) return hipErrorInvalidValue;
-    const int nchunks_pad = (a.nchunks + 7) / 8 * 8;
-    const dim3 grid((unsigned)(nchunks_pad * a.ngroups));
     switch (a.KP) {
 #define X(K)                                                                                  \
     case K:                                                                                   \
+        if constexpr (Q) return hipErrorInvalidValue;                                         \
         if (prec == 2)                                                                        \
             hipLaunchKernelGGL((stats_mixed_kernel<K, _Float16>), grid, dim3(256), 0, s, a); \
         else                                                                                  \
@@ -1000,13 +1047,13 @@ hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s) {
     case K:                                                                                       \
         if (a.st_nw == 8 && (K == 64 || K == 128)) {                                              \
             if (prec == 2)                                                                        \
-                hipLaunchKernelGGL((stats_mixed_tr_kernel<K, _Float16, (K == 64 || K == 128 ? 8 : 4)>), grid, dim3(512), 0, s, a); \
+                hipLaunchKernelGGL((stats_mixed_tr_kernel<K, _Float16, (K == 64 || K == 128 ? 8 : 4), Q>), grid, dim3(512), 0, s, a); \
             else                                                                                  \
-                hipLaunchKernelGGL((stats_mixed_tr_kernel<K, __bf16, (K == 64 || K == 128 ? 8 : 4)>), grid, dim3(512), 0, s, a);   \
+                hipLaunchKernelGGL((stats_mixed_tr_kernel<K, __bf16, (K == 64 || K == 128 ? 8 : 4), Q>), grid, dim3(512), 0, s, a);   \
         } else if (prec == 2)                                                                     \
-            hipLaunchKernelGGL((stats_mixed_tr_kernel<K, _Float16, 4>), grid, dim3(256), 0, s, a); \
+            hipLaunchKernelGGL((stats_mixed_tr_kernel<K, _Float16, 4, Q>), grid, dim3(256), 0, s, a); \
         else                                                                                      \
-            hipLaunchKernelGGL((stats_mixed_tr_kernel<K, __bf16, 4>), grid, dim3(256), 0, s, a);   \
+            hipLaunchKernelGGL((stats_mixed_tr_kernel<K, __bf16, 4, Q>), grid, dim3(256), 0, s, a);   \
         break;
         XD(32) XD(64) X(96) XD(128) X(160) X(192) X(224) XD(256)
 #undef X
@@ -1014,6 +1061,24 @@ hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s) {
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s) {
+    const int nchunks_pad = (a.nchunks + 7) / 8 * 8;
+    return launch_stats_mixed_form<false>(a, prec, dim3((unsigned)(nchunks_pad * a.ngroups)), s);
+}
+
+int mixed_stats_queue_blocks(int KP, int st_nw, int num_cus, int nitems) {
+    // resident blocks: two per CU for the 4-wave kernel at depths up to 128
+    const int per_cu = (st_nw == 4 && KP <= 128) ? 2 : 1;
+    int n = per_cu * (num_cus > 0 ? num_cus : 256);
+    if (n > nitems) n = (nitems + 7) / 8 * 8;
+    return n < 8 ? 8 : n;
+}
+
+hipError_t launch_stats_mixed_queue(const MixedArgs &a, int prec, hipStream_t s) {
+    if (!a.queue || a.queue_blocks < 1 || mixed_needs_zb(a.KP)) return hipErrorInvalidValue;
+    return launch_stats_mixed_form<true>(a, prec, dim3((unsigned)a.queue_blocks), s);
 }
 
 }  // namespace gmmhip

@@ -14,7 +14,8 @@
 //   FP32 form:  loglik_kernel over all frames, stats_queue_kernel over the
 //               chunks of the MAP and MLLR requests only;
 //   mixed form: prep_frames (z = (x - o) * s split hi/lo), loglik_mixed,
-//               stats_mixed over the MAP/MLLR chunks;
+//               stats_mixed over the MAP/MLLR chunks (the persistent
+//               per-XCD work-queue form where the depth allows);
 //   seg_reduce (n, F per request, MAP means of the MAP requests), seg_sum
 //   (LL per request), mllr_check / the two FP64 G, k GEMMs /
 //   mllr_solve (MLLR requests),
@@ -100,7 +101,7 @@ Layout layout_of(const Shape &s, const Dims &d) {
     L.LLnat = take(s.N * 4);
     L.out_end = p;
     L.out_copy = (s.frame_ll ? L.out_end : L.LLnat) - L.ll;
-    L.queue = take(4);
+    L.queue = take(8 * 4);  // work-queue counters (FP32: one; mixed: one per XCD)
     L.LL2 = take(pad128(s.N) * 4);  // the mixed statistics kernel stages 32-frame steps
     L.blockLL = take(pad128(s.N) / 128 * 8);
     L.part = take((size_t)s.nchunks * Mp * KP * 4);
@@ -358,7 +359,15 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
             m.partials = (float *)(A + L.part);
             m.chunk_lo = clo;
             m.chunk_hi = chi;
-            SRV_HIPCHK(launch_stats_mixed(m, s->prec, st));
+            if (!mixed_needs_zb(d.KPm)) {
+                m.queue = (unsigned int *)(A + L.queue);
+                m.queue_blocks = mixed_stats_queue_blocks(d.KPm, m.st_nw, s->num_cus,
+                                                          sh.nchunks * m.ngroups);
+                SRV_HIPCHK(launch_fill_u32(m.queue, 0u, 8, st));
+                SRV_HIPCHK(launch_stats_mixed_queue(m, s->prec, st));
+            } else {
+                SRV_HIPCHK(launch_stats_mixed(m, s->prec, st));
+            }
         }
         roff = s->d_zoff;
         rscale = s->d_zscale;
