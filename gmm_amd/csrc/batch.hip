@@ -381,33 +381,50 @@ hipError_t launch_mllr_check(const double *n, int U, int M, int D, int *status, 
 }
 
 // ---------------------------------------------------------------------------
-// Blocked solve for E = D+1 <= 64 on FP64 MFMA.  One wave per (u, i).  G
-// (symmetric, unpacked from the packed upper triangle of G plus the prior,
-// staged in LDS) is padded to 64 x 64 with an identity block and held in
-// registers in the accumulator layout of v_mfma_f64_16x16x4f64: tile (I, J)
-// of 16 x 16, lane (q, c), register r = element (16 I + q + 4 r, 16 J + c);
-// the right-hand side k sits one row per lane.  Gauss-Jordan elimination
-// without pivoting, as the reference inverts G_i (MATRIX::_Invert_Gen,
-// Matrix4_LMa.cpp:2232; stable for SPD G, whose pivots are the ratios of
-// its leading principal minors and so all positive), four columns at a
-// time: step s takes the rows and columns K = 4s .. 4s+3, the 4 x 4 pivot
-// block B = A[K, K], the row block R = A[K, :] and the column block
-// P = A[:, K] (through LDS), forms R' = B^-1 R (lane j solves for column j
-// with an LU of B that every lane repeats) and
-//   A <- A - P_eff R',   P_eff = P with B - I4 in the rows K,
-// which eliminates the columns K from every other row and leaves R' in the
-// rows K, as one rank-4 MFMA update per 16 x 16 tile (tiles of finished
-// columns skipped: their R' is zero).  k takes the same update with vector
-// FMAs.  After the last step A is the identity and k holds w.  The pivots
-// of B's LU are the pivots of the unblocked elimination, so a non-positive
-// one (G not positive definite) marks the utterance's status.  Blocking turns the E^2 single-element broadcasts of a
-// one-row-per-lane elimination into 8 KB of LDS traffic per step.
-// two waves per SIMD: the compiler then keeps the 64 x 64 block in 251
-// VGPRs without spilling instead of 379 VGPR + AGPR at one wave
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mllr_solve_mfma_kernel(
+// Blocked LDL^T solve for E = D+1 <= 64 on FP64 MFMA.  One wave per (u, i).
+// W_i = G_i^-1 k_i of ComputeW_MLLR, which the reference forms by inverting
+// G_i with Gauss-Jordan elimination (MATRIX::_Invert_Gen,
+// Matrix4_LMa.cpp:2232); G_i is symmetric positive definite, so here it is
+// factored instead and k_i substituted through the factor.
+// G (symmetric, unpacked from the packed upper triangle plus the prior) is
+// padded to 64 x 64 with an identity block and held in registers in the
+// accumulator layout of v_mfma_f64_16x16x4f64: tile (I, J) of 16 x 16, lane
+// (q, c), register r = element (16 I + q + 4 r, 16 J + c); k sits one row per
+// lane.  Step s factors the columns K = 4s .. 4s+3: with P = A[:, K] (through
+// LDS) and the 4 x 4 block B = A[K, K] = L_K D_K L_K^T (unit lower L_K,
+// every lane repeats it), X = P L_K^-T for the rows below K (lane = row), and
+//   A <- A - X D_K^-1 X^T
+// on the tiles on or below the diagonal from tile column K / 16 on (rows and
+// columns up to K carry X = 0, so finished columns and the block itself are
+// untouched): one rank-4 MFMA per tile, 80 per system against 160 for the
+// full Gauss-Jordan update.  The lanes holding the columns K keep
+// L[:, K] = X D_K^-1 (the unit lower factor) for the back substitution, and
+// k takes the forward substitution L z = k in the same step (z_K by L_K,
+// the rows below by X D_K^-1 z_K).  After the last step u = D^-1 z, and
+// L^T w = u runs block by block from the last: w_K = L_K^-T u_K, then
+// u_i -= L[K, i] . w_K for every row i above K, L[K, :] staged from the
+// registers that hold row block K.  A non-positive D_K entry (G not
+// positive definite) marks the utterance's status.
+// The solve is latency-bound (a serial chain of 4 x 4 factorisations, LDS
+// round trips and dependent FP64 MFMAs per step), so the kernel is held to
+// four waves per SIMD: only the ten tiles on and below the diagonal are
+// live (128 VGPRs) and G is read from the packed triangle straight into
+// them (8 KB of LDS per wave).  Measured on the 4,096-utterance batch
+// (245,760 systems of 61): 3.07 ms, against 3.56 ms at three waves and
+// 3.62 ms for a blocked Gauss-Jordan with the full 64 x 64 system (160
+// MFMAs per system, two waves per SIMD).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void mllr_solve_ldl_kernel(
     MllrSolveArgs a) {
-    constexpr int EMAX = 64;
-    __shared__ double sG[EMAX * (EMAX + 1) / 2];
+    // LDS: P (64 x 4; the back substitution's row block of L in the same
+    // place), X D^-1 and X (64 x 4 each), the factored blocks (16 per step:
+    // L_K below the diagonal, 1/D_K on it), k of the rows K
+    __shared__ double sW[4 * 256 + 8];
+    double *sP = sW;
+    double *sR = sW;
+    double *sX = sW + 256;
+    double *sXr = sW + 512;
+    double *sL = sW + 768;
+    double *sK = sW + 1024;
     const int E = a.D + 1;
     const int NP = E * (E + 1) / 2;
     const uint64_t ui = blockIdx.x;
@@ -417,151 +434,176 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ml
         if (lane < E) a.W[ui * E + lane] = 0.0;
         return;
     }
-    // all of this lane's loads in flight at once (one HBM round trip, not
-    // one per element)
-    constexpr int NIT = (EMAX * (EMAX + 1) / 2 + 63) / 64;
+    // the tiles on and below the diagonal, straight from the packed upper
+    // triangle (and the prior); the tiles above it are never read
     const double *gp = a.Gp + ui * a.ldg;
     const double *gip = a.Ginit ? a.Ginit + (size_t)i * NP : nullptr;
-    double t[NIT];
-#pragma unroll
-    for (int it = 0; it < NIT; it++) {
-        const int p = lane + 64 * it;
-        t[it] = p < NP ? gp[p] : 0.0;
-    }
-    if (gip) {
-#pragma unroll
-        for (int it = 0; it < NIT; it++) {
-            const int p = lane + 64 * it;
-            if (p < NP) t[it] += gip[p];
-        }
-    }
-    double rhs = 0.0;
-    if (lane < E) {
-        rhs = a.Kp[ui * a.ldk + lane];
-        if (a.kinit) rhs += a.kinit[(size_t)i * E + lane];
-    }
-#pragma unroll
-    for (int it = 0; it < NIT; it++) {
-        const int p = lane + 64 * it;
-        if (p < NP) sG[p] = t[it];
-    }
-    __syncthreads();
     f64x4 A[4][4];
 #pragma unroll
     for (int I = 0; I < 4; I++)
 #pragma unroll
-        for (int J = 0; J < 4; J++)
+        for (int J = 0; J <= I; J++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = 16 * I + q + 4 * r, col = 16 * J + c;
                 double v = row == col ? 1.0 : 0.0;
                 if (row < E && col < E) {
                     const int lo = row < col ? row : col, hi = row < col ? col : row;
-                    v = sG[lo * E - (lo * (lo - 1)) / 2 + (hi - lo)];
+                    const int pk = lo * E - (lo * (lo - 1)) / 2 + (hi - lo);
+                    v = gp[pk];
+                    if (gip) v += gip[pk];
                 }
                 A[I][J][r] = v;
             }
-    // LDS from here on: R and R' (4 x 64 each), P (64 x 4), k of the rows K
-    double *sR = sG;
-    double *sRp = sG + 256;
-    double *sP = sG + 512;
-    double *sK = sG + 768;
+    double rhs = 0.0;
+    if (lane < E) {
+        rhs = a.Kp[ui * a.ldk + lane];
+        if (a.kinit) rhs += a.kinit[(size_t)i * E + lane];
+    }
     bool bad = false;
     const int S = (E + 3) / 4;
 #pragma unroll
     for (int s = 0; s < 16; s++) {
         if (s < S) {
-            const int Ib = s >> 2, rr = s & 3, cb = 4 * (s & 3);
+            const int Ib = s >> 2, cb = 4 * (s & 3);
+            const int k0 = 4 * s;
             __syncthreads();  // the previous step's reads are done
-            // R[q][16 J + c] is register rr of tile (Ib, J): row 4s + q
+            // (rows above tile row Ib are not read: the tiles above the
+            // diagonal stay dead)
+            if (c >= cb && c < cb + 4) {
 #pragma unroll
-            for (int J = 0; J < 4; J++) sR[q * 64 + 16 * J + c] = A[Ib][J][rr];
-            // P[row][t], t = c - cb, from the lanes holding columns 4s .. 4s+3
+                for (int I = 0; I < 4; I++)
+                    if (I >= Ib) {
+#pragma unroll
+                        for (int r = 0; r < 4; r++) sP[(16 * I + q + 4 * r) * 4 + (c - cb)] = A[I][Ib][r];
+                    }
+            }
+            if (lane >= k0 && lane < k0 + 4) sK[lane - k0] = rhs;
+            __syncthreads();
+            // B = L_K D_K L_K^T (every lane)
+            double l[4][4], dv[4], dinv[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                // d_j = B[j][j] - sum_p L[j][p]^2 d_p, then column j of L
+                double d = sP[(k0 + j) * 4 + j];
+#pragma unroll
+                for (int p = 0; p < j; p++) d = fma(-l[j][p] * dv[p], l[j][p], d);
+                bad = bad || !(d > 0.0);
+                const double dd = d > 0.0 ? d : 1.0;
+                dv[j] = dd;
+                // 1/d: the hardware reciprocal and two Newton steps
+                double rc = __builtin_amdgcn_rcp(dd);
+                rc = fma(rc, fma(-dd, rc, 1.0), rc);
+                dinv[j] = fma(rc, fma(-dd, rc, 1.0), rc);
+#pragma unroll
+                for (int tt = j + 1; tt < 4; tt++) {
+                    double v = sP[(k0 + tt) * 4 + j];
+#pragma unroll
+                    for (int p = 0; p < j; p++) v = fma(-l[tt][p] * dv[p], l[j][p], v);
+                    l[tt][j] = v * dinv[j];
+                }
+            }
+            // this lane's row: x = P[row] L_K^-T (rows below K; zero above)
+            double x[4] = {0.0, 0.0, 0.0, 0.0};
+            if (lane >= k0 + 4) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    double v = sP[lane * 4 + j];
+#pragma unroll
+                    for (int p = 0; p < j; p++) v = fma(-x[p], l[j][p], v);
+                    x[j] = v;
+                }
+            }
+            // X and L[row][K] = X D_K^-1 (the factor kept for the back substitution)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                sXr[lane * 4 + j] = x[j];
+                sX[lane * 4 + j] = x[j] * dinv[j];
+            }
+            // forward substitution: z_K = L_K^-1 k_K, rows below -= L[row][K] z_K
+            double z[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                double v = sK[j];
+#pragma unroll
+                for (int p = 0; p < j; p++) v = fma(-l[j][p], z[p], v);
+                z[j] = v;
+            }
+            if (lane >= k0 + 4) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) rhs = fma(-x[j] * dinv[j], z[j], rhs);
+            } else if (lane >= k0) {
+                // u_K = D_K^-1 z_K (selected, not indexed: no scratch array)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (lane - k0 == j) rhs = z[j] * dinv[j];
+            }
+            if (lane < 16) {
+                double val = 0.0;
+#pragma unroll
+                for (int tt = 0; tt < 4; tt++)
+#pragma unroll
+                    for (int v = 0; v <= tt; v++)
+                        if (lane == 4 * tt + v) val = tt > v ? l[tt][v] : dinv[tt];
+                sL[s * 16 + lane] = val;
+            }
+            __syncthreads();
+            // operands: A[row c][k q] = -X[16 I + c][q], B[k q][col c] = (X D^-1)[16 J + c][q]
+            double aop[4], bop[4];
+#pragma unroll
+            for (int I = 0; I < 4; I++) {
+                aop[I] = -sXr[(16 * I + c) * 4 + q];
+                bop[I] = sX[(16 * I + c) * 4 + q];
+            }
+            // the column block K in registers becomes L[:, K] below the block
             if (c >= cb && c < cb + 4) {
 #pragma unroll
                 for (int I = 0; I < 4; I++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) sP[(16 * I + q + 4 * r) * 4 + (c - cb)] = A[I][Ib][r];
+                    for (int r = 0; r < 4; r++) {
+                        const int row = 16 * I + q + 4 * r;
+                        if (I >= Ib && row >= k0 + 4) A[I][Ib][r] = sX[row * 4 + (c - cb)];
+                    }
             }
-            if (lane >= 4 * s && lane < 4 * s + 4) sK[lane - 4 * s] = rhs;
-            __syncthreads();
-            // B = L U without pivoting (every lane; B is 4 x 4)
-            double b[4][4];
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+            for (int I = 0; I < 4; I++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) b[t][v] = sR[t * 64 + 4 * s + v];
-            double rp[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const double piv = b[j][j];
-                bad = bad || !(piv > 0.0);
-                // 1/piv: the hardware reciprocal and two Newton steps
-                const double pp = piv > 0.0 ? piv : 1.0;
-                double rc = __builtin_amdgcn_rcp(pp);
-                rc = fma(rc, fma(-pp, rc, 1.0), rc);
-                rp[j] = fma(rc, fma(-pp, rc, 1.0), rc);
-#pragma unroll
-                for (int t = j + 1; t < 4; t++) {
-                    const double l = b[t][j] * rp[j];
-                    b[t][j] = l;
-#pragma unroll
-                    for (int v = j + 1; v < 4; v++) b[t][v] = fma(-l, b[j][v], b[t][v]);
+                for (int J = 0; J < 4; J++) {
+                    // (tiles wholly in the identity padding add X = 0)
+                    if (J < Ib || I < J || 16 * I >= E) continue;
+                    A[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[I], bop[J], A[I][J], 0, 0, 0);
                 }
-            }
-            // x = B^-1 y: forward with the unit lower factor, backward with U
-            auto solve = [&](double y0, double y1, double y2, double y3, double *x) {
-                double z[4] = {y0, y1, y2, y3};
+        }
+    }
+    // back substitution L^T w = u, blocks from the last
 #pragma unroll
-                for (int t = 1; t < 4; t++)
-#pragma unroll
-                    for (int j = 0; j < t; j++) z[t] = fma(-b[t][j], z[j], z[t]);
-#pragma unroll
-                for (int t = 3; t >= 0; t--) {
-                    double v = z[t];
-#pragma unroll
-                    for (int j = t + 1; j < 4; j++) v = fma(-b[t][j], x[j], v);
-                    x[t] = v * rp[t];
-                }
-            };
-            // R'[:, lane] = B^-1 R[:, lane]
-            double x[4];
-            solve(sR[lane], sR[64 + lane], sR[128 + lane], sR[192 + lane], x);
-#pragma unroll
-            for (int t = 0; t < 4; t++) sRp[t * 64 + lane] = x[t];
-            // k: rhs[row] -= P_eff[row][:] . (B^-1 k_K)
-            double xk[4];
-            solve(sK[0], sK[1], sK[2], sK[3], xk);
-            double rk = 0.0;
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const double pv = sP[lane * 4 + t] - (lane == 4 * s + t ? 1.0 : 0.0);
-                rk = fma(pv, xk[t], rk);
-            }
-            rhs -= rk;
+    for (int s = 15; s >= 0; s--) {
+        if (s < S) {
+            const int Ib = s >> 2, rr = s & 3;
+            const int k0 = 4 * s;
             __syncthreads();
-            // B operand R'[q][16 J + c], A operand -P_eff[16 I + c][q]
-            double bop[4] = {0.0, 0.0, 0.0, 0.0}, aop[4];
+            // row 4s + q of L is register rr of tiles (Ib, J), columns 16 J + c
 #pragma unroll
             for (int J = 0; J < 4; J++)
-                if (J >= Ib) bop[J] = sRp[q * 64 + 16 * J + c];
+                if (J <= Ib) sR[q * 64 + 16 * J + c] = A[Ib][J][rr];
+            if (lane >= k0 && lane < k0 + 4) sK[lane - k0] = rhs;
+            __syncthreads();
+            double w[4];
 #pragma unroll
-            for (int I = 0; I < 4; I++) {
-                const int row = 16 * I + c;
-                aop[I] = (row == 4 * s + q ? 1.0 : 0.0) - sP[row * 4 + q];
+            for (int j = 3; j >= 0; j--) {
+                double v = sK[j];
+#pragma unroll
+                for (int p = j + 1; p < 4; p++) v = fma(-sL[s * 16 + p * 4 + j], w[p], v);
+                w[j] = v;
             }
-            // the tiles the next step reads first (its pivot tile row and
-            // column), so that step can start while the rest still run
-            const int In = (s + 1) >> 2;
+            if (lane < k0) {
 #pragma unroll
-            for (int pass = 0; pass < 2; pass++)
+                for (int j = 0; j < 4; j++) rhs = fma(-sR[j * 64 + lane], w[j], rhs);
+            } else if (lane < k0 + 4) {
 #pragma unroll
-                for (int I = 0; I < 4; I++)
-#pragma unroll
-                    for (int J = 0; J < 4; J++) {
-                        if (J < Ib || ((I == In || J == In) != (pass == 0))) continue;
-                        A[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[I], bop[J], A[I][J], 0, 0, 0);
-                    }
+                for (int j = 0; j < 4; j++)
+                    if (lane - k0 == j) rhs = w[j];
+            }
         }
     }
     if (lane < E) a.W[ui * E + lane] = rhs;
@@ -572,7 +614,7 @@ hipError_t launch_mllr_solve(const MllrSolveArgs &a, hipStream_t s) {
     const int E = a.D + 1;
     const dim3 grid((unsigned)((uint64_t)a.U * a.D));
     if (E <= 64) {
-        hipLaunchKernelGGL(mllr_solve_mfma_kernel, grid, dim3(64), 0, s, a);
+        hipLaunchKernelGGL(mllr_solve_ldl_kernel, grid, dim3(64), 0, s, a);
     } else {
         const size_t shm = ((size_t)E * (E + 1) + E) * sizeof(double);
         hipLaunchKernelGGL(mllr_solve_kernel, grid, dim3(64), shm, s, a);
