@@ -159,15 +159,19 @@ int model_operands(gmmhip_ctx *h, bool mllr, ModelOperands *o) {
     if (!h->d_mu_f32 || h->mu_f32_version != h->model_version) {
         HIPCHK(h, ensure(&h->d_mu_f32, &h->mu_f32_cap, MD));
         HIPCHK(h, ensure(&h->d_var_f32, &h->var_f32_cap, MD));
+        HIPCHK(h, ensure(&h->d_gconst, &h->gconst_cap, (size_t)h->M));
         HIPCHK(h, hipMemcpyAsync(h->d_mu_f32, h->m_mu.data(), MD * 4, hipMemcpyHostToDevice,
                                  h->stream));
         HIPCHK(h, hipMemcpyAsync(h->d_var_f32, h->m_var.data(), MD * 4, hipMemcpyHostToDevice,
                                  h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->d_gconst, h->m_gconst.data(), (size_t)h->M * 8,
+                                 hipMemcpyHostToDevice, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         h->mu_f32_version = h->model_version;
     }
     o->mu = h->d_mu_f32;
     o->var = h->d_var_f32;
+    o->gconst = h->d_gconst;
     o->P = o->Xi = o->Ginit = o->kinit = o->ivar = nullptr;
     if (mllr) {
         const int rc = build_mllr_operands(h, b);
@@ -228,20 +232,18 @@ int batch_mixed_operands(gmmhip_ctx *h, gmmhip_batch_state *b, int prec, int KPm
         zoff[d] = (float)o;
         if (std::isfinite(v) && v > 0.0) zscale[d] = (float)(1.0 / std::sqrt(v));
     }
+    HIPCHK(h, ensure(&b->d_zoff, &b->zoff_cap, (size_t)D));
+    HIPCHK(h, ensure(&b->d_zscale, &b->zscale_cap, (size_t)D));
+    HIPCHK(h, ensure(&b->d_P16, &b->P16_cap, (size_t)((M + 31) / 32) * KPm * 64));
+    HIPCHK(h, hipMemcpyAsync(b->d_zoff, zoff.data(), D * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(b->d_zscale, zscale.data(), D * 4, hipMemcpyHostToDevice, h->stream));
     bool overflow = false;
-    const std::vector<uint16_t> P = pack_split_model(h, prec, KPm, zoff.data(), zscale.data(), &overflow);
+    int rc = pack_split_model(h, prec, KPm, b->d_zoff, b->d_zscale, b->d_P16, &overflow);
+    if (rc) return rc;
     b->p16_model = h->model_version;
     b->p16_prec = prec;
     b->p16_ok = !overflow;
     *ok = b->p16_ok;
-    if (overflow) return GMMHIP_OK;
-    HIPCHK(h, ensure(&b->d_zoff, &b->zoff_cap, (size_t)D));
-    HIPCHK(h, ensure(&b->d_zscale, &b->zscale_cap, (size_t)D));
-    HIPCHK(h, ensure(&b->d_P16, &b->P16_cap, P.size()));
-    HIPCHK(h, hipMemcpyAsync(b->d_zoff, zoff.data(), D * 4, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(b->d_zscale, zscale.data(), D * 4, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(b->d_P16, P.data(), P.size() * 2, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
     return GMMHIP_OK;
 }
 

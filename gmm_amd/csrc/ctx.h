@@ -106,6 +106,9 @@ struct gmmhip_ctx {
     gmmhip_batch_state *batch = nullptr;
     float *d_mu_f32 = nullptr, *d_var_f32 = nullptr;  // UBM means / variances
     size_t mu_f32_cap = 0, var_f32_cap = 0;
+    double *d_gconst = nullptr;  // per-component Gaussian constants (FP64)
+    size_t gconst_cap = 0;
+    unsigned int *d_flag = nullptr;  // split-model overflow flag
     uint64_t mu_f32_version = ~0ull;
     int num_cus = 0;
 
@@ -222,16 +225,19 @@ int gemm_posteriors(gmmhip_ctx *h, float *ll, float *post);
 // per row i, packed like P), the prior k and 1/var in FP64 (batch_api.hip).  Valid until the next gmmhip_set_model.
 struct ModelOperands {
     const float *mu, *var;
+    const double *gconst;
     const double *P, *Xi, *Ginit, *kinit, *ivar;
 };
 int model_operands(gmmhip_ctx *h, bool mllr, ModelOperands *o);
 
-// Split model of the mixed form for the feature map z = (x - zoff) * zscale:
-// [Mt32][KPm/16][hi, lo][64 lanes][8] 16-bit pieces of the log2-unit
-// coefficients [const, nu*omega, -omega/2] (gmmhip_api.hip).  *overflow is
-// set when a coefficient does not fit the 16-bit type.
-std::vector<uint16_t> pack_split_model(const gmmhip_ctx *h, int prec, int KPm, const float *zoff,
-                                       const float *zscale, bool *overflow);
+// Split model of the mixed form for the feature map z = (x - zoff) * zscale
+// (device arrays of D): [Mt32][KPm/16][hi, lo][64 lanes][8] 16-bit pieces of
+// the log2-unit coefficients [const, nu*omega, -omega/2], formed on the
+// device into d_out (Mt32 * KPm * 64 values) on the handle's stream
+// (gmmhip_api.hip).  *overflow is set when a coefficient does not fit the
+// 16-bit type; the call waits for the stream.
+int pack_split_model(gmmhip_ctx *h, int prec, int KPm, const float *d_zoff, const float *d_zscale,
+                     uint16_t *d_out, bool *overflow);
 
 // Frames per statistics work item of an E-step over N frames: the
 // opts.chunk_frames override, else enough items for ~4096 blocks, capped by

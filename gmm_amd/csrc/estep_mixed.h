@@ -57,6 +57,22 @@ struct MixedArgs {
     int queue_blocks;
 };
 
+// Split model of the mixed form: for the feature map z = (x - off) * scale,
+// the log2-unit coefficients [const, nu*omega, -omega/2] of component m
+// (nu = scale (mu - off), omega = 1 / (var scale^2)), formed in FP64 and
+// split hi = T(v), lo = T(v - hi), in the MFMA B-operand layout
+// [Mt32][KP/16][hi, lo][64 lanes][8]; component slots past M are zero.
+struct SplitModelArgs {
+    const float *mu, *var;     // [M][D]
+    const double *gconst;      // [M]
+    const float *off, *scale;  // [D]
+    int M, D, KP, Mt32;
+    uint16_t *out;             // Mt32 * 32 slots * KP * 2 (hi, lo) values
+    unsigned int *overflow;    // set non-zero when a hi piece is not finite
+};
+// prec: GMMHIP_PREC_BF16X3 (1) or GMMHIP_PREC_FP16X3 (2)
+hipError_t launch_pack_split_model(const SplitModelArgs &a, int prec, hipStream_t s);
+
 hipError_t launch_colstats(const float *X, uint64_t N, int D, int blocks, uint64_t per_block,
                            double *part, hipStream_t s);
 // prec: GMMHIP_PREC_BF16X3 (1) or GMMHIP_PREC_FP16X3 (2)

@@ -65,7 +65,9 @@ __device__ __forceinline__ float log2f_fast(float x) { return __builtin_amdgcn_l
 
 // ---------------------------------------------------------------------------
 // Per-dimension sum and sum of squares in FP64, fixed order.  Thread
-// (g, d) of a block walks frames start+g, start+g+G, ... (G = 256 / D).
+// (g, d) of a block walks frames start+g, start+g+G, ... (G = 256 / D),
+// eight frames a round with their eight loads in flight before the adds
+// (one load at a time left the kernel latency-bound at 2 TB/s).
 __global__ __launch_bounds__(256) void colstats_kernel(const float *__restrict__ X, uint64_t N,
                                                        int D, uint64_t per_block,
                                                        double *__restrict__ part) {
@@ -78,7 +80,21 @@ __global__ __launch_bounds__(256) void colstats_kernel(const float *__restrict__
     uint64_t end = start + per_block;
     if (end > N) end = N;
     if (g < G) {
-        for (uint64_t n = start + g; n < end; n += G) {
+        uint64_t n = start + g;
+        const uint64_t step = (uint64_t)G * D;
+        for (; n + 7 * (uint64_t)G < end; n += 8 * (uint64_t)G) {
+            const float *xp = X + n * D + d;
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = xp[j * step];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const double w = v[j];
+                s1 += w;
+                s2 += w * w;
+            }
+        }
+        for (; n < end; n += G) {
             const double v = X[n * D + d];
             s1 += v;
             s2 += v * v;
@@ -99,6 +115,104 @@ __global__ __launch_bounds__(256) void colstats_kernel(const float *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Split model.  binary16 pieces round once from the FP64 coefficient (round
+// to nearest even), bfloat16 pieces from its binary32 value, in integer
+// arithmetic (the same bits the 16-bit casts give).
+__device__ __forceinline__ uint16_t f64_to_f16_bits(double v) {
+    const uint64_t x = __builtin_bit_cast(uint64_t, v);
+    const uint32_t sign = (uint32_t)((x >> 48) & 0x8000u);
+    const uint64_t ax = x & 0x7fffffffffffffffull;
+    // normal range: rebias the exponent (1023 -> 15), round the 52-bit mantissa to 10
+    const uint32_t rn = (uint32_t)((ax - (1008ull << 52) + ((1ull << 41) - 1) + ((ax >> 42) & 1)) >> 42);
+    // below 2^-14: a multiple of 2^-24 (exact scaling), rounded to even by the 2^52 shift
+    const double units = __builtin_bit_cast(double, ax) * 16777216.0;
+    const uint32_t rs = (uint32_t)((units + 4503599627370496.0) - 4503599627370496.0);
+    uint32_t r = ax < 0x3f10000000000000ull ? rs : rn;
+    r = ax >= 0x40effe0000000000ull ? 0x7c00u : r;  // >= 65520 rounds to infinity
+    r = ax > 0x7ff0000000000000ull ? 0x7e00u : r;   // NaN
+    return (uint16_t)(sign | r);
+}
+__device__ __forceinline__ float f16_bits_to_f32(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    const float sub = (float)m * 5.9604644775390625e-08f;  // m * 2^-24
+    const uint32_t b = e == 31 ? (sign | 0x7f800000u | (m << 13)) : (sign | ((e + 112u) << 23) | (m << 13));
+    return e == 0 ? (sign ? -sub : sub) : __builtin_bit_cast(float, b);
+}
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+    const uint32_t x = __builtin_bit_cast(uint32_t, f);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+    return (uint16_t)((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+}
+
+// One wave per component slot: the wave forms the D pairs (omega, nu) and
+// the linear and quadratic coefficients, lane 0 the constant (its sum over
+// d in order), then the lanes split and store the KP coefficients.  FP64
+// arithmetic as written: no contraction into fused multiply-adds.
+template <bool FP16>
+__global__ __launch_bounds__(64) void pack_split_model_kernel(SplitModelArgs a) {
+#pragma clang fp contract(off)
+    __shared__ double om[PREP_MAX_D], nu[PREP_MAX_D], coef[256];
+    const int m = blockIdx.x, lane = threadIdx.x;
+    const int D = a.D, KP = a.KP;
+    const bool real = m < a.M;
+    for (int d = lane; d < D; d += 64) {
+        double o = 0.0, n = 0.0;
+        if (real) {
+            const double sd = a.scale[d];
+            n = sd * ((double)a.mu[(size_t)m * D + d] - (double)a.off[d]);
+            o = 1.0 / ((double)a.var[(size_t)m * D + d] * sd * sd);
+        }
+        om[d] = o;
+        nu[d] = n;
+        coef[1 + d] = o * n;
+        coef[1 + D + d] = -0.5 * o;
+    }
+    for (int k = 2 * D + 1 + lane; k < KP; k += 64) coef[k] = 0.0;
+    __syncthreads();
+    if (lane == 0) {
+        double cq = 0.0;
+        for (int d = 0; d < D; d++) cq += om[d] * nu[d] * nu[d];
+        coef[0] = real ? a.gconst[m] - 0.5 * cq : 0.0;
+    }
+    __syncthreads();
+    const double log2e = 1.4426950408889634;
+    const int t = m >> 5, c = m & 31, KS = KP / 16;
+    bool bad = false;
+    for (int k = lane; k < KP; k += 64) {
+        const double v = coef[k] * log2e;
+        uint16_t hb, lb;
+        float hf;
+        if constexpr (FP16) {
+            hb = f64_to_f16_bits(v);
+            hf = f16_bits_to_f32(hb);
+            lb = f64_to_f16_bits(v - (double)hf);
+        } else {
+            hb = f32_to_bf16_bits((float)v);
+            hf = __builtin_bit_cast(float, (uint32_t)hb << 16);
+            lb = f32_to_bf16_bits((float)(v - (double)hf));
+        }
+        bad |= !__builtin_isfinite(hf);
+        const int ks = k >> 4, hh = (k >> 3) & 1, j = k & 7;
+        const size_t ih = ((((size_t)t * KS + ks) * 2 + 0) * 64 + 32 * hh + c) * 8 + j;
+        a.out[ih] = hb;
+        a.out[ih + 64 * 8] = lb;
+    }
+    if (bad) atomicOr(a.overflow, 1u);
+}
+
+hipError_t launch_pack_split_model(const SplitModelArgs &a, int prec, hipStream_t s) {
+    if (a.D < 1 || a.D > PREP_MAX_D || a.KP < 2 * a.D + 1 || a.KP > 256 || a.KP % 32 ||
+        a.Mt32 * 32 < a.M)
+        return hipErrorInvalidValue;
+    if (prec == 2)
+        hipLaunchKernelGGL(pack_split_model_kernel<true>, dim3((unsigned)(a.Mt32 * 32)), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(pack_split_model_kernel<false>, dim3((unsigned)(a.Mt32 * 32)), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Split frame features into hi/lo pieces in both layouts.  One block per
 // 32-frame block: the block's centred features are staged in LDS once, then
 // every store is one 16-byte piece (8 features of a frame for ZA, 8 frames
@@ -114,35 +228,41 @@ __device__ __forceinline__ unsigned int split_bits(float v, unsigned int &lo) {
     return __builtin_bit_cast(uint16_t, hi);
 }
 
-template <typename T>
+template <typename T, int KP>
 __global__ __launch_bounds__(256) void prep_frames_kernel(PrepArgs p) {
     __shared__ float zs[32 * PREP_MAX_D];
     __shared__ unsigned int flags[32];
     __shared__ unsigned int cnt;
     const int tid = threadIdx.x;
-    const int D = p.D, KP = p.KP;
+    const int D = p.D;
     const uint64_t blk = blockIdx.x;
     const uint64_t n0 = blk * 32;
     const int nf = (n0 >= p.N) ? 0 : (int)((p.N - n0) < 32 ? (p.N - n0) : 32);
     if (tid < 32) flags[tid] = 0u;
     if (tid == 0) cnt = 0u;
     __syncthreads();
-    // the block's frames are 32 x D consecutive floats
-    for (int e = tid; e < nf * D; e += 256) {
-        const int f = e / D, d = e - f * D;
-        const float x = p.X[n0 * D + e];
-        float z = (x - p.off[d]) * p.scale[d];
-        if (!isfinite(x)) {
-            // non-finite frame: its log-likelihood is clamped, no statistics
-            z = 0.0f;
-            atomicOr(&flags[f], 1u);
-        } else if (fabsf(z) > p.zlimit) {
-            // outside the 16-bit range: the call runs in FP32 instead
-            z = 0.0f;
-            atomicOr(&flags[f], 2u);
-            atomicAdd(&cnt, 1u);
+    // the block's frames are 32 x D consecutive floats; thread (g, d) takes
+    // dimension d of frames g, g + G, ... (G = 256 / D rows per pass, one
+    // division per thread instead of one per element)
+    const int G = 256 / D;
+    const int g = tid / D, d = tid - g * D;
+    if (g < G) {
+        const float o = p.off[d], sc = p.scale[d];
+        for (int f = g; f < nf; f += G) {
+            const float x = p.X[(n0 + f) * D + d];
+            float z = (x - o) * sc;
+            if (!isfinite(x)) {
+                // non-finite frame: its log-likelihood is clamped, no statistics
+                z = 0.0f;
+                atomicOr(&flags[f], 1u);
+            } else if (fabsf(z) > p.zlimit) {
+                // outside the 16-bit range: the call runs in FP32 instead
+                z = 0.0f;
+                atomicOr(&flags[f], 2u);
+                atomicAdd(&cnt, 1u);
+            }
+            zs[f * D + d] = z;
         }
-        zs[e] = z;
     }
     __syncthreads();
     if (tid < nf) {
@@ -162,8 +282,8 @@ __global__ __launch_bounds__(256) void prep_frames_kernel(PrepArgs p) {
         }
         return 0.0f;
     };
-    const int U = KP / 8, CT = KP / 32;
-    const int pieces = 32 * U;  // per layout and part
+    constexpr int U = KP / 8, CT = KP / 32;
+    constexpr int pieces = 32 * U;  // per layout and part
     u32x4 *zah = reinterpret_cast<u32x4 *>(p.za_hi_raw) + blk * pieces;
     u32x4 *zal = reinterpret_cast<u32x4 *>(p.za_lo_raw) + blk * pieces;
     u32x4 *zbh = p.zb_hi_raw ? reinterpret_cast<u32x4 *>(p.zb_hi_raw) + blk * pieces : nullptr;
@@ -990,10 +1110,18 @@ hipError_t launch_prep_frames(const PrepArgs &p, int prec, hipStream_t s) {
     if (p.D < 1 || p.D > PREP_MAX_D || p.KP % 32 || p.KP < 2 * p.D + 1 || p.Npad % 32)
         return hipErrorInvalidValue;
     const dim3 grid((unsigned)(p.Npad / 32));
-    if (prec == 2)
-        hipLaunchKernelGGL(prep_frames_kernel<_Float16>, grid, dim3(256), 0, s, p);
-    else
-        hipLaunchKernelGGL(prep_frames_kernel<__bf16>, grid, dim3(256), 0, s, p);
+    switch (p.KP) {
+#define X(K)                                                                        \
+    case K:                                                                         \
+        if (prec == 2)                                                              \
+            hipLaunchKernelGGL((prep_frames_kernel<_Float16, K>), grid, dim3(256), 0, s, p); \
+        else                                                                        \
+            hipLaunchKernelGGL((prep_frames_kernel<__bf16, K>), grid, dim3(256), 0, s, p);   \
+        break;
+        GMMHIP_KP32_CASES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

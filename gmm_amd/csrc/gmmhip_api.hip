@@ -6,6 +6,7 @@
 // the MFMA B-operand layout; frames stay resident in HBM row-major, so the
 // reference's host transpose into 8-frame tiles (alignVectors,
 // trainer/OL_GMMStatsEstimator_CUDA.cpp:292-328) has no counterpart here.
+#include "batch.h"
 #include "ctx.h"
 #include "tune.h"
 
@@ -88,15 +89,6 @@ void add_time(gmmhip_ctx *h, int cls, int e0, int e1) {
 // Frames per mixed-precision feature row block (the log-likelihood kernel
 // works on 128 frames per block).
 
-template <typename T>
-void split_store(double v, uint16_t *hi, uint16_t *lo, bool *overflow) {
-    const T h = (T)(float)v;
-    const T l = (T)(float)(v - (double)(float)h);
-    if (!std::isfinite((float)h)) *overflow = true;
-    *hi = __builtin_bit_cast(uint16_t, h);
-    *lo = __builtin_bit_cast(uint16_t, l);
-}
-
 }  // namespace
 
 namespace gmmhip_detail {
@@ -141,36 +133,31 @@ uint64_t tuned_chunk(const gmmhip_ctx *h, int path, int prec) {
     return (cf + ST_FRAMES_PER_ITER - 1) / ST_FRAMES_PER_ITER * ST_FRAMES_PER_ITER;
 }
 
-std::vector<uint16_t> pack_split_model(const gmmhip_ctx *h, int prec, int KPm, const float *zoff,
-                                       const float *zscale, bool *overflow) {
-    const int M = h->M, D = h->D, Mt32 = (M + 31) / 32, KS = KPm / 16;
-    std::vector<uint16_t> P((size_t)Mt32 * KS * 2 * 64 * 8, 0);
-    std::vector<double> coef(KPm);
-    const double log2e = 1.4426950408889634;
-    for (int m = 0; m < M; m++) {
-        std::fill(coef.begin(), coef.end(), 0.0);
-        double cq = 0.0;
-        for (int d = 0; d < D; d++) {
-            const double sd = zscale[d];
-            const double nu = sd * ((double)h->m_mu[(size_t)m * D + d] - (double)zoff[d]);
-            const double om = 1.0 / ((double)h->m_var[(size_t)m * D + d] * sd * sd);
-            coef[1 + d] = om * nu;
-            coef[1 + D + d] = -0.5 * om;
-            cq += om * nu * nu;
-        }
-        coef[0] = h->m_gconst[m] - 0.5 * cq;
-        const int t = m / 32, c = m % 32;
-        for (int k = 0; k < KPm; k++) {
-            const int ks = k / 16, hh = (k % 16) / 8, j = k % 8;
-            const size_t ih = ((((size_t)t * KS + ks) * 2 + 0) * 64 + 32 * hh + c) * 8 + j;
-            const size_t il = ((((size_t)t * KS + ks) * 2 + 1) * 64 + 32 * hh + c) * 8 + j;
-            if (prec == GMMHIP_PREC_FP16X3)
-                split_store<_Float16>(coef[k] * log2e, &P[ih], &P[il], overflow);
-            else
-                split_store<__bf16>(coef[k] * log2e, &P[ih], &P[il], overflow);
-        }
-    }
-    return P;
+int pack_split_model(gmmhip_ctx *h, int prec, int KPm, const float *d_zoff, const float *d_zscale,
+                     uint16_t *d_out, bool *overflow) {
+    ModelOperands mo;
+    int rc = model_operands(h, false, &mo);
+    if (rc) return rc;
+    if (!h->d_flag) HIPCHK(h, hipMalloc((void **)&h->d_flag, sizeof(unsigned int)));
+    HIPCHK(h, launch_fill_u32(h->d_flag, 0u, 1, h->stream));
+    SplitModelArgs a{};
+    a.mu = mo.mu;
+    a.var = mo.var;
+    a.gconst = mo.gconst;
+    a.off = d_zoff;
+    a.scale = d_zscale;
+    a.M = h->M;
+    a.D = h->D;
+    a.KP = KPm;
+    a.Mt32 = (h->M + 31) / 32;
+    a.out = d_out;
+    a.overflow = h->d_flag;
+    HIPCHK(h, launch_pack_split_model(a, prec, h->stream));
+    unsigned int flag = 0;
+    HIPCHK(h, hipMemcpyAsync(&flag, h->d_flag, sizeof(flag), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    *overflow = flag != 0;
+    return GMMHIP_OK;
 }
 
 }  // namespace gmmhip_detail
@@ -272,14 +259,11 @@ int prepare_mixed(gmmhip_ctx *h, bool *use_mixed) {
     if (h->z_range_fallback) return GMMHIP_OK;
     if (h->p16_model != h->model_version || h->p16_z != h->z_version || h->p16_prec != prec) {
         bool overflow = false;
-        const std::vector<uint16_t> P =
-            pack_split_model(h, prec, KPm, h->h_zoff.data(), h->h_zscale.data(), &overflow);
         uint16_t *p16 = static_cast<uint16_t *>(h->d_P16);
-        HIPCHK(h, ensure(&p16, &h->P16_cap, P.size()));
+        HIPCHK(h, ensure(&p16, &h->P16_cap, (size_t)((h->M + 31) / 32) * KPm * 64));
         h->d_P16 = p16;
-        HIPCHK(h, hipMemcpyAsync(h->d_P16, P.data(), P.size() * sizeof(uint16_t),
-                                 hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
+        const int rc = pack_split_model(h, prec, KPm, h->d_zoff, h->d_zscale, p16, &overflow);
+        if (rc) return rc;
         h->p16_model = h->model_version;
         h->p16_z = h->z_version;
         h->p16_prec = prec;
@@ -386,6 +370,8 @@ int gmmhip_destroy(gmmhip_t h) {
     hipFree(h->d_mask);
     hipFree(h->d_mu_f32);
     hipFree(h->d_var_f32);
+    hipFree(h->d_gconst);
+    hipFree(h->d_flag);
     if (h->h_res) hipHostFree(h->h_res);
     if (h->ev_ok)
         for (auto &e : h->ev) hipEventDestroy(e);

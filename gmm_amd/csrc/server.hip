@@ -631,17 +631,17 @@ int setup_mixed(gmmhip_server *s, bool *ok) {
         s->h_zoff[d] = (float)o;
         if (std::isfinite(v) && v > 0.0) s->h_zscale[d] = (float)(1.0 / std::sqrt(v));
     }
-    bool overflow = false;
-    const std::vector<uint16_t> P =
-        pack_split_model(b, s->prec, s->d.KPm, s->h_zoff.data(), s->h_zscale.data(), &overflow);
-    *ok = !overflow;
-    if (overflow) return GMMHIP_OK;
     SRV_HIPCHK(hipMalloc((void **)&s->d_zoff, D * 4));
     SRV_HIPCHK(hipMalloc((void **)&s->d_zscale, D * 4));
-    SRV_HIPCHK(hipMalloc(&s->d_P16, P.size() * 2));
+    SRV_HIPCHK(hipMalloc(&s->d_P16, (size_t)((M + 31) / 32) * s->d.KPm * 64 * 2));
     SRV_HIPCHK(hipMemcpy(s->d_zoff, s->h_zoff.data(), D * 4, hipMemcpyHostToDevice));
     SRV_HIPCHK(hipMemcpy(s->d_zscale, s->h_zscale.data(), D * 4, hipMemcpyHostToDevice));
-    SRV_HIPCHK(hipMemcpy(s->d_P16, P.data(), P.size() * 2, hipMemcpyHostToDevice));
+    bool overflow = false;
+    const int rc = pack_split_model(b, s->prec, s->d.KPm, s->d_zoff, s->d_zscale,
+                                    static_cast<uint16_t *>(s->d_P16), &overflow);
+    if (rc) return rc;
+    *ok = !overflow;
+    if (overflow) return GMMHIP_OK;
     // frames with |z| above this run in FP32 (prep_frames flags > 240)
     s->zlimit = s->prec == GMMHIP_PREC_FP16X3 ? 240.0f : INFINITY;
     return GMMHIP_OK;
