@@ -453,7 +453,16 @@ __global__ __launch_bounds__(256) void reduce_kernel(const float *__restrict__ p
     const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= E) return;
     double s = 0.0;
-    for (int c = 0; c < nchunks; c++) s += (double)part[(size_t)c * E + e];
+    int c = 0;
+    // eight chunks' loads in flight, added in chunk order
+    for (; c + 8 <= nchunks; c += 8) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = part[(size_t)(c + j) * E + e];
+#pragma unroll
+        for (int j = 0; j < 8; j++) s += (double)v[j];
+    }
+    for (; c < nchunks; c++) s += (double)part[(size_t)c * E + e];
     out[e] = s;
 }
 
@@ -463,7 +472,16 @@ __global__ __launch_bounds__(256) void sum_blockll_kernel(const double *__restri
                                                           double *__restrict__ out) {
     __shared__ double sh[256];
     double s = 0.0;
-    for (uint64_t i = threadIdx.x; i < n; i += 256) s += v[i];
+    uint64_t i = threadIdx.x;
+    // eight loads in flight, added in index order
+    for (; i + 7 * 256 < n; i += 8 * 256) {
+        double t[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) t[j] = v[i + 256 * j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) s += t[j];
+    }
+    for (; i < n; i += 256) s += v[i];
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
