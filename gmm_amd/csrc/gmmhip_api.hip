@@ -135,6 +135,7 @@ uint64_t tuned_chunk(const gmmhip_ctx *h, int path, int prec) {
 
 int pack_split_model(gmmhip_ctx *h, int prec, int KPm, const float *d_zoff, const float *d_zscale,
                      uint16_t *d_out, bool *overflow) {
+    DevGuard g(h->dev);
     ModelOperands mo;
     int rc = model_operands(h, false, &mo);
     if (rc) return rc;
