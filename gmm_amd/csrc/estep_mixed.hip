@@ -1031,8 +1031,12 @@ __global__ __launch_bounds__(64 * NW) void loglik_mixed_dma_kernel(MixedArgs a) 
     // Fast path: no value of the wave's tile lies more than LSE_HEADROOM
     // above its row reference, and sm += 2^d (sm stays below 2^(11+64)).
     // Otherwise the whole tile takes the general update, which moves the
-    // reference of every value that exceeded it.
-#define GMMHIP_LL_STEP(ACC_, NACC_, T_, PR_, PW_)                                    \
+    // reference of every value that exceeded it.  The running sums alternate
+    // between two register sets: the fast-path sums SMN = SMC + 2^d are
+    // formed among the tile's MFMAs, before the ballot that decides the path
+    // (after it, the adds found no MFMA left to overlap), and the general
+    // update rebuilds SMN from the untouched SMC.
+#define GMMHIP_LL_STEP(ACC_, NACC_, T_, PR_, PW_, SMC, SMN)                          \
     {                                                                                \
         __builtin_amdgcn_s_waitcnt(0);                                               \
         __syncthreads();                                                             \
@@ -1047,20 +1051,16 @@ __global__ __launch_bounds__(64 * NW) void loglik_mixed_dma_kernel(MixedArgs a) 
         _Pragma("unroll") for (int r = 0; r < 16; r++) {                             \
             d[r] = ACC_[r] - mx[r];                                                  \
             e[r] = exp2f_fast(d[r]);                                                 \
+            SMN[r] = SMC[r] + e[r];                                                  \
         }                                                                            \
         float dm = fmaxf(fmaxf(d[0], d[1]), d[2]);                                   \
         _Pragma("unroll") for (int r = 3; r < 16; r += 2) dm = fmaxf(fmaxf(dm, d[r]), d[r + 1 < 16 ? r + 1 : r]); \
         GMMHIP_LL_SCHED                                                              \
-        if (__builtin_amdgcn_ballot_w64(dm > LSE_HEADROOM) == 0) {                   \
-            _Pragma("unroll") for (int r = 0; r < 16; r++) sm[r] += e[r];            \
-        } else {                                                                     \
-            /* a real branch (sm opaque here, so the common path's sums are   */ \
-            /* not hoisted above it into separate registers)                     */ \
-            _Pragma("unroll") for (int r = 0; r < 16; r++) asm volatile("" : "+v"(sm[r])); \
+        if (__builtin_amdgcn_ballot_w64(dm > LSE_HEADROOM) != 0) {                   \
             _Pragma("unroll") for (int r = 0; r < 16; r++) {                         \
                 const bool up = d[r] > LSE_HEADROOM;                                 \
                 const float ee = up ? exp2f_fast(-d[r]) : e[r];                      \
-                sm[r] = up ? fmaf(sm[r], ee, 1.0f) : sm[r] + ee;                     \
+                SMN[r] = up ? fmaf(SMC[r], ee, 1.0f) : SMC[r] + ee;                  \
                 mx[r] = up ? ACC_[r] : mx[r];                                        \
             }                                                                        \
         }                                                                            \
@@ -1076,19 +1076,22 @@ __global__ __launch_bounds__(64 * NW) void loglik_mixed_dma_kernel(MixedArgs a) 
         if (ks + 2 < KS) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);          \
         _Pragma("unroll") for (int i = 0; i < 3; i++) {                              \
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                       \
-            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);                       \
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);                       \
         }                                                                            \
     }
     f32x16 acc2;
+    float sm2[16];
     int t = 1;
     // tile t sits in Pl1 for odd t, Pl0 for even t
     for (; t + 1 < a.Mt32; t += 2) {
-        GMMHIP_LL_STEP(acc, acc2, t, Pl1, Pl0)
-        GMMHIP_LL_STEP(acc2, acc, t + 1, Pl0, Pl1)
+        GMMHIP_LL_STEP(acc, acc2, t, Pl1, Pl0, sm, sm2)
+        GMMHIP_LL_STEP(acc2, acc, t + 1, Pl0, Pl1, sm2, sm)
     }
     if (t < a.Mt32) {
-        GMMHIP_LL_STEP(acc, acc2, t, Pl1, Pl0)
+        GMMHIP_LL_STEP(acc, acc2, t, Pl1, Pl0, sm, sm2)
         acc = acc2;
+#pragma unroll
+        for (int r = 0; r < 16; r++) sm[r] = sm2[r];
     }
 #undef GMMHIP_LL_STEP
 #undef GMMHIP_LL_READ_ORDER
