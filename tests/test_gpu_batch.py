@@ -157,15 +157,16 @@ def test_batch_mllr_wide_and_many_components(acc, D, M):
 
 
 @pytest.mark.parametrize("prec,tol", [(gmm_amd.PREC_FP16X3, 2e-4), (gmm_amd.PREC_BF16X3, 2e-3)])
-@pytest.mark.parametrize("D", [12, 30, 45])
+@pytest.mark.parametrize("D", [12, 30, 45, 120])
 def test_batch_mixed_precision(acc, prec, tol, D):
     """The mixed form in the batched path: per-utterance statistics, MAP
     means and MLLR transforms against the oracle (mixed-precision
     tolerances, as the E-step's), graph replays bit-identical to eager runs,
     and each utterance's result independent of its batch-mates (UBM-based
     centring and scaling, chunks relative to the utterance start).  Depths
-    KP = 32, 64 (statistics by transposing LDS reads) and 96 (the
-    register-staged statistics kernel with the ZB copy)."""
+    KP = 32, 64, 256 (the persistent work-queue form of the statistics kernel
+    with transposing LDS reads) and 96 (the register-staged statistics kernel
+    with the ZB copy, launched as a grid)."""
     x, off, w, mu, var = make_batch(11, 30, 64, D, lo=20, hi=600, extra=(0, 5))
     acc.set_model(w, mu, var)
     ref = per_utt_oracle(x, off, w, mu, var)
@@ -239,3 +240,23 @@ def test_batch_graph_recaptured_after_reallocation(acc):
     for k in eager:
         for g in (g1, g1b, g2, g2b):
             assert np.array_equal(eager[k], g[k]), k
+
+
+def test_batch_mixed_model_overflow_runs_fp32(acc):
+    """A UBM coefficient beyond the FP16 range (a variance of 2e-6) is flagged
+    by the split-model packing on the GPU and sends the fp16x3 batch to the
+    FP32 kernels: the results equal the FP32 run bit for bit."""
+    x, off, w, mu, var = make_batch(14, 8, 32, 6, lo=50, hi=300)
+    var = var.copy()
+    var[7, 2] = 2e-6  # just above the 1e-6 variance floor
+    acc.set_model(w, mu, var)
+    fp32 = acc.batch_adapt(x, off, tau=16)
+    acc.set_opts(precision=gmm_amd.PREC_FP16X3)
+    try:
+        mixed = acc.batch_adapt(x, off, tau=16)
+        tiles = acc.tiles_used()
+    finally:
+        acc.set_opts(precision=gmm_amd.PREC_FP32)
+    assert tiles[2] == 0, tiles
+    for k in fp32:
+        assert np.array_equal(fp32[k], mixed[k]), k

@@ -170,3 +170,38 @@ def test_mixed_error_report_ubm_like(acc):
         _check.check_stats((n, F, S, tot), rst, x, w, mu, var, rtol=max(L[2], 1e-5),
                            tot_rtol=L[1], band=_check.BAND[name], what=name + " ")
     acc.set_opts(precision=gmm_amd.PREC_FP32)
+
+
+def test_split_model_overflow_runs_fp32(acc):
+    """A model coefficient beyond the FP16 range (a variance of 2e-6 in one
+    dimension: its quadratic coefficient is ~1e6 after the frames' scaling)
+    is flagged by the split-model packing on the GPU and sends the fp16x3 call
+    to the FP32 kernels: log-likelihoods and statistics equal the FP32 form's
+    bit for bit and no mixed statistics kernel ran.  bf16x3, whose range holds
+    the coefficient, stays mixed."""
+    x, w, mu, var = case(31, 3000, 12, 20)
+    var = var.copy()
+    var[3, 5] = 2e-6  # just above the 1e-6 variance floor
+    acc.set_model(w, mu, var)
+    acc.set_frames(x)
+    acc.set_opts(precision=gmm_amd.PREC_FP32)
+    ll32 = acc.loglikes()
+    r32 = acc.accumulate()
+    try:
+        acc.set_opts(precision=gmm_amd.PREC_FP16X3)
+        ll16 = acc.loglikes()
+        r16 = acc.accumulate()
+        tiles16 = acc.tiles_used()
+        acc.set_opts(precision=gmm_amd.PREC_BF16X3)
+        llb = acc.loglikes().astype(np.float64)
+        acc.accumulate()
+        tilesb = acc.tiles_used()
+    finally:
+        acc.set_opts(precision=gmm_amd.PREC_FP32)
+    assert np.array_equal(ll16, ll32)
+    for a, b in zip(r16, r32):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    assert tiles16[2] == 0, tiles16
+    assert tilesb[2] != 0, tilesb
+    rll = ll32.astype(np.float64)
+    assert np.all(np.abs(llb - rll) <= TOL[gmm_amd.PREC_BF16X3][0] * np.abs(rll) + 1e-6)
