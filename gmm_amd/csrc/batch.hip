@@ -555,15 +555,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ml
                 aop[I] = -sXr[(16 * I + c) * 4 + q];
                 bop[I] = sX[(16 * I + c) * 4 + q];
             }
-            // the column block K in registers becomes L[:, K] below the block
+            // the column block K in registers becomes L[:, K] below the block:
+            // row 16 I + q + 4 r lies below K = 16 Ib + 4 rr .. + 3 exactly
+            // when I > Ib, or I == Ib and r > rr (q < 4), a condition known at
+            // compile time, so the loads write the registers under the lane
+            // mask alone
             if (c >= cb && c < cb + 4) {
+                const int rr = s & 3;
 #pragma unroll
                 for (int I = 0; I < 4; I++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int row = 16 * I + q + 4 * r;
-                        if (I >= Ib && row >= k0 + 4) A[I][Ib][r] = sX[row * 4 + (c - cb)];
-                    }
+                    for (int r = 0; r < 4; r++)
+                        if (I > Ib || (I == Ib && r > rr))
+                            A[I][Ib][r] = sX[(16 * I + q + 4 * r) * 4 + (c - cb)];
             }
 #pragma unroll
             for (int I = 0; I < 4; I++)
