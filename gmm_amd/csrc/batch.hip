@@ -46,9 +46,30 @@ __global__ __launch_bounds__(256) void pad_rows_kernel(const float *__restrict__
     for (uint64_t e = threadIdx.x; e < np; e += 256) y[e] = e < n ? x[e] : 0.0f;
 }
 
+// The same copy in 16-byte pieces (D a multiple of 4, both bases 16-byte
+// aligned): a row of D floats is D/4 pieces.
+__global__ __launch_bounds__(256) void pad_rows4_kernel(const float4 *__restrict__ X,
+                                                        const uint64_t *__restrict__ src,
+                                                        const uint64_t *__restrict__ dst, int D4,
+                                                        float4 *__restrict__ Xp) {
+    const int u = blockIdx.x;
+    const uint64_t n = (src[u + 1] - src[u]) * (uint64_t)D4;
+    const uint64_t np = (dst[u + 1] - dst[u]) * (uint64_t)D4;
+    const float4 *x = X + src[u] * (uint64_t)D4;
+    float4 *y = Xp + dst[u] * (uint64_t)D4;
+    for (uint64_t e = threadIdx.x; e < np; e += 256)
+        y[e] = e < n ? x[e] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 hipError_t launch_pad_rows(const float *X, const uint64_t *src, const uint64_t *dst, int U, int D,
                            float *Xp, hipStream_t s) {
-    hipLaunchKernelGGL(pad_rows_kernel, dim3(U), dim3(256), 0, s, X, src, dst, D, Xp);
+    if (D % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0 &&
+        (reinterpret_cast<uintptr_t>(Xp) & 15) == 0)
+        hipLaunchKernelGGL(pad_rows4_kernel, dim3(U), dim3(256), 0, s,
+                           reinterpret_cast<const float4 *>(X), src, dst, D / 4,
+                           reinterpret_cast<float4 *>(Xp));
+    else
+        hipLaunchKernelGGL(pad_rows_kernel, dim3(U), dim3(256), 0, s, X, src, dst, D, Xp);
     return hipGetLastError();
 }
 
