@@ -1034,8 +1034,12 @@ __global__ __launch_bounds__(64 * NW) void loglik_mixed_dma_kernel(MixedArgs a) 
     // Tile t's MFMAs interleave with the log-sum-exp update of tile t-1.
     // Fast path: no value of the wave's tile lies more than LSE_HEADROOM
     // above its row reference, and sm += 2^d (sm stays below 2^(11+64)).
-    // Otherwise the whole tile takes the general update, which moves the
-    // reference of every value that exceeded it.  The running sums alternate
+    // Otherwise the general update moves the reference of every value that
+    // exceeded it, row register by row register, skipping the registers in
+    // which no lane did: with a narrow-variance model a frame's own
+    // component lies hundreds of log2 units above the first tile's maximum,
+    // so about half the tiles hold such a jump, mostly in one or two of the
+    // sixteen rows a lane carries.  The running sums alternate
     // between two register sets: the fast-path sums SMN = SMC + 2^d are
     // formed among the tile's MFMAs, before the ballot that decides the path
     // (after it, the adds found no MFMA left to overlap), and the general
@@ -1063,6 +1067,7 @@ __global__ __launch_bounds__(64 * NW) void loglik_mixed_dma_kernel(MixedArgs a) 
         if (__builtin_amdgcn_ballot_w64(dm > LSE_HEADROOM) != 0) {                   \
             _Pragma("unroll") for (int r = 0; r < 16; r++) {                         \
                 const bool up = d[r] > LSE_HEADROOM;                                 \
+                if (__builtin_amdgcn_ballot_w64(up) == 0) continue;                  \
                 const float ee = up ? exp2f_fast(-d[r]) : e[r];                      \
                 SMN[r] = up ? fmaf(SMC[r], ee, 1.0f) : SMC[r] + ee;                  \
                 mx[r] = up ? ACC_[r] : mx[r];                                        \
