@@ -260,3 +260,43 @@ def test_batch_mixed_model_overflow_runs_fp32(acc):
     assert tiles[2] == 0, tiles
     for k in fp32:
         assert np.array_equal(fp32[k], mixed[k]), k
+
+
+@pytest.mark.parametrize("prec", [gmm_amd.PREC_FP32, gmm_amd.PREC_FP16X3])
+def test_batch_device_frames_any_alignment(acc, prec):
+    """Device-resident frames at a 16-byte aligned address and at one float
+    past it: the row padding copies 16-byte pieces in the first case and
+    single floats in the second, and every output is the same bit for bit."""
+    import torch
+    x, off, w, mu, var = make_batch(31, 24, 64, 12, lo=20, hi=500, extra=(0, 7))
+    N, D = x.shape
+    M = mu.shape[0]
+    U = len(off) - 1
+    acc.set_model(w, mu, var)
+    buf = torch.zeros(N * D + 8, dtype=torch.float32, device="cuda")
+    base = buf.data_ptr()
+    assert base % 16 == 0
+    xt = torch.from_numpy(x.reshape(-1))
+
+    def run(shift):
+        buf.zero_()
+        buf[shift:shift + N * D] = xt.to("cuda")
+        o = {"means": torch.empty((U, M, D), dtype=torch.float32, device="cuda"),
+             "occ": torch.empty((U, M), dtype=torch.float64, device="cuda"),
+             "first": torch.empty((U, M, D), dtype=torch.float64, device="cuda"),
+             "W": torch.empty((U, D, D + 1), dtype=torch.float64, device="cuda"),
+             "status": torch.empty(U, dtype=torch.int32, device="cuda")}
+        acc.batch_adapt_device(base + 4 * shift, off, D, {k: v.data_ptr() for k, v in o.items()},
+                               tau=16, prior=True)
+        torch.cuda.synchronize()
+        return {k: v.cpu().numpy() for k, v in o.items()}
+
+    acc.set_opts(precision=prec)
+    try:
+        aligned, shifted = run(0), run(1)
+        host = acc.batch_adapt(x, off, tau=16, prior=True)
+    finally:
+        acc.set_opts(precision=gmm_amd.PREC_FP32)
+    for k in aligned:
+        assert np.array_equal(aligned[k], shifted[k]), k
+        assert np.array_equal(aligned[k], host[k]), k
