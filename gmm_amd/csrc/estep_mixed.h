@@ -6,8 +6,18 @@
 
 namespace gmmhip {
 
-// Contraction depth of the mixed form: [1, z(D), z^2(D)] padded to 32.
-inline int kp_mixed_for_dim(int D) { return (2 * D + 1 + 31) / 32 * 32; }
+// Contraction depth of the mixed log-likelihoods: [1, z(D), z^2(D)] padded
+// to 32.
+inline int kp_mixed_contract(int D) { return (2 * D + 1 + 31) / 32 * 32; }
+// Layout depth of the split feature rows and the split model (KP below):
+// the contraction depth rounded up to a power of two, zero columns past
+// 2D+1, so that every depth takes the statistics kernel with transposing
+// LDS reads.  The log-likelihood kernel reads only the first
+// kp_mixed_contract(D) columns.
+inline int kp_mixed_for_dim(int D) {
+    const int k = kp_mixed_contract(D);
+    return k <= 32 ? 32 : k <= 64 ? 64 : k <= 128 ? 128 : 256;
+}
 
 // The 32-frame-block copy ZB is read only by the statistics kernel of depths
 // that are not a power-of-two number of 16-byte pieces; at KP = 32, 64, 128,
@@ -78,10 +88,11 @@ hipError_t launch_colstats(const float *X, uint64_t N, int D, int blocks, uint64
 // prec: GMMHIP_PREC_BF16X3 (1) or GMMHIP_PREC_FP16X3 (2)
 hipError_t launch_prep_frames(const PrepArgs &p, int prec, hipStream_t s);
 hipError_t launch_loglik_mixed(const MixedArgs &a, int prec, hipStream_t s);
-// Frames per block of the mixed log-likelihood kernel at depth KP with
-// ll_nw waves (4, or 8 where KP is a multiple of 64): the number of
-// per-block log-likelihood sums it writes is ceil(Npad / that).
-int mixed_ll_frames_per_block(int KP, int nw);
+// Frames per block of the mixed log-likelihood kernel at feature dimension D
+// with nw waves (4, or 8 where the contraction depth is a multiple of 64):
+// the number of per-block log-likelihood sums it writes is ceil(Npad /
+// that).
+int mixed_ll_frames_per_block(int D, int nw);
 // Statistics waves per block for a component tile of comp_tile components
 // (128 or 256) at depth KP: 8 for 256 at KP 64 or 128, else 4.
 int mixed_stats_waves(int KP, int comp_tile);

@@ -952,11 +952,16 @@ __device__ __forceinline__ void ll_finish(const MixedArgs &a, const f32x16 &acc,
 // the split model through LDS by direct-to-LDS copies (two LDS buffers, one
 // barrier per tile).  hi*hi + hi*lo + lo*hi accumulate into one chain per
 // tile; the log-sum-exp of tile t runs on the VALU between tile t+1's MFMAs.
-template <int KP, typename T, int NW>
+// KP: contraction depth (2D+1 rounded up to 32); KPL: the feature rows' and
+// the split model's layout depth (KP rounded up to a power of two, zero
+// columns past 2D+1), of which the first KP columns are read.
+template <int KP, int KPL, typename T, int NW>
 __global__ __launch_bounds__(64 * NW) void loglik_mixed_dma_kernel(MixedArgs a) {
     typedef typename Vec8<T>::type V8;
+    static_assert(KP <= KPL && KPL % 32 == 0, "contraction within the layout");
     constexpr int KS = KP / 16;
-    constexpr int TILE16 = KS * 2 * 64;      // 16-byte pieces per component tile
+    constexpr int TILE16 = KS * 2 * 64;      // 16-byte pieces per component tile read
+    constexpr int TILEL16 = (KPL / 16) * 2 * 64;  // ... per tile of the layout
     constexpr int PF = TILE16 / (64 * NW);
     static_assert(TILE16 % (64 * NW) == 0, "component tile must split evenly over the block");
     // one __shared__ variable per buffer: the compiler then sees that a copy
@@ -974,8 +979,8 @@ __global__ __launch_bounds__(64 * NW) void loglik_mixed_dma_kernel(MixedArgs a) 
         // rows past Npad (a multiple of 128) belong to no frame: zero operands
         const uint64_t fr = base + 32 * wave + c;
         const bool ok = fr < a.Npad;
-        const u32x4 *zh = reinterpret_cast<const u32x4 *>(a.za_hi) + (ok ? fr : 0) * (KP / 8);
-        const u32x4 *zl = reinterpret_cast<const u32x4 *>(a.za_lo) + (ok ? fr : 0) * (KP / 8);
+        const u32x4 *zh = reinterpret_cast<const u32x4 *>(a.za_hi) + (ok ? fr : 0) * (KPL / 8);
+        const u32x4 *zl = reinterpret_cast<const u32x4 *>(a.za_lo) + (ok ? fr : 0) * (KPL / 8);
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const u32x4 z = (u32x4){0u, 0u, 0u, 0u};
@@ -998,7 +1003,7 @@ __global__ __launch_bounds__(64 * NW) void loglik_mixed_dma_kernel(MixedArgs a) 
 #define GMMHIP_DMA_TILE(T_, PW_)                                                     \
     _Pragma("unroll") for (int j = 0; j < PF; j++) {                                 \
         const int o = (wave * PF + j) * 64;                                          \
-        lds_dma16(dma_rsrc(P16 + (size_t)(T_) * TILE16, TILE16 * 16),                \
+        lds_dma16(dma_rsrc(P16 + (size_t)(T_) * TILEL16, TILE16 * 16),               \
                   (unsigned int)(o + lane) * 16u, &PW_[o]);                          \
     }
 #define GMMHIP_TILE_MFMA(ACC_)                                                       \
@@ -1137,24 +1142,30 @@ hipError_t launch_prep_frames(const PrepArgs &p, int prec, hipStream_t s) {
     return hipGetLastError();
 }
 
-int mixed_ll_frames_per_block(int KP, int nw) { return (nw == 8 && KP % 64 == 0) ? 256 : 128; }
+int mixed_ll_frames_per_block(int D, int nw) {
+    return (nw == 8 && kp_mixed_contract(D) % 64 == 0) ? 256 : 128;
+}
 
 hipError_t launch_loglik_mixed(const MixedArgs &a, int prec, hipStream_t s) {
-    const int fpb = mixed_ll_frames_per_block(a.KP, a.ll_nw);
+    const int KPc = kp_mixed_contract(a.D);
+    if (a.D < 1 || a.D > PREP_MAX_D || a.KP != kp_mixed_for_dim(a.D)) return hipErrorInvalidValue;
+    const int fpb = mixed_ll_frames_per_block(a.D, a.ll_nw);
     const dim3 grid((unsigned)((a.Npad + fpb - 1) / fpb));
-    switch (a.KP) {
+    switch (KPc) {
 #define X(K)                                                                                      \
-    case K:                                                                                       \
+    case K: {                                                                                     \
+        constexpr int KL = K <= 32 ? 32 : K <= 64 ? 64 : K <= 128 ? 128 : 256;                    \
         if (fpb == 256) {                                                                         \
             if (prec == 2)                                                                        \
-                hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, _Float16, (K % 64 == 0 ? 8 : 4)>), grid, dim3(512), 0, s, a); \
+                hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, KL, _Float16, (K % 64 == 0 ? 8 : 4)>), grid, dim3(512), 0, s, a); \
             else                                                                                  \
-                hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, __bf16, (K % 64 == 0 ? 8 : 4)>), grid, dim3(512), 0, s, a); \
+                hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, KL, __bf16, (K % 64 == 0 ? 8 : 4)>), grid, dim3(512), 0, s, a); \
         } else if (prec == 2)                                                                     \
-            hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, _Float16, 4>), grid, dim3(256), 0, s, a); \
+            hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, KL, _Float16, 4>), grid, dim3(256), 0, s, a); \
         else                                                                                      \
-            hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, __bf16, 4>), grid, dim3(256), 0, s, a);   \
-        break;
+            hipLaunchKernelGGL((loglik_mixed_dma_kernel<K, KL, __bf16, 4>), grid, dim3(256), 0, s, a);   \
+        break;                                                                                    \
+    }
         GMMHIP_KP32_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;

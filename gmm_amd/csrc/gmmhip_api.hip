@@ -123,7 +123,7 @@ int choose_mixed_ll_nw(const gmmhip_ctx *h, int path) {
         gmmhip_tune::Entry e;
         f = gmmhip_tune::lookup(path, h->opt.precision, h->D, h->M, &e) ? e.ll_frames : 128;
     }
-    return (f >= 256 && kp_mixed_for_dim(h->D) % 64 == 0) ? 8 : 4;
+    return (f >= 256 && kp_mixed_contract(h->D) % 64 == 0) ? 8 : 4;
 }
 
 uint64_t tuned_chunk(const gmmhip_ctx *h, int path, int prec) {
@@ -602,7 +602,7 @@ int gmmhip_loglikes_device(gmmhip_t h, float *d_out) {
         m.LL2 = h->d_LL2;
         m.LLnat = d_out;
         m.blockLL = h->d_blockLL;
-        h->used_ll_frames = mixed_ll_frames_per_block(m.KP, m.ll_nw);
+        h->used_ll_frames = mixed_ll_frames_per_block(m.D, m.ll_nw);
         HIPCHK(h, launch_loglik_mixed(m, h->opt.precision, h->stream));
         return GMMHIP_OK;
     }
@@ -671,7 +671,7 @@ int gmmhip_accumulate(gmmhip_t h, int flags, double *n, double *F, double *S, do
     const int ngroups = (Mtx + st_nw - 1) / st_nw;
     const uint64_t chunk_frames = pick_chunk_frames(h, ngroups, mixed);
     const int ll_sub =
-        mixed ? mixed_ll_frames_per_block(h->KPm, choose_mixed_ll_nw(h, GMMHIP_TUNE_ESTEP)) / 64
+        mixed ? mixed_ll_frames_per_block(h->D, choose_mixed_ll_nw(h, GMMHIP_TUNE_ESTEP)) / 64
               : choose_ll_sub(h, GMMHIP_TUNE_ESTEP);
     h->used_ll_frames = 64 * ll_sub;
     h->used_chunk = (int)chunk_frames;

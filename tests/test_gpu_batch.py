@@ -163,10 +163,10 @@ def test_batch_mixed_precision(acc, prec, tol, D):
     means and MLLR transforms against the oracle (mixed-precision
     tolerances, as the E-step's), graph replays bit-identical to eager runs,
     and each utterance's result independent of its batch-mates (UBM-based
-    centring and scaling, chunks relative to the utterance start).  Depths
-    KP = 32, 64, 256 (the persistent work-queue form of the statistics kernel
-    with transposing LDS reads) and 96 (the register-staged statistics kernel
-    with the ZB copy, launched as a grid)."""
+    centring and scaling, chunks relative to the utterance start).  Feature
+    rows of 32, 64, 128 and 256 columns (contraction depths 32, 64, 96 and
+    256), every one through the persistent work-queue form of the statistics
+    kernel."""
     x, off, w, mu, var = make_batch(11, 30, 64, D, lo=20, hi=600, extra=(0, 5))
     acc.set_model(w, mu, var)
     ref = per_utt_oracle(x, off, w, mu, var)

@@ -67,9 +67,10 @@ def check(acc, prec, x, w, mu, var, ref_ll=None, ref_st=None):
                                    (3000, 39, 64), (400, 127, 40), (2049, 60, 257),
                                    (600, 70, 20), (500, 90, 9), (300, 100, 12)])
 def test_mixed_shapes(acc, prec, N, D, M):
-    """Depths KP = 32, 64, 128, 256 (statistics from the ZA rows by
-    transposing LDS reads) and 96, 160, 192, 224 (register-staged statistics
-    kernel with the ZB copy)."""
+    """Contraction depths 2D+1 rounded up to 32 = 32, 64, 128, 256 (feature
+    rows of that depth) and 96, 160, 192, 224 (feature rows padded with zero
+    columns to 128 or 256, of which the log-likelihood kernel reads only the
+    contraction depth; the statistics kernel reads the whole row)."""
     x, w, mu, var = case(500 + N + D + M, N, D, M)
     check(acc, prec, x, w, mu, var)
 

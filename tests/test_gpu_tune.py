@@ -117,7 +117,8 @@ def test_mixed_frame_tiles_identical(acc, prec, D, M, N):
                                    (60, 2048, 4096 + 5)])
 def test_mixed_comp_tiles_identical(acc, prec, D, M, N):
     """The mixed statistics component tile (128 or 256 components per block;
-    256 only at depths 64 and 128, else the handle keeps 128): every
+    256 only for feature rows of 64 and 128 columns, else the handle keeps
+    128): every
     component's statistics come from the same wave-level contraction over
     the same frame steps, so n, F, S, the per-frame log-likelihoods and the
     batched path's outputs are bit-identical.  M = 300 and 70 leave waves of
@@ -129,17 +130,18 @@ def test_mixed_comp_tiles_identical(acc, prec, D, M, N):
     x = synth.sample_frames_np(w, mu, var, N, 6)
     acc.set_model(w.astype(np.float32), mu, var)
     acc.set_frames(x)
-    kpm = (2 * D + 1 + 31) // 32 * 32
+    kpc = (2 * D + 1 + 31) // 32 * 32
+    kpl = 32 if kpc <= 32 else 64 if kpc <= 64 else 128 if kpc <= 128 else 256  # row layout
     res = {}
     try:
         for ct in (128, 256):
             acc.set_opts(precision=prec, ll_frames=128, chunk_frames=1024, comp_tile=ct)
             st = acc.accumulate()
             used = acc.tiles_used()
-            assert used == (128, 1024, ct if kpm in (64, 128) else 128), used
+            assert used == (128, 1024, ct if kpl in (64, 128) else 128), used
             off = np.array([0, 300, N // 2, N], np.uint64)
             bt = acc.batch_adapt(x, off, mllr=False)
-            assert acc.tiles_used()[2] == (ct if kpm in (64, 128) else 128)
+            assert acc.tiles_used()[2] == (ct if kpl in (64, 128) else 128)
             res[ct] = (st, bt)
     finally:
         acc.set_opts(precision=gmm_amd.PREC_FP32, ll_frames=0, chunk_frames=0, comp_tile=0)
