@@ -14,13 +14,10 @@
 // every utterance is copied to a 32-row-aligned place (pad_rows_kernel),
 // prep_frames splits the frames centred and scaled by the UBM's global mean
 // and deviation (so each utterance's result does not depend on the batch),
-// then loglik_mixed and the statistics over the utterances' chunks -- at
-// depths of a power-of-two number of 16-byte pieces (D <= 127 but for 2D+1
-// in 65..96 or 129..224) the persistent form of stats_mixed_tr, whose blocks
-// pull (chunk, group) items from per-XCD device counters, else the grid
-// form of the register-staged kernel.  A batch
-// with a frame value outside the 16-bit range, or a model whose
-// coefficients overflow it, runs in FP32.
+// then loglik_mixed and the statistics over the utterances' chunks in the
+// persistent form of stats_mixed_tr, whose blocks pull (chunk, group) items
+// from per-XCD device counters.  A batch with a frame value outside the
+// 16-bit range, or a model whose coefficients overflow it, runs in FP32.
 // With cfg->graph the whole pipeline is captured once into a hipGraph and
 // replayed while the batch shape and buffers stay the same.
 #include "batch.h"
@@ -53,8 +50,8 @@ struct gmmhip_batch_state {
     // centring / scaling and the split model for them
     float *d_xpad = nullptr;
     size_t xpad_cap = 0;
-    uint16_t *d_zah = nullptr, *d_zal = nullptr, *d_zbh = nullptr, *d_zbl = nullptr;
-    size_t zah_cap = 0, zal_cap = 0, zbh_cap = 0, zbl_cap = 0;
+    uint16_t *d_zah = nullptr, *d_zal = nullptr;
+    size_t zah_cap = 0, zal_cap = 0;
     uint8_t *d_bad = nullptr;
     size_t bad_cap = 0;
     unsigned int *d_nrange = nullptr;
@@ -83,8 +80,8 @@ void batch_state_free(gmmhip_batch_state *b) {
                     (void *)b->d_n, (void *)b->d_F, (void *)b->d_means, (void *)b->d_G,
                     (void *)b->d_K, (void *)b->d_W, (void *)b->d_status, (void *)b->d_P,
                     (void *)b->d_Xi, (void *)b->d_Ginit, (void *)b->d_kinit, (void *)b->d_ivar,
-                    (void *)b->d_xpad, (void *)b->d_zah, (void *)b->d_zal, (void *)b->d_zbh,
-                    (void *)b->d_zbl, (void *)b->d_bad, (void *)b->d_nrange, (void *)b->d_src,
+                    (void *)b->d_xpad, (void *)b->d_zah, (void *)b->d_zal, (void *)b->d_bad,
+                    (void *)b->d_nrange, (void *)b->d_src,
                     (void *)b->d_dst, (void *)b->d_zoff, (void *)b->d_zscale, (void *)b->d_P16})
         if (p) hipFree(p);
     delete b;
@@ -279,8 +276,6 @@ int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
         m.KP = p.KPm;
         m.za_hi = b->d_zah;
         m.za_lo = b->d_zal;
-        m.zb_hi = b->d_zbh;
-        m.zb_lo = b->d_zbl;
         m.bad = b->d_bad;
         m.P16 = b->d_P16;
         m.M = p.M;
@@ -299,17 +294,13 @@ int enqueue_pipeline(gmmhip_ctx *h, gmmhip_batch_state *b, const BatchPlan &p) {
         m.partials = h->d_part;
         m.chunk_lo = b->d_lo;
         m.chunk_hi = b->d_hi;
-        if (!mixed_needs_zb(p.KPm)) {
-            // persistent grid over the (chunk, group) items; counters reset
-            // by a kernel so the pipeline stays capturable
-            m.queue = b->d_queue;
-            m.queue_blocks =
-                mixed_stats_queue_blocks(p.KPm, p.st_nw, h->num_cus, p.nchunks * m.ngroups);
-            HIPCHK(h, launch_fill_u32(b->d_queue, 0u, 8, s));
-            HIPCHK(h, launch_stats_mixed_queue(m, p.prec, s));
-        } else {
-            HIPCHK(h, launch_stats_mixed(m, p.prec, s));
-        }
+        // persistent grid over the (chunk, group) items; counters reset by a
+        // kernel so the pipeline stays capturable
+        m.queue = b->d_queue;
+        m.queue_blocks =
+            mixed_stats_queue_blocks(p.KPm, p.st_nw, h->num_cus, p.nchunks * m.ngroups);
+        HIPCHK(h, launch_fill_u32(b->d_queue, 0u, 8, s));
+        HIPCHK(h, launch_stats_mixed_queue(m, p.prec, s));
     } else if (!p.per_utt) {
         HIPCHK(h, launch_loglik_fp32(a, s));
         a.nchunks = p.nchunks;
@@ -484,11 +475,6 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
             const size_t zn = (size_t)p.Npad * p.KPm;
             HIPCHK(h, ensure(&b->d_zah, &b->zah_cap, zn));
             HIPCHK(h, ensure(&b->d_zal, &b->zal_cap, zn));
-            const bool need_zb = mixed_needs_zb(p.KPm);
-            if (need_zb) {
-                HIPCHK(h, ensure(&b->d_zbh, &b->zbh_cap, zn));
-                HIPCHK(h, ensure(&b->d_zbl, &b->zbl_cap, zn));
-            }
             HIPCHK(h, ensure(&b->d_bad, &b->bad_cap, (size_t)p.Npad));
             if (!b->d_nrange) HIPCHK(h, hipMalloc((void **)&b->d_nrange, sizeof(unsigned int)));
             HIPCHK(h, hipMemcpyAsync(b->d_src, offsets, (U + 1) * 8, hipMemcpyHostToDevice, h->stream));
@@ -506,8 +492,6 @@ extern "C" int gmmhip_batch_adapt(gmmhip_t h, const float *frames, int frames_on
             pa.scale = b->d_zscale;
             pa.za_hi_raw = b->d_zah;
             pa.za_lo_raw = b->d_zal;
-            pa.zb_hi_raw = need_zb ? b->d_zbh : nullptr;
-            pa.zb_lo_raw = need_zb ? b->d_zbl : nullptr;
             pa.bad = b->d_bad;
             pa.nrange = b->d_nrange;
             pa.zlimit = prec == GMMHIP_PREC_FP16X3 ? 240.0f : INFINITY;

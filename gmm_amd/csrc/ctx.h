@@ -66,13 +66,12 @@ struct gmmhip_ctx {
     std::vector<double> m_gconst;
     uint64_t model_version = 0, frames_version = 0;
 
-    // mixed-precision frame features (ZA hi/lo, and ZB hi/lo for depths
-    // that need it) and split model
+    // mixed-precision frame features (ZA hi/lo) and split model
     int z_prec = 0, KPm = 0;
     uint64_t z_version = 0, z_N = 0, z_Npad = 0;
     bool z_range_fallback = false;
-    uint16_t *d_za_hi = nullptr, *d_za_lo = nullptr, *d_zb_hi = nullptr, *d_zb_lo = nullptr;
-    size_t z_cap = 0, zb_cap = 0;
+    uint16_t *d_za_hi = nullptr, *d_za_lo = nullptr;
+    size_t z_cap = 0;
     uint8_t *d_bad = nullptr;
     size_t bad_cap = 0;
     unsigned int *d_nrange = nullptr;

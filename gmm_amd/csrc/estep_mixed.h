@@ -19,12 +19,6 @@ inline int kp_mixed_for_dim(int D) {
     return k <= 32 ? 32 : k <= 64 ? 64 : k <= 128 ? 128 : 256;
 }
 
-// The 32-frame-block copy ZB is read only by the statistics kernel of depths
-// that are not a power-of-two number of 16-byte pieces; at KP = 32, 64, 128,
-// 256 the statistics kernel forms it from ZA with transposing LDS reads and
-// ZB is neither allocated nor written.
-inline bool mixed_needs_zb(int KP) { return KP != 32 && KP != 64 && KP != 128 && KP != 256; }
-
 // largest feature dimension of the mixed forms (2D+1 <= 256)
 constexpr int PREP_MAX_D = 127;
 
@@ -35,7 +29,6 @@ struct PrepArgs {
     const float *off;      // [D] centring
     const float *scale;    // [D] 1/std
     uint16_t *za_hi_raw, *za_lo_raw;  // [Npad][KP]
-    uint16_t *zb_hi_raw, *zb_lo_raw;  // [Npad/32][32*KP]; null: not written
     uint8_t *bad;          // [N] written for every frame: 1 non-finite, 2 out of the 16-bit range, 0 ok
     unsigned int *nrange;  // count of (frame, dim) values flagged 2
     float zlimit;          // |z| above this is out of range (fp16: 240)
@@ -44,14 +37,15 @@ struct PrepArgs {
 struct MixedArgs {
     uint64_t N, Npad;
     int D, KP;
-    const void *za_hi, *za_lo, *zb_hi, *zb_lo;
+    const void *za_hi, *za_lo;
     const uint8_t *bad;
     const void *P16;       // split model [Mt32][KP/16][2][64] x 16 B
     int M, Mt32;
     float min_ll, min_gamma;
     float *LL2, *LLnat;
     double *blockLL;
-    int ll_nw;             // log-likelihood waves per block: 4 (128 frames) or 8 (256, KP % 64 == 0)
+    int ll_nw;             // log-likelihood waves per block: 4 (128 frames) or 8 (256, the
+                           // contraction depth a multiple of 64)
     int st_nw;             // statistics waves per block: 4 (128 components) or 8 (256; KP 64
                            // or 128); ngroups = ceil(Mt32 / st_nw)
     uint64_t chunk_frames;
@@ -97,9 +91,8 @@ int mixed_ll_frames_per_block(int D, int nw);
 // (128 or 256) at depth KP: 8 for 256 at KP 64 or 128, else 4.
 int mixed_stats_waves(int KP, int comp_tile);
 hipError_t launch_stats_mixed(const MixedArgs &a, int prec, hipStream_t s);
-// Persistent form of the statistics kernel at depths whose feature row is a
-// power-of-two number of 16-byte pieces (mixed_needs_zb false): queue_blocks
-// blocks pull (chunk, group) items, chunk-major, from eight per-XCD lists
+// Persistent form of the statistics kernel: queue_blocks blocks pull
+// (chunk, group) items, chunk-major, from eight per-XCD lists
 // (counters a.queue[0..7], zero at launch) until all are empty.  Same
 // partials as launch_stats_mixed, bit for bit.
 hipError_t launch_stats_mixed_queue(const MixedArgs &a, int prec, hipStream_t s);

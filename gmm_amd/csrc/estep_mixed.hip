@@ -10,19 +10,11 @@
 // offset data; the model coefficients absorb o and s.
 //
 // Frame features [1, z, z^2, 0...] are split once per frame set
-// (prep_frames_kernel) into
-//   ZA[n][k]                   row-major: the log-likelihood contraction
-//                              reads 8 consecutive k of one frame per lane;
-//                              the statistics kernel of power-of-two depths
-//                              also takes its frame-contraction operand from
-//                              these rows, by transposing LDS reads;
-//   ZB[blk][u][ct][h][c][j]    (other depths only) per 32-frame block: the
-//                              register-staged statistics kernel's frame
-//                              contraction reads, for column 32ct+c, the 8
-//                              frames 16u + 8(j/4) + 4h + j%4 that lane
-//                              (h, c) holds after the first contraction (the
-//                              32 lanes of a half-wave read consecutive
-//                              16-byte pieces).
+// (prep_frames_kernel) into rows ZA[n][k] of KP columns, KP a power of two
+// (zero columns past 2D+1): the log-likelihood contraction reads 8
+// consecutive k of one frame per lane (the first 2D+1 rounded up to 32 of
+// them), and the statistics kernel takes its frame-contraction operand from
+// the same rows by transposing LDS reads.
 // MFMA 32x32x16 lane layout (lane = 32h + c): A[row c][k 8h+j],
 // B[k 8h+j][col c], C reg r -> row 8(r/4) + 4h + r%4, col c.
 #include "estep.h"
@@ -213,10 +205,10 @@ hipError_t launch_pack_split_model(const SplitModelArgs &a, int prec, hipStream_
 }
 
 // ---------------------------------------------------------------------------
-// Split frame features into hi/lo pieces in both layouts.  One block per
-// 32-frame block: the block's centred features are staged in LDS once, then
-// every store is one 16-byte piece (8 features of a frame for ZA, 8 frames
-// of a feature for ZB) and consecutive threads write consecutive pieces.
+// Split frame features into hi/lo pieces.  One block per 32-frame block:
+// the block's centred features are staged in LDS once, then every store is
+// one 16-byte piece (8 features of a frame) and consecutive threads write
+// consecutive pieces.
 // Frames past N (padding to a multiple of 32) are zero rows.  bad[n] is
 // written for every frame: 1 if the frame holds a non-finite value, else 2
 // if a centred value lies outside the 16-bit range, else 0.
@@ -282,12 +274,10 @@ __global__ __launch_bounds__(256) void prep_frames_kernel(PrepArgs p) {
         }
         return 0.0f;
     };
-    constexpr int U = KP / 8, CT = KP / 32;
-    constexpr int pieces = 32 * U;  // per layout and part
+    constexpr int U = KP / 8;
+    constexpr int pieces = 32 * U;  // per part
     u32x4 *zah = reinterpret_cast<u32x4 *>(p.za_hi_raw) + blk * pieces;
     u32x4 *zal = reinterpret_cast<u32x4 *>(p.za_lo_raw) + blk * pieces;
-    u32x4 *zbh = p.zb_hi_raw ? reinterpret_cast<u32x4 *>(p.zb_hi_raw) + blk * pieces : nullptr;
-    u32x4 *zbl = p.zb_lo_raw ? reinterpret_cast<u32x4 *>(p.zb_lo_raw) + blk * pieces : nullptr;
     // ZA[n][k]: piece q of frame f holds k = 8q .. 8q+7
     for (int e = tid; e < pieces; e += 256) {
         const int f = e / U, k0 = (e - f * U) * 8;
@@ -303,184 +293,9 @@ __global__ __launch_bounds__(256) void prep_frames_kernel(PrepArgs p) {
         zah[e] = H;
         zal[e] = L;
     }
-    // ZB[blk][u][ct][h][c][j]: slot ((u*CT + ct)*2 + h)*32 + c holds feature
-    // 32ct + c of frames 16u + 8(j/4) + 4h + j%4, j = 0..7
-    if (!zbh) return;
-    for (int e = tid; e < pieces; e += 256) {
-        const int c = e & 31, h = (e >> 5) & 1, rest = e >> 6;
-        const int ct = rest % CT, u = rest / CT;
-        const int k = 32 * ct + c;
-        u32x4 H, L;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int j0 = 2 * i, j1 = 2 * i + 1;
-            const int f0 = 16 * u + 8 * (j0 >> 2) + 4 * h + (j0 & 3);
-            const int f1 = 16 * u + 8 * (j1 >> 2) + 4 * h + (j1 & 3);
-            unsigned int l0, l1;
-            const unsigned int h0 = split_bits<T>(val(f0, k), l0);
-            const unsigned int h1 = split_bits<T>(val(f1, k), l1);
-            H[i] = h0 | (h1 << 16);
-            L[i] = l0 | (l1 << 16);
-        }
-        zbh[e] = H;
-        zbl[e] = L;
-    }
 }
 
 // ---------------------------------------------------------------------------
-// Statistics: block = (frame chunk, group of 128 components); wave w owns the
-// 32-component tile 4*group + w and its 32 x KP accumulators.  Per 32 frames
-// the block stages the ZA rows and the ZB block of hi/lo pieces in LDS
-// (register prefetch of the next step), forms posteriors from the first
-// contraction, splits them and contracts them with ZB.
-template <int KP, typename T>
-__global__ __launch_bounds__(256, (KP <= 192 ? 2 : 1)) void stats_mixed_kernel(MixedArgs a) {
-    typedef typename Vec8<T>::type V8;
-    constexpr int KS = KP / 16;
-    constexpr int CT = KP / 32;
-    constexpr int ZAS = KP / 8 + 1;                 // ZA row stride in u32x4 (padded)
-    constexpr int ZA16 = 32 * (KP / 8);             // u32x4 per hi (or lo) ZA block
-    constexpr int ZB16 = 32 * (KP / 8);             // u32x4 per hi (or lo) ZB block
-    constexpr int NPF = (2 * ZA16 + 2 * ZB16) / 256;  // u32x4 per thread per step
-    __shared__ u32x4 ZAl[2][32 * ZAS];
-    __shared__ u32x4 ZBl[2][ZB16];
-    __shared__ float Ls[32];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int h = lane >> 5, c = lane & 31;
-
-    const int b = blockIdx.x;
-    const int xcd = b & 7, idx = b >> 3;
-    const int group = idx % a.ngroups;
-    const int chunk = (idx / a.ngroups) * 8 + xcd;
-    if (chunk >= a.nchunks) return;
-    const int tile = group * 4 + wave;
-    const bool active = tile < a.Mt32;
-    const bool cvalid = (32 * tile + c) < a.M;
-
-    V8 bhi[KS], blo[KS];
-    {
-        const u32x4 *P16 = reinterpret_cast<const u32x4 *>(a.P16);
-#pragma unroll
-        for (int ks = 0; ks < KS; ks++) {
-            u32x4 x = (u32x4){0u, 0u, 0u, 0u}, y = (u32x4){0u, 0u, 0u, 0u};
-            if (active) {
-                x = P16[((size_t)tile * KS * 2 + ks * 2 + 0) * 64 + lane];
-                y = P16[((size_t)tile * KS * 2 + ks * 2 + 1) * 64 + lane];
-            }
-            bhi[ks] = __builtin_bit_cast(V8, x);
-            blo[ks] = __builtin_bit_cast(V8, y);
-        }
-    }
-    f32x16 acc2[CT];
-#pragma unroll
-    for (int j = 0; j < CT; j++) acc2[j] = (f32x16){};
-
-    uint64_t f0, f1;
-    if (a.chunk_lo) {
-        f0 = a.chunk_lo[chunk];
-        f1 = a.chunk_hi[chunk];
-    } else {
-        f0 = (uint64_t)chunk * a.chunk_frames;
-        f1 = f0 + a.chunk_frames;
-        if (f1 > a.N) f1 = a.N;
-    }
-
-    const u32x4 *zah = reinterpret_cast<const u32x4 *>(a.za_hi);
-    const u32x4 *zal = reinterpret_cast<const u32x4 *>(a.za_lo);
-    const u32x4 *zbh = reinterpret_cast<const u32x4 *>(a.zb_hi);
-    const u32x4 *zbl = reinterpret_cast<const u32x4 *>(a.zb_lo);
-    u32x4 pf[NPF];
-    float pl = INFINITY;
-    // base is a multiple of 32; the 32 ZA rows and the ZB block of a step are
-    // contiguous runs of ZA16 pieces each
-#define GMMHIP_PREFETCH_STEP(B_)                                                   \
-    _Pragma("unroll") for (int j = 0; j < NPF; j++) {                              \
-        const int i = tid + 256 * j;                                               \
-        const int part = i / ZA16, off = i - part * ZA16;                          \
-        const u32x4 *src = part == 0 ? zah : part == 1 ? zal : part == 2 ? zbh : zbl; \
-        pf[j] = src[(B_) * (KP / 8) + off];                                        \
-    }                                                                              \
-    if (tid < 32) pl = ((B_) + tid < f1) ? a.LL2[(B_) + tid] : INFINITY;
-    GMMHIP_PREFETCH_STEP(f0)
-
-    for (uint64_t base = f0; base < f1; base += 32) {
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < NPF; j++) {
-            const int i = tid + 256 * j;
-            const int part = i / ZA16, off = i - part * ZA16;
-            if (part < 2) {
-                const int row = off / (KP / 8), col = off - row * (KP / 8);
-                ZAl[part][row * ZAS + col] = pf[j];
-            } else {
-                ZBl[part - 2][off] = pf[j];
-            }
-        }
-        if (tid < 32) Ls[tid] = pl;
-        __syncthreads();
-        if (base + 32 < f1) {
-            GMMHIP_PREFETCH_STEP(base + 32)
-        }
-        if (!active) continue;
-
-        f32x16 accA = {}, accB = {};
-#pragma unroll
-        for (int ks = 0; ks < KS; ks++) {
-            const u32x4 xh = ZAl[0][c * ZAS + 2 * ks + h];
-            const u32x4 xl = ZAl[1][c * ZAS + 2 * ks + h];
-            const V8 zh = __builtin_bit_cast(V8, xh);
-            const V8 zl = __builtin_bit_cast(V8, xl);
-            accA = mfma32(zh, bhi[ks], accA);
-            accB = mfma32(zh, blo[ks], accB);
-            accB = mfma32(zl, bhi[ks], accB);
-        }
-
-        // posteriors, split into hi/lo A operands of the frame contraction:
-        // register r = 8u + j holds frame 16u + 8(j/4) + 4h + j%4
-        V8 ghi[2], glo[2];
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int row = 8 * (r >> 2) + 4 * h + (r & 3);
-            float g = exp2f_fast(accA[r] + accB[r] - Ls[row]);
-            g = (cvalid && g > a.min_gamma) ? g : 0.0f;
-            const T gh = (T)g;
-            const T gl = (T)(g - (float)gh);
-            ghi[r >> 3][r & 7] = gh;
-            glo[r >> 3][r & 7] = gl;
-        }
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-            // a 16-frame half with every posterior under the cut adds
-            // nothing: skip its products
-            const u32x4 gw = __builtin_bit_cast(u32x4, ghi[u]);
-            if (__builtin_amdgcn_ballot_w64((gw[0] | gw[1] | gw[2] | gw[3]) != 0u) == 0) continue;
-#pragma unroll
-            for (int ct = 0; ct < CT; ct++) {
-                const u32x4 yh4 = ZBl[0][((u * CT + ct) * 2 + h) * 32 + c];
-                const u32x4 yl4 = ZBl[1][((u * CT + ct) * 2 + h) * 32 + c];
-                const V8 yh = __builtin_bit_cast(V8, yh4);
-                const V8 yl = __builtin_bit_cast(V8, yl4);
-                acc2[ct] = mfma32(ghi[u], yh, acc2[ct]);
-                acc2[ct] = mfma32(ghi[u], yl, acc2[ct]);
-                acc2[ct] = mfma32(glo[u], yh, acc2[ct]);
-            }
-        }
-    }
-
-    if (!active) return;
-    const size_t Mp = (size_t)a.Mt32 * 32;
-    float *out = a.partials + ((size_t)chunk * Mp + (size_t)tile * 32) * KP;
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++)
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int row = 8 * (r >> 2) + 4 * h + (r & 3);
-            out[(size_t)row * KP + 32 * ct + c] = acc2[ct][r];
-        }
-}
-
 // Two FP32 values split into packed 16-bit hi and lo words (hi = T(v),
 // lo = T(v - hi), round to nearest), as the A operand words of the frame
 // contraction.
@@ -1115,7 +930,10 @@ __global__ __launch_bounds__(64 * NW) void loglik_mixed_dma_kernel(MixedArgs a) 
 }
 
 // ---------------------------------------------------------------------------
+// contraction depths of the log-likelihood kernel, and the layout depths of
+// the feature rows and the split model
 #define GMMHIP_KP32_CASES(X) X(32) X(64) X(96) X(128) X(160) X(192) X(224) X(256)
+#define GMMHIP_KPL_CASES(X) X(32) X(64) X(128) X(256)
 
 hipError_t launch_colstats(const float *X, uint64_t N, int D, int blocks, uint64_t per_block,
                            double *part, hipStream_t s) {
@@ -1124,7 +942,7 @@ hipError_t launch_colstats(const float *X, uint64_t N, int D, int blocks, uint64
 }
 
 hipError_t launch_prep_frames(const PrepArgs &p, int prec, hipStream_t s) {
-    if (p.D < 1 || p.D > PREP_MAX_D || p.KP % 32 || p.KP < 2 * p.D + 1 || p.Npad % 32)
+    if (p.D < 1 || p.D > PREP_MAX_D || p.KP != kp_mixed_for_dim(p.D) || p.Npad % 32)
         return hipErrorInvalidValue;
     const dim3 grid((unsigned)(p.Npad / 32));
     switch (p.KP) {
@@ -1135,7 +953,7 @@ hipError_t launch_prep_frames(const PrepArgs &p, int prec, hipStream_t s) {
         else                                                                        \
             hipLaunchKernelGGL((prep_frames_kernel<__bf16, K>), grid, dim3(256), 0, s, p);   \
         break;
-        GMMHIP_KP32_CASES(X)
+        GMMHIP_KPL_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;
     }
@@ -1173,8 +991,6 @@ hipError_t launch_loglik_mixed(const MixedArgs &a, int prec, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Feature rows of a power-of-two number of 16-byte pieces take the
-// direct-to-LDS statistics kernel; the others the register-staged one.
 int mixed_stats_waves(int KP, int comp_tile) {
     // (at KP = 256 eight waves would share a 256-register budget and spill)
     return (comp_tile == 256 && (KP == 64 || KP == 128)) ? 8 : 4;
@@ -1185,16 +1001,9 @@ static hipError_t launch_stats_mixed_form(const MixedArgs &a, int prec, dim3 gri
     if (a.st_nw != 4 && a.st_nw != 8) return hipErrorInvalidValue;
     if (a.st_nw == 8 && a.KP != 64 && a.KP != 128) return hipErrorInvalidValue;
     if (a.ngroups != (a.Mt32 + a.st_nw - 1) / a.st_nw) return hipErrorInvalidValue;
+    if (a.KP != kp_mixed_for_dim(a.D)) return hipErrorInvalidValue;
     switch (a.KP) {
-#define X(K)                                                                                  \
-    case K:                                                                                   \
-        if constexpr (Q) return hipErrorInvalidValue;                                         \
-        if (prec == 2)                                                                        \
-            hipLaunchKernelGGL((stats_mixed_kernel<K, _Float16>), grid, dim3(256), 0, s, a); \
-        else                                                                                  \
-            hipLaunchKernelGGL((stats_mixed_kernel<K, __bf16>), grid, dim3(256), 0, s, a);   \
-        break;
-#define XD(K)                                                                                     \
+#define X(K)                                                                                      \
     case K:                                                                                       \
         if (a.st_nw == 8 && (K == 64 || K == 128)) {                                              \
             if (prec == 2)                                                                        \
@@ -1206,9 +1015,8 @@ static hipError_t launch_stats_mixed_form(const MixedArgs &a, int prec, dim3 gri
         else                                                                                      \
             hipLaunchKernelGGL((stats_mixed_tr_kernel<K, __bf16, 4, Q>), grid, dim3(256), 0, s, a);   \
         break;
-        XD(32) XD(64) X(96) XD(128) X(160) X(192) X(224) XD(256)
+        GMMHIP_KPL_CASES(X)
 #undef X
-#undef XD
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1228,7 +1036,7 @@ int mixed_stats_queue_blocks(int KP, int st_nw, int num_cus, int nitems) {
 }
 
 hipError_t launch_stats_mixed_queue(const MixedArgs &a, int prec, hipStream_t s) {
-    if (!a.queue || a.queue_blocks < 1 || mixed_needs_zb(a.KP)) return hipErrorInvalidValue;
+    if (!a.queue || a.queue_blocks < 1) return hipErrorInvalidValue;
     return launch_stats_mixed_form<true>(a, prec, dim3((unsigned)a.queue_blocks), s);
 }
 

@@ -216,17 +216,6 @@ int prepare_mixed(gmmhip_ctx *h, bool *use_mixed) {
             HIPCHK(h, hipMalloc((void **)&h->d_za_lo, zn * 2));
             h->z_cap = zn;
         }
-        const bool need_zb = mixed_needs_zb(KPm);
-        if (need_zb && h->zb_cap < zn) {
-            bump_alloc_generation();
-            hipFree(h->d_zb_hi);
-            hipFree(h->d_zb_lo);
-            h->d_zb_hi = h->d_zb_lo = nullptr;
-            h->zb_cap = 0;
-            HIPCHK(h, hipMalloc((void **)&h->d_zb_hi, zn * 2));
-            HIPCHK(h, hipMalloc((void **)&h->d_zb_lo, zn * 2));
-            h->zb_cap = zn;
-        }
         HIPCHK(h, ensure(&h->d_bad, &h->bad_cap, (size_t)N));
         if (!h->d_nrange) HIPCHK(h, hipMalloc((void **)&h->d_nrange, sizeof(unsigned int)));
         HIPCHK(h, hipMemsetAsync(h->d_nrange, 0, sizeof(unsigned int), h->stream));
@@ -240,8 +229,6 @@ int prepare_mixed(gmmhip_ctx *h, bool *use_mixed) {
         p.scale = h->d_zscale;
         p.za_hi_raw = h->d_za_hi;
         p.za_lo_raw = h->d_za_lo;
-        p.zb_hi_raw = need_zb ? h->d_zb_hi : nullptr;
-        p.zb_lo_raw = need_zb ? h->d_zb_lo : nullptr;
         p.bad = h->d_bad;
         p.nrange = h->d_nrange;
         p.zlimit = (prec == GMMHIP_PREC_FP16X3) ? 240.0f : INFINITY;
@@ -282,8 +269,6 @@ MixedArgs make_mixed_args(gmmhip_ctx *h) {
     m.KP = h->KPm;
     m.za_hi = h->d_za_hi;
     m.za_lo = h->d_za_lo;
-    m.zb_hi = h->d_zb_hi;
-    m.zb_lo = h->d_zb_lo;
     m.bad = h->d_bad;
     m.P16 = h->d_P16;
     m.M = h->M;
@@ -357,8 +342,6 @@ int gmmhip_destroy(gmmhip_t h) {
     hipFree(h->d_tot);
     hipFree(h->d_za_hi);
     hipFree(h->d_za_lo);
-    hipFree(h->d_zb_hi);
-    hipFree(h->d_zb_lo);
     hipFree(h->d_bad);
     hipFree(h->d_nrange);
     hipFree(h->d_colpart);

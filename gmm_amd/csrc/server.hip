@@ -72,7 +72,7 @@ struct Layout {
     size_t beg, end, lo, hi, first, X, in_end;
     size_t ll, status, n, W, means, LLnat, out_end, out_copy;
     size_t queue, LL2, blockLL, part, F, G, K;
-    size_t zah, zal, zbh, zbl, bad, nrange, total;
+    size_t zah, zal, bad, nrange, total;
 };
 
 Layout layout_of(const Shape &s, const Dims &d) {
@@ -108,12 +108,9 @@ Layout layout_of(const Shape &s, const Dims &d) {
     L.F = take((size_t)Us * d.M * d.D * 8);
     L.G = take((size_t)s.Umllr * d.D * d.NP * 8);
     L.K = take((size_t)s.Umllr * d.D * d.E * 8);
-    const size_t zb = s.mixed ? Nz * KP * 2 : 0;
-    L.zah = take(zb);
-    L.zal = take(zb);
-    const size_t zbb = mixed_needs_zb(KP) ? zb : 0;
-    L.zbh = take(zbb);
-    L.zbl = take(zbb);
+    const size_t za = s.mixed ? Nz * KP * 2 : 0;
+    L.zah = take(za);
+    L.zal = take(za);
     L.bad = take(s.mixed ? s.N : 0);
     L.nrange = take(s.mixed ? 4 : 0);
     L.total = p;
@@ -326,8 +323,6 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
         p.scale = s->d_zscale;
         p.za_hi_raw = (uint16_t *)(A + L.zah);
         p.za_lo_raw = (uint16_t *)(A + L.zal);
-        p.zb_hi_raw = mixed_needs_zb(d.KPm) ? (uint16_t *)(A + L.zbh) : nullptr;
-        p.zb_lo_raw = mixed_needs_zb(d.KPm) ? (uint16_t *)(A + L.zbl) : nullptr;
         p.bad = bad;
         p.nrange = nrange;
         p.zlimit = s->zlimit;
@@ -339,8 +334,6 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
         m.KP = d.KPm;
         m.za_hi = p.za_hi_raw;
         m.za_lo = p.za_lo_raw;
-        m.zb_hi = p.zb_hi_raw;
-        m.zb_lo = p.zb_lo_raw;
         m.bad = bad;
         m.P16 = s->d_P16;
         m.M = d.M;
@@ -359,15 +352,11 @@ int launch_batch(gmmhip_server *s, gmmhip_server::Slot &sl) {
             m.partials = (float *)(A + L.part);
             m.chunk_lo = clo;
             m.chunk_hi = chi;
-            if (!mixed_needs_zb(d.KPm)) {
-                m.queue = (unsigned int *)(A + L.queue);
-                m.queue_blocks = mixed_stats_queue_blocks(d.KPm, m.st_nw, s->num_cus,
-                                                          sh.nchunks * m.ngroups);
-                SRV_HIPCHK(launch_fill_u32(m.queue, 0u, 8, st));
-                SRV_HIPCHK(launch_stats_mixed_queue(m, s->prec, st));
-            } else {
-                SRV_HIPCHK(launch_stats_mixed(m, s->prec, st));
-            }
+            m.queue = (unsigned int *)(A + L.queue);
+            m.queue_blocks = mixed_stats_queue_blocks(d.KPm, m.st_nw, s->num_cus,
+                                                      sh.nchunks * m.ngroups);
+            SRV_HIPCHK(launch_fill_u32(m.queue, 0u, 8, st));
+            SRV_HIPCHK(launch_stats_mixed_queue(m, s->prec, st));
         }
         roff = s->d_zoff;
         rscale = s->d_zscale;
