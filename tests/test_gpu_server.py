@@ -206,7 +206,21 @@ MIXED_TOL = {gmm_amd.PREC_FP16X3: (1e-5, 2e-4), gmm_amd.PREC_BF16X3: (2e-4, 2e-3
 @pytest.mark.parametrize("prec", [gmm_amd.PREC_FP16X3, gmm_amd.PREC_BF16X3])
 def test_server_mixed_alone_vs_mix_and_oracle(ubm, prec):
     w, mu, var = ubm
-    reqs = make_requests(21, w, mu, var, 40, lo=20, hi=1200)
+    check_mixed_server(w, mu, var, prec, make_requests(21, w, mu, var, 40, lo=20, hi=1200))
+
+
+def test_server_mixed_padded_rows():
+    """D = 40: the split feature rows hold 128 columns, of which the
+    log-likelihood kernel contracts 96 (2D+1 = 81 rounded up to 32) and the
+    statistics kernel reads all; alone = in the mix, and the oracle."""
+    if gmm_amd.device_count() < 1:
+        pytest.skip("no GPU")
+    w, mu, var = make_ubm(6, 64, 40)
+    check_mixed_server(w, mu, var, gmm_amd.PREC_FP16X3,
+                       make_requests(23, w, mu, var, 16, lo=20, hi=800))
+
+
+def check_mixed_server(w, mu, var, prec, reqs):
     lt, st = MIXED_TOL[prec]
     srv = gmm_amd.Server(w, mu, var, tau=16, chunk_frames=256, precision=prec)
     try:
