@@ -16,7 +16,8 @@ block; the mixed form's 128 / 256 frames per block (four or eight waves of 32
 frames sharing each model tile staged in LDS; 256 only where the depth
 2D+1 rounded up to 32 is a multiple of 64).  Component tiles: the mixed
 statistics kernel's 128 / 256 components per block (four or eight waves of 32
-components sharing each staged frame step; 256 at depths 64 and 128);
+components sharing each staged frame step; 256 where the split feature
+rows hold 64 or 128 columns);
 the FP32 form has one (written as 128).  The work-item size is swept for
 every form, in the E-step and (--batch) in the batched path, per precision.
 Frames come from a seeded true mixture (means ~ N(0, 2^2), variances
@@ -46,9 +47,11 @@ def frame_tiles(prec, D):
 
 
 def comp_tiles(prec, D):
-    """Candidate statistics component tiles of a form at dimension D."""
+    """Candidate statistics component tiles of a form at dimension D (256 where
+    the split feature rows, 2D+1 rounded up to 32 and then to a power of two,
+    hold 64 or 128 columns)."""
     kpm = (2 * D + 1 + 31) // 32 * 32
-    if prec == PREC["fp32"] or kpm not in (64, 128):
+    if prec == PREC["fp32"] or not 32 < kpm <= 128:
         return [128]
     return [128, 256]
 

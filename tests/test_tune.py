@@ -94,13 +94,15 @@ def test_autotune_frame_tile_candidates():
 
 
 def test_autotune_comp_tile_candidates():
-    """Mixed forms: 128 / 256 components per statistics block at depths 64
-    and 128 (at 256 eight waves would spill); other depths and FP32: 128
+    """Mixed forms: 128 / 256 components per statistics block where the
+    split feature rows hold 64 or 128 columns (contraction depths 64, 96 and
+    128; at 256 columns eight waves would spill); other depths and FP32: 128
     only."""
     fp32, fp16 = autotune.PREC["fp32"], autotune.PREC["fp16x3"]
     assert autotune.comp_tiles(fp32, 60) == [128]
     assert autotune.comp_tiles(fp16, 60) == [128, 256]   # depth 128
     assert autotune.comp_tiles(fp16, 20) == [128, 256]   # depth 64
-    assert autotune.comp_tiles(fp16, 39) == [128]        # depth 96
+    assert autotune.comp_tiles(fp16, 39) == [128, 256]   # depth 96, rows of 128
+    assert autotune.comp_tiles(fp16, 70) == [128]        # depth 160, rows of 256
     assert autotune.comp_tiles(fp16, 10) == [128]        # depth 32
-    assert autotune.comp_tiles(fp16, 127) == [128]        # depth 256
+    assert autotune.comp_tiles(fp16, 127) == [128]       # depth 256
